@@ -1,0 +1,208 @@
+/*
+ * piv.h -- C ABI of the MI355X PIV analysis engine (libpiv_amd.so).
+ *
+ * A drop-in for the per-pair hot path of the Rust `piv analyze` pipeline (agoodman/piv,
+ * crate piv-core).  Plain C types only: pointers, sizes, POD structs.  Every call returns a
+ * piv_status; the message of the last failure on the calling thread is piv_last_error().
+ * No C++ exception crosses this boundary.  Handles are safe to use from several host threads
+ * (calls on one analyzer are serialised internally).
+ *
+ * Each entry point names the reference interface it replaces (paths relative to
+ * rust/crates/piv-core/src unless stated).
+ */
+#ifndef PIV_AMD_H
+#define PIV_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PIV_ABI_VERSION 1
+#define PIV_MAX_PASSES 16
+
+/* PivError (error.rs:3-20) plus device-side conditions. */
+typedef enum {
+    PIV_OK = 0,
+    PIV_ERR_CONFIG = 1,          /* PivError::Config */
+    PIV_ERR_SHAPE_MISMATCH = 2,  /* PivError::ShapeMismatch */
+    PIV_ERR_IMAGE_TOO_SMALL = 3, /* PivError::ImageTooSmall */
+    PIV_ERR_DEVICE = 4,          /* HIP runtime / kernel failure */
+    PIV_ERR_OOM = 5,             /* device allocation failed */
+    PIV_ERR_ADMISSION = 6,       /* serving: request refused by the HBM budget */
+    PIV_ERR_CANCELLED = 7,       /* serving: request cancelled */
+    PIV_ERR_INVALID_ARGUMENT = 8,/* null pointer, bad enum, buffer too small */
+    PIV_ERR_UNSUPPORTED = 9      /* valid in the reference, not implemented on this device */
+} piv_status;
+
+/* Sample formats; u8 and u16 are normalised as decode.rs:227-234 does (v/255, v/65535). */
+typedef enum { PIV_PIXEL_U8 = 0, PIV_PIXEL_U16 = 1, PIV_PIXEL_F32 = 2 } piv_pixel_format;
+
+/* flags (field.rs:6-20) */
+#define PIV_FLAG_NO_SIGNAL 1u
+#define PIV_FLAG_LOW_PPR 2u
+#define PIV_FLAG_MEDIAN_OUTLIER 4u
+#define PIV_FLAG_REPLACED 8u
+#define PIV_FLAG_OUT_OF_RANGE 16u
+#define PIV_FLAG_INVALID 23u
+
+/* PivConfig + MaskConfig (multipass.rs:22-63), ValidationConfig (validate.rs:7-37) and
+ * PreprocessConfig (preprocess.rs:8-29) in one struct. */
+typedef struct {
+    uint32_t windows[PIV_MAX_PASSES]; /* window size per pass, largest first */
+    uint32_t num_windows;
+    float overlap;                    /* [0, 1) */
+    uint32_t final_iterations;        /* >= 1 */
+    float min_mean, min_std;          /* MaskConfig */
+    float ppr_threshold, median_threshold, median_eps; /* ValidationConfig */
+    int32_t replace;
+    float smooth_sigma;               /* PreprocessConfig */
+    uint32_t normalize_radius;
+    float normalize_eps;
+} piv_config;
+
+/* Grid (grid.rs:9-17) */
+typedef struct {
+    uint32_t window, step, nx, ny, x0, y0;
+} piv_grid;
+
+/* ValidationSummary (validate.rs:39-46) */
+typedef struct {
+    uint64_t no_signal, low_ppr, median_outliers, out_of_range, replaced;
+} piv_summary;
+
+typedef struct piv_analyzer piv_analyzer; /* PivAnalyzer (multipass.rs:85-88) */
+typedef struct piv_frame piv_frame;       /* PreparedFrame (preprocess.rs:130-135), on device */
+
+/* Caller-owned output of one pair: VectorField (field.rs:25-44) + summary. Arrays [ny*nx]. */
+typedef struct {
+    float *dx, *dy, *peak_ratio;
+    uint8_t *flags;
+    size_t capacity;     /* elements available in each array */
+    piv_grid grid;       /* filled */
+    piv_summary summary; /* filled (final pass, as PairResult::summary) */
+} piv_field_out;
+
+/* PassTrace (multipass.rs:72-81) plus the integer peak index the north-star parity needs.
+ * All arrays caller-owned, [nx*ny] (origins [nx*ny*4]); any may be NULL. */
+typedef struct {
+    size_t capacity;
+    piv_grid grid;             /* filled */
+    float *origins;            /* [ax, ay, bx, by] per cell */
+    float *raw_dx, *raw_dy, *raw_ppr;
+    uint8_t *raw_flags;
+    int32_t *peak_index;       /* iy*n + ix of the correlation maximum, -1 if no peak */
+    float *dx, *dy, *ppr;      /* validated */
+    uint8_t *flags;
+    piv_summary summary;
+} piv_pass_trace;
+
+const char *piv_last_error(void);
+int piv_abi_version(void);
+piv_status piv_device_count(int *count);
+
+/* PivConfig::default() etc.: windows [96, 64], overlap 0.5, final_iterations 2, mask 0/0,
+ * validation 1.3 / 2 / 0.1 / replace, preprocessing sigma 1, radius 4, eps 0.02. */
+void piv_config_default(piv_config *cfg);
+
+/* Grid::new (grid.rs:20-44) */
+piv_status piv_grid_new(uint32_t height, uint32_t width, uint32_t window, float overlap,
+                        piv_grid *out);
+
+/* PivAnalyzer::new (multipass.rs:91-119): validates the config, binds a device. */
+piv_status piv_analyzer_create(const piv_config *cfg, int device, piv_analyzer **out);
+void piv_analyzer_destroy(piv_analyzer *a);
+/* Number of passes: windows ++ [last] * (final_iterations - 1) (multipass.rs:165-171) */
+uint32_t piv_analyzer_num_passes(const piv_analyzer *a);
+
+/* PivAnalyzer::output_grid (multipass.rs:126-133) */
+piv_status piv_output_grid(const piv_analyzer *a, uint32_t height, uint32_t width,
+                           piv_grid *out);
+
+/* PreparedFrame::new (preprocess.rs:137-153) from HOST pixels (row_stride in bytes,
+ * background: optional host float [h*w]).  The frame lives on the analyzer's device. */
+piv_status piv_frame_prepare(piv_analyzer *a, const void *pixels, piv_pixel_format fmt,
+                             uint32_t height, uint32_t width, size_t row_stride,
+                             const float *background, piv_frame **out);
+/* Same from DEVICE pixels already resident in HBM (no host copy).  When no conditioning is
+ * configured (sigma 0, radius 0, no background) the frame references `pixels` without a copy
+ * and the caller keeps them alive until piv_frame_release; otherwise it is conditioned into
+ * frame-owned memory.  background: optional device float [h*w]. */
+piv_status piv_frame_prepare_device(piv_analyzer *a, const void *pixels, piv_pixel_format fmt,
+                                    uint32_t height, uint32_t width, size_t row_stride,
+                                    const float *background, piv_frame **out);
+void piv_frame_release(piv_frame *f);
+/* PreparedFrame's public `image` (and the background-subtracted base behind `stats`), as
+ * host float [h*w]; either pointer may be NULL. */
+piv_status piv_frame_download(const piv_frame *f, float *image, float *base);
+
+/* PivAnalyzer::analyze (multipass.rs:136-138) */
+piv_status piv_analyze_pair(piv_analyzer *a, const piv_frame *fa, const piv_frame *fb,
+                            piv_field_out *out);
+/* PivAnalyzer::analyze_traced (multipass.rs:142-150); traces[num_traces] with
+ * num_traces == piv_analyzer_num_passes */
+piv_status piv_analyze_traced(piv_analyzer *a, const piv_frame *fa, const piv_frame *fb,
+                              piv_field_out *out, piv_pass_trace *traces, uint32_t num_traces);
+
+/* Batched analysis of `npairs` independent pairs (the CLI's per-pair loop, analyze.rs:422-523,
+ * run as one launch per pass): fa[i], fb[i] -> DEVICE outputs dx/dy/ppr/flags of
+ * [npairs][ny][nx] and summaries [npairs] (host, may be NULL).  `stream` is a hipStream_t
+ * (NULL: the analyzer's stream); the call returns once the work is enqueued when
+ * `async` != 0, else after it completes. */
+piv_status piv_analyze_batch_device(piv_analyzer *a, const piv_frame *const *fa,
+                                    const piv_frame *const *fb, uint32_t npairs, float *dx,
+                                    float *dy, float *ppr, uint8_t *flags,
+                                    piv_summary *summaries, void *stream, int async);
+
+/* ---- stage-level entry points (parity tests replay each stage from the Rust fixtures) --- */
+
+/* multipass.rs:207-231 window placement for `grid`; prev == NULL on the first pass.
+ * prev_dx/prev_dy host float [prev->ny*prev->nx]; origins host float [grid->nx*ny*4]. */
+piv_status piv_place_windows(piv_analyzer *a, const piv_grid *grid, uint32_t height,
+                             uint32_t width, const piv_grid *prev, const float *prev_dx,
+                             const float *prev_dy, float *origins);
+
+/* PivAnalyzer::pass given origins (multipass.rs:233-259 with correlate.rs:99-139): mask,
+ * correlation and assembly of the raw field.  origins host [count*4]; outputs host [count]
+ * (peak_index may be NULL). */
+piv_status piv_correlate_windows(piv_analyzer *a, const piv_frame *fa, const piv_frame *fb,
+                                 uint32_t window, const float *origins, uint32_t count,
+                                 float *dx, float *dy, float *ppr, uint8_t *flags,
+                                 int32_t *peak_index);
+
+/* validate (validate.rs:51-130) in place on a host field [ny*nx] using the analyzer's
+ * ValidationConfig and max_displacement. */
+piv_status piv_validate_field(piv_analyzer *a, uint32_t ny, uint32_t nx, float *dx, float *dy,
+                              const float *ppr, uint8_t *flags, float max_displacement,
+                              piv_summary *summary);
+
+/* ---- measurement (no counterpart in the reference) ------------------------------------ */
+
+/* Per-kernel timing: when enabled, every kernel the analyzer launches is bracketed by HIP
+ * events recorded on the stream it is launched on.  `bytes` is the algorithmic traffic of
+ * the launches (window-gather model: 2 n^2 4 + 49 bytes per window for a correlation pass). */
+typedef struct {
+    char name[32];       /* "correlate", "validate", "prepare" */
+    uint32_t window;     /* window size for correlate, else 0 */
+    uint64_t launches;
+    uint64_t units;      /* windows (correlate) or vectors (validate) processed */
+    double total_ms;
+    double bytes;
+} piv_kernel_stat;
+
+/* enable != 0 turns timing on and clears the counters; 0 turns it off. */
+piv_status piv_profile_enable(piv_analyzer *a, int enable);
+/* Waits for the analyzer's work, then copies up to `capacity` stats; *count = total. */
+piv_status piv_profile_read(piv_analyzer *a, piv_kernel_stat *out, uint32_t capacity,
+                            uint32_t *count);
+
+/* decode.rs:227-234 sample normalisation on the device (host in/out), for parity checks. */
+piv_status piv_normalize_pixels(int device, const void *pixels, piv_pixel_format fmt,
+                                size_t count, float *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PIV_AMD_H */

@@ -237,7 +237,7 @@ void or_prepare(const float *raw, int32_t h, int32_t w, const float *bg, float s
         float v = raw[i];
         if (bg) {
             v = v - bg[i];
-            v = fmaxf(v, 0.0f); /* f32::max(v, 0.0): NaN -> 0 */
+            v = v > 0.0f ? v : 0.0f; /* f32::max(v, 0.0): NaN -> 0, -0 -> +0 */
         }
         base[i] = v;
     }

@@ -1,0 +1,493 @@
+"""MI355X-native PIV analysis engine: Python mirror of the piv-core API over the C ABI.
+
+Names, argument meaning and error behaviour follow the Rust crate piv-core (agoodman/piv):
+
+    PivConfig / MaskConfig / ValidationConfig / PreprocessConfig   (multipass.rs:22-63,
+                                                                    validate.rs:7-37,
+                                                                    preprocess.rs:8-29)
+    PivAnalyzer.new(cfg).analyze(a, b) -> PairResult             (multipass.rs:91-150)
+    PivAnalyzer.analyze_traced(a, b) -> (PairResult, [PassTrace])
+    PivAnalyzer.output_grid(h, w) -> Grid                         (multipass.rs:126-133)
+    PreparedFrame.new(raw, preprocess, background, analyzer)      (preprocess.rs:137-153)
+    Grid.new(h, w, window, overlap)                               (grid.rs:20-44)
+    validate(field, cfg, max_displacement)                        (validate.rs:51-130)
+
+Every computation runs in the HIP library (libpiv_amd.so) on the GPU; errors come back as
+PivError subclasses carrying the library's message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from . import _lib as L
+
+__all__ = ["PivConfig", "MaskConfig", "ValidationConfig", "PreprocessConfig", "PivAnalyzer",
+           "PreparedFrame", "Grid", "VectorField", "PairResult", "PassTrace",
+           "ValidationSummary", "PivError", "ConfigError", "ShapeMismatch", "ImageTooSmall",
+           "DeviceError", "flags", "validate", "place_windows", "correlate_windows",
+           "normalize_pixels"]
+
+
+class flags:  # field.rs:6-20
+    NO_SIGNAL = 1
+    LOW_PPR = 2
+    MEDIAN_OUTLIER = 4
+    REPLACED = 8
+    OUT_OF_RANGE = 16
+    INVALID = NO_SIGNAL | LOW_PPR | MEDIAN_OUTLIER | OUT_OF_RANGE
+
+
+class PivError(Exception):
+    """PivError (error.rs:3-20) and device-side failures."""
+    status = None
+
+
+class ConfigError(PivError):
+    status = L.ERR_CONFIG
+
+
+class ShapeMismatch(PivError):
+    status = L.ERR_SHAPE_MISMATCH
+
+
+class ImageTooSmall(PivError):
+    status = L.ERR_IMAGE_TOO_SMALL
+
+
+class DeviceError(PivError):
+    status = L.ERR_DEVICE
+
+
+class Unsupported(PivError):
+    status = L.ERR_UNSUPPORTED
+
+
+class InvalidArgument(PivError):
+    status = L.ERR_INVALID_ARGUMENT
+
+
+_ERRORS = {c.status: c for c in (ConfigError, ShapeMismatch, ImageTooSmall, DeviceError,
+                                 Unsupported, InvalidArgument)}
+
+
+def _check(rc):
+    if rc != L.OK:
+        msg = L.load().piv_last_error().decode(errors="replace")
+        raise _ERRORS.get(rc, PivError)(msg)
+
+
+@dataclass
+class MaskConfig:
+    min_mean: float = 0.0
+    min_std: float = 0.0
+
+
+@dataclass
+class ValidationConfig:
+    ppr_threshold: float = 1.3
+    median_threshold: float = 2.0
+    median_eps: float = 0.1
+    replace: bool = True
+
+
+@dataclass
+class PreprocessConfig:
+    smooth_sigma: float = 1.0
+    normalize_radius: int = 4
+    normalize_eps: float = 0.02
+
+    @staticmethod
+    def raw():
+        return PreprocessConfig(0.0, 0, 0.0)
+
+
+@dataclass
+class PivConfig:
+    windows: List[int] = field(default_factory=lambda: [96, 64])
+    overlap: float = 0.5
+    final_iterations: int = 2
+    mask: MaskConfig = field(default_factory=MaskConfig)
+    validation: ValidationConfig = field(default_factory=ValidationConfig)
+
+    def to_c(self, pre: PreprocessConfig) -> L.piv_config:
+        c = L.piv_config()
+        if len(self.windows) > L.MAX_PASSES:
+            raise ConfigError(f"invalid configuration: at most {L.MAX_PASSES} window sizes")
+        for i, w in enumerate(self.windows):
+            if not (0 <= int(w) < 2 ** 32):
+                raise ConfigError(f"invalid configuration: window size {w}")
+            c.windows[i] = int(w)
+        c.num_windows = len(self.windows)
+        c.overlap = self.overlap
+        c.final_iterations = max(int(self.final_iterations), 0)
+        c.min_mean = self.mask.min_mean
+        c.min_std = self.mask.min_std
+        c.ppr_threshold = self.validation.ppr_threshold
+        c.median_threshold = self.validation.median_threshold
+        c.median_eps = self.validation.median_eps
+        c.replace = 1 if self.validation.replace else 0
+        c.smooth_sigma = pre.smooth_sigma
+        c.normalize_radius = int(pre.normalize_radius)
+        c.normalize_eps = pre.normalize_eps
+        return c
+
+
+@dataclass(frozen=True)
+class Grid:
+    window: int
+    step: int
+    nx: int
+    ny: int
+    x0: int
+    y0: int
+
+    @staticmethod
+    def new(height, width, window, overlap) -> "Grid":
+        g = L.piv_grid()
+        _check(L.load().piv_grid_new(height, width, window, overlap, C.byref(g)))
+        return Grid._from_c(g)
+
+    @staticmethod
+    def _from_c(g) -> "Grid":
+        return Grid(g.window, g.step, g.nx, g.ny, g.x0, g.y0)
+
+    def _to_c(self):
+        return L.piv_grid(self.window, self.step, self.nx, self.ny, self.x0, self.y0)
+
+    def __len__(self):
+        return self.nx * self.ny
+
+    def center_x(self, ix):
+        return np.float32(self.x0 + ix * self.step) + (np.float32(self.window) - 1) / 2
+
+    def center_y(self, iy):
+        return np.float32(self.y0 + iy * self.step) + (np.float32(self.window) - 1) / 2
+
+    def xs(self):
+        return np.array([self.center_x(i) for i in range(self.nx)], np.float32)
+
+    def ys(self):
+        return np.array([self.center_y(i) for i in range(self.ny)], np.float32)
+
+
+@dataclass
+class ValidationSummary:
+    no_signal: int = 0
+    low_ppr: int = 0
+    median_outliers: int = 0
+    out_of_range: int = 0
+    replaced: int = 0
+
+    @staticmethod
+    def _from_c(s):
+        return ValidationSummary(int(s.no_signal), int(s.low_ppr), int(s.median_outliers),
+                                 int(s.out_of_range), int(s.replaced))
+
+
+@dataclass
+class VectorField:
+    grid: Grid
+    dx: np.ndarray
+    dy: np.ndarray
+    peak_ratio: np.ndarray
+    flags: np.ndarray
+
+    def valid_count(self):
+        return int(np.count_nonzero((self.flags & flags.INVALID) == 0))
+
+
+@dataclass
+class PairResult:
+    field: VectorField
+    summary: ValidationSummary
+
+
+@dataclass
+class PassTrace:
+    origins: np.ndarray     # [ny*nx, 4]
+    raw: VectorField
+    validated: VectorField
+    peak_index: np.ndarray  # [ny, nx] iy*n + ix, -1 where no peak
+    summary: ValidationSummary
+
+
+_FMT = {np.dtype(np.uint8): L.PIXEL_U8, np.dtype(np.uint16): L.PIXEL_U16,
+        np.dtype(np.float32): L.PIXEL_F32}
+
+
+class PivAnalyzer:
+    """PivAnalyzer (multipass.rs:85-150) on one GPU."""
+
+    def __init__(self, cfg: Optional[PivConfig] = None,
+                 preprocess: Optional[PreprocessConfig] = None, device: int = 0):
+        self.cfg = cfg if cfg is not None else PivConfig()
+        self.preprocess = preprocess if preprocess is not None else PreprocessConfig()
+        self._lib = L.load()
+        self._c = self.cfg.to_c(self.preprocess)
+        h = C.c_void_p()
+        _check(self._lib.piv_analyzer_create(C.byref(self._c), device, C.byref(h)))
+        self._h = h
+        self.device = device
+
+    @staticmethod
+    def new(cfg: PivConfig, preprocess: Optional[PreprocessConfig] = None, device: int = 0):
+        return PivAnalyzer(cfg, preprocess, device)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.piv_analyzer_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def config(self):
+        return self.cfg
+
+    @property
+    def num_passes(self):
+        return int(self._lib.piv_analyzer_num_passes(self._h))
+
+    def output_grid(self, height, width) -> Grid:
+        g = L.piv_grid()
+        _check(self._lib.piv_output_grid(self._h, height, width, C.byref(g)))
+        return Grid._from_c(g)
+
+    def prepare(self, raw, background=None) -> "PreparedFrame":
+        return PreparedFrame._prepare(self, raw, background)
+
+    def _pass_grids(self, h, w):
+        wins = list(self.cfg.windows) + [self.cfg.windows[-1]] * (self.cfg.final_iterations - 1)
+        return [Grid.new(h, w, n, self.cfg.overlap) for n in wins]
+
+    def analyze(self, a: "PreparedFrame", b: "PreparedFrame") -> PairResult:
+        g = self.output_grid(a.height, a.width) if a.shape == b.shape else None
+        if g is None:
+            raise ShapeMismatch(f"frames have different shapes: {a.shape} vs {b.shape}")
+        n = len(g)
+        dx, dy, pr = (np.empty(n, np.float32) for _ in range(3))
+        fl = np.empty(n, np.uint8)
+        out = L.piv_field_out(L.ptr(dx), L.ptr(dy), L.ptr(pr), L.ptr(fl), n)
+        _check(self._lib.piv_analyze_pair(self._h, a._h, b._h, C.byref(out)))
+        g = Grid._from_c(out.grid)
+        shp = (g.ny, g.nx)
+        return PairResult(VectorField(g, dx.reshape(shp), dy.reshape(shp), pr.reshape(shp),
+                                      fl.reshape(shp)), ValidationSummary._from_c(out.summary))
+
+    def analyze_traced(self, a: "PreparedFrame", b: "PreparedFrame"):
+        if a.shape != b.shape:
+            raise ShapeMismatch(f"frames have different shapes: {a.shape} vs {b.shape}")
+        grids = self._pass_grids(a.height, a.width)
+        n = len(grids[-1])
+        dx, dy, pr = (np.empty(n, np.float32) for _ in range(3))
+        fl = np.empty(n, np.uint8)
+        out = L.piv_field_out(L.ptr(dx), L.ptr(dy), L.ptr(pr), L.ptr(fl), n)
+        bufs = []
+        traces = (L.piv_pass_trace * len(grids))()
+        for i, g in enumerate(grids):
+            c = len(g)
+            b_ = dict(origins=np.empty(c * 4, np.float32), raw_dx=np.empty(c, np.float32),
+                      raw_dy=np.empty(c, np.float32), raw_ppr=np.empty(c, np.float32),
+                      raw_flags=np.empty(c, np.uint8), peak_index=np.empty(c, np.int32),
+                      dx=np.empty(c, np.float32), dy=np.empty(c, np.float32),
+                      ppr=np.empty(c, np.float32), flags=np.empty(c, np.uint8))
+            bufs.append(b_)
+            t = traces[i]
+            t.capacity = c
+            for k, v in b_.items():
+                setattr(t, k, L.ptr(v))
+        _check(self._lib.piv_analyze_traced(self._h, a._h, b._h, C.byref(out), traces,
+                                            len(grids)))
+        res = []
+        for i, g in enumerate(grids):
+            shp = (g.ny, g.nx)
+            b_ = bufs[i]
+            raw = VectorField(g, b_["raw_dx"].reshape(shp), b_["raw_dy"].reshape(shp),
+                              b_["raw_ppr"].reshape(shp), b_["raw_flags"].reshape(shp))
+            val = VectorField(g, b_["dx"].reshape(shp), b_["dy"].reshape(shp),
+                              b_["ppr"].reshape(shp), b_["flags"].reshape(shp))
+            res.append(PassTrace(b_["origins"].reshape(-1, 4), raw, val,
+                                 b_["peak_index"].reshape(shp),
+                                 ValidationSummary._from_c(traces[i].summary)))
+        g = Grid._from_c(out.grid)
+        shp = (g.ny, g.nx)
+        pair = PairResult(VectorField(g, dx.reshape(shp), dy.reshape(shp), pr.reshape(shp),
+                                      fl.reshape(shp)), ValidationSummary._from_c(out.summary))
+        return pair, res
+
+    def profile(self, enable=True):
+        """Turn per-kernel HIP-event timing on (clearing counters) or off."""
+        _check(self._lib.piv_profile_enable(self._h, 1 if enable else 0))
+
+    def profile_read(self):
+        """[{name, window, launches, units, total_ms, bytes}] since profile(True)."""
+        cnt = C.c_uint32(0)
+        _check(self._lib.piv_profile_read(self._h, None, 0, C.byref(cnt)))
+        arr = (L.piv_kernel_stat * max(cnt.value, 1))()
+        _check(self._lib.piv_profile_read(self._h, C.cast(arr, C.c_void_p), cnt.value,
+                                          C.byref(cnt)))
+        return [dict(name=arr[i].name.decode(), window=int(arr[i].window),
+                     launches=int(arr[i].launches), units=int(arr[i].units),
+                     total_ms=float(arr[i].total_ms), bytes=float(arr[i].bytes))
+                for i in range(cnt.value)]
+
+    def analyze_batch_device(self, frames_a, frames_b, dx, dy, ppr, flags_out,
+                             summaries=True, stream=None, sync=True):
+        """Analyse len(frames_a) pairs; outputs are device tensors [npairs, ny, nx]."""
+        n = len(frames_a)
+        A = (C.c_void_p * n)(*[f._h.value for f in frames_a])
+        B = (C.c_void_p * n)(*[f._h.value for f in frames_b])
+        sums = (L.piv_summary * n)() if summaries else None
+        _check(self._lib.piv_analyze_batch_device(
+            self._h, A, B, n, L.ptr(dx), L.ptr(dy), L.ptr(ppr), L.ptr(flags_out),
+            C.cast(sums, C.c_void_p) if sums is not None else None,
+            None if stream is None else C.c_void_p(stream), 0 if sync else 1))
+        if sums is None:
+            return None
+        return [ValidationSummary._from_c(s) for s in sums]
+
+
+class PreparedFrame:
+    """PreparedFrame (preprocess.rs:130-153): conditioned image + signal statistics, on the
+    analyzer's device."""
+
+    def __init__(self):
+        self._h = None
+
+    @staticmethod
+    def new(raw, preprocess: PreprocessConfig, background, analyzer: PivAnalyzer):
+        if (preprocess.smooth_sigma, preprocess.normalize_radius, preprocess.normalize_eps) != (
+                analyzer.preprocess.smooth_sigma, analyzer.preprocess.normalize_radius,
+                analyzer.preprocess.normalize_eps):
+            raise ConfigError("preprocess config differs from the analyzer's")
+        return PreparedFrame._prepare(analyzer, raw, background)
+
+    @staticmethod
+    def _prepare(an: PivAnalyzer, raw, background=None):
+        raw = np.ascontiguousarray(raw)
+        if raw.ndim != 2 or raw.dtype not in _FMT:
+            raise InvalidArgument(f"expected a 2-D uint8/uint16/float32 image, got {raw.dtype} "
+                                  f"{raw.shape}")
+        bg = None if background is None else np.ascontiguousarray(background, np.float32)
+        if bg is not None and bg.shape != raw.shape:
+            raise ShapeMismatch(f"background {bg.shape} vs frame {raw.shape}")
+        h = C.c_void_p()
+        _check(an._lib.piv_frame_prepare(an._h, L.ptr(raw), _FMT[raw.dtype], raw.shape[0],
+                                         raw.shape[1], raw.strides[0], L.ptr(bg), C.byref(h)))
+        f = PreparedFrame()
+        f._h, f._lib, f.shape, f._owner = h, an._lib, raw.shape, an
+        return f
+
+    @staticmethod
+    def from_device(an: PivAnalyzer, tensor, background=None):
+        """Wrap device-resident samples (a torch tensor on the analyzer's GPU, 2-D,
+        uint8 / int16-viewed uint16 / float32).  `fmt` follows the tensor dtype."""
+        import torch
+
+        fmt = {torch.uint8: L.PIXEL_U8, torch.uint16: L.PIXEL_U16,
+               torch.int16: L.PIXEL_U16, torch.float32: L.PIXEL_F32}[tensor.dtype]
+        if tensor.dim() != 2 or tensor.stride(1) != 1:
+            raise InvalidArgument("expected a 2-D row-major device tensor")
+        h = C.c_void_p()
+        _check(an._lib.piv_frame_prepare_device(
+            an._h, C.c_void_p(tensor.data_ptr()), fmt, tensor.shape[0], tensor.shape[1],
+            tensor.stride(0) * tensor.element_size(),
+            None if background is None else C.c_void_p(background.data_ptr()), C.byref(h)))
+        f = PreparedFrame()
+        f._h, f._lib, f.shape, f._owner = h, an._lib, tuple(tensor.shape), an
+        f._keep = (tensor, background)
+        return f
+
+    @property
+    def height(self):
+        return self.shape[0]
+
+    @property
+    def width(self):
+        return self.shape[1]
+
+    def dim(self):
+        return self.shape
+
+    def download(self):
+        """(image, base): the conditioned image and the background-subtracted samples."""
+        img = np.empty(self.shape, np.float32)
+        base = np.empty(self.shape, np.float32)
+        _check(self._lib.piv_frame_download(self._h, L.ptr(img), L.ptr(base)))
+        return img, base
+
+    @property
+    def image(self):
+        return self.download()[0]
+
+    def close(self):
+        if self._h:
+            self._lib.piv_frame_release(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---- stage-level entry points ---------------------------------------------------------
+
+def place_windows(an: PivAnalyzer, grid: Grid, height, width, prev: Optional[Grid] = None,
+                  prev_dx=None, prev_dy=None):
+    """Window origins [ax, ay, bx, by] per cell (multipass.rs:207-231)."""
+    o = np.empty((len(grid), 4), np.float32)
+    pdx = None if prev is None else np.ascontiguousarray(prev_dx, np.float32)
+    pdy = None if prev is None else np.ascontiguousarray(prev_dy, np.float32)
+    pg = None if prev is None else C.byref(prev._to_c())
+    g = grid._to_c()
+    _check(an._lib.piv_place_windows(an._h, C.byref(g), height, width, pg, L.ptr(pdx),
+                                     L.ptr(pdy), L.ptr(o)))
+    return o
+
+
+def correlate_windows(an: PivAnalyzer, a: PreparedFrame, b: PreparedFrame, window, origins):
+    """Raw field cells for given origins: (dx, dy, ppr, flags, peak_index)."""
+    o = np.ascontiguousarray(origins, np.float32).reshape(-1, 4)
+    n = o.shape[0]
+    dx, dy, pr = (np.empty(n, np.float32) for _ in range(3))
+    fl = np.empty(n, np.uint8)
+    pk = np.empty(n, np.int32)
+    _check(an._lib.piv_correlate_windows(an._h, a._h, b._h, window, L.ptr(o), n, L.ptr(dx),
+                                         L.ptr(dy), L.ptr(pr), L.ptr(fl), L.ptr(pk)))
+    return dx, dy, pr, fl, pk
+
+
+def validate(an: PivAnalyzer, dx, dy, ppr, flags_, max_displacement):
+    """validate (validate.rs:51-130) on copies; returns (dx, dy, flags, ValidationSummary)."""
+    dx = np.array(dx, np.float32, copy=True, order="C")
+    dy = np.array(dy, np.float32, copy=True, order="C")
+    fl = np.array(flags_, np.uint8, copy=True, order="C")
+    pr = np.ascontiguousarray(ppr, np.float32)
+    ny, nx = dx.shape
+    s = L.piv_summary()
+    _check(an._lib.piv_validate_field(an._h, ny, nx, L.ptr(dx), L.ptr(dy), L.ptr(pr), L.ptr(fl),
+                                      max_displacement, C.byref(s)))
+    return dx, dy, fl, ValidationSummary._from_c(s)
+
+
+def normalize_pixels(pixels, device=0):
+    """decode.rs:227-234 on the device."""
+    p = np.ascontiguousarray(pixels)
+    out = np.empty(p.shape, np.float32)
+    _check(L.load().piv_normalize_pixels(device, L.ptr(p), _FMT[p.dtype], p.size, L.ptr(out)))
+    return out
+
+
+def device_count():
+    n = C.c_int(0)
+    _check(L.load().piv_device_count(C.byref(n)))
+    return n.value

@@ -1,0 +1,10 @@
+// k_correlate instantiations for 24 px windows.
+#include "correlate.hpp"
+
+namespace piv {
+hipError_t launch_correlate_24(int fmt, const PassPair *pairs, const uint32_t *begins, int npairs,
+                              uint32_t total, uint32_t wpp, hipStream_t st)
+{
+    return launch_corr_n<24>(fmt, pairs, begins, npairs, total, wpp, st);
+}
+}  // namespace piv

@@ -1,0 +1,1179 @@
+// C-ABI implementation (include/piv.h): PivAnalyzer / PreparedFrame lifecycle, frame
+// conditioning and the multi-pass orchestration over the HIP kernels.
+//
+// Per pass (multipass.rs:152-192) one k_correlate launch covers every window of every pair
+// in the call, then one k_validate launch validates every pair's field; the validated field
+// stays in HBM as the next pass's predictor.  All per-pair descriptors of a call are
+// uploaded with a single host-to-device copy before the first launch.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/piv.h"
+#include "piv_device.hpp"
+#include "piv_kernels.hpp"
+
+using namespace piv;
+
+namespace {
+
+thread_local std::string g_err;
+
+piv_status fail(piv_status st, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return st;
+}
+
+#define HIP_TRY(expr)                                                                         \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(e_ == hipErrorOutOfMemory ? PIV_ERR_OOM : PIV_ERR_DEVICE, "%s: %s (%s:%d)", \
+                        #expr, hipGetErrorString(e_), __FILE__, __LINE__);                    \
+    } while (0)
+
+#define PIV_TRY(expr)                                                                         \
+    do {                                                                                      \
+        piv_status s_ = (expr);                                                               \
+        if (s_ != PIV_OK) return s_;                                                          \
+    } while (0)
+
+// Device buffer that grows on demand (contents not preserved).
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    piv_status reserve(size_t bytes)
+    {
+        if (bytes <= cap) return PIV_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t want = bytes < 256 ? 256 : bytes;
+        HIP_TRY(hipMalloc(&p, want));
+        cap = want;
+        return PIV_OK;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+struct HostBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    piv_status reserve(size_t bytes)
+    {
+        if (bytes <= cap) return PIV_OK;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t want = bytes < 4096 ? 4096 : bytes;
+        HIP_TRY(hipHostMalloc(&p, want, hipHostMallocDefault));
+        cap = want;
+        return PIV_OK;
+    }
+    void release()
+    {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+int bytes_per_px(int fmt) { return fmt == FMT_U8 ? 1 : (fmt == FMT_U16 ? 2 : 4); }
+
+// grid.rs:20-44
+piv_status grid_new(uint32_t h, uint32_t w, uint32_t window, float overlap, GridDev &g)
+{
+    if (!(overlap >= 0.0f && overlap < 1.0f))
+        return fail(PIV_ERR_CONFIG, "invalid configuration: overlap %g must be in [0, 1)", overlap);
+    if (window == 0 || window > w || window > h)
+        return fail(PIV_ERR_IMAGE_TOO_SMALL,
+                    "image (%ux%u) is smaller than the %u px interrogation window", h, w, window);
+    volatile float prod = (float)window * (1.0f - overlap);  // f32, as grid.rs:33
+    const float s = std::round(prod);
+    int step = s > 0.0f ? (int)s : 0;
+    if (step < 1) step = 1;
+    g.window = (int)window;
+    g.step = step;
+    g.nx = ((int)w - (int)window) / step + 1;
+    g.ny = ((int)h - (int)window) / step + 1;
+    g.x0 = ((int)w - (int)window - (g.nx - 1) * step) / 2;
+    g.y0 = ((int)h - (int)window - (g.ny - 1) * step) / 2;
+    return PIV_OK;
+}
+
+piv_grid to_pub(const GridDev &g)
+{
+    return piv_grid{(uint32_t)g.window, (uint32_t)g.step, (uint32_t)g.nx,
+                    (uint32_t)g.ny,     (uint32_t)g.x0,   (uint32_t)g.y0};
+}
+
+// preprocess.rs:161-168 kernel taps (host expf, as the reference's f32 exp)
+int gauss_taps(float sigma, std::vector<float> &k)
+{
+    float rf = std::ceil(3.0f * sigma);
+    if (rf < 1.0f) rf = 1.0f;
+    const int r = (int)rf;
+    k.assign(2 * r + 1, 0.0f);
+    for (int i = -r; i <= r; ++i) {
+        volatile float num = -(float)(i * i);
+        volatile float den = 2.0f * sigma * sigma;
+        k[i + r] = std::exp((float)(num / den));
+    }
+    float total = 0.0f;
+    for (int i = 0; i <= 2 * r; ++i) total += k[i];
+    for (int i = 0; i <= 2 * r; ++i) k[i] /= total;
+    return r;
+}
+
+}  // namespace
+
+struct piv_frame {
+    int device = 0;
+    int H = 0, W = 0;
+    int fmt = FMT_F32;         // format the correlation kernel reads `img` in
+    const void *img = nullptr; // device
+    int pitch = 0;             // pixels
+    DevBuf raw;                // owned copy of the input pixels (host input)
+    DevBuf image;              // owned conditioned f32 image
+    DevBuf base;               // owned background-subtracted base (f32), when materialised
+    DevBuf sat;                // owned f64 SAT of base (sum, then sum of squares)
+    int64_t cells = 0;
+    bool has_sat = false;
+    int in_fmt = FMT_F32;      // original sample format
+    const void *raw_ptr = nullptr;  // device pointer of the original samples
+    int raw_pitch = 0;
+    bool has_bg = false;
+    ~piv_frame()
+    {
+        raw.release();
+        image.release();
+        base.release();
+        sat.release();
+    }
+};
+
+struct piv_analyzer {
+    piv_config cfg{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    std::vector<int> passes;
+    std::vector<float> taps;
+    int taps_r = 0;
+    DevBuf d_taps;
+    // workspace
+    DevBuf fields;       // per-pass raw + validated fields of the current call
+    DevBuf desc;         // descriptors
+    HostBuf h_desc;
+    DevBuf tile_counts;
+    DevBuf summaries;    // u32 [npass][npairs][5]
+    HostBuf h_summaries;
+    DevBuf prep_tmp, prep_tmp2, prep_sat;  // conditioning scratch
+    DevBuf stage_in, stage_out;            // stage-level entry points
+    // kernel timing (piv_profile_*)
+    struct ProfRec {
+        std::string name;
+        int window;
+        double bytes;
+        uint64_t units;
+        hipEvent_t e0, e1;
+    };
+    bool prof_on = false;
+    std::vector<ProfRec> prof_pending;
+    std::vector<hipEvent_t> prof_pool;
+    std::map<std::pair<std::string, int>, piv_kernel_stat> prof_stats;
+    ~piv_analyzer()
+    {
+        for (auto &r : prof_pending) {
+            (void)hipEventDestroy(r.e0);
+            (void)hipEventDestroy(r.e1);
+        }
+        for (auto e : prof_pool) (void)hipEventDestroy(e);
+        fields.release();
+        desc.release();
+        h_desc.release();
+        tile_counts.release();
+        summaries.release();
+        h_summaries.release();
+        prep_tmp.release();
+        prep_tmp2.release();
+        prep_sat.release();
+        stage_in.release();
+        stage_out.release();
+        d_taps.release();
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+// PivAnalyzer::new checks (multipass.rs:91-119)
+piv_status check_config(const piv_config &c)
+{
+    if (c.num_windows == 0)
+        return fail(PIV_ERR_CONFIG, "invalid configuration: at least one window size is required");
+    if (c.num_windows > PIV_MAX_PASSES)
+        return fail(PIV_ERR_CONFIG, "invalid configuration: at most %d window sizes", PIV_MAX_PASSES);
+    for (uint32_t i = 0; i < c.num_windows; ++i)
+        if (c.windows[i] < 8 || c.windows[i] % 2 != 0)
+            return fail(PIV_ERR_CONFIG, "invalid configuration: window size %u must be even and >= 8",
+                        c.windows[i]);
+    if (!(c.overlap >= 0.0f && c.overlap < 1.0f))
+        return fail(PIV_ERR_CONFIG, "invalid configuration: overlap %g must be in [0, 1)", c.overlap);
+    if (c.final_iterations == 0)
+        return fail(PIV_ERR_CONFIG, "invalid configuration: final_iterations must be at least 1");
+    if (c.num_windows + c.final_iterations - 1 > 64)
+        return fail(PIV_ERR_CONFIG, "invalid configuration: too many passes");
+    for (uint32_t i = 0; i < c.num_windows; ++i)
+        if (!correlate_supported((int)c.windows[i]))
+            return fail(PIV_ERR_UNSUPPORTED,
+                        "window size %u is valid in the reference but has no gfx950 kernel yet "
+                        "(supported: 8, 12, 16, 24, 32, 48, 64, 96)",
+                        c.windows[i]);
+    if (c.smooth_sigma > 0.0f && std::ceil(3.0f * c.smooth_sigma) > 255.0f)
+        return fail(PIV_ERR_CONFIG, "smooth_sigma %g too large", c.smooth_sigma);
+    return PIV_OK;
+}
+
+// The window signal mask (multipass.rs:233-241) can be skipped without changing any result
+// only when it provably passes for every window: min_std <= 0 (std >= 0 always) and either
+// min_mean = -inf, or the base image is u8 samples without background (all values >= 0 and
+// every SAT sum exact in f64: multiples of 2^-31 below 2^22 fit 53 bits), so mean >= 0.
+bool mask_needed(const piv_config &c, int in_fmt, bool has_bg)
+{
+    if (!(c.min_std <= 0.0f)) return true;
+    if (c.min_mean == -std::numeric_limits<float>::infinity()) return false;
+    if (in_fmt == FMT_U8 && !has_bg && c.min_mean <= 0.0f) return false;
+    return true;
+}
+
+piv_status set_device(int dev)
+{
+    HIP_TRY(hipSetDevice(dev));
+    return PIV_OK;
+}
+
+hipError_t prof_event(piv_analyzer *a, hipEvent_t *e)
+{
+    if (!a->prof_pool.empty()) {
+        *e = a->prof_pool.back();
+        a->prof_pool.pop_back();
+        return hipSuccess;
+    }
+    return hipEventCreate(e);
+}
+
+// Launch `fn` bracketed by timing events when profiling is on.
+template <class F>
+hipError_t timed(piv_analyzer *a, const char *name, int window, double bytes, uint64_t units, F &&fn)
+{
+    if (!a->prof_on) return fn();
+    piv_analyzer::ProfRec r{name, window, bytes, units, nullptr, nullptr};
+    hipError_t e = prof_event(a, &r.e0);
+    if (e == hipSuccess) e = prof_event(a, &r.e1);
+    if (e == hipSuccess) e = hipEventRecord(r.e0, a->stream);
+    if (e != hipSuccess) return e;
+    e = fn();
+    if (e != hipSuccess) return e;
+    e = hipEventRecord(r.e1, a->stream);
+    if (e == hipSuccess) a->prof_pending.push_back(r);
+    return e;
+}
+
+// Fold completed timing events into the counters (the stream must be idle).
+void prof_harvest(piv_analyzer *a)
+{
+    for (auto &r : a->prof_pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, r.e0, r.e1) == hipSuccess) {
+            auto &st = a->prof_stats[{r.name, r.window}];
+            std::snprintf(st.name, sizeof st.name, "%s", r.name.c_str());
+            st.window = (uint32_t)r.window;
+            st.launches += 1;
+            st.units += r.units;
+            st.total_ms += ms;
+            st.bytes += r.bytes;
+        }
+        a->prof_pool.push_back(r.e0);
+        a->prof_pool.push_back(r.e1);
+    }
+    a->prof_pending.clear();
+}
+
+// PreparedFrame::new (preprocess.rs:137-153) on device samples.
+piv_status prepare_frame(piv_analyzer *a, piv_frame *f, const float *bg_dev)
+{
+    const piv_config &c = a->cfg;
+    const int H = f->H, W = f->W;
+    const size_t n = (size_t)H * W;
+    const bool cond = c.smooth_sigma > 0.0f || c.normalize_radius > 0 || bg_dev != nullptr;
+    const bool mask = mask_needed(c, f->in_fmt, bg_dev != nullptr);
+    f->has_bg = bg_dev != nullptr;
+    f->cells = (int64_t)(H + 1) * (W + 1);
+    hipStream_t st = a->stream;
+    if (cond || mask) {
+        PIV_TRY(f->base.reserve(n * sizeof(float)));
+        HIP_TRY(launch_base(f->in_fmt, f->raw_ptr, f->raw_pitch, bg_dev, H, W, f->base.as<float>(), st));
+    }
+    if (mask) {
+        PIV_TRY(f->sat.reserve((size_t)f->cells * 2 * sizeof(double)));
+        HIP_TRY(launch_sat(f->base.as<float>(), H, W, f->sat.as<double>(),
+                           f->sat.as<double>() + f->cells, st));
+        f->has_sat = true;
+    }
+    if (!cond) {
+        f->fmt = f->in_fmt;
+        f->img = f->raw_ptr;
+        f->pitch = f->raw_pitch;
+        return PIV_OK;
+    }
+    PIV_TRY(f->image.reserve(n * sizeof(float)));
+    const float *cur = f->base.as<float>();
+    if (c.smooth_sigma > 0.0f) {
+        PIV_TRY(a->prep_tmp.reserve(n * sizeof(float)));
+        float *dst = c.normalize_radius > 0 ? nullptr : f->image.as<float>();
+        if (!dst) {
+            PIV_TRY(a->prep_tmp2.reserve(n * sizeof(float)));
+            dst = a->prep_tmp2.as<float>();
+        }
+        HIP_TRY(launch_blur(cur, H, W, a->d_taps.as<float>(), a->taps_r, a->prep_tmp.as<float>(),
+                            dst, st));
+        cur = dst;
+    }
+    if (c.normalize_radius > 0) {
+        PIV_TRY(a->prep_sat.reserve((size_t)f->cells * 2 * sizeof(double)));
+        double *s = a->prep_sat.as<double>();
+        HIP_TRY(launch_sat(cur, H, W, s, s + f->cells, st));
+        HIP_TRY(launch_local_norm(cur, H, W, (int)c.normalize_radius, c.normalize_eps, s,
+                                  s + f->cells, f->image.as<float>(), st));
+    } else if (cur != f->image.as<float>()) {
+        HIP_TRY(hipMemcpyAsync(f->image.p, cur, n * sizeof(float), hipMemcpyDeviceToDevice, st));
+    }
+    f->fmt = FMT_F32;
+    f->img = f->image.p;
+    f->pitch = W;
+    return PIV_OK;
+}
+
+struct PassPlan {
+    GridDev grid;
+    int n;
+    size_t cells;  // per pair
+};
+
+piv_status plan_passes(piv_analyzer *a, int H, int W, std::vector<PassPlan> &plans)
+{
+    plans.clear();
+    for (int n : a->passes) {
+        PassPlan p{};
+        PIV_TRY(grid_new((uint32_t)H, (uint32_t)W, (uint32_t)n, a->cfg.overlap, p.grid));
+        p.n = n;
+        p.cells = (size_t)p.grid.nx * p.grid.ny;
+        plans.push_back(p);
+    }
+    return PIV_OK;
+}
+
+// Output targets of the last pass (device pointers, [npairs][cells]); null = workspace.
+struct BatchOut {
+    float *dx = nullptr, *dy = nullptr, *ppr = nullptr;
+    uint8_t *flags = nullptr;
+};
+
+struct FieldPtrs {
+    float *dx, *dy, *ppr;
+    uint8_t *flags;
+};
+
+// Runs every pass for npairs pairs.  Returns device pointers of each pass's raw and validated
+// fields (in the workspace unless `out` redirects the final validated field).
+piv_status run_batch(piv_analyzer *a, const piv_frame *const *fa, const piv_frame *const *fb,
+                     uint32_t npairs, const BatchOut &out, bool traced,
+                     std::vector<PassPlan> &plans, std::vector<FieldPtrs> &raw,
+                     std::vector<FieldPtrs> &val, std::vector<float *> &origins,
+                     std::vector<int32_t *> &peaks, uint32_t **summaries_dev)
+{
+    const int H = fa[0]->H, W = fa[0]->W;
+    for (uint32_t i = 0; i < npairs; ++i) {
+        if (!fa[i] || !fb[i]) return fail(PIV_ERR_INVALID_ARGUMENT, "null frame");
+        if (fa[i]->H != fb[i]->H || fa[i]->W != fb[i]->W)
+            return fail(PIV_ERR_SHAPE_MISMATCH, "frames have different shapes: (%d, %d) vs (%d, %d)",
+                        fa[i]->H, fa[i]->W, fb[i]->H, fb[i]->W);
+        if (fa[i]->H != H || fa[i]->W != W)
+            return fail(PIV_ERR_SHAPE_MISMATCH, "all pairs of a batch must share one frame size");
+        if (fa[i]->fmt != fa[0]->fmt || fb[i]->fmt != fa[0]->fmt)
+            return fail(PIV_ERR_INVALID_ARGUMENT, "all frames of a batch must share one sample format");
+        if (fa[i]->device != a->device || fb[i]->device != a->device)
+            return fail(PIV_ERR_INVALID_ARGUMENT, "frame belongs to another device");
+    }
+    PIV_TRY(plan_passes(a, H, W, plans));
+    const size_t np = plans.size();
+    // ---- workspace layout ----
+    size_t off = 0;
+    std::vector<size_t> o_raw(np), o_val(np), o_org(np), o_pk(np);
+    for (size_t i = 0; i < np; ++i) {
+        const size_t c = plans[i].cells * npairs;
+        o_raw[i] = off;
+        off = align_up(off + c * (3 * sizeof(float) + 1), 256);
+        o_val[i] = off;
+        off = align_up(off + c * (3 * sizeof(float) + 1), 256);
+        if (traced) {
+            o_org[i] = off;
+            off = align_up(off + c * 4 * sizeof(float), 256);
+            o_pk[i] = off;
+            off = align_up(off + c * sizeof(int32_t), 256);
+        }
+    }
+    PIV_TRY(a->fields.reserve(off));
+    char *base = a->fields.as<char>();
+    raw.assign(np, {});
+    val.assign(np, {});
+    origins.assign(np, nullptr);
+    peaks.assign(np, nullptr);
+    for (size_t i = 0; i < np; ++i) {
+        const size_t c = plans[i].cells * npairs;
+        auto carve = [&](size_t o) {
+            FieldPtrs f;
+            f.dx = reinterpret_cast<float *>(base + o);
+            f.dy = f.dx + c;
+            f.ppr = f.dy + c;
+            f.flags = reinterpret_cast<uint8_t *>(f.ppr + c);
+            return f;
+        };
+        raw[i] = carve(o_raw[i]);
+        val[i] = carve(o_val[i]);
+        if (traced) {
+            origins[i] = reinterpret_cast<float *>(base + o_org[i]);
+            peaks[i] = reinterpret_cast<int32_t *>(base + o_pk[i]);
+        }
+    }
+    if (out.dx) {
+        FieldPtrs &l = val[np - 1];
+        l.dx = out.dx;
+        l.dy = out.dy;
+        l.ppr = out.ppr;
+        l.flags = out.flags;
+    }
+    // ---- descriptors ----
+    std::vector<uint32_t> tiles_pp(np);
+    size_t dsz = 0;
+    std::vector<size_t> o_pp(np), o_pb(np), o_vp(np), o_tb(np);
+    for (size_t i = 0; i < np; ++i) {
+        const GridDev &g = plans[i].grid;
+        tiles_pp[i] = (uint32_t)(((g.nx + VALIDATE_TILE - 1) / VALIDATE_TILE) *
+                                 ((g.ny + VALIDATE_TILE - 1) / VALIDATE_TILE));
+        o_pp[i] = dsz;
+        dsz = align_up(dsz + npairs * sizeof(PassPair), 64);
+        o_pb[i] = dsz;
+        dsz = align_up(dsz + npairs * sizeof(uint32_t), 64);
+        o_vp[i] = dsz;
+        dsz = align_up(dsz + npairs * sizeof(ValidatePair), 64);
+        o_tb[i] = dsz;
+        dsz = align_up(dsz + npairs * sizeof(uint32_t), 64);
+    }
+    PIV_TRY(a->desc.reserve(dsz));
+    PIV_TRY(a->h_desc.reserve(dsz));
+    PIV_TRY(a->summaries.reserve(np * npairs * 5 * sizeof(uint32_t)));
+    uint32_t max_tiles = 0;
+    for (size_t i = 0; i < np; ++i) max_tiles = std::max(max_tiles, tiles_pp[i] * npairs);
+    PIV_TRY(a->tile_counts.reserve((size_t)max_tiles * 5 * sizeof(uint32_t)));
+    // The pinned staging buffer may still be read by a previous call's copy: wait for it.
+    HIP_TRY(hipStreamSynchronize(a->stream));
+    prof_harvest(a);
+    char *hd = static_cast<char *>(a->h_desc.p);
+    char *dd = a->desc.as<char>();
+    uint32_t *sum_dev = a->summaries.as<uint32_t>();
+    const float maxdisp = (float)a->cfg.windows[0] / 2.0f;
+    for (size_t i = 0; i < np; ++i) {
+        const PassPlan &pl = plans[i];
+        auto *pp = reinterpret_cast<PassPair *>(hd + o_pp[i]);
+        auto *pb = reinterpret_cast<uint32_t *>(hd + o_pb[i]);
+        auto *vp = reinterpret_cast<ValidatePair *>(hd + o_vp[i]);
+        auto *tb = reinterpret_cast<uint32_t *>(hd + o_tb[i]);
+        for (uint32_t k = 0; k < npairs; ++k) {
+            PassPair d{};
+            const piv_frame *A = fa[k], *B = fb[k];
+            d.img_a = A->img;
+            d.img_b = B->img;
+            d.use_mask = (A->has_sat && B->has_sat) ? 1 : 0;
+            d.sat_a = A->has_sat ? A->sat.as<double>() : nullptr;
+            d.sat_b = B->has_sat ? B->sat.as<double>() : nullptr;
+            d.sat_cells = A->cells;
+            if (i > 0) {
+                d.prev_dx = val[i - 1].dx + (size_t)k * plans[i - 1].cells;
+                d.prev_dy = val[i - 1].dy + (size_t)k * plans[i - 1].cells;
+                d.prev = plans[i - 1].grid;
+                d.has_prev = 1;
+            }
+            const size_t co = (size_t)k * pl.cells;
+            d.out_dx = raw[i].dx + co;
+            d.out_dy = raw[i].dy + co;
+            d.out_ppr = raw[i].ppr + co;
+            d.out_flags = raw[i].flags + co;
+            d.out_origins = traced ? origins[i] + 4 * co : nullptr;
+            d.out_peak = traced ? peaks[i] + co : nullptr;
+            d.H = H;
+            d.W = W;
+            d.pitch = A->pitch;
+            d.fmt = A->fmt;
+            d.grid = pl.grid;
+            d.min_mean = a->cfg.min_mean;
+            d.min_std = a->cfg.min_std;
+            d.win_begin = (uint32_t)(k * pl.cells);
+            if (B->pitch != A->pitch)
+                return fail(PIV_ERR_INVALID_ARGUMENT, "frames of a pair must share a row pitch");
+            pp[k] = d;
+            pb[k] = d.win_begin;
+            ValidatePair v{};
+            v.raw_dx = d.out_dx;
+            v.raw_dy = d.out_dy;
+            v.raw_ppr = d.out_ppr;
+            v.raw_flags = d.out_flags;
+            v.dx = val[i].dx + co;
+            v.dy = val[i].dy + co;
+            v.ppr = val[i].ppr + co;
+            v.flags = val[i].flags + co;
+            v.summary = sum_dev + (i * npairs + k) * 5;
+            v.nx = pl.grid.nx;
+            v.ny = pl.grid.ny;
+            v.ppr_threshold = a->cfg.ppr_threshold;
+            v.median_threshold = a->cfg.median_threshold;
+            v.median_eps = a->cfg.median_eps;
+            v.max_displacement = maxdisp;
+            v.replace = a->cfg.replace ? 1 : 0;
+            v.tile_begin = k * tiles_pp[i];
+            v.tiles_x = (pl.grid.nx + VALIDATE_TILE - 1) / VALIDATE_TILE;
+            vp[k] = v;
+            tb[k] = v.tile_begin;
+        }
+    }
+    hipStream_t st = a->stream;
+    HIP_TRY(hipMemcpyAsync(dd, hd, dsz, hipMemcpyHostToDevice, st));
+    for (size_t i = 0; i < np; ++i) {
+        const PassPlan &pl = plans[i];
+        const uint32_t total = (uint32_t)(pl.cells * npairs);
+        const double cbytes = (double)total * (2.0 * pl.n * pl.n * 4.0 + 49.0);
+        HIP_TRY(timed(a, "correlate", pl.n, cbytes, total, [&] {
+            return launch_correlate(pl.n, fa[0]->fmt, reinterpret_cast<PassPair *>(dd + o_pp[i]),
+                                    reinterpret_cast<uint32_t *>(dd + o_pb[i]), (int)npairs,
+                                    total, (uint32_t)pl.cells, st);
+        }));
+        // validation reads the raw field (13 B/vector) and writes the validated one (13 B)
+        HIP_TRY(timed(a, "validate", 0, (double)total * 26.0, total, [&] {
+            return launch_validate(reinterpret_cast<ValidatePair *>(dd + o_vp[i]),
+                                   reinterpret_cast<uint32_t *>(dd + o_tb[i]), (int)npairs,
+                                   tiles_pp[i] * npairs, a->tile_counts.as<uint32_t>(), st);
+        }));
+    }
+    *summaries_dev = sum_dev;
+    return PIV_OK;
+}
+
+void to_summary(const uint32_t *c, piv_summary *s)
+{
+    s->no_signal = c[0];
+    s->low_ppr = c[1];
+    s->median_outliers = c[2];
+    s->out_of_range = c[3];
+    s->replaced = c[4];
+}
+
+piv_status frame_from_device(piv_analyzer *a, const void *pixels, int fmt, uint32_t h,
+                             uint32_t w, size_t row_stride, bool owned, const float *bg_dev,
+                             piv_frame **out)
+{
+    auto *f = new (std::nothrow) piv_frame();
+    if (!f) return fail(PIV_ERR_OOM, "host allocation failed");
+    f->device = a->device;
+    f->H = (int)h;
+    f->W = (int)w;
+    f->in_fmt = fmt;
+    f->raw_ptr = pixels;
+    f->raw_pitch = (int)(row_stride / bytes_per_px(fmt));
+    (void)owned;
+    piv_status s = prepare_frame(a, f, bg_dev);
+    if (s != PIV_OK) {
+        delete f;
+        return s;
+    }
+    *out = f;
+    return PIV_OK;
+}
+
+piv_status check_pixels_args(const void *pixels, int fmt, uint32_t h, uint32_t w,
+                             size_t row_stride)
+{
+    if (!pixels) return fail(PIV_ERR_INVALID_ARGUMENT, "pixels is null");
+    if (fmt != FMT_U8 && fmt != FMT_U16 && fmt != FMT_F32)
+        return fail(PIV_ERR_INVALID_ARGUMENT, "unknown pixel format %d", fmt);
+    if (h == 0 || w == 0) return fail(PIV_ERR_INVALID_ARGUMENT, "empty frame");
+    if (h > (1u << 15) || w > (1u << 15)) return fail(PIV_ERR_INVALID_ARGUMENT, "frame too large");
+    const size_t bpp = (size_t)bytes_per_px(fmt);
+    if (row_stride < w * bpp || row_stride % bpp != 0)
+        return fail(PIV_ERR_INVALID_ARGUMENT, "row_stride %zu invalid for width %u", row_stride, w);
+    return PIV_OK;
+}
+
+}  // namespace
+
+// =========================================================================================
+// C ABI
+// =========================================================================================
+extern "C" {
+
+const char *piv_last_error(void) { return g_err.c_str(); }
+
+int piv_abi_version(void) { return PIV_ABI_VERSION; }
+
+piv_status piv_device_count(int *count)
+{
+    if (!count) return fail(PIV_ERR_INVALID_ARGUMENT, "count is null");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    *count = n;
+    return PIV_OK;
+}
+
+void piv_config_default(piv_config *c)
+{
+    if (!c) return;
+    std::memset(c, 0, sizeof *c);
+    c->windows[0] = 96;
+    c->windows[1] = 64;
+    c->num_windows = 2;
+    c->overlap = 0.5f;
+    c->final_iterations = 2;
+    c->min_mean = 0.0f;
+    c->min_std = 0.0f;
+    c->ppr_threshold = 1.3f;
+    c->median_threshold = 2.0f;
+    c->median_eps = 0.1f;
+    c->replace = 1;
+    c->smooth_sigma = 1.0f;
+    c->normalize_radius = 4;
+    c->normalize_eps = 0.02f;
+}
+
+piv_status piv_grid_new(uint32_t height, uint32_t width, uint32_t window, float overlap,
+                        piv_grid *out)
+{
+    if (!out) return fail(PIV_ERR_INVALID_ARGUMENT, "out is null");
+    GridDev g{};
+    PIV_TRY(grid_new(height, width, window, overlap, g));
+    *out = to_pub(g);
+    return PIV_OK;
+}
+
+piv_status piv_analyzer_create(const piv_config *cfg, int device, piv_analyzer **out)
+{
+    if (!cfg || !out) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    PIV_TRY(check_config(*cfg));
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(PIV_ERR_DEVICE, "no HIP device available");
+    if (device < 0 || device >= ndev)
+        return fail(PIV_ERR_INVALID_ARGUMENT, "device %d out of range (%d devices)", device, ndev);
+    PIV_TRY(set_device(device));
+    auto *a = new (std::nothrow) piv_analyzer();
+    if (!a) return fail(PIV_ERR_OOM, "host allocation failed");
+    a->cfg = *cfg;
+    a->device = device;
+    for (uint32_t i = 0; i < cfg->num_windows; ++i) a->passes.push_back((int)cfg->windows[i]);
+    for (uint32_t i = 1; i < cfg->final_iterations; ++i)
+        a->passes.push_back((int)cfg->windows[cfg->num_windows - 1]);
+    hipError_t e = hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete a;
+        return fail(PIV_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    if (cfg->smooth_sigma > 0.0f) {
+        a->taps_r = gauss_taps(cfg->smooth_sigma, a->taps);
+        piv_status s = a->d_taps.reserve(a->taps.size() * sizeof(float));
+        if (s == PIV_OK &&
+            hipMemcpy(a->d_taps.p, a->taps.data(), a->taps.size() * sizeof(float),
+                      hipMemcpyHostToDevice) != hipSuccess)
+            s = fail(PIV_ERR_DEVICE, "copying blur taps failed");
+        if (s != PIV_OK) {
+            delete a;
+            return s;
+        }
+    }
+    *out = a;
+    return PIV_OK;
+}
+
+void piv_analyzer_destroy(piv_analyzer *a)
+{
+    if (!a) return;
+    (void)hipSetDevice(a->device);
+    if (a->stream) (void)hipStreamSynchronize(a->stream);
+    delete a;
+}
+
+uint32_t piv_analyzer_num_passes(const piv_analyzer *a) { return a ? (uint32_t)a->passes.size() : 0; }
+
+piv_status piv_output_grid(const piv_analyzer *a, uint32_t height, uint32_t width, piv_grid *out)
+{
+    if (!a || !out) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+    return piv_grid_new(height, width, a->cfg.windows[a->cfg.num_windows - 1], a->cfg.overlap,
+                        out);
+}
+
+piv_status piv_frame_prepare(piv_analyzer *a, const void *pixels, piv_pixel_format fmt,
+                             uint32_t height, uint32_t width, size_t row_stride,
+                             const float *background, piv_frame **out)
+{
+    if (!a || !out) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    PIV_TRY(check_pixels_args(pixels, fmt, height, width, row_stride));
+    std::lock_guard<std::mutex> lk(a->mu);
+    PIV_TRY(set_device(a->device));
+    auto *f = new (std::nothrow) piv_frame();
+    if (!f) return fail(PIV_ERR_OOM, "host allocation failed");
+    f->device = a->device;
+    f->H = (int)height;
+    f->W = (int)width;
+    f->in_fmt = fmt;
+    const size_t bpp = (size_t)bytes_per_px(fmt);
+    piv_status s = f->raw.reserve((size_t)height * width * bpp);
+    DevBuf bg;
+    if (s == PIV_OK) {
+        hipError_t e = hipMemcpy2DAsync(f->raw.p, width * bpp, pixels, row_stride, width * bpp,
+                                        height, hipMemcpyHostToDevice, a->stream);
+        if (e != hipSuccess) s = fail(PIV_ERR_DEVICE, "upload: %s", hipGetErrorString(e));
+    }
+    if (s == PIV_OK && background) {
+        s = bg.reserve((size_t)height * width * sizeof(float));
+        if (s == PIV_OK &&
+            hipMemcpyAsync(bg.p, background, (size_t)height * width * sizeof(float),
+                           hipMemcpyHostToDevice, a->stream) != hipSuccess)
+            s = fail(PIV_ERR_DEVICE, "background upload failed");
+    }
+    if (s == PIV_OK) {
+        f->raw_ptr = f->raw.p;
+        f->raw_pitch = (int)width;
+        s = prepare_frame(a, f, background ? bg.as<float>() : nullptr);
+    }
+    if (s == PIV_OK && hipStreamSynchronize(a->stream) != hipSuccess)
+        s = fail(PIV_ERR_DEVICE, "frame preparation failed");
+    bg.release();
+    if (s != PIV_OK) {
+        delete f;
+        return s;
+    }
+    *out = f;
+    return PIV_OK;
+}
+
+piv_status piv_frame_prepare_device(piv_analyzer *a, const void *pixels, piv_pixel_format fmt,
+                                    uint32_t height, uint32_t width, size_t row_stride,
+                                    const float *background, piv_frame **out)
+{
+    if (!a || !out) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    PIV_TRY(check_pixels_args(pixels, fmt, height, width, row_stride));
+    std::lock_guard<std::mutex> lk(a->mu);
+    PIV_TRY(set_device(a->device));
+    PIV_TRY(frame_from_device(a, pixels, fmt, height, width, row_stride, false, background, out));
+    HIP_TRY(hipStreamSynchronize(a->stream));
+    return PIV_OK;
+}
+
+void piv_frame_release(piv_frame *f)
+{
+    if (!f) return;
+    (void)hipSetDevice(f->device);
+    delete f;
+}
+
+piv_status piv_frame_download(const piv_frame *f, float *image, float *base)
+{
+    if (!f) return fail(PIV_ERR_INVALID_ARGUMENT, "null frame");
+    PIV_TRY(set_device(f->device));
+    const size_t n = (size_t)f->H * f->W;
+    DevBuf tmp;
+    auto normalized = [&](float **p) -> piv_status {
+        PIV_TRY(tmp.reserve(n * sizeof(float)));
+        // the original samples, normalised (pitch-aware via base kernel without background)
+        HIP_TRY(launch_base(f->in_fmt, f->raw_ptr, f->raw_pitch, nullptr, f->H, f->W,
+                            tmp.as<float>(), nullptr));
+        *p = tmp.as<float>();
+        return PIV_OK;
+    };
+    if (image) {
+        if (f->fmt == FMT_F32 && f->img == f->image.p && f->image.p) {
+            HIP_TRY(hipMemcpy(image, f->image.p, n * sizeof(float), hipMemcpyDeviceToHost));
+        } else {
+            float *p = nullptr;
+            PIV_TRY(normalized(&p));
+            HIP_TRY(hipMemcpy(image, p, n * sizeof(float), hipMemcpyDeviceToHost));
+        }
+    }
+    if (base) {
+        if (f->base.p) {
+            HIP_TRY(hipMemcpy(base, f->base.p, n * sizeof(float), hipMemcpyDeviceToHost));
+        } else {
+            float *p = nullptr;
+            PIV_TRY(normalized(&p));
+            HIP_TRY(hipMemcpy(base, p, n * sizeof(float), hipMemcpyDeviceToHost));
+        }
+    }
+    tmp.release();
+    return PIV_OK;
+}
+
+static piv_status analyze_impl(piv_analyzer *a, const piv_frame *fa, const piv_frame *fb,
+                               piv_field_out *out, piv_pass_trace *traces, uint32_t num_traces)
+{
+    if (!a || !fa || !fb || !out) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+    if (fa->H != fb->H || fa->W != fb->W)
+        return fail(PIV_ERR_SHAPE_MISMATCH, "frames have different shapes: (%d, %d) vs (%d, %d)",
+                    fa->H, fa->W, fb->H, fb->W);
+    if (fa->fmt != fb->fmt)
+        return fail(PIV_ERR_INVALID_ARGUMENT, "frames of a pair must share a sample format");
+    if (traces && num_traces != a->passes.size())
+        return fail(PIV_ERR_INVALID_ARGUMENT, "need %zu traces", a->passes.size());
+    std::lock_guard<std::mutex> lk(a->mu);
+    PIV_TRY(set_device(a->device));
+    std::vector<PassPlan> plans;
+    PIV_TRY(plan_passes(a, fa->H, fa->W, plans));
+    const size_t cells = plans.back().cells;
+    if (out->capacity < cells || !out->dx || !out->dy || !out->peak_ratio || !out->flags)
+        return fail(PIV_ERR_INVALID_ARGUMENT, "output arrays need %zu elements", cells);
+    if (traces)
+        for (size_t i = 0; i < plans.size(); ++i)
+            if (traces[i].capacity < plans[i].cells)
+                return fail(PIV_ERR_INVALID_ARGUMENT, "trace %zu needs %zu elements", i, plans[i].cells);
+    std::vector<FieldPtrs> raw, val;
+    std::vector<float *> org;
+    std::vector<int32_t *> pk;
+    uint32_t *sums = nullptr;
+    const piv_frame *A[1] = {fa}, *B[1] = {fb};
+    PIV_TRY(run_batch(a, A, B, 1, BatchOut{}, traces != nullptr, plans, raw, val, org, pk, &sums));
+    hipStream_t st = a->stream;
+    const size_t np = plans.size();
+    PIV_TRY(a->h_summaries.reserve(np * 5 * sizeof(uint32_t)));
+    HIP_TRY(hipMemcpyAsync(a->h_summaries.p, sums, np * 5 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    const FieldPtrs &l = val[np - 1];
+    HIP_TRY(hipMemcpyAsync(out->dx, l.dx, cells * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(out->dy, l.dy, cells * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(out->peak_ratio, l.ppr, cells * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(out->flags, l.flags, cells, hipMemcpyDeviceToHost, st));
+    if (traces) {
+        for (size_t i = 0; i < np; ++i) {
+            piv_pass_trace &t = traces[i];
+            const size_t c = plans[i].cells;
+            auto cp = [&](void *dst, const void *src, size_t bytes) -> piv_status {
+                if (dst) HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+                return PIV_OK;
+            };
+            PIV_TRY(cp(t.origins, org[i], c * 16));
+            PIV_TRY(cp(t.raw_dx, raw[i].dx, c * 4));
+            PIV_TRY(cp(t.raw_dy, raw[i].dy, c * 4));
+            PIV_TRY(cp(t.raw_ppr, raw[i].ppr, c * 4));
+            PIV_TRY(cp(t.raw_flags, raw[i].flags, c));
+            PIV_TRY(cp(t.peak_index, pk[i], c * 4));
+            PIV_TRY(cp(t.dx, val[i].dx, c * 4));
+            PIV_TRY(cp(t.dy, val[i].dy, c * 4));
+            PIV_TRY(cp(t.ppr, val[i].ppr, c * 4));
+            PIV_TRY(cp(t.flags, val[i].flags, c));
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    const uint32_t *hs = static_cast<const uint32_t *>(a->h_summaries.p);
+    out->grid = to_pub(plans.back().grid);
+    to_summary(hs + (np - 1) * 5, &out->summary);
+    if (traces)
+        for (size_t i = 0; i < np; ++i) {
+            traces[i].grid = to_pub(plans[i].grid);
+            to_summary(hs + i * 5, &traces[i].summary);
+        }
+    return PIV_OK;
+}
+
+piv_status piv_analyze_pair(piv_analyzer *a, const piv_frame *fa, const piv_frame *fb,
+                            piv_field_out *out)
+{
+    return analyze_impl(a, fa, fb, out, nullptr, 0);
+}
+
+piv_status piv_analyze_traced(piv_analyzer *a, const piv_frame *fa, const piv_frame *fb,
+                              piv_field_out *out, piv_pass_trace *traces, uint32_t num_traces)
+{
+    if (!traces) return fail(PIV_ERR_INVALID_ARGUMENT, "traces is null");
+    return analyze_impl(a, fa, fb, out, traces, num_traces);
+}
+
+piv_status piv_analyze_batch_device(piv_analyzer *a, const piv_frame *const *fa,
+                                    const piv_frame *const *fb, uint32_t npairs, float *dx,
+                                    float *dy, float *ppr, uint8_t *flags,
+                                    piv_summary *summaries, void *stream, int async)
+{
+    if (!a || !fa || !fb || !dx || !dy || !ppr || !flags)
+        return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+    if (npairs == 0) return PIV_OK;
+    std::lock_guard<std::mutex> lk(a->mu);
+    PIV_TRY(set_device(a->device));
+    hipStream_t saved = a->stream;
+    if (stream) a->stream = static_cast<hipStream_t>(stream);
+    std::vector<PassPlan> plans;
+    std::vector<FieldPtrs> raw, val;
+    std::vector<float *> org;
+    std::vector<int32_t *> pk;
+    uint32_t *sums = nullptr;
+    BatchOut bo;
+    bo.dx = dx;
+    bo.dy = dy;
+    bo.ppr = ppr;
+    bo.flags = flags;
+    piv_status s = run_batch(a, fa, fb, npairs, bo, false, plans, raw, val, org, pk, &sums);
+    if (s == PIV_OK && summaries) {
+        const size_t np = plans.size();
+        s = a->h_summaries.reserve(np * npairs * 5 * sizeof(uint32_t));
+        if (s == PIV_OK &&
+            (hipMemcpyAsync(a->h_summaries.p, sums + (np - 1) * npairs * 5,
+                            npairs * 5 * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                            a->stream) != hipSuccess ||
+             hipStreamSynchronize(a->stream) != hipSuccess))
+            s = fail(PIV_ERR_DEVICE, "summary download failed");
+        if (s == PIV_OK) {
+            const uint32_t *hs = static_cast<const uint32_t *>(a->h_summaries.p);
+            for (uint32_t k = 0; k < npairs; ++k) to_summary(hs + k * 5, &summaries[k]);
+        }
+    } else if (s == PIV_OK && !async) {
+        if (hipStreamSynchronize(a->stream) != hipSuccess) s = fail(PIV_ERR_DEVICE, "batch failed");
+    }
+    a->stream = saved;
+    return s;
+}
+
+piv_status piv_place_windows(piv_analyzer *a, const piv_grid *grid, uint32_t height,
+                             uint32_t width, const piv_grid *prev, const float *prev_dx,
+                             const float *prev_dy, float *origins)
+{
+    if (!a || !grid || !origins || (prev && (!prev_dx || !prev_dy)))
+        return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+    std::lock_guard<std::mutex> lk(a->mu);
+    PIV_TRY(set_device(a->device));
+    GridDev g{(int)grid->window, (int)grid->step, (int)grid->nx, (int)grid->ny, (int)grid->x0, (int)grid->y0};
+    GridDev pg{};
+    const size_t n = (size_t)g.nx * g.ny;
+    size_t pn = 0;
+    if (prev) {
+        pg = GridDev{(int)prev->window, (int)prev->step, (int)prev->nx, (int)prev->ny, (int)prev->x0, (int)prev->y0};
+        pn = (size_t)pg.nx * pg.ny;
+    }
+    PIV_TRY(a->stage_in.reserve(2 * pn * sizeof(float) + 16));
+    PIV_TRY(a->stage_out.reserve(n * 4 * sizeof(float)));
+    float *pdx = a->stage_in.as<float>(), *pdy = pdx + pn;
+    hipStream_t st = a->stream;
+    if (prev) {
+        HIP_TRY(hipMemcpyAsync(pdx, prev_dx, pn * 4, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(pdy, prev_dy, pn * 4, hipMemcpyHostToDevice, st));
+    }
+    HIP_TRY(launch_place(g, pg, prev ? 1 : 0, pdx, pdy, (int)height, (int)width,
+                         a->stage_out.as<float>(), st));
+    HIP_TRY(hipMemcpyAsync(origins, a->stage_out.p, n * 16, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return PIV_OK;
+}
+
+piv_status piv_correlate_windows(piv_analyzer *a, const piv_frame *fa, const piv_frame *fb,
+                                 uint32_t window, const float *origins, uint32_t count,
+                                 float *dx, float *dy, float *ppr, uint8_t *flags,
+                                 int32_t *peak_index)
+{
+    if (!a || !fa || !fb || !origins || !dx || !dy || !ppr || !flags)
+        return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+    if (fa->H != fb->H || fa->W != fb->W || fa->fmt != fb->fmt || fa->pitch != fb->pitch)
+        return fail(PIV_ERR_SHAPE_MISMATCH, "frames differ in shape or format");
+    if (!correlate_supported((int)window))
+        return fail(PIV_ERR_UNSUPPORTED, "window size %u has no gfx950 kernel", window);
+    if ((int)window > fa->W || (int)window > fa->H)
+        return fail(PIV_ERR_IMAGE_TOO_SMALL, "window %u larger than the frame", window);
+    if (count == 0) return PIV_OK;
+    for (uint32_t k = 0; k < count; ++k) {
+        // origins must keep the (n+1)^2 bilinear footprint inside the frame
+        const float ax = origins[4 * k], ay = origins[4 * k + 1], bx = origins[4 * k + 2], by = origins[4 * k + 3];
+        if (!(ax >= 0 && ay >= 0 && bx >= 0 && by >= 0 && ax + window <= fa->W && bx + window <= fa->W &&
+              ay + window <= fa->H && by + window <= fa->H))
+            return fail(PIV_ERR_INVALID_ARGUMENT, "origin %u outside the frame", k);
+    }
+    std::lock_guard<std::mutex> lk(a->mu);
+    PIV_TRY(set_device(a->device));
+    const size_t in_bytes = align_up(count * 16, 256) + align_up(sizeof(PassPair), 256) + 256;
+    PIV_TRY(a->stage_in.reserve(in_bytes));
+    PIV_TRY(a->stage_out.reserve((size_t)count * 17 + 1024));
+    char *din = a->stage_in.as<char>();
+    float *d_org = reinterpret_cast<float *>(din);
+    auto *d_pp = reinterpret_cast<PassPair *>(din + align_up(count * 16, 256));
+    auto *d_begin = reinterpret_cast<uint32_t *>(din + align_up(count * 16, 256) + align_up(sizeof(PassPair), 256));
+    float *o_dx = a->stage_out.as<float>(), *o_dy = o_dx + count, *o_ppr = o_dy + count;
+    int32_t *o_pk = reinterpret_cast<int32_t *>(o_ppr + count);
+    uint8_t *o_fl = reinterpret_cast<uint8_t *>(o_pk + count);
+    PassPair d{};
+    d.img_a = fa->img;
+    d.img_b = fb->img;
+    d.use_mask = (fa->has_sat && fb->has_sat) ? 1 : 0;
+    d.sat_a = fa->has_sat ? fa->sat.as<double>() : nullptr;
+    d.sat_b = fb->has_sat ? fb->sat.as<double>() : nullptr;
+    d.sat_cells = fa->cells;
+    d.in_origins = d_org;
+    d.out_dx = o_dx;
+    d.out_dy = o_dy;
+    d.out_ppr = o_ppr;
+    d.out_flags = o_fl;
+    d.out_peak = o_pk;
+    d.H = fa->H;
+    d.W = fa->W;
+    d.pitch = fa->pitch;
+    d.fmt = fa->fmt;
+    d.grid = GridDev{(int)window, 1, (int)count, 1, 0, 0};
+    d.min_mean = a->cfg.min_mean;
+    d.min_std = a->cfg.min_std;
+    hipStream_t st = a->stream;
+    uint32_t zero = 0;
+    HIP_TRY(hipMemcpyAsync(d_org, origins, count * 16, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_pp, &d, sizeof d, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_begin, &zero, sizeof zero, hipMemcpyHostToDevice, st));
+    HIP_TRY(launch_correlate((int)window, fa->fmt, d_pp, d_begin, 1, count, count, st));
+    HIP_TRY(hipMemcpyAsync(dx, o_dx, count * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(dy, o_dy, count * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(ppr, o_ppr, count * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(flags, o_fl, count, hipMemcpyDeviceToHost, st));
+    if (peak_index) HIP_TRY(hipMemcpyAsync(peak_index, o_pk, count * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return PIV_OK;
+}
+
+piv_status piv_validate_field(piv_analyzer *a, uint32_t ny, uint32_t nx, float *dx, float *dy,
+                              const float *ppr, uint8_t *flags, float max_displacement,
+                              piv_summary *summary)
+{
+    if (!a || !dx || !dy || !ppr || !flags) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+    if (nx == 0 || ny == 0) return fail(PIV_ERR_INVALID_ARGUMENT, "empty field");
+    std::lock_guard<std::mutex> lk(a->mu);
+    PIV_TRY(set_device(a->device));
+    const size_t n = (size_t)nx * ny;
+    const size_t fb = align_up(n * 13, 256);
+    PIV_TRY(a->stage_in.reserve(2 * fb + 1024));
+    const uint32_t tiles = ((nx + VALIDATE_TILE - 1) / VALIDATE_TILE) * ((ny + VALIDATE_TILE - 1) / VALIDATE_TILE);
+    PIV_TRY(a->tile_counts.reserve((size_t)tiles * 5 * 4 + 64));
+    char *b = a->stage_in.as<char>();
+    float *r_dx = reinterpret_cast<float *>(b), *r_dy = r_dx + n, *r_ppr = r_dy + n;
+    uint8_t *r_fl = reinterpret_cast<uint8_t *>(r_ppr + n);
+    float *v_dx = reinterpret_cast<float *>(b + fb), *v_dy = v_dx + n, *v_ppr = v_dy + n;
+    uint8_t *v_fl = reinterpret_cast<uint8_t *>(v_ppr + n);
+    auto *d_job = reinterpret_cast<ValidatePair *>(b + 2 * fb);
+    auto *d_tb = reinterpret_cast<uint32_t *>(b + 2 * fb + 512);
+    auto *d_sum = reinterpret_cast<uint32_t *>(b + 2 * fb + 768);
+    ValidatePair v{};
+    v.raw_dx = r_dx;
+    v.raw_dy = r_dy;
+    v.raw_ppr = r_ppr;
+    v.raw_flags = r_fl;
+    v.dx = v_dx;
+    v.dy = v_dy;
+    v.ppr = v_ppr;
+    v.flags = v_fl;
+    v.summary = d_sum;
+    v.nx = (int)nx;
+    v.ny = (int)ny;
+    v.ppr_threshold = a->cfg.ppr_threshold;
+    v.median_threshold = a->cfg.median_threshold;
+    v.median_eps = a->cfg.median_eps;
+    v.max_displacement = max_displacement;
+    v.replace = a->cfg.replace ? 1 : 0;
+    v.tile_begin = 0;
+    v.tiles_x = (int)((nx + VALIDATE_TILE - 1) / VALIDATE_TILE);
+    uint32_t zero = 0, hs[5];
+    hipStream_t st = a->stream;
+    HIP_TRY(hipMemcpyAsync(r_dx, dx, n * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(r_dy, dy, n * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(r_ppr, ppr, n * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(r_fl, flags, n, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_job, &v, sizeof v, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_tb, &zero, sizeof zero, hipMemcpyHostToDevice, st));
+    HIP_TRY(launch_validate(d_job, d_tb, 1, tiles, a->tile_counts.as<uint32_t>(), st));
+    HIP_TRY(hipMemcpyAsync(dx, v_dx, n * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(dy, v_dy, n * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(flags, v_fl, n, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(hs, d_sum, sizeof hs, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (summary) to_summary(hs, summary);
+    return PIV_OK;
+}
+
+piv_status piv_profile_enable(piv_analyzer *a, int enable)
+{
+    if (!a) return fail(PIV_ERR_INVALID_ARGUMENT, "null analyzer");
+    std::lock_guard<std::mutex> lk(a->mu);
+    PIV_TRY(set_device(a->device));
+    HIP_TRY(hipStreamSynchronize(a->stream));
+    prof_harvest(a);
+    a->prof_stats.clear();
+    a->prof_on = enable != 0;
+    return PIV_OK;
+}
+
+piv_status piv_profile_read(piv_analyzer *a, piv_kernel_stat *out, uint32_t capacity,
+                            uint32_t *count)
+{
+    if (!a || !count) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+    std::lock_guard<std::mutex> lk(a->mu);
+    PIV_TRY(set_device(a->device));
+    HIP_TRY(hipStreamSynchronize(a->stream));
+    prof_harvest(a);
+    uint32_t i = 0;
+    for (auto &kv : a->prof_stats) {
+        if (out && i < capacity) out[i] = kv.second;
+        ++i;
+    }
+    *count = i;
+    return PIV_OK;
+}
+
+piv_status piv_normalize_pixels(int device, const void *pixels, piv_pixel_format fmt,
+                                size_t count, float *out)
+{
+    if (!pixels || !out) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+    if ((int)fmt != FMT_U8 && (int)fmt != FMT_U16 && (int)fmt != FMT_F32)
+        return fail(PIV_ERR_INVALID_ARGUMENT, "unknown pixel format %d", (int)fmt);
+    PIV_TRY(set_device(device));
+    DevBuf in, o;
+    const size_t bpp = (size_t)bytes_per_px(fmt);
+    PIV_TRY(in.reserve(count * bpp + 4));
+    PIV_TRY(o.reserve(count * 4 + 4));
+    HIP_TRY(hipMemcpy(in.p, pixels, count * bpp, hipMemcpyHostToDevice));
+    HIP_TRY(launch_normalize(fmt, in.p, (long)count, o.as<float>(), nullptr));
+    HIP_TRY(hipMemcpy(out, o.p, count * 4, hipMemcpyDeviceToHost));
+    in.release();
+    o.release();
+    return PIV_OK;
+}
+
+}  // extern "C"

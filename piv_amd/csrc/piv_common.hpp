@@ -1,0 +1,156 @@
+// Device helpers shared by the PIV kernels: scalar restatements of piv-core that must be
+// bit-exact (placement, predictor sampling, SAT statistics, pixel normalisation, bilinear
+// window load).  Built with -ffp-contract=off: every expression rounds as the Rust source.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "piv_device.hpp"
+
+namespace piv {
+
+
+// ---------------------------------------------------------------------------------------
+// scalar helpers that restate piv-core exactly
+// ---------------------------------------------------------------------------------------
+
+// Rust f32::clamp: if x < min { min } ; if x > max { max }
+__device__ __forceinline__ float clamp_rs(float x, float lo, float hi)
+{
+    if (x < lo) x = lo;
+    if (x > hi) x = hi;
+    return x;
+}
+
+// grid.rs:55-66
+__device__ __forceinline__ float center(int o0, int i, int step, int window)
+{
+    return (float)(o0 + i * step) + ((float)window - 1.0f) / 2.0f;
+}
+
+// field.rs:56-81 VectorField::sample (bilinear, NaN-skipping, clamped)
+__device__ __forceinline__ void sample_field(const GridDev &g, const float *__restrict__ dx,
+                             const float *__restrict__ dy, float x, float y, float &u, float &v)
+{
+    const float fx = clamp_rs((x - center(g.x0, 0, g.step, g.window)) / (float)g.step, 0.0f,
+                              (float)(g.nx - 1));
+    const float fy = clamp_rs((y - center(g.y0, 0, g.step, g.window)) / (float)g.step, 0.0f,
+                              (float)(g.ny - 1));
+    const int ix = (int)floorf(fx), iy = (int)floorf(fy);
+    const float tx = fx - (float)ix, ty = fy - (float)iy;
+    const int ix1 = min(ix + 1, g.nx - 1), iy1 = min(iy + 1, g.ny - 1);
+    const int rr[4] = {iy, iy, iy1, iy1};
+    const int cc[4] = {ix, ix1, ix, ix1};
+    const float ww[4] = {(1.0f - tx) * (1.0f - ty), tx * (1.0f - ty), (1.0f - tx) * ty, tx * ty};
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float su = dx[rr[k] * g.nx + cc[k]], sv = dy[rr[k] * g.nx + cc[k]];
+        if (ww[k] > 0.0f && isfinite(su) && isfinite(sv)) {
+            a0 = a0 + ww[k] * su;
+            a1 = a1 + ww[k] * sv;
+            a2 = a2 + ww[k];
+        }
+    }
+    if (a2 > 0.0f) {
+        u = a0 / a2;
+        v = a1 / a2;
+    } else {
+        u = 0.0f;
+        v = 0.0f;
+    }
+}
+
+// multipass.rs:224-229 place closure
+__device__ __forceinline__ void place(int origin, float shift, float maxv, float &a, float &b)
+{
+    shift = clamp_rs(shift, -maxv, maxv);
+    const float lo = fmaxf(-shift, 0.0f);
+    const float hi = fminf(maxv - shift, maxv);
+    const float a0 = clamp_rs((float)origin - shift / 2.0f, lo, hi);
+    a = a0;
+    b = a0 + shift;
+}
+
+// preprocess.rs:112-124 IntegralImage::stats
+__device__ __forceinline__ void sat_stats(const double *__restrict__ sum,
+                                          const double *__restrict__ sq, int W, int x, int y,
+                                          int bw, int bh, float &mean, float &sd)
+{
+    const long s = (long)W + 1;
+    const double n = (double)((long)bw * bh);
+    const long i11 = (long)(y + bh) * s + (x + bw), i01 = (long)y * s + (x + bw);
+    const long i10 = (long)(y + bh) * s + x, i00 = (long)y * s + x;
+    const double m = (sum[i11] - sum[i01] - sum[i10] + sum[i00]) / n;
+    const double var = (sq[i11] - sq[i01] - sq[i10] + sq[i00]) / n - m * m;
+    mean = (float)m;
+    sd = (float)sqrt(var > 0.0 ? var : 0.0);
+}
+
+// decode.rs:227-234: b / 255 and v / 65535 as IEEE quotients.  q = v * (1/D) is off by one
+// ulp for some v; one fma residual step r = v - q*D, q += r * (1/D) gives the correctly
+// rounded quotient for every u8 and u16 value (checked exhaustively in the tests).
+template <int FMT>
+__device__ __forceinline__ float load_pixel(const void *__restrict__ base, long idx)
+{
+    if constexpr (FMT == FMT_U8) {
+        const float v = (float)static_cast<const uint8_t *>(base)[idx];
+        constexpr float inv = 1.0f / 255.0f;
+        const float q = v * inv;
+        const float r = __builtin_fmaf(-q, 255.0f, v);
+        return __builtin_fmaf(r, inv, q);
+    } else if constexpr (FMT == FMT_U16) {
+        const float v = (float)static_cast<const uint16_t *>(base)[idx];
+        constexpr float inv = 1.0f / 65535.0f;
+        const float q = v * inv;
+        const float r = __builtin_fmaf(-q, 65535.0f, v);
+        return __builtin_fmaf(r, inv, q);
+    } else {
+        return static_cast<const float *>(base)[idx];
+    }
+}
+
+// correlate.rs:182-202 load_window, one column per lane: col[r] = window(r, c).
+// Lanes of a window read consecutive pixels of each image row (coalesced).  Reads are
+// clamped into the image: a window exactly as wide (or tall) as the image has a zero weight
+// on the extra column (row) the reference slices past the edge.
+template <int N, int FMT>
+__device__ __forceinline__ void load_column(const void *__restrict__ img, int pitch, int H, int W,
+                                            float x, float y, int c, float (&col)[N])
+{
+    const int x0 = (int)floorf(x), y0 = (int)floorf(y);
+    const float tx = x - (float)x0, ty = y - (float)y0;
+    const int xa = min(x0 + c, W - 1), xb = min(x0 + c + 1, W - 1);
+    if (tx < 1e-4f && ty < 1e-4f) {
+#pragma unroll
+        for (int r = 0; r < N; ++r) col[r] = load_pixel<FMT>(img, (long)min(y0 + r, H - 1) * pitch + xa);
+    } else {
+        const float w00 = (1.0f - tx) * (1.0f - ty), w01 = tx * (1.0f - ty);
+        const float w10 = (1.0f - tx) * ty, w11 = tx * ty;
+        const long r0 = (long)min(y0, H - 1) * pitch;
+        float t0 = load_pixel<FMT>(img, r0 + xa), t1 = load_pixel<FMT>(img, r0 + xb);
+#pragma unroll
+        for (int r = 0; r < N; ++r) {
+            const long rb = (long)min(y0 + r + 1, H - 1) * pitch;
+            const float b0 = load_pixel<FMT>(img, rb + xa), b1 = load_pixel<FMT>(img, rb + xb);
+            col[r] = w00 * t0 + w01 * t1 + w10 * b0 + w11 * b1;
+            t0 = b0;
+            t1 = b1;
+        }
+    }
+}
+
+__device__ __forceinline__ uint32_t find_pair_u(const uint32_t *begins, int npairs, uint32_t k)
+{
+    int lo = 0, hi = npairs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (begins[mid] <= k) lo = mid; else hi = mid - 1;
+    }
+    return (uint32_t)lo;
+}
+
+}  // namespace piv

@@ -1,0 +1,60 @@
+// Structures shared by the host orchestration (piv_api.cpp) and the HIP kernels.
+#pragma once
+
+#include <cstdint>
+
+namespace piv {
+
+enum PixelFormat : int32_t { FMT_U8 = 0, FMT_U16 = 1, FMT_F32 = 2 };
+
+// Grid::new result (piv-core/src/grid.rs:9-44)
+struct GridDev {
+    int32_t window, step, nx, ny, x0, y0;
+};
+
+// flags (piv-core/src/field.rs:6-20)
+constexpr uint8_t NO_SIGNAL = 1, LOW_PPR = 2, MEDIAN_OUTLIER = 4, REPLACED = 8, OUT_OF_RANGE = 16;
+constexpr uint8_t INVALID = NO_SIGNAL | LOW_PPR | MEDIAN_OUTLIER | OUT_OF_RANGE;
+
+// One image pair's share of one correlation pass.  A launch covers many pairs (the whole
+// batch, or windows of one size from several serving requests); window k of the launch
+// belongs to the pair whose [win_begin, win_begin + nx*ny) range holds k.
+struct PassPair {
+    const void *img_a;        // conditioned image (FMT_F32) or raw pixels (fused u8/u16/f32)
+    const void *img_b;
+    const double *sat_a;      // f64 summed-area tables of the background-subtracted frame:
+    const double *sat_b;      //   sum at [0, cells), sum of squares at [cells, 2 cells); or null
+    const float *prev_dx;     // previous pass's validated field (predictor), or null
+    const float *prev_dy;
+    const float *in_origins;  // optional [nx*ny*4]: use these origins instead of placing
+    float *out_dx, *out_dy, *out_ppr;
+    uint8_t *out_flags;
+    float *out_origins;       // optional [nx*ny*4] (traced runs)
+    int32_t *out_peak;        // optional [nx*ny] integer peak iy*n+ix, -1 if no peak
+    int64_t sat_cells;        // (H+1)*(W+1)
+    int32_t H, W, pitch;      // pitch in pixels
+    int32_t fmt;
+    GridDev grid, prev;
+    int32_t has_prev;
+    int32_t use_mask;
+    float min_mean, min_std;
+    uint32_t win_begin;
+    uint32_t pad_;
+};
+
+// One image pair's validation job (piv-core/src/validate.rs:51-130).
+struct ValidatePair {
+    const float *raw_dx, *raw_dy, *raw_ppr;
+    const uint8_t *raw_flags;
+    float *dx, *dy, *ppr;
+    uint8_t *flags;
+    uint32_t *summary;        // 5 counters: no_signal, low_ppr, median_outliers, out_of_range, replaced
+    int32_t nx, ny;
+    float ppr_threshold, median_threshold, median_eps, max_displacement;
+    int32_t replace;
+    uint32_t tile_begin;      // first tile index of this pair in the launch
+    int32_t tiles_x;          // tiles across
+    int32_t pad_;
+};
+
+}  // namespace piv

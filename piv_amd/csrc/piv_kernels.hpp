@@ -1,0 +1,28 @@
+// Launchers for the HIP kernels (piv_kernels.hip), called by the host orchestration.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "piv_device.hpp"
+
+namespace piv {
+
+bool correlate_supported(int n);
+hipError_t launch_correlate(int n, int fmt, const PassPair *pairs, const uint32_t *begins,
+                            int npairs, uint32_t total, uint32_t wpp, hipStream_t st);
+hipError_t launch_validate(const ValidatePair *jobs, const uint32_t *tile_begins, int njobs,
+                           uint32_t ntiles, uint32_t *tile_counts, hipStream_t st);
+hipError_t launch_base(int fmt, const void *raw, int pitch, const float *bg, int H, int W,
+                       float *out, hipStream_t st);
+hipError_t launch_sat(const float *img, int H, int W, double *sum, double *sq, hipStream_t st);
+hipError_t launch_blur(const float *in, int H, int W, const float *taps, int r, float *tmp,
+                       float *out, hipStream_t st);
+hipError_t launch_local_norm(const float *in, int H, int W, int radius, float eps,
+                             const double *sum, const double *sq, float *out, hipStream_t st);
+hipError_t launch_normalize(int fmt, const void *raw, long count, float *out, hipStream_t st);
+hipError_t launch_place(const GridDev &g, const GridDev &prev, int has_prev, const float *pdx,
+                        const float *pdy, int H, int W, float *origins, hipStream_t st);
+
+constexpr int VALIDATE_TILE = 16;
+
+}  // namespace piv
