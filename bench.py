@@ -87,6 +87,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--latency-requests", type=int, default=50)
+    ap.add_argument("--only-batched", action="store_true",
+                    help="time the batched steps only (for profiler runs)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -152,6 +154,11 @@ def main():
             tr = json.load(f)
         key = f"correlate{dom['window']}_pairs{P}"
         traffic = tr.get(key)
+
+    if args.only_batched:
+        print(json.dumps({"fields_per_s": fields_per_s, "ms_per_step": ms_per_step,
+                          "kernel_avg_ms": kernels}), flush=True)
+        return
 
     # single-request latency (one pair per call, arrival to result on the host)
     lat = []

@@ -1,63 +1,144 @@
 // The correlation-pass kernel template (one instantiation per window size, see corr_n*.hip).
+//
+// One launch covers every window of every pair of a pass.  Work decomposition:
+//   - a workgroup holds WPB windows of ONE pair, so the pair descriptor and the image base
+//     pointers are wave-uniform (scalar registers, SGPR-based global loads);
+//   - one lane per window line: lane c first owns column c of the window, later row c;
+//   - workgroup ids are remapped so that each of the 8 XCDs works on a contiguous range of
+//     windows (consecutive windows share most of their pixels: one XCD's L2 serves them).
+//
+// Per window (correlate.rs:99-139 restated for the GPU):
+//   1. placement from the previous pass's validated field (multipass.rs:207-231) and the
+//      signal mask from the f64 summed-area tables (multipass.rs:233-241);
+//   2. fused ingest: raw u8/u16 samples are read straight from HBM and the window is sampled
+//      bilinearly in raw units (the correlation coefficient is invariant to the decode scale
+//      1/255 or 1/65535, which only rescales the energy threshold);
+//   3. z = a + i b, 2-D FFT: column FFT in registers, LDS transpose, row FFT;
+//   4. cross spectrum conj(FA) FB from Z[k] and Z[-k], with the DC bin cleared (the mean
+//      removal of load_window) and the window energies from Parseval;
+//   5. Hermitian inverse: row IFFT of rows 0..N/2, then column-pair IFFTs (two real
+//      columns per complex transform), scaled into an LDS correlation plane;
+//   6. find_peak (correlate.rs:215-303): first row-major argmax, footprint half width,
+//      highest local maximum outside it, 3-point Gaussian fit, peak ratio.
 #pragma once
 
 #include "fft.hpp"
 #include "piv_common.hpp"
 
 namespace piv {
-// ---------------------------------------------------------------------------------------
-// correlation pass kernel
-// ---------------------------------------------------------------------------------------
 
-// Windows per workgroup: one wave when N divides 64, three waves (192 lanes) when N divides
-// 192, otherwise one window.  The complex transpose plane per window is N x (N+1) floats
-// for each of re/im (odd row stride: conflict-free column and row access).
 template <int N>
 struct KCfg {
-    static constexpr int WPB = (64 % N == 0) ? 64 / N : ((192 % N == 0 && N <= 48 && N != 48) ? 192 / N : 1);
+    // windows per workgroup: one wave when N divides 64, three waves when N divides 192
+    static constexpr bool IN_WAVE = (64 % N == 0);
+    static constexpr int WPB = IN_WAVE ? 64 / N : ((192 % N == 0 && N < 48) ? 192 / N : 1);
     static constexpr int BLOCK = WPB * N;
-    static constexpr int P = N + 1;
+    static constexpr int P = N + 1;   // odd LDS row stride: conflict-free rows and columns
     static constexpr int PL = N * P;
+    // minimum waves per SIMD the register allocator must allow
+    static constexpr int WAVES = N <= 16 ? 4 : (N <= 32 ? 3 : 2);
 };
 
+// Sum over the N lanes of one window (all lanes receive the total).
+template <int N>
+__device__ __forceinline__ float window_sum(float v, float *red, int lane)
+{
+    if constexpr (KCfg<N>::IN_WAVE) {
+#pragma unroll
+        for (int m = N / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, N);
+        return v;
+    } else {
+        __syncthreads();
+        red[lane] = v;
+        __syncthreads();
+        float s = 0.0f;
+        for (int i = 0; i < N; ++i) s += red[i];
+        return s;
+    }
+}
+
+template <int FMT>
+__device__ __forceinline__ float raw_sample(const void *__restrict__ base, uint32_t idx)
+{
+    if constexpr (FMT == FMT_U8) {
+        return (float)((const __attribute__((address_space(1))) uint8_t *)base)[idx];
+    } else if constexpr (FMT == FMT_U16) {
+        return (float)((const __attribute__((address_space(1))) uint16_t *)base)[idx];
+    } else {
+        return ((const __attribute__((address_space(1))) float *)base)[idx];
+    }
+}
+
+// correlate.rs:182-202: window column c, sampled at fractional origin (x, y) in raw units.
+// The integer path (tx, ty < 1e-4) copies samples without interpolation, as the reference.
+// Rows and columns are clamped into the frame: a window exactly as wide (tall) as the frame
+// has zero weight on the sample the reference would slice past the edge.
 template <int N, int FMT>
-__global__ __launch_bounds__(KCfg<N>::BLOCK) void k_correlate(const PassPair *__restrict__ pairs,
-                                                              const uint32_t *__restrict__ begins,
-                                                              int npairs, uint32_t total,
-                                                              uint32_t wpp)
+__device__ __forceinline__ void load_column_raw(const void *__restrict__ img, int pitch, int H,
+                                                int W, float x, float y, int c, float (&col)[N])
+{
+    const int x0 = (int)floorf(x), y0 = (int)floorf(y);
+    const float tx = x - (float)x0, ty = y - (float)y0;
+    const bool ints = tx < 1e-4f && ty < 1e-4f;
+    const uint32_t xa = (uint32_t)min(x0 + c, W - 1), xb = (uint32_t)min(x0 + c + 1, W - 1);
+    const uint32_t off0 = (uint32_t)y0 * (uint32_t)pitch, last = (uint32_t)(H - 1) * (uint32_t)pitch;
+    float hprev = 0.0f;
+#pragma unroll
+    for (int r = 0; r <= N; ++r) {
+        const uint32_t row = min(off0 + (uint32_t)r * (uint32_t)pitch, last);
+        const float p0 = raw_sample<FMT>(img, row + xa);
+        float h = p0;
+        if (!ints) {
+            const float p1 = raw_sample<FMT>(img, row + xb);
+            h = __builtin_fmaf(tx, p1 - p0, p0);
+        }
+        if (r > 0) col[r - 1] = ints ? hprev : __builtin_fmaf(ty, h - hprev, hprev);
+        hprev = h;
+        // bound the loads in flight (and their registers) to 16 rows at a time
+        if ((r & 15) == 15) __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// energy threshold of correlate.rs:110 in raw sample units
+template <int FMT>
+__device__ __forceinline__ double energy_floor()
+{
+    if constexpr (FMT == FMT_U8) return 1e-12 * 255.0 * 255.0;
+    else if constexpr (FMT == FMT_U16) return 1e-12 * 65535.0 * 65535.0;
+    else return 1e-12;
+}
+
+template <int N, int FMT>
+__global__ __launch_bounds__(KCfg<N>::BLOCK) __attribute__((amdgpu_waves_per_eu(KCfg<N>::WAVES)))
+void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblocks)
 {
     using C = KCfg<N>;
     constexpr int WPB = C::WPB, P = C::P, PL = C::PL, H2 = N / 2;
     __shared__ float s_re[WPB * PL];
-    __shared__ float s_im[WPB * PL];
-    __shared__ double s_red[WPB * 2 * N];
+    __shared__ float s_im[C::IN_WAVE ? 1 : WPB * PL];
+    __shared__ float s_red[WPB * N];
     __shared__ float s_pkv[WPB * N];
     __shared__ int s_pki[WPB * N];
+
+    // ---- XCD-aware block order: XCD x (blocks L = x, x+8, ...) gets a contiguous range ----
+    const uint32_t L = blockIdx.x;
+    const uint32_t q = nblocks >> 3, rem = nblocks & 7, xcd = L & 7;
+    const uint32_t Lr = xcd * q + min(xcd, rem) + (L >> 3);
+    const uint32_t p = Lr / bpp, blk = Lr - p * bpp;
 
     const int tid = threadIdx.x;
     const int wl = tid / N;
     const int lane = tid - wl * N;
-    const uint32_t gw = blockIdx.x * WPB + wl;
-    const bool active = gw < total;
-    float *pr = s_re + wl * PL;
-    float *pi = s_im + wl * PL;
-    double *red = s_red + wl * 2 * N;
-    float *pkv = s_pkv + wl * N;
-    int *pki = s_pki + wl * N;
-
-    // ---- which pair / cell ------------------------------------------------------------
-    uint32_t p = 0, k = 0;
-    if (active) {
-        if (wpp) {
-            p = gw / wpp;
-            k = gw - p * wpp;
-        } else {
-            p = find_pair_u(begins, npairs, gw);
-            k = gw - begins[p];
-        }
-    }
     const PassPair &pp = pairs[p];
     const GridDev g = pp.grid;
+    const uint32_t cells = (uint32_t)(g.nx * g.ny);
+    const uint32_t k = blk * WPB + wl;
+    const bool active = k < cells;
+    float *pr = s_re + wl * PL;
+    float *pim = C::IN_WAVE ? s_im : s_im + wl * PL;
+    float *red = s_red + wl * N;
+    float *pkv = s_pkv + wl * N;
+    int *pki = s_pki + wl * N;
     const int ix = (int)(k % (uint32_t)g.nx), iy = (int)(k / (uint32_t)g.nx);
 
     // ---- placement (multipass.rs:207-231) -------------------------------------------
@@ -92,11 +173,11 @@ __global__ __launch_bounds__(KCfg<N>::BLOCK) void k_correlate(const PassPair *__
         if (!(m >= pp.min_mean && sd >= pp.min_std)) sig = false;
     }
 
-    // ---- fused ingest: window columns (correlate.rs:182-202) -----------------------------
+    // ---- fused ingest + mean removal --------------------------------------------------
     float zr[N], zi[N];
     if (sig) {
-        load_column<N, FMT>(pp.img_a, pp.pitch, pp.H, pp.W, ax, ay, lane, zr);
-        load_column<N, FMT>(pp.img_b, pp.pitch, pp.H, pp.W, bx, by, lane, zi);
+        load_column_raw<N, FMT>(pp.img_a, pp.pitch, pp.H, pp.W, ax, ay, lane, zr);
+        load_column_raw<N, FMT>(pp.img_b, pp.pitch, pp.H, pp.W, bx, by, lane, zi);
     } else {
 #pragma unroll
         for (int r = 0; r < N; ++r) {
@@ -104,155 +185,251 @@ __global__ __launch_bounds__(KCfg<N>::BLOCK) void k_correlate(const PassPair *__
             zi[r] = 0.0f;
         }
     }
-
-    // ---- mean removal and energy (correlate.rs:203-209) -------------------------------
     {
-        double sa = 0.0, sb = 0.0;
+        // Approximate means only reduce the dynamic range of the transform: the DC bin is
+        // cleared exactly in the cross spectrum, which is what the reference's mean removal
+        // amounts to.
+        float sa = 0.0f, sb = 0.0f;
 #pragma unroll
         for (int r = 0; r < N; ++r) {
-            sa += (double)zr[r];
-            sb += (double)zi[r];
+            sa += zr[r];
+            sb += zi[r];
         }
-        red[lane] = sa;
-        red[N + lane] = sb;
-    }
-    __syncthreads();
-    double ea = 0.0, eb = 0.0;
-    {
-        double ta = 0.0, tb = 0.0;
-        for (int i = 0; i < N; ++i) {
-            ta += red[i];
-            tb += red[N + i];
-        }
-        const float ma = (float)(ta / (double)(N * N));
-        const float mb = (float)(tb / (double)(N * N));
+        constexpr float inv = 1.0f / (float)(N * N);
+        const float ma = window_sum<N>(sa, red, lane) * inv;
+        const float mb = window_sum<N>(sb, red, lane) * inv;
 #pragma unroll
         for (int r = 0; r < N; ++r) {
             zr[r] -= ma;
             zi[r] -= mb;
-            ea += (double)zr[r] * (double)zr[r];
-            eb += (double)zi[r] * (double)zi[r];
         }
     }
-    __syncthreads();
-    red[lane] = ea;
-    red[N + lane] = eb;
-    __syncthreads();
-    ea = 0.0;
-    eb = 0.0;
-    for (int i = 0; i < N; ++i) {
-        ea += red[i];
-        eb += red[N + i];
-    }
-    const bool has_peak = sig && !(ea <= 1e-12 || eb <= 1e-12);
 
     // ---- forward transform of z = a + i b ----------------------------------------------
-    pivfft::fft<N, -1>(zr, zi);  // along rows r, lane = column c
-    // transpose: lane c holds Z1[ky][c]; lane ky takes row ky
+    // Transposes go through LDS.  A window inside one wave uses a single float plane and
+    // moves real and imaginary parts in two passes (the barriers of a one-wave workgroup cost
+    // nothing), halving the LDS footprint, which bounds occupancy at N = 32 and 64.
+    pivfft::fft<N, -1>(zr, zi);  // along rows r; lane = column
+    if constexpr (C::IN_WAVE) {
 #pragma unroll
-    for (int r = 0; r < N; ++r) {
-        pr[r * P + lane] = zr[r];
-        pi[r * P + lane] = zi[r];
-    }
-    __syncthreads();
+        for (int r = 0; r < N; ++r) pr[r * P + lane] = zr[r];
+        __syncthreads();
 #pragma unroll
-    for (int x = 0; x < N; ++x) {
-        zr[x] = pr[lane * P + x];
-        zi[x] = pi[lane * P + x];
+        for (int x = 0; x < N; ++x) zr[x] = pr[lane * P + x];
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < N; ++r) pr[r * P + lane] = zi[r];
+        __syncthreads();
+#pragma unroll
+        for (int x = 0; x < N; ++x) zi[x] = pr[lane * P + x];
+    } else {
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < N; ++r) {
+            pr[r * P + lane] = zr[r];
+            pim[r * P + lane] = zi[r];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int x = 0; x < N; ++x) {
+            zr[x] = pr[lane * P + x];
+            zi[x] = pim[lane * P + x];
+        }
     }
     pivfft::fft<N, -1>(zr, zi);  // lane ky holds Z[ky][kx]
-    __syncthreads();
-#pragma unroll
-    for (int x = 0; x < N; ++x) {
-        pr[lane * P + x] = zr[x];
-        pi[lane * P + x] = zi[x];
-    }
-    __syncthreads();
 
-    // ---- cross spectrum conj(FA) * FB (correlate.rs:128-131) --------------------------
-    // 2A = Z[k] + conj(Z[-k]),  2B = (Z[k] - conj(Z[-k])) / i ;  X' = conj(2A) (2B) = 4 X
+    // ---- cross spectrum conj(FA) FB, DC cleared; energies by Parseval ---------------------
+    // 2A = Z[k] + conj(Z[-k]),  2B = (Z[k] - conj(Z[-k])) / i,  X' = conj(2A) 2B = 4 X
+    float ea2 = 0.0f, eb2 = 0.0f;
     {
         const int mrow = (N - lane) % N;
+        if constexpr (C::IN_WAVE) {
+            // The partner lane (row -ky) supplies Z[-ky][-kx] by shuffle.  Columns x and N-x
+            // are done together so both originals are read before either is overwritten.
+            const int src = (tid - lane) + mrow;  // partner lane in this wave
 #pragma unroll
-        for (int x = 0; x < N; ++x) {
-            const int mx = (N - x) % N;
-            const float qr = pr[mrow * P + mx], qi = pi[mrow * P + mx];
-            const float a2r = zr[x] + qr, a2i = zi[x] - qi;
-            const float b2r = zi[x] + qi, b2i = qr - zr[x];
-            zr[x] = __builtin_fmaf(a2r, b2r, a2i * b2i);
-            zi[x] = __builtin_fmaf(a2r, b2i, -(a2i * b2r));
+            for (int x = 0; x <= H2; ++x) {
+                const int mx = (N - x) % N;
+                const float qr = __shfl(zr[mx], src, 64), qi = __shfl(zi[mx], src, 64);
+                float qr2 = 0.0f, qi2 = 0.0f;
+                if (mx != x) {
+                    qr2 = __shfl(zr[x], src, 64);
+                    qi2 = __shfl(zi[x], src, 64);
+                }
+                {
+                    const float a2r = zr[x] + qr, a2i = zi[x] - qi;
+                    const float b2r = zi[x] + qi, b2i = qr - zr[x];
+                    const bool dc = (x == 0) && (lane == 0);
+                    ea2 = dc ? ea2 : __builtin_fmaf(a2r, a2r, __builtin_fmaf(a2i, a2i, ea2));
+                    eb2 = dc ? eb2 : __builtin_fmaf(b2r, b2r, __builtin_fmaf(b2i, b2i, eb2));
+                    const float xr = __builtin_fmaf(a2r, b2r, a2i * b2i);
+                    const float xi = __builtin_fmaf(a2r, b2i, -(a2i * b2r));
+                    zr[x] = dc ? 0.0f : xr;
+                    zi[x] = dc ? 0.0f : xi;
+                }
+                if (mx != x) {
+                    const float a2r = zr[mx] + qr2, a2i = zi[mx] - qi2;
+                    const float b2r = zi[mx] + qi2, b2i = qr2 - zr[mx];
+                    ea2 = __builtin_fmaf(a2r, a2r, __builtin_fmaf(a2i, a2i, ea2));
+                    eb2 = __builtin_fmaf(b2r, b2r, __builtin_fmaf(b2i, b2i, eb2));
+                    zr[mx] = __builtin_fmaf(a2r, b2r, a2i * b2i);
+                    zi[mx] = __builtin_fmaf(a2r, b2i, -(a2i * b2r));
+                }
+                if ((x & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+            }
+        } else {
+            __syncthreads();
+#pragma unroll
+            for (int x = 0; x < N; ++x) {
+                pr[lane * P + x] = zr[x];
+                pim[lane * P + x] = zi[x];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int x = 0; x < N; ++x) {
+                const int mx = (N - x) % N;
+                const float qr = pr[mrow * P + mx], qi = pim[mrow * P + mx];
+                const float a2r = zr[x] + qr, a2i = zi[x] - qi;
+                const float b2r = zi[x] + qi, b2i = qr - zr[x];
+                const bool dc = (x == 0) && (lane == 0);
+                ea2 = dc ? ea2 : __builtin_fmaf(a2r, a2r, __builtin_fmaf(a2i, a2i, ea2));
+                eb2 = dc ? eb2 : __builtin_fmaf(b2r, b2r, __builtin_fmaf(b2i, b2i, eb2));
+                const float xr = __builtin_fmaf(a2r, b2r, a2i * b2i);
+                const float xi = __builtin_fmaf(a2r, b2i, -(a2i * b2r));
+                zr[x] = dc ? 0.0f : xr;
+                zi[x] = dc ? 0.0f : xi;
+            }
         }
     }
-    // rows ky = 0..N/2 of the Hermitian cross spectrum: inverse along kx
+    // sum_k |2A_k|^2 = 4 N^2 sum (a - mean)^2  (Parseval), in raw units squared
+    const double norm = 1.0 / (4.0 * (double)(N * N));
+    const double ea = (double)window_sum<N>(ea2, red, lane) * norm;
+    const double eb = (double)window_sum<N>(eb2, red, lane) * norm;
+    const double floor_e = energy_floor<FMT>();
+    const bool has_peak = sig && !(ea <= floor_e || eb <= floor_e);
+
+    // ---- Hermitian inverse ------------------------------------------------------------
+    // Rows ky = 0..N/2 are inverted along kx (Y[ky][x]); the other rows are conjugates.
+    // Then column pairs (2j, 2j+1) are inverted along ky as one complex transform
+    //   V[ky] = Y[ky][2j] + i Y[ky][2j+1]  ->  C[.][2j] + i C[.][2j+1].
     pivfft::fft<N, 1>(zr, zi);  // lane ky (<= N/2) holds Y[ky][x]
-    __syncthreads();
-    if (lane <= H2) {
+    const int jj = lane < H2 ? lane : lane - H2;
+    if constexpr (C::IN_WAVE) {
+        // Row owners write V's real parts (virtual rows ky and N-ky of the half plane), keep
+        // V's imaginary parts in zi, then the column-pair lanes gather them in two passes.
+        constexpr int HW = H2 + 1;  // odd row stride of the half-width plane
+        const int s = lane;
+        const bool own = s <= H2, mirror = s >= 1 && s < H2;
+        __syncthreads();
 #pragma unroll
-        for (int x = 0; x < N; ++x) {
-            pr[lane * P + x] = zr[x];
-            pi[lane * P + x] = zi[x];
+        for (int j = 0; j < H2; ++j) {
+            const float yr0 = zr[2 * j], yi0 = zi[2 * j], yr1 = zr[2 * j + 1], yi1 = zi[2 * j + 1];
+            if (own) pr[s * HW + j] = yr0 - yi1;
+            if (mirror) pr[(N - s) * HW + j] = yr0 + yi1;
+            zi[2 * j] = yi0 + yr1;
+            zi[2 * j + 1] = yr1 - yi0;
         }
-    }
-    __syncthreads();
-    // column pair (2j, 2j+1), inverse along ky: C[.][2j] + i C[.][2j+1]
-    {
-        const int j = lane < H2 ? lane : lane - H2;
-        const int c0 = 2 * j, c1 = 2 * j + 1;
+        __syncthreads();
+#pragma unroll
+        for (int ky = 0; ky < N; ++ky) zr[ky] = pr[ky * HW + jj];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < H2; ++j) {
+            if (own) pr[s * HW + j] = zi[2 * j];
+            if (mirror) pr[(N - s) * HW + j] = zi[2 * j + 1];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ky = 0; ky < N; ++ky) zi[ky] = pr[ky * HW + jj];
+    } else {
+        __syncthreads();
+        if (lane <= H2) {
+#pragma unroll
+            for (int x = 0; x < N; ++x) {
+                pr[lane * P + x] = zr[x];
+                pim[lane * P + x] = zi[x];
+            }
+        }
+        __syncthreads();
+        const int c0 = 2 * jj, c1 = 2 * jj + 1;
 #pragma unroll
         for (int ky = 0; ky < N; ++ky) {
             if (ky <= H2) {
-                const float y0r = pr[ky * P + c0], y0i = pi[ky * P + c0];
-                const float y1r = pr[ky * P + c1], y1i = pi[ky * P + c1];
+                const float y0r = pr[ky * P + c0], y0i = pim[ky * P + c0];
+                const float y1r = pr[ky * P + c1], y1i = pim[ky * P + c1];
                 zr[ky] = y0r - y1i;
                 zi[ky] = y0i + y1r;
             } else {
                 const int s = N - ky;
-                const float y0r = pr[s * P + c0], y0i = pi[s * P + c0];
-                const float y1r = pr[s * P + c1], y1i = pi[s * P + c1];
+                const float y0r = pr[s * P + c0], y0i = pim[s * P + c0];
+                const float y1r = pr[s * P + c1], y1i = pim[s * P + c1];
                 zr[ky] = y0r + y1i;
                 zi[ky] = y1r - y0i;
             }
+            if ((ky & 7) == 7) __builtin_amdgcn_sched_barrier(0);
         }
-        pivfft::fft<N, 1>(zr, zi);
-        // normalisation (correlate.rs:134-137): `1.0 / (n^2 sqrt(ea eb)) as f32` divides in
-        // f32 after the cast; the 1/4 removes the factor of the packed cross spectrum.
+    }
+    pivfft::fft<N, 1>(zr, zi);
+    {
+        // correlate.rs:134-137: corr = C / (n^2 sqrt(ea eb)), the 1/4 undoes X' = 4 X
         const float denom = (float)((double)(N * N) * sqrt(ea * eb));
-        const float scale = 1.0f / denom;
+        const float scale = 0.25f / denom;
+        const int c0 = 2 * jj, c1 = 2 * jj + 1;
         __syncthreads();
         if (lane < H2) {
 #pragma unroll
             for (int y = 0; y < N; ++y) {
-                pr[y * P + c0] = (zr[y] * 0.25f) * scale;
-                pr[y * P + c1] = (zi[y] * 0.25f) * scale;
+                pr[y * P + c0] = zr[y] * scale;
+                pr[y * P + c1] = zi[y] * scale;
             }
         }
     }
     __syncthreads();
 
     // ---- find_peak (correlate.rs:215-303), lane = row -----------------------------------
-    // (1) first maximum in row-major order (strict >)
-    {
-        float best = -INFINITY;
-        int bi = 0;
-        for (int x = 0; x < N; ++x) {
-            const float v = pr[lane * P + x];
-            if (v > best) {
-                best = v;
-                bi = lane * N + x;
-            }
-        }
-        pkv[lane] = best;
-        pki[lane] = bi;
-    }
-    __syncthreads();
+    // NaN can only reach the plane from non-finite float input; the integer formats take
+    // the comparison shortcuts that assume finite values.
+    constexpr bool MAYBE_NAN = (FMT == FMT_F32);
+    // (1) first maximum in row-major order (strict >): row scan, then across rows
+#pragma unroll
+    for (int x = 0; x < N; ++x) zr[x] = pr[lane * P + x];
     float best = -INFINITY;
     int bidx = 0;
-    for (int r = 0; r < N; ++r) {
-        const float v = pkv[r];
-        if (v > best) {
-            best = v;
-            bidx = pki[r];
+    {
+        float rb = -INFINITY;
+        int ri = 0;
+#pragma unroll
+        for (int x = 0; x < N; ++x) {
+            const bool gt = zr[x] > rb;
+            rb = gt ? zr[x] : rb;
+            ri = gt ? x : ri;
         }
+        ri += lane * N;
+        if constexpr (C::IN_WAVE) {
+            // lexicographic (value desc, index asc) maximum: same as the first maximum
+#pragma unroll
+            for (int m = 1; m < N; m <<= 1) {
+                const float ov = __shfl_xor(rb, m, N);
+                const int oi = __shfl_xor(ri, m, N);
+                const bool take = (ov > rb) || (ov == rb && oi < ri);
+                rb = take ? ov : rb;
+                ri = take ? oi : ri;
+            }
+            best = rb;
+            bidx = ri;
+        } else {
+            pkv[lane] = rb;
+            pki[lane] = ri;
+            __syncthreads();
+            for (int r = 0; r < N; ++r) {
+                const float v = pkv[r];
+                if (v > best) {
+                    best = v;
+                    bidx = pki[r];
+                }
+            }
+        }
+        if (!(best > -INFINITY)) bidx = 0;  // no maximum at all: (0, 0) as the reference
     }
     const int pby = bidx / N, pbx = bidx - pby * N;
     // (2) footprint: half width at half maximum along +-x, +-y
@@ -271,36 +448,53 @@ __global__ __launch_bounds__(KCfg<N>::BLOCK) void k_correlate(const PassPair *__
     }
     int exclude = (int)ceilf(1.5f * (float)hw);
     exclude = exclude < 2 ? 2 : exclude;
-    // (3) highest local maximum outside the footprint
+    // (3) highest local maximum outside the footprint (all 8 neighbours <= v)
+    float second = -INFINITY;
     {
         float cand = -INFINITY;
         const int ru = (lane + N - 1) % N, rd = (lane + 1) % N;
         int dyc = abs(lane - pby);
         dyc = dyc < N - dyc ? dyc : N - dyc;
-        float u0 = pr[ru * P + N - 1], u1 = pr[ru * P + 0];
-        float m0 = pr[lane * P + N - 1], m1 = pr[lane * P + 0];
-        float d0 = pr[rd * P + N - 1], d1 = pr[rd * P + 0];
+        const bool row_far = dyc > exclude;
+        // vertical max of (row above, row, row below) per column, sliding along x
+        float vm_prev = fmaxf(fmaxf(pr[ru * P + N - 1], zr[N - 1]), pr[rd * P + N - 1]);
+        float u1 = pr[ru * P + 0], d1 = pr[rd * P + 0];
+        float vm_cur = fmaxf(fmaxf(u1, zr[0]), d1);
+#pragma unroll
         for (int x = 0; x < N; ++x) {
             const int xn = x + 1 < N ? x + 1 : 0;
-            const float u2 = pr[ru * P + xn], m2 = pr[lane * P + xn], d2 = pr[rd * P + xn];
+            const float u2 = pr[ru * P + xn], d2 = pr[rd * P + xn];
+            const float vm_next = fmaxf(fmaxf(u2, zr[xn]), d2);
             int dxc = abs(x - pbx);
             dxc = dxc < N - dxc ? dxc : N - dxc;
-            const int cd = dyc > dxc ? dyc : dxc;
-            const float v = m1;
-            if (v > cand && cd > exclude && u0 <= v && u1 <= v && u2 <= v && m0 <= v && m2 <= v &&
-                d0 <= v && d1 <= v && d2 <= v)
-                cand = v;
-            u0 = u1; u1 = u2;
-            m0 = m1; m1 = m2;
-            d0 = d1; d1 = d2;
+            const float v = zr[x];
+            bool local;
+            if constexpr (MAYBE_NAN) {
+                const float m0 = zr[(x + N - 1) % N], m2 = zr[xn];
+                const float u0 = pr[ru * P + (x + N - 1) % N], d0 = pr[rd * P + (x + N - 1) % N];
+                local = u0 <= v && u1 <= v && u2 <= v && m0 <= v && m2 <= v && d0 <= v &&
+                        d1 <= v && d2 <= v;
+            } else {
+                local = fmaxf(fmaxf(vm_prev, vm_next), fmaxf(u1, d1)) <= v;
+            }
+            if (local && (row_far || dxc > exclude) && v > cand) cand = v;
+            vm_prev = vm_cur;
+            vm_cur = vm_next;
+            u1 = u2;
+            d1 = d2;
         }
-        __syncthreads();
-        pkv[lane] = cand;
+        if constexpr (C::IN_WAVE) {
+#pragma unroll
+            for (int m = 1; m < N; m <<= 1) cand = fmaxf(cand, __shfl_xor(cand, m, N));
+            second = cand;
+        } else {
+            __syncthreads();
+            pkv[lane] = cand;
+            __syncthreads();
+            for (int r = 0; r < N; ++r) second = pkv[r] > second ? pkv[r] : second;
+        }
     }
-    __syncthreads();
     if (lane == 0 && active) {
-        float second = -INFINITY;
-        for (int r = 0; r < N; ++r) second = pkv[r] > second ? pkv[r] : second;
         // (4) three-point Gaussian (parabolic fallback) sub-pixel fit
         float sub[2];
 #pragma unroll
@@ -310,8 +504,7 @@ __global__ __launch_bounds__(KCfg<N>::BLOCK) void k_correlate(const PassPair *__
             const float q = axis == 0 ? pr[pby * P + (pbx + 1) % N] : pr[((pby + 1) % N) * P + pbx];
             float d;
             if (m > 0.0f && c > 0.0f && q > 0.0f) {
-                const float lm = (float)log((double)m), lc = (float)log((double)c),
-                            lq = (float)log((double)q);
+                const float lm = logf(m), lc = logf(c), lq = logf(q);
                 d = (lm - lq) / (2.0f * (lm - 2.0f * lc + lq));
             } else {
                 d = (m - q) / (2.0f * (m - 2.0f * c + q));
@@ -349,27 +542,26 @@ __global__ __launch_bounds__(KCfg<N>::BLOCK) void k_correlate(const PassPair *__
     }
 }
 
-
 template <int N, int FMT>
-hipError_t launch_corr_t(const PassPair *pairs, const uint32_t *begins, int npairs,
-                                uint32_t total, uint32_t wpp, hipStream_t st)
+hipError_t launch_corr_t(const PassPair *pairs, int npairs, uint32_t max_cells, hipStream_t st)
 {
     using C = KCfg<N>;
-    const uint32_t blocks = (total + C::WPB - 1) / C::WPB;
+    const uint32_t bpp = (max_cells + C::WPB - 1) / C::WPB;
+    const uint32_t blocks = bpp * (uint32_t)npairs;
     if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_correlate<N, FMT>), dim3(blocks), dim3(C::BLOCK), 0, st, pairs, begins,
-                       npairs, total, wpp);
+    hipLaunchKernelGGL((k_correlate<N, FMT>), dim3(blocks), dim3(C::BLOCK), 0, st, pairs, bpp,
+                       blocks);
     return hipGetLastError();
 }
 
 template <int N>
-hipError_t launch_corr_n(int fmt, const PassPair *pairs, const uint32_t *begins,
-                                int npairs, uint32_t total, uint32_t wpp, hipStream_t st)
+hipError_t launch_corr_n(int fmt, const PassPair *pairs, int npairs, uint32_t max_cells,
+                         hipStream_t st)
 {
     switch (fmt) {
-    case FMT_U8: return launch_corr_t<N, FMT_U8>(pairs, begins, npairs, total, wpp, st);
-    case FMT_U16: return launch_corr_t<N, FMT_U16>(pairs, begins, npairs, total, wpp, st);
-    default: return launch_corr_t<N, FMT_F32>(pairs, begins, npairs, total, wpp, st);
+    case FMT_U8: return launch_corr_t<N, FMT_U8>(pairs, npairs, max_cells, st);
+    case FMT_U16: return launch_corr_t<N, FMT_U16>(pairs, npairs, max_cells, st);
+    default: return launch_corr_t<N, FMT_F32>(pairs, npairs, max_cells, st);
     }
 }
 

@@ -200,6 +200,10 @@ __device__ __forceinline__ void stockham(float (&xr)[N], float (&xi)[N])
                 yr[d + r * NS] = vr[r];
                 yi[d + r * NS] = vi[r];
             }
+            // large transforms: keep the scheduler from interleaving too many butterflies
+            // (each in-flight butterfly holds extra registers)
+            if constexpr (N >= 64)
+                if ((j & 3) == 3) __builtin_amdgcn_sched_barrier(0);
         }
         stockham<N, S, NS * R, REM / R>(yr, yi);
 #pragma unroll

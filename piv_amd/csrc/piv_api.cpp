@@ -577,8 +577,7 @@ piv_status run_batch(piv_analyzer *a, const piv_frame *const *fa, const piv_fram
         const double cbytes = (double)total * (2.0 * pl.n * pl.n * 4.0 + 49.0);
         HIP_TRY(timed(a, "correlate", pl.n, cbytes, total, [&] {
             return launch_correlate(pl.n, fa[0]->fmt, reinterpret_cast<PassPair *>(dd + o_pp[i]),
-                                    reinterpret_cast<uint32_t *>(dd + o_pb[i]), (int)npairs,
-                                    total, (uint32_t)pl.cells, st);
+                                    (int)npairs, (uint32_t)pl.cells, st);
         }));
         // validation reads the raw field (13 B/vector) and writes the validated one (13 B)
         HIP_TRY(timed(a, "validate", 0, (double)total * 26.0, total, [&] {
@@ -1060,7 +1059,7 @@ piv_status piv_correlate_windows(piv_analyzer *a, const piv_frame *fa, const piv
     HIP_TRY(hipMemcpyAsync(d_org, origins, count * 16, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(d_pp, &d, sizeof d, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(d_begin, &zero, sizeof zero, hipMemcpyHostToDevice, st));
-    HIP_TRY(launch_correlate((int)window, fa->fmt, d_pp, d_begin, 1, count, count, st));
+    HIP_TRY(launch_correlate((int)window, fa->fmt, d_pp, 1, count, st));
     HIP_TRY(hipMemcpyAsync(dx, o_dx, count * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(dy, o_dy, count * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(ppr, o_ppr, count * 4, hipMemcpyDeviceToHost, st));

@@ -25,22 +25,22 @@
 
 namespace piv {
 
-hipError_t launch_correlate_8(int fmt, const PassPair *pairs, const uint32_t *begins, int npairs,
-                              uint32_t total, uint32_t wpp, hipStream_t st);
-hipError_t launch_correlate_12(int fmt, const PassPair *pairs, const uint32_t *begins, int npairs,
-                              uint32_t total, uint32_t wpp, hipStream_t st);
-hipError_t launch_correlate_16(int fmt, const PassPair *pairs, const uint32_t *begins, int npairs,
-                              uint32_t total, uint32_t wpp, hipStream_t st);
-hipError_t launch_correlate_24(int fmt, const PassPair *pairs, const uint32_t *begins, int npairs,
-                              uint32_t total, uint32_t wpp, hipStream_t st);
-hipError_t launch_correlate_32(int fmt, const PassPair *pairs, const uint32_t *begins, int npairs,
-                              uint32_t total, uint32_t wpp, hipStream_t st);
-hipError_t launch_correlate_48(int fmt, const PassPair *pairs, const uint32_t *begins, int npairs,
-                              uint32_t total, uint32_t wpp, hipStream_t st);
-hipError_t launch_correlate_64(int fmt, const PassPair *pairs, const uint32_t *begins, int npairs,
-                              uint32_t total, uint32_t wpp, hipStream_t st);
-hipError_t launch_correlate_96(int fmt, const PassPair *pairs, const uint32_t *begins, int npairs,
-                              uint32_t total, uint32_t wpp, hipStream_t st);
+hipError_t launch_correlate_8(int fmt, const PassPair *pairs, int npairs, uint32_t max_cells,
+                              hipStream_t st);
+hipError_t launch_correlate_12(int fmt, const PassPair *pairs, int npairs, uint32_t max_cells,
+                              hipStream_t st);
+hipError_t launch_correlate_16(int fmt, const PassPair *pairs, int npairs, uint32_t max_cells,
+                              hipStream_t st);
+hipError_t launch_correlate_24(int fmt, const PassPair *pairs, int npairs, uint32_t max_cells,
+                              hipStream_t st);
+hipError_t launch_correlate_32(int fmt, const PassPair *pairs, int npairs, uint32_t max_cells,
+                              hipStream_t st);
+hipError_t launch_correlate_48(int fmt, const PassPair *pairs, int npairs, uint32_t max_cells,
+                              hipStream_t st);
+hipError_t launch_correlate_64(int fmt, const PassPair *pairs, int npairs, uint32_t max_cells,
+                              hipStream_t st);
+hipError_t launch_correlate_96(int fmt, const PassPair *pairs, int npairs, uint32_t max_cells,
+                              hipStream_t st);
 
 // ---------------------------------------------------------------------------------------
 // validation (validate.rs:51-171)
@@ -418,18 +418,18 @@ bool correlate_supported(int n)
     }
 }
 
-hipError_t launch_correlate(int n, int fmt, const PassPair *pairs, const uint32_t *begins,
-                            int npairs, uint32_t total, uint32_t wpp, hipStream_t st)
+hipError_t launch_correlate(int n, int fmt, const PassPair *pairs, int npairs,
+                            uint32_t max_cells, hipStream_t st)
 {
     switch (n) {
-    case 8: return launch_correlate_8(fmt, pairs, begins, npairs, total, wpp, st);
-    case 12: return launch_correlate_12(fmt, pairs, begins, npairs, total, wpp, st);
-    case 16: return launch_correlate_16(fmt, pairs, begins, npairs, total, wpp, st);
-    case 24: return launch_correlate_24(fmt, pairs, begins, npairs, total, wpp, st);
-    case 32: return launch_correlate_32(fmt, pairs, begins, npairs, total, wpp, st);
-    case 48: return launch_correlate_48(fmt, pairs, begins, npairs, total, wpp, st);
-    case 64: return launch_correlate_64(fmt, pairs, begins, npairs, total, wpp, st);
-    case 96: return launch_correlate_96(fmt, pairs, begins, npairs, total, wpp, st);
+    case 8: return launch_correlate_8(fmt, pairs, npairs, max_cells, st);
+    case 12: return launch_correlate_12(fmt, pairs, npairs, max_cells, st);
+    case 16: return launch_correlate_16(fmt, pairs, npairs, max_cells, st);
+    case 24: return launch_correlate_24(fmt, pairs, npairs, max_cells, st);
+    case 32: return launch_correlate_32(fmt, pairs, npairs, max_cells, st);
+    case 48: return launch_correlate_48(fmt, pairs, npairs, max_cells, st);
+    case 64: return launch_correlate_64(fmt, pairs, npairs, max_cells, st);
+    case 96: return launch_correlate_96(fmt, pairs, npairs, max_cells, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -8,8 +8,8 @@
 namespace piv {
 
 bool correlate_supported(int n);
-hipError_t launch_correlate(int n, int fmt, const PassPair *pairs, const uint32_t *begins,
-                            int npairs, uint32_t total, uint32_t wpp, hipStream_t st);
+hipError_t launch_correlate(int n, int fmt, const PassPair *pairs, int npairs,
+                            uint32_t max_cells, hipStream_t st);
 hipError_t launch_validate(const ValidatePair *jobs, const uint32_t *tile_begins, int njobs,
                            uint32_t ntiles, uint32_t *tile_counts, hipStream_t st);
 hipError_t launch_base(int fmt, const void *raw, int pitch, const float *bg, int H, int W,
