@@ -187,7 +187,6 @@ struct piv_analyzer {
     DevBuf fields;       // per-pass raw + validated fields of the current call
     DevBuf desc;         // descriptors
     HostBuf h_desc;
-    DevBuf tile_counts;
     DevBuf summaries;    // u32 [npass][npairs][5]
     HostBuf h_summaries;
     DevBuf prep_tmp, prep_tmp2, prep_sat;  // conditioning scratch
@@ -214,7 +213,6 @@ struct piv_analyzer {
         fields.release();
         desc.release();
         h_desc.release();
-        tile_counts.release();
         summaries.release();
         h_summaries.release();
         prep_tmp.release();
@@ -432,13 +430,16 @@ piv_status run_batch(piv_analyzer *a, const piv_frame *const *fa, const piv_fram
     const size_t np = plans.size();
     // ---- workspace layout ----
     size_t off = 0;
-    std::vector<size_t> o_raw(np), o_val(np), o_org(np), o_pk(np);
+    std::vector<size_t> o_raw(np), o_val(np), o_snap(np), o_org(np), o_pk(np);
+    std::vector<uint8_t *> snap(np);
     for (size_t i = 0; i < np; ++i) {
         const size_t c = plans[i].cells * npairs;
         o_raw[i] = off;
         off = align_up(off + c * (3 * sizeof(float) + 1), 256);
         o_val[i] = off;
         off = align_up(off + c * (3 * sizeof(float) + 1), 256);
+        o_snap[i] = off;
+        off = align_up(off + c, 256);
         if (traced) {
             o_org[i] = off;
             off = align_up(off + c * 4 * sizeof(float), 256);
@@ -464,6 +465,7 @@ piv_status run_batch(piv_analyzer *a, const piv_frame *const *fa, const piv_fram
         };
         raw[i] = carve(o_raw[i]);
         val[i] = carve(o_val[i]);
+        snap[i] = reinterpret_cast<uint8_t *>(base + o_snap[i]);
         if (traced) {
             origins[i] = reinterpret_cast<float *>(base + o_org[i]);
             peaks[i] = reinterpret_cast<int32_t *>(base + o_pk[i]);
@@ -477,28 +479,17 @@ piv_status run_batch(piv_analyzer *a, const piv_frame *const *fa, const piv_fram
         l.flags = out.flags;
     }
     // ---- descriptors ----
-    std::vector<uint32_t> tiles_pp(np);
     size_t dsz = 0;
-    std::vector<size_t> o_pp(np), o_pb(np), o_vp(np), o_tb(np);
+    std::vector<size_t> o_pp(np), o_vp(np);
     for (size_t i = 0; i < np; ++i) {
-        const GridDev &g = plans[i].grid;
-        tiles_pp[i] = (uint32_t)(((g.nx + VALIDATE_TILE - 1) / VALIDATE_TILE) *
-                                 ((g.ny + VALIDATE_TILE - 1) / VALIDATE_TILE));
         o_pp[i] = dsz;
         dsz = align_up(dsz + npairs * sizeof(PassPair), 64);
-        o_pb[i] = dsz;
-        dsz = align_up(dsz + npairs * sizeof(uint32_t), 64);
         o_vp[i] = dsz;
         dsz = align_up(dsz + npairs * sizeof(ValidatePair), 64);
-        o_tb[i] = dsz;
-        dsz = align_up(dsz + npairs * sizeof(uint32_t), 64);
     }
     PIV_TRY(a->desc.reserve(dsz));
     PIV_TRY(a->h_desc.reserve(dsz));
     PIV_TRY(a->summaries.reserve(np * npairs * 5 * sizeof(uint32_t)));
-    uint32_t max_tiles = 0;
-    for (size_t i = 0; i < np; ++i) max_tiles = std::max(max_tiles, tiles_pp[i] * npairs);
-    PIV_TRY(a->tile_counts.reserve((size_t)max_tiles * 5 * sizeof(uint32_t)));
     // The pinned staging buffer may still be read by a previous call's copy: wait for it.
     HIP_TRY(hipStreamSynchronize(a->stream));
     prof_harvest(a);
@@ -509,9 +500,7 @@ piv_status run_batch(piv_analyzer *a, const piv_frame *const *fa, const piv_fram
     for (size_t i = 0; i < np; ++i) {
         const PassPlan &pl = plans[i];
         auto *pp = reinterpret_cast<PassPair *>(hd + o_pp[i]);
-        auto *pb = reinterpret_cast<uint32_t *>(hd + o_pb[i]);
         auto *vp = reinterpret_cast<ValidatePair *>(hd + o_vp[i]);
-        auto *tb = reinterpret_cast<uint32_t *>(hd + o_tb[i]);
         for (uint32_t k = 0; k < npairs; ++k) {
             PassPair d{};
             const piv_frame *A = fa[k], *B = fb[k];
@@ -545,7 +534,6 @@ piv_status run_batch(piv_analyzer *a, const piv_frame *const *fa, const piv_fram
             if (B->pitch != A->pitch)
                 return fail(PIV_ERR_INVALID_ARGUMENT, "frames of a pair must share a row pitch");
             pp[k] = d;
-            pb[k] = d.win_begin;
             ValidatePair v{};
             v.raw_dx = d.out_dx;
             v.raw_dy = d.out_dy;
@@ -555,6 +543,7 @@ piv_status run_batch(piv_analyzer *a, const piv_frame *const *fa, const piv_fram
             v.dy = val[i].dy + co;
             v.ppr = val[i].ppr + co;
             v.flags = val[i].flags + co;
+            v.snapshot = snap[i] + co;
             v.summary = sum_dev + (i * npairs + k) * 5;
             v.nx = pl.grid.nx;
             v.ny = pl.grid.ny;
@@ -563,14 +552,12 @@ piv_status run_batch(piv_analyzer *a, const piv_frame *const *fa, const piv_fram
             v.median_eps = a->cfg.median_eps;
             v.max_displacement = maxdisp;
             v.replace = a->cfg.replace ? 1 : 0;
-            v.tile_begin = k * tiles_pp[i];
-            v.tiles_x = (pl.grid.nx + VALIDATE_TILE - 1) / VALIDATE_TILE;
             vp[k] = v;
-            tb[k] = v.tile_begin;
         }
     }
     hipStream_t st = a->stream;
     HIP_TRY(hipMemcpyAsync(dd, hd, dsz, hipMemcpyHostToDevice, st));
+    HIP_TRY(launch_zero_u32(sum_dev, (uint32_t)(np * npairs * 5), st));
     for (size_t i = 0; i < np; ++i) {
         const PassPlan &pl = plans[i];
         const uint32_t total = (uint32_t)(pl.cells * npairs);
@@ -581,9 +568,8 @@ piv_status run_batch(piv_analyzer *a, const piv_frame *const *fa, const piv_fram
         }));
         // validation reads the raw field (13 B/vector) and writes the validated one (13 B)
         HIP_TRY(timed(a, "validate", 0, (double)total * 26.0, total, [&] {
-            return launch_validate(reinterpret_cast<ValidatePair *>(dd + o_vp[i]),
-                                   reinterpret_cast<uint32_t *>(dd + o_tb[i]), (int)npairs,
-                                   tiles_pp[i] * npairs, a->tile_counts.as<uint32_t>(), st);
+            return launch_validate(reinterpret_cast<ValidatePair *>(dd + o_vp[i]), (int)npairs,
+                                   (uint32_t)pl.cells, st);
         }));
     }
     *summaries_dev = sum_dev;
@@ -1079,17 +1065,16 @@ piv_status piv_validate_field(piv_analyzer *a, uint32_t ny, uint32_t nx, float *
     PIV_TRY(set_device(a->device));
     const size_t n = (size_t)nx * ny;
     const size_t fb = align_up(n * 13, 256);
-    PIV_TRY(a->stage_in.reserve(2 * fb + 1024));
-    const uint32_t tiles = ((nx + VALIDATE_TILE - 1) / VALIDATE_TILE) * ((ny + VALIDATE_TILE - 1) / VALIDATE_TILE);
-    PIV_TRY(a->tile_counts.reserve((size_t)tiles * 5 * 4 + 64));
+    PIV_TRY(a->stage_in.reserve(2 * fb + align_up(n, 256) + 1024));
     char *b = a->stage_in.as<char>();
     float *r_dx = reinterpret_cast<float *>(b), *r_dy = r_dx + n, *r_ppr = r_dy + n;
     uint8_t *r_fl = reinterpret_cast<uint8_t *>(r_ppr + n);
     float *v_dx = reinterpret_cast<float *>(b + fb), *v_dy = v_dx + n, *v_ppr = v_dy + n;
     uint8_t *v_fl = reinterpret_cast<uint8_t *>(v_ppr + n);
-    auto *d_job = reinterpret_cast<ValidatePair *>(b + 2 * fb);
-    auto *d_tb = reinterpret_cast<uint32_t *>(b + 2 * fb + 512);
-    auto *d_sum = reinterpret_cast<uint32_t *>(b + 2 * fb + 768);
+    uint8_t *snapd = reinterpret_cast<uint8_t *>(b + 2 * fb);
+    char *tail = b + 2 * fb + align_up(n, 256);
+    auto *d_job = reinterpret_cast<ValidatePair *>(tail);
+    auto *d_sum = reinterpret_cast<uint32_t *>(tail + 512);
     ValidatePair v{};
     v.raw_dx = r_dx;
     v.raw_dy = r_dy;
@@ -1099,6 +1084,7 @@ piv_status piv_validate_field(piv_analyzer *a, uint32_t ny, uint32_t nx, float *
     v.dy = v_dy;
     v.ppr = v_ppr;
     v.flags = v_fl;
+    v.snapshot = snapd;
     v.summary = d_sum;
     v.nx = (int)nx;
     v.ny = (int)ny;
@@ -1107,17 +1093,15 @@ piv_status piv_validate_field(piv_analyzer *a, uint32_t ny, uint32_t nx, float *
     v.median_eps = a->cfg.median_eps;
     v.max_displacement = max_displacement;
     v.replace = a->cfg.replace ? 1 : 0;
-    v.tile_begin = 0;
-    v.tiles_x = (int)((nx + VALIDATE_TILE - 1) / VALIDATE_TILE);
-    uint32_t zero = 0, hs[5];
+    uint32_t hs[5];
     hipStream_t st = a->stream;
     HIP_TRY(hipMemcpyAsync(r_dx, dx, n * 4, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(r_dy, dy, n * 4, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(r_ppr, ppr, n * 4, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(r_fl, flags, n, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(d_job, &v, sizeof v, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(d_tb, &zero, sizeof zero, hipMemcpyHostToDevice, st));
-    HIP_TRY(launch_validate(d_job, d_tb, 1, tiles, a->tile_counts.as<uint32_t>(), st));
+    HIP_TRY(launch_zero_u32(d_sum, 5, st));
+    HIP_TRY(launch_validate(d_job, 1, (uint32_t)n, st));
     HIP_TRY(hipMemcpyAsync(dx, v_dx, n * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(dy, v_dy, n * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(flags, v_fl, n, hipMemcpyDeviceToHost, st));

@@ -48,12 +48,11 @@ struct ValidatePair {
     const uint8_t *raw_flags;
     float *dx, *dy, *ppr;
     uint8_t *flags;
+    uint8_t *snapshot;        // flags after the median test ([nx*ny] scratch)
     uint32_t *summary;        // 5 counters: no_signal, low_ppr, median_outliers, out_of_range, replaced
     int32_t nx, ny;
     float ppr_threshold, median_threshold, median_eps, max_displacement;
     int32_t replace;
-    uint32_t tile_begin;      // first tile index of this pair in the launch
-    int32_t tiles_x;          // tiles across
     int32_t pad_;
 };
 

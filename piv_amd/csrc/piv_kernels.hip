@@ -43,63 +43,94 @@ hipError_t launch_correlate_96(int fmt, const PassPair *pairs, int npairs, uint3
                               hipStream_t st);
 
 // ---------------------------------------------------------------------------------------
-// validation (validate.rs:51-171)
+// validation (validate.rs:51-171): two launches per pass, one thread per vector
+//   k_val_median : per-vector tests of the vector and its 3x3 neighbours (validate.rs:59-78),
+//                  then the normalised median test (validate.rs:80-100) -> flags after test
+//   k_val_replace: replacement of rejected vectors by the median of usable neighbours at
+//                  radius 1, else 2 (validate.rs:102-128) -> validated field
+// Medians sort total_cmp keys (f32::total_cmp, validate.rs:164) with a sorting network.
 // ---------------------------------------------------------------------------------------
 
-constexpr int VT = 16;            // output tile
-constexpr int VH = 3;             // halo: median test +-1 feeding replacement +-2
-constexpr int VR = VT + 2 * VH;   // 22
-constexpr uint8_t ABSENT = 0x81;  // outside the grid: NO_SIGNAL-like, never usable
-
-// f32::total_cmp order as a signed key
 __device__ __forceinline__ int32_t total_key(float v)
 {
     const int32_t b = __float_as_int(v);
     return b ^ (int32_t)((uint32_t)(b >> 31) >> 1);
 }
 
-// k-th smallest (0-based) of v[0..cnt) in total_cmp order
-template <int M>
-__device__ __forceinline__ float select_k(const float (&v)[M], int cnt, int kth)
+__device__ __forceinline__ float from_key(int32_t k)
 {
-    float out = 0.0f;
-#pragma unroll
-    for (int i = 0; i < M; ++i) {
-        if (i < cnt) {
-            const int32_t ki = total_key(v[i]);
-            int lt = 0, le = 0;
-#pragma unroll
-            for (int j = 0; j < M; ++j) {
-                if (j < cnt) {
-                    const int32_t kj = total_key(v[j]);
-                    lt += kj < ki;
-                    le += kj <= ki;
-                }
-            }
-            if (lt <= kth && kth < le) out = v[i];
-        }
-    }
-    return out;
+    return __int_as_float(k ^ (int32_t)((uint32_t)(k >> 31) >> 1));
 }
 
-// validate.rs:163-171 median (sort by total_cmp; even count -> 0.5 (a + b))
-template <int M>
-__device__ __forceinline__ float median_of(const float (&v)[M], int cnt)
+#define CSWAP(a, b)                   \
+    do {                              \
+        const int32_t lo_ = min(a, b); \
+        b = max(a, b);                \
+        a = lo_;                      \
+    } while (0)
+
+// Batcher odd-even merge network, 19 comparators; empty slots hold INT_MAX (sorted last)
+__device__ __forceinline__ void sort8(int32_t (&k)[8])
 {
-    if (cnt & 1) return select_k<M>(v, cnt, cnt / 2);
-    const float a = select_k<M>(v, cnt, cnt / 2 - 1), b = select_k<M>(v, cnt, cnt / 2);
-    return 0.5f * (a + b);
+    CSWAP(k[0], k[1]); CSWAP(k[2], k[3]); CSWAP(k[4], k[5]); CSWAP(k[6], k[7]);
+    CSWAP(k[0], k[2]); CSWAP(k[1], k[3]); CSWAP(k[4], k[6]); CSWAP(k[5], k[7]);
+    CSWAP(k[1], k[2]); CSWAP(k[5], k[6]);
+    CSWAP(k[0], k[4]); CSWAP(k[1], k[5]); CSWAP(k[2], k[6]); CSWAP(k[3], k[7]);
+    CSWAP(k[2], k[4]); CSWAP(k[3], k[5]);
+    CSWAP(k[1], k[2]); CSWAP(k[3], k[4]); CSWAP(k[5], k[6]);
+}
+
+// validate.rs:163-171 median of the cnt smallest (valid) sorted keys
+template <int M>
+__device__ __forceinline__ float median_sorted(const int32_t (&k)[M], int cnt)
+{
+    const int ilo = (cnt - 1) >> 1, ihi = cnt >> 1;
+    int32_t lo = k[0], hi = k[0];
+#pragma unroll
+    for (int i = 1; i < M; ++i) {
+        lo = i == ilo ? k[i] : lo;
+        hi = i == ihi ? k[i] : hi;
+    }
+    if (cnt & 1) return from_key(hi);
+    return 0.5f * (from_key(lo) + from_key(hi));
+}
+
+__device__ __forceinline__ float median8(const float (&v)[8], int cnt)
+{
+    int32_t k[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) k[i] = i < cnt ? total_key(v[i]) : INT32_MAX;
+    sort8(k);
+    return median_sorted<8>(k, cnt);
 }
 
 // validate.rs:156-161 residual
-template <int M>
-__device__ __forceinline__ float residual(float value, const float (&nb)[M], int cnt, float eps)
+__device__ __forceinline__ float residual8(float value, const float (&nb)[8], int cnt, float eps)
 {
-    const float med = median_of<M>(nb, cnt);
-    float dev[M];
+    const float med = median8(nb, cnt);
+    float dev[8];
 #pragma unroll
-    for (int i = 0; i < M; ++i) dev[i] = fabsf(nb[i] - med);
-    return fabsf(value - med) / (median_of<M>(dev, cnt) + eps);
+    for (int i = 0; i < 8; ++i) dev[i] = fabsf(nb[i] - med);
+    return fabsf(value - med) / (median8(dev, cnt) + eps);
+}
+
+// median of up to 24 values (radius-2 replacement, only when no radius-1 neighbour is
+// usable): insertion into a sorted key list
+__device__ float median24(const float *v, int cnt)
+{
+    int32_t k[24];
+    for (int i = 0; i < cnt; ++i) {
+        int32_t x = total_key(v[i]);
+        int j = i;
+        while (j > 0 && k[j - 1] > x) {
+            k[j] = k[j - 1];
+            --j;
+        }
+        k[j] = x;
+    }
+    const int ilo = (cnt - 1) >> 1, ihi = cnt >> 1;
+    if (cnt & 1) return from_key(k[ihi]);
+    return 0.5f * (from_key(k[ilo]) + from_key(k[ihi]));
 }
 
 // glibc hypotf: sqrt of the exact double sum of squares, rounded to float
@@ -110,59 +141,40 @@ __device__ __forceinline__ float hypot_rs(float u, float v)
     return (float)sqrt(du * du + dv * dv);
 }
 
-__global__ __launch_bounds__(256) void k_validate(const ValidatePair *__restrict__ jobs,
-                                                  const uint32_t *__restrict__ tile_begins,
-                                                  int njobs, uint32_t *__restrict__ tile_counts)
+// validate.rs:59-78 per-vector tests
+__device__ __forceinline__ uint8_t vector_flags(const ValidatePair &J, long i)
 {
-    __shared__ float s_dx[VR * VR], s_dy[VR * VR];
-    __shared__ uint8_t s_f1[VR * VR], s_f2[VR * VR];
-    __shared__ uint32_t s_cnt[5];
+    uint8_t f = J.raw_flags[i];
+    if (f & NO_SIGNAL) return f;
+    if (hypot_rs(J.raw_dx[i], J.raw_dy[i]) > J.max_displacement) f |= OUT_OF_RANGE;
+    if (J.raw_ppr[i] < J.ppr_threshold) f |= LOW_PPR;
+    return f;
+}
 
-    const uint32_t tile = blockIdx.x;
-    const uint32_t jp = find_pair_u(tile_begins, njobs, tile);
-    const ValidatePair &J = jobs[jp];
-    const uint32_t lt = tile - J.tile_begin;
-    const int tx0 = (int)(lt % (uint32_t)J.tiles_x) * VT, ty0 = (int)(lt / (uint32_t)J.tiles_x) * VT;
+// add per-wave counts to a pair's counters (one atomic per wave and counter)
+__device__ __forceinline__ void count_add(uint32_t *ctr, bool pred)
+{
+    const unsigned long long m = __ballot(pred);
+    if (m && (threadIdx.x & 63) == (uint32_t)__ffsll((long long)m) - 1)
+        atomicAdd(ctr, (uint32_t)__popcll(m));
+}
+
+constexpr int VBLOCK = 256;
+
+__global__ __launch_bounds__(VBLOCK) void k_val_median(const ValidatePair *__restrict__ jobs,
+                                                       uint32_t bpp)
+{
+    const uint32_t p = blockIdx.x / bpp, blk = blockIdx.x - p * bpp;
+    const ValidatePair &J = jobs[p];
     const int nx = J.nx, ny = J.ny;
-    if (threadIdx.x < 5) s_cnt[threadIdx.x] = 0;
-    __syncthreads();
-
-    // per-vector tests (validate.rs:59-78) over tile + halo
-    uint32_t c_ns = 0, c_oor = 0, c_low = 0, c_med = 0, c_rep = 0;
-    for (int i = threadIdx.x; i < VR * VR; i += blockDim.x) {
-        const int gx = tx0 - VH + i % VR, gy = ty0 - VH + i / VR;
-        uint8_t f = ABSENT;
-        float u = NAN, v = NAN;
-        if (gx >= 0 && gy >= 0 && gx < nx && gy < ny) {
-            const long gi = (long)gy * nx + gx;
-            f = J.raw_flags[gi];
-            u = J.raw_dx[gi];
-            v = J.raw_dy[gi];
-            const bool own = gx >= tx0 && gy >= ty0 && gx < tx0 + VT && gy < ty0 + VT;
-            if (f & NO_SIGNAL) {
-                c_ns += own;
-            } else {
-                if (hypot_rs(u, v) > J.max_displacement) {
-                    f |= OUT_OF_RANGE;
-                    c_oor += own;
-                }
-                if (J.raw_ppr[gi] < J.ppr_threshold) {
-                    f |= LOW_PPR;
-                    c_low += own;
-                }
-            }
-        }
-        s_dx[i] = u;
-        s_dy[i] = v;
-        s_f1[i] = f;
-    }
-    __syncthreads();
-
-    // normalised median test (validate.rs:80-100) on tile + 2-cell halo
-    for (int i = threadIdx.x; i < VR * VR; i += blockDim.x) {
-        const int lx = i % VR, ly = i / VR;
-        uint8_t f = s_f1[i];
-        if (lx >= 1 && ly >= 1 && lx < VR - 1 && ly < VR - 1 && (f & INVALID) == 0) {
+    const int c = (int)(blk * VBLOCK + threadIdx.x);
+    const bool in = c < nx * ny;
+    uint8_t f = 0;
+    bool med_out = false;
+    if (in) {
+        const int ix = c % nx, iy = c / nx;
+        f = vector_flags(J, c);
+        if ((f & INVALID) == 0) {
             float nu[8], nv[8];
             int cnt = 0;
 #pragma unroll
@@ -170,100 +182,121 @@ __global__ __launch_bounds__(256) void k_validate(const ValidatePair *__restrict
 #pragma unroll
                 for (int ox = -1; ox <= 1; ++ox) {
                     if (oy == 0 && ox == 0) continue;
-                    const int j = i + oy * VR + ox;
-                    if ((s_f1[j] & INVALID) == 0) {
-#pragma unroll
-                        for (int q = 0; q < 8; ++q)
-                            if (q == cnt) {
-                                nu[q] = s_dx[j];
-                                nv[q] = s_dy[j];
-                            }
-                        ++cnt;
+                    const int x = ix + ox, y = iy + oy;
+                    bool use = false;
+                    float u = 0.0f, v = 0.0f;
+                    if (x >= 0 && y >= 0 && x < nx && y < ny) {
+                        const long j = (long)y * nx + x;
+                        use = (vector_flags(J, j) & INVALID) == 0;
+                        u = J.raw_dx[j];
+                        v = J.raw_dy[j];
                     }
+                    // compact: slot cnt takes this neighbour
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                        if (use && q == cnt) {
+                            nu[q] = u;
+                            nv[q] = v;
+                        }
+                    cnt += use ? 1 : 0;
                 }
             if (cnt >= 2) {
-                const float ru = residual<8>(s_dx[i], nu, cnt, J.median_eps);
-                const float rv = residual<8>(s_dy[i], nv, cnt, J.median_eps);
+                const float ru = residual8(J.raw_dx[c], nu, cnt, J.median_eps);
+                const float rv = residual8(J.raw_dy[c], nv, cnt, J.median_eps);
                 if (sqrtf(ru * ru + rv * rv) > J.median_threshold) {
                     f |= MEDIAN_OUTLIER;
-                    const int gx = tx0 - VH + lx, gy = ty0 - VH + ly;
-                    c_med += (gx >= tx0 && gy >= ty0 && gx < tx0 + VT && gy < ty0 + VT);
+                    med_out = true;
                 }
             }
         }
-        s_f2[i] = f;
+        J.snapshot[c] = f;  // flags after the median test (snapshot for replacement)
     }
-    __syncthreads();
+    const bool ns = in && (f & NO_SIGNAL);
+    count_add(J.summary + 0, ns);
+    count_add(J.summary + 1, in && !ns && (f & LOW_PPR));
+    count_add(J.summary + 2, med_out);
+    count_add(J.summary + 3, in && !ns && (f & OUT_OF_RANGE));
+}
 
-    // replacement (validate.rs:102-128) and output for the tile
-    {
-        const int lx = VH + threadIdx.x % VT, ly = VH + threadIdx.x / VT;
-        const int gx = tx0 + threadIdx.x % VT, gy = ty0 + threadIdx.x / VT;
-        if (gx < nx && gy < ny) {
-            const int i = ly * VR + lx;
-            uint8_t f = s_f2[i];
-            float u = s_dx[i], v = s_dy[i];
-            if (J.replace && (f & NO_SIGNAL) == 0 && (f & INVALID) != 0) {
-                bool done = false;
+__global__ __launch_bounds__(VBLOCK) void k_val_replace(const ValidatePair *__restrict__ jobs,
+                                                        uint32_t bpp)
+{
+    const uint32_t p = blockIdx.x / bpp, blk = blockIdx.x - p * bpp;
+    const ValidatePair &J = jobs[p];
+    const int nx = J.nx, ny = J.ny;
+    const int c = (int)(blk * VBLOCK + threadIdx.x);
+    const bool in = c < nx * ny;
+    bool rep = false;
+    if (in) {
+        const int ix = c % nx, iy = c / nx;
+        const uint8_t f = J.snapshot[c];
+        float u = J.raw_dx[c], v = J.raw_dy[c];
+        uint8_t fo = f;
+        if (J.replace && (f & NO_SIGNAL) == 0 && (f & INVALID) != 0) {
+            float nu[8], nv[8];
+            int cnt = 0;
 #pragma unroll
-                for (int radius = 1; radius <= 2 && !done; ++radius) {
-                    float nu[24], nv[24];
-                    int cnt = 0;
-                    for (int oy = -radius; oy <= radius; ++oy)
-                        for (int ox = -radius; ox <= radius; ++ox) {
-                            if (oy == 0 && ox == 0) continue;
-                            const int j = i + oy * VR + ox;
-                            if ((s_f2[j] & INVALID) == 0) {
+            for (int oy = -1; oy <= 1; ++oy)
 #pragma unroll
-                                for (int q = 0; q < 24; ++q)
-                                    if (q == cnt) {
-                                        nu[q] = s_dx[j];
-                                        nv[q] = s_dy[j];
-                                    }
-                                ++cnt;
-                            }
-                        }
-                    if (cnt > 0) {
-                        u = median_of<24>(nu, cnt);
-                        v = median_of<24>(nv, cnt);
-                        f |= REPLACED;
-                        ++c_rep;
-                        done = true;
+                for (int ox = -1; ox <= 1; ++ox) {
+                    if (oy == 0 && ox == 0) continue;
+                    const int x = ix + ox, y = iy + oy;
+                    bool use = false;
+                    float a = 0.0f, b = 0.0f;
+                    if (x >= 0 && y >= 0 && x < nx && y < ny) {
+                        const long j = (long)y * nx + x;
+                        use = (J.snapshot[j] & INVALID) == 0;
+                        a = J.raw_dx[j];
+                        b = J.raw_dy[j];
                     }
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                        if (use && q == cnt) {
+                            nu[q] = a;
+                            nv[q] = b;
+                        }
+                    cnt += use ? 1 : 0;
                 }
-                if (!done) {
+            if (cnt > 0) {
+                u = median8(nu, cnt);
+                v = median8(nv, cnt);
+                rep = true;
+            } else {
+                float wu[24], wv[24];
+                int n2 = 0;
+                for (int y = max(iy - 2, 0); y <= min(iy + 2, ny - 1); ++y)
+                    for (int x = max(ix - 2, 0); x <= min(ix + 2, nx - 1); ++x) {
+                        if (x == ix && y == iy) continue;
+                        const long j = (long)y * nx + x;
+                        if ((J.snapshot[j] & INVALID) == 0) {
+                            wu[n2] = J.raw_dx[j];
+                            wv[n2] = J.raw_dy[j];
+                            ++n2;
+                        }
+                    }
+                if (n2 > 0) {
+                    u = median24(wu, n2);
+                    v = median24(wv, n2);
+                    rep = true;
+                } else {
                     u = NAN;
                     v = NAN;
                 }
             }
-            const long gi = (long)gy * nx + gx;
-            J.dx[gi] = u;
-            J.dy[gi] = v;
-            J.ppr[gi] = J.raw_ppr[gi];
-            J.flags[gi] = f;
+            if (rep) fo = f | REPLACED;
         }
+        J.dx[c] = u;
+        J.dy[c] = v;
+        J.ppr[c] = J.raw_ppr[c];
+        J.flags[c] = fo;
     }
-    atomicAdd(&s_cnt[0], c_ns);
-    atomicAdd(&s_cnt[1], c_low);
-    atomicAdd(&s_cnt[2], c_med);
-    atomicAdd(&s_cnt[3], c_oor);
-    atomicAdd(&s_cnt[4], c_rep);
-    __syncthreads();
-    if (threadIdx.x < 5) tile_counts[5 * (size_t)tile + threadIdx.x] = s_cnt[threadIdx.x];
+    count_add(J.summary + 4, rep);
 }
 
-// per-pair sum of tile counters (deterministic, no global atomics)
-__global__ void k_summarize(const ValidatePair *__restrict__ jobs, int njobs,
-                            const uint32_t *__restrict__ tile_counts, uint32_t ntiles_total)
+__global__ void k_zero_u32(uint32_t *__restrict__ p, uint32_t n)
 {
-    const int jp = blockIdx.x * blockDim.x + threadIdx.x;
-    if (jp >= njobs) return;
-    const ValidatePair &J = jobs[jp];
-    const uint32_t end = jp + 1 < njobs ? jobs[jp + 1].tile_begin : ntiles_total;
-    uint32_t acc[5] = {0, 0, 0, 0, 0};
-    for (uint32_t t = J.tile_begin; t < end; ++t)
-        for (int c = 0; c < 5; ++c) acc[c] += tile_counts[5 * (size_t)t + c];
-    for (int c = 0; c < 5; ++c) J.summary[c] = acc[c];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -434,16 +467,23 @@ hipError_t launch_correlate(int n, int fmt, const PassPair *pairs, int npairs,
     }
 }
 
-hipError_t launch_validate(const ValidatePair *jobs, const uint32_t *tile_begins, int njobs,
-                           uint32_t ntiles, uint32_t *tile_counts, hipStream_t st)
+hipError_t launch_validate(const ValidatePair *jobs, int njobs, uint32_t max_cells,
+                           hipStream_t st)
 {
-    if (ntiles == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_validate, dim3(ntiles), dim3(256), 0, st, jobs, tile_begins, njobs,
-                       tile_counts);
+    const uint32_t bpp = (max_cells + VBLOCK - 1) / VBLOCK;
+    const uint32_t blocks = bpp * (uint32_t)njobs;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_val_median, dim3(blocks), dim3(VBLOCK), 0, st, jobs, bpp);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_summarize, dim3((njobs + 63) / 64), dim3(64), 0, st, jobs, njobs,
-                       tile_counts, ntiles);
+    hipLaunchKernelGGL(k_val_replace, dim3(blocks), dim3(VBLOCK), 0, st, jobs, bpp);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
     return hipGetLastError();
 }
 

@@ -10,8 +10,10 @@ namespace piv {
 bool correlate_supported(int n);
 hipError_t launch_correlate(int n, int fmt, const PassPair *pairs, int npairs,
                             uint32_t max_cells, hipStream_t st);
-hipError_t launch_validate(const ValidatePair *jobs, const uint32_t *tile_begins, int njobs,
-                           uint32_t ntiles, uint32_t *tile_counts, hipStream_t st);
+// both validation launches of one pass; summaries must be zeroed beforehand
+hipError_t launch_validate(const ValidatePair *jobs, int njobs, uint32_t max_cells,
+                           hipStream_t st);
+hipError_t launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t st);
 hipError_t launch_base(int fmt, const void *raw, int pitch, const float *bg, int H, int W,
                        float *out, hipStream_t st);
 hipError_t launch_sat(const float *img, int H, int W, double *sum, double *sq, hipStream_t st);
@@ -23,6 +25,5 @@ hipError_t launch_normalize(int fmt, const void *raw, long count, float *out, hi
 hipError_t launch_place(const GridDev &g, const GridDev &prev, int has_prev, const float *pdx,
                         const float *pdy, int H, int W, float *origins, hipStream_t st);
 
-constexpr int VALIDATE_TILE = 16;
 
 }  // namespace piv

@@ -149,3 +149,51 @@ def test_full_analysis_matches_rust(piv, name):
     assert np.array_equal(bits(r2.field.dx), bits(r.field.dx))
     assert np.array_equal(r2.field.flags, r.field.flags)
     assert np.array_equal(bits(traces[0].origins), bits(c.passes[0].origins))
+
+
+def _random_field(rng, ny, nx):
+    """A noisy uniform field with every validation path present: NO_SIGNAL holes, low peak
+    ratios, out-of-range and outlier vectors, blocks of invalid vectors (radius-2
+    replacement, and cells with no usable neighbour at all), and +-0 ties."""
+    dx = (1.5 + 0.05 * rng.standard_normal((ny, nx))).astype(np.float32)
+    dy = (-0.7 + 0.05 * rng.standard_normal((ny, nx))).astype(np.float32)
+    ppr = (1.0 + 4.0 * rng.random((ny, nx))).astype(np.float32)
+    fl = np.zeros((ny, nx), np.uint8)
+    out = rng.random((ny, nx)) < 0.04
+    dx[out] += rng.choice([-6.0, 6.0, 40.0], out.sum()).astype(np.float32)
+    hole = rng.random((ny, nx)) < 0.05
+    dx[hole] = dy[hole] = np.nan
+    ppr[hole] = 0.0
+    fl[hole] = 1
+    # a 3x3 block of low-PPR vectors: the centre has no usable radius-1 neighbour
+    if ny > 8 and nx > 8:
+        ppr[3:6, 4:7] = 1.01
+        # a 5x5 block of holes: its centre has no usable neighbour within radius 2
+        dx[ny - 7:ny - 2, nx - 7:nx - 2] = np.nan
+        dy[ny - 7:ny - 2, nx - 7:nx - 2] = np.nan
+        fl[ny - 7:ny - 2, nx - 7:nx - 2] = 1
+        ppr[ny - 6:ny - 3, nx - 6:nx - 3] = 1.0
+        fl[ny - 5, nx - 5] = 0
+        dx[ny - 5, nx - 5], dy[ny - 5, nx - 5] = 9.0, 9.0
+    z = rng.random((ny, nx)) < 0.02
+    dx[z] = np.where(rng.random(z.sum()) < 0.5, np.float32(0.0), np.float32(-0.0))
+    return dx, dy, ppr, fl
+
+
+@pytest.mark.parametrize("seed,shape,replace", [(1, (31, 31), True), (2, (127, 127), True),
+                                                (3, (17, 40), False), (4, (2, 3), True),
+                                                (5, (1, 9), True), (6, (64, 65), True)])
+def test_validation_random_fields_bit_exact(piv, oracle, seed, shape, replace):
+    rng = np.random.default_rng(seed)
+    dx, dy, ppr, fl = _random_field(rng, *shape)
+    cfg = piv.PivConfig(windows=[32], final_iterations=1,
+                        validation=piv.ValidationConfig(replace=replace))
+    an = piv.PivAnalyzer(cfg, piv.PreprocessConfig.raw())
+    maxd = np.float32(16.0)
+    gdx, gdy, gfl, gs = piv.validate(an, dx, dy, ppr, fl, maxd)
+    f = oracle.Field(None, dx, dy, ppr, fl)
+    ref, rs = oracle.validate(f, 1.3, 2.0, 0.1, replace, maxd)
+    assert np.array_equal(gfl, ref.flags)
+    assert np.array_equal(bits(gdx), bits(ref.dx))
+    assert np.array_equal(bits(gdy), bits(ref.dy))
+    assert gs.__dict__ == rs
