@@ -89,6 +89,9 @@ def main():
     ap.add_argument("--latency-requests", type=int, default=50)
     ap.add_argument("--only-batched", action="store_true",
                     help="time the batched steps only (for profiler runs)")
+    ap.add_argument("--copy-calib-mib", type=int, default=0,
+                    help="run one device-to-device copy of this many MiB first (a kernel of "
+                         "known traffic for scaling profiler byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -122,6 +125,15 @@ def main():
     dy = torch.empty_like(dx)
     pr = torch.empty_like(dx)
     fl = torch.empty((P, g.ny, g.nx), dtype=torch.uint8, device=dev)
+
+    if args.copy_calib_mib:
+        n = args.copy_calib_mib << 20
+        src = torch.ones(n // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     for _ in range(args.warmup):
         an.analyze_batch_device(fa, fb, dx, dy, pr, fl, summaries=False, sync=True)

@@ -82,16 +82,15 @@ __device__ __forceinline__ void load_column_raw(const void *__restrict__ img, in
     const bool ints = tx < 1e-4f && ty < 1e-4f;
     const uint32_t xa = (uint32_t)min(x0 + c, W - 1), xb = (uint32_t)min(x0 + c + 1, W - 1);
     const uint32_t off0 = (uint32_t)y0 * (uint32_t)pitch, last = (uint32_t)(H - 1) * (uint32_t)pitch;
+    // Branch-free: both samples are always loaded so the row loads issue back to back; the
+    // integer path selects the unweighted sample.
     float hprev = 0.0f;
 #pragma unroll
     for (int r = 0; r <= N; ++r) {
         const uint32_t row = min(off0 + (uint32_t)r * (uint32_t)pitch, last);
         const float p0 = raw_sample<FMT>(img, row + xa);
-        float h = p0;
-        if (!ints) {
-            const float p1 = raw_sample<FMT>(img, row + xb);
-            h = __builtin_fmaf(tx, p1 - p0, p0);
-        }
+        const float p1 = raw_sample<FMT>(img, row + xb);
+        const float h = ints ? p0 : __builtin_fmaf(tx, p1 - p0, p0);
         if (r > 0) col[r - 1] = ints ? hprev : __builtin_fmaf(ty, h - hprev, hprev);
         hprev = h;
         // bound the loads in flight (and their registers) to 16 rows at a time
@@ -319,13 +318,15 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
         // V's imaginary parts in zi, then the column-pair lanes gather them in two passes.
         constexpr int HW = H2 + 1;  // odd row stride of the half-width plane
         const int s = lane;
-        const bool own = s <= H2, mirror = s >= 1 && s < H2;
+        // rows a lane does not own are written to a dummy row N (branch-free stores)
+        const int row_own = s <= H2 ? s : N;
+        const int row_mir = (s >= 1 && s < H2) ? N - s : N;
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < H2; ++j) {
             const float yr0 = zr[2 * j], yi0 = zi[2 * j], yr1 = zr[2 * j + 1], yi1 = zi[2 * j + 1];
-            if (own) pr[s * HW + j] = yr0 - yi1;
-            if (mirror) pr[(N - s) * HW + j] = yr0 + yi1;
+            pr[row_own * HW + j] = yr0 - yi1;
+            pr[row_mir * HW + j] = yr0 + yi1;
             zi[2 * j] = yi0 + yr1;
             zi[2 * j + 1] = yr1 - yi0;
         }
@@ -335,8 +336,8 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < H2; ++j) {
-            if (own) pr[s * HW + j] = zi[2 * j];
-            if (mirror) pr[(N - s) * HW + j] = zi[2 * j + 1];
+            pr[row_own * HW + j] = zi[2 * j];
+            pr[row_mir * HW + j] = zi[2 * j + 1];
         }
         __syncthreads();
 #pragma unroll
@@ -432,18 +433,48 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
         if (!(best > -INFINITY)) bidx = 0;  // no maximum at all: (0, 0) as the reference
     }
     const int pby = bidx / N, pbx = bidx - pby * N;
-    // (2) footprint: half width at half maximum along +-x, +-y
+    // (2) footprint: half width at half maximum along +-x, +-y (k < N/2 while the value at
+    //     distance k is > best / 2).  Lane l of the window reads the row of the peak at
+    //     distance l to the right and the column at distance l downwards; distances to the
+    //     left/up are lanes N - l.  The walk lengths are runs of set bits in ballots.
     int hw = 0;
     {
         const float half_best = 0.5f * best;
-        const int sy[4] = {1, N - 1, 0, 0};
-        const int sx[4] = {0, 0, 1, N - 1};
+        const float hv = pr[pby * P + (pbx + lane) % N];
+        const float vv = pr[((pby + lane) % N) * P + pbx];
+        if constexpr (C::IN_WAVE) {
+            const int base = tid - lane;
+            const unsigned long long full = N == 64 ? ~0ull : ((1ull << N) - 1);
+            const unsigned long long hm = (__ballot(hv > half_best) >> base) & full;
+            const unsigned long long vm = (__ballot(vv > half_best) >> base) & full;
+            // forward: lanes 0, 1, 2, ...; backward: lanes 0, N-1, N-2, ...
+            auto run_fwd = [&](unsigned long long m) {
+                const unsigned long long inv = ~m & full;
+                return inv ? __builtin_ctzll(inv) : N;
+            };
+            auto run_bwd = [&](unsigned long long m) {
+                if (!(m & 1ull)) return 0;
+                // bits N-1, N-2, ... : count leading ones of the N-bit field below bit 0
+                const unsigned long long rest = (m >> 1) << (64 - (N - 1)) ;
+                const unsigned long long inv = ~rest;
+                const int lead = N - 1 > 0 ? (inv ? __builtin_clzll(inv) : 64) : 0;
+                return 1 + (lead < N - 1 ? lead : N - 1);
+            };
+            const int k0 = min(run_fwd(vm), H2), k1 = min(run_bwd(vm), H2);
+            const int k2 = min(run_fwd(hm), H2), k3 = min(run_bwd(hm), H2);
+            hw = max(max(k0, k1), max(k2, k3));
+        } else {
+            const int sy[4] = {1, N - 1, 0, 0};
+            const int sx[4] = {0, 0, 1, N - 1};
+            (void)hv;
+            (void)vv;
 #pragma unroll
-        for (int d = 0; d < 4; ++d) {
-            int kk = 0;
-            while (kk < H2 && pr[((pby + kk * sy[d]) % N) * P + (pbx + kk * sx[d]) % N] > half_best)
-                ++kk;
-            hw = kk > hw ? kk : hw;
+            for (int d = 0; d < 4; ++d) {
+                int kk = 0;
+                while (kk < H2 && pr[((pby + kk * sy[d]) % N) * P + (pbx + kk * sx[d]) % N] > half_best)
+                    ++kk;
+                hw = kk > hw ? kk : hw;
+            }
         }
     }
     int exclude = (int)ceilf(1.5f * (float)hw);
