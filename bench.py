@@ -144,6 +144,7 @@ def main():
     t0 = time.perf_counter()
     for _ in range(args.steps):
         an.analyze_batch_device(fa, fb, dx, dy, pr, fl, summaries=False, sync=False)
+    an.synchronize()  # the analyzer's stream (same HIP runtime as torch: torch is loaded first)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     stats = an.profile_read()

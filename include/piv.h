@@ -138,6 +138,21 @@ void piv_frame_release(piv_frame *f);
  * host float [h*w]; either pointer may be NULL. */
 piv_status piv_frame_download(const piv_frame *f, float *image, float *base);
 
+/* BackgroundEstimator (preprocess.rs:31-72): per-pixel min or mean over sample frames, on
+ * the analyzer's device.  add() takes HOST samples (on_device = 0) or DEVICE samples
+ * (on_device = 1), normalised as decode.rs:227-234; finish() downloads the estimate (host
+ * float [h*w]) and keeps it on the device, where piv_background_device() returns it for
+ * piv_frame_prepare_device. */
+typedef enum { PIV_BACKGROUND_MIN = 0, PIV_BACKGROUND_MEAN = 1 } piv_background_mode;
+typedef struct piv_background piv_background;
+piv_status piv_background_create(piv_analyzer *a, piv_background_mode mode, uint32_t height,
+                                 uint32_t width, piv_background **out);
+piv_status piv_background_add(piv_background *bg, const void *pixels, piv_pixel_format fmt,
+                              size_t row_stride, int on_device);
+piv_status piv_background_finish(piv_background *bg, float *out);
+piv_status piv_background_device(const piv_background *bg, const float **device_ptr);
+void piv_background_destroy(piv_background *bg);
+
 /* PivAnalyzer::analyze (multipass.rs:136-138) */
 piv_status piv_analyze_pair(piv_analyzer *a, const piv_frame *fa, const piv_frame *fb,
                             piv_field_out *out);
@@ -177,6 +192,15 @@ piv_status piv_correlate_windows(piv_analyzer *a, const piv_frame *fa, const piv
 piv_status piv_validate_field(piv_analyzer *a, uint32_t ny, uint32_t nx, float *dx, float *dy,
                               const float *ppr, uint8_t *flags, float max_displacement,
                               piv_summary *summary);
+
+/* ---- device memory plumbing (no counterpart in the reference) --------------------------
+ * For callers without their own HIP allocator (tests, the Python binding): plain device
+ * buffers and copies on the given device, and a wait for an analyzer's queued work. */
+piv_status piv_device_alloc(int device, size_t bytes, void **out);
+piv_status piv_device_free(int device, void *ptr);
+piv_status piv_copy_to_device(int device, void *dst, const void *src, size_t bytes);
+piv_status piv_copy_to_host(int device, void *dst, const void *src, size_t bytes);
+piv_status piv_analyzer_synchronize(piv_analyzer *a);
 
 /* ---- measurement (no counterpart in the reference) ------------------------------------ */
 

@@ -29,7 +29,7 @@ __all__ = ["PivConfig", "MaskConfig", "ValidationConfig", "PreprocessConfig", "P
            "PreparedFrame", "Grid", "VectorField", "PairResult", "PassTrace",
            "ValidationSummary", "PivError", "ConfigError", "ShapeMismatch", "ImageTooSmall",
            "DeviceError", "flags", "validate", "place_windows", "correlate_windows",
-           "normalize_pixels"]
+           "normalize_pixels", "BackgroundEstimator", "DeviceArray"]
 
 
 class flags:  # field.rs:6-20
@@ -322,6 +322,10 @@ class PivAnalyzer:
                                       fl.reshape(shp)), ValidationSummary._from_c(out.summary))
         return pair, res
 
+    def synchronize(self):
+        """Wait for all work queued by this analyzer."""
+        _check(self._lib.piv_analyzer_synchronize(self._h))
+
     def profile(self, enable=True):
         """Turn per-kernel HIP-event timing on (clearing counters) or off."""
         _check(self._lib.piv_profile_enable(self._h, 1 if enable else 0))
@@ -386,23 +390,28 @@ class PreparedFrame:
         return f
 
     @staticmethod
-    def from_device(an: PivAnalyzer, tensor, background=None):
-        """Wrap device-resident samples (a torch tensor on the analyzer's GPU, 2-D,
-        uint8 / int16-viewed uint16 / float32).  `fmt` follows the tensor dtype."""
-        import torch
-
-        fmt = {torch.uint8: L.PIXEL_U8, torch.uint16: L.PIXEL_U16,
-               torch.int16: L.PIXEL_U16, torch.float32: L.PIXEL_F32}[tensor.dtype]
-        if tensor.dim() != 2 or tensor.stride(1) != 1:
-            raise InvalidArgument("expected a 2-D row-major device tensor")
+    def from_device(an: PivAnalyzer, arr, background=None):
+        """Wrap device-resident samples: a 2-D DeviceArray, or a row-major device tensor of
+        a torch that shares this process's HIP runtime (u8, u16 or f32)."""
+        if isinstance(arr, DeviceArray):
+            if len(arr.shape) != 2 or arr.dtype not in _FMT:
+                raise InvalidArgument("expected a 2-D uint8/uint16/float32 DeviceArray")
+            fmt, shape, ptr_, stride = _FMT[arr.dtype], arr.shape, arr.device_ptr, arr.shape[1] * arr.dtype.itemsize
+        else:
+            name = str(arr.dtype).replace("torch.", "")
+            fmt = {"uint8": L.PIXEL_U8, "uint16": L.PIXEL_U16, "int16": L.PIXEL_U16,
+                   "float32": L.PIXEL_F32}[name]
+            if arr.dim() != 2 or arr.stride(1) != 1:
+                raise InvalidArgument("expected a 2-D row-major device tensor")
+            shape, ptr_ = tuple(arr.shape), arr.data_ptr()
+            stride = arr.stride(0) * arr.element_size()
         h = C.c_void_p()
         _check(an._lib.piv_frame_prepare_device(
-            an._h, C.c_void_p(tensor.data_ptr()), fmt, tensor.shape[0], tensor.shape[1],
-            tensor.stride(0) * tensor.element_size(),
-            None if background is None else C.c_void_p(background.data_ptr()), C.byref(h)))
+            an._h, C.c_void_p(ptr_), fmt, shape[0], shape[1], stride,
+            None if background is None else L.ptr(background), C.byref(h)))
         f = PreparedFrame()
-        f._h, f._lib, f.shape, f._owner = h, an._lib, tuple(tensor.shape), an
-        f._keep = (tensor, background)
+        f._h, f._lib, f.shape, f._owner = h, an._lib, tuple(shape), an
+        f._keep = (arr, background)
         return f
 
     @property
@@ -430,6 +439,98 @@ class PreparedFrame:
     def close(self):
         if self._h:
             self._lib.piv_frame_release(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceArray:
+    """A plain device buffer with a numpy dtype and shape (no torch needed)."""
+
+    def __init__(self, shape, dtype, device=0):
+        self.shape = tuple(shape)
+        self.dtype = np.dtype(dtype)
+        self.device = device
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        _check(L.load().piv_device_alloc(device, self.nbytes, C.byref(p)))
+        self.device_ptr = p.value
+
+    @staticmethod
+    def from_numpy(a, device=0):
+        a = np.ascontiguousarray(a)
+        d = DeviceArray(a.shape, a.dtype, device)
+        _check(L.load().piv_copy_to_device(device, C.c_void_p(d.device_ptr), L.ptr(a), d.nbytes))
+        return d
+
+    def upload(self, a):
+        """Overwrite the buffer with the bytes of a host array of the same size."""
+        a = np.ascontiguousarray(a, self.dtype)
+        if a.nbytes != self.nbytes:
+            raise ShapeMismatch(f"{a.shape} {a.dtype} into device array {self.shape} {self.dtype}")
+        _check(L.load().piv_copy_to_device(self.device, C.c_void_p(self.device_ptr), L.ptr(a),
+                                           self.nbytes))
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        _check(L.load().piv_copy_to_host(self.device, L.ptr(out), C.c_void_p(self.device_ptr),
+                                         self.nbytes))
+        return out
+
+    def slice(self, i):
+        """Device view of a[i] along the first axis (shares memory)."""
+        v = DeviceArray.__new__(DeviceArray)
+        v.shape, v.dtype, v.device = self.shape[1:], self.dtype, self.device
+        v.nbytes = int(np.prod(v.shape)) * v.dtype.itemsize
+        v.device_ptr = self.device_ptr + i * v.nbytes
+        v._base = self
+        return v
+
+    def close(self):
+        if getattr(self, "_base", None) is None and getattr(self, "device_ptr", None):
+            L.load().piv_device_free(self.device, C.c_void_p(self.device_ptr))
+            self.device_ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BackgroundEstimator:
+    """BackgroundEstimator (preprocess.rs:39-72): per-pixel min or mean over sample frames,
+    accumulated on the analyzer's GPU."""
+
+    MIN, MEAN = 0, 1
+
+    def __init__(self, an: PivAnalyzer, mode, height, width):
+        self._an = an
+        self._lib = an._lib
+        self.shape = (height, width)
+        h = C.c_void_p()
+        m = {"min": 0, "mean": 1}.get(mode, mode)
+        _check(self._lib.piv_background_create(an._h, int(m), height, width, C.byref(h)))
+        self._h = h
+
+    def add(self, frame):
+        f = np.ascontiguousarray(frame)
+        if f.shape != self.shape:
+            raise ShapeMismatch(f"frame {f.shape} vs background {self.shape}")
+        _check(self._lib.piv_background_add(self._h, L.ptr(f), _FMT[f.dtype], f.strides[0], 0))
+
+    def finish(self):
+        out = np.empty(self.shape, np.float32)
+        _check(self._lib.piv_background_finish(self._h, L.ptr(out)))
+        return out
+
+    def close(self):
+        if self._h:
+            self._lib.piv_background_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -89,6 +89,16 @@ _SIGS = {
                                C.c_void_p], C.c_int),
     "piv_validate_field": ([C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
                             C.c_void_p, C.c_void_p, C.c_float, C.POINTER(piv_summary)], C.c_int),
+    "piv_background_create": ([C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)], C.c_int),
+    "piv_background_add": ([C.c_void_p, C.c_void_p, C.c_int, C.c_size_t, C.c_int], C.c_int),
+    "piv_background_finish": ([C.c_void_p, C.c_void_p], C.c_int),
+    "piv_background_device": ([C.c_void_p, C.POINTER(C.c_void_p)], C.c_int),
+    "piv_background_destroy": ([C.c_void_p], None),
+    "piv_device_alloc": ([C.c_int, C.c_size_t, C.POINTER(C.c_void_p)], C.c_int),
+    "piv_device_free": ([C.c_int, C.c_void_p], C.c_int),
+    "piv_copy_to_device": ([C.c_int, C.c_void_p, C.c_void_p, C.c_size_t], C.c_int),
+    "piv_copy_to_host": ([C.c_int, C.c_void_p, C.c_void_p, C.c_size_t], C.c_int),
+    "piv_analyzer_synchronize": ([C.c_void_p], C.c_int),
     "piv_profile_enable": ([C.c_void_p, C.c_int], C.c_int),
     "piv_profile_read": ([C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)], C.c_int),
     "piv_normalize_pixels": ([C.c_int, C.c_void_p, C.c_int, C.c_size_t, C.c_void_p], C.c_int),
@@ -118,6 +128,8 @@ def ptr(a):
         return None
     if isinstance(a, np.ndarray):
         return a.ctypes.data_as(C.c_void_p)
+    if hasattr(a, "device_ptr"):  # piv_amd.DeviceArray
+        return C.c_void_p(a.device_ptr)
     if hasattr(a, "data_ptr"):  # torch tensor on the device
         return C.c_void_p(a.data_ptr())
     return C.c_void_p(int(a))

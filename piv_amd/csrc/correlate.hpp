@@ -36,7 +36,7 @@ struct KCfg {
     static constexpr int P = N + 1;   // odd LDS row stride: conflict-free rows and columns
     static constexpr int PL = N * P;
     // minimum waves per SIMD the register allocator must allow
-    static constexpr int WAVES = N <= 16 ? 4 : (N <= 32 ? 3 : 2);
+    static constexpr int WAVES = N <= 16 ? 4 : (N <= 32 ? 3 : (N <= 64 ? 2 : 1));
 };
 
 // Sum over the N lanes of one window (all lanes receive the total).

@@ -408,6 +408,28 @@ __global__ void k_local_norm(const float *__restrict__ in, int H, int W, int rad
     out[i] = (in[i] - m) / (sd + eps);
 }
 
+// BackgroundEstimator::add (preprocess.rs:56-63): the first frame initialises the estimate,
+// later frames are folded in with f32::min (NaN-ignoring) or added in f32.
+template <int FMT>
+__global__ void k_bg_add(const void *__restrict__ raw, int pitch, int H, int W, int mode,
+                         int first, float *__restrict__ acc)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)H * W) return;
+    const int y = (int)(i / W), x = (int)(i - (long)y * W);
+    const float v = load_pixel<FMT>(raw, (long)y * pitch + x);
+    if (first) acc[i] = v;
+    else if (mode == 0) acc[i] = fminf(acc[i], v);
+    else acc[i] = acc[i] + v;
+}
+
+// BackgroundEstimator::finish (preprocess.rs:65-71): mean = sum / count in f32
+__global__ void k_bg_finish(float *__restrict__ acc, long n, float count)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) acc[i] = acc[i] / count;
+}
+
 // pixel normalisation only (for the exhaustive u8/u16 check and frame downloads)
 template <int FMT>
 __global__ void k_normalize(const void *__restrict__ raw, long count, float *__restrict__ out)
@@ -527,6 +549,25 @@ hipError_t launch_local_norm(const float *in, int H, int W, int radius, float ep
     const long n = (long)H * W;
     hipLaunchKernelGGL(k_local_norm, dim3(nblk(n, 256)), dim3(256), 0, st, in, H, W, radius, eps,
                        sum, sq, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_bg_add(int fmt, const void *raw, int pitch, int H, int W, int mode, int first,
+                         float *acc, hipStream_t st)
+{
+    const long n = (long)H * W;
+    if (fmt == FMT_U8)
+        hipLaunchKernelGGL(k_bg_add<FMT_U8>, dim3(nblk(n, 256)), dim3(256), 0, st, raw, pitch, H, W, mode, first, acc);
+    else if (fmt == FMT_U16)
+        hipLaunchKernelGGL(k_bg_add<FMT_U16>, dim3(nblk(n, 256)), dim3(256), 0, st, raw, pitch, H, W, mode, first, acc);
+    else
+        hipLaunchKernelGGL(k_bg_add<FMT_F32>, dim3(nblk(n, 256)), dim3(256), 0, st, raw, pitch, H, W, mode, first, acc);
+    return hipGetLastError();
+}
+
+hipError_t launch_bg_finish(float *acc, long n, float count, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_bg_finish, dim3(nblk(n, 256)), dim3(256), 0, st, acc, n, count);
     return hipGetLastError();
 }
 

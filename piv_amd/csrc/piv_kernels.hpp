@@ -22,6 +22,9 @@ hipError_t launch_blur(const float *in, int H, int W, const float *taps, int r, 
 hipError_t launch_local_norm(const float *in, int H, int W, int radius, float eps,
                              const double *sum, const double *sq, float *out, hipStream_t st);
 hipError_t launch_normalize(int fmt, const void *raw, long count, float *out, hipStream_t st);
+hipError_t launch_bg_add(int fmt, const void *raw, int pitch, int H, int W, int mode, int first,
+                         float *acc, hipStream_t st);
+hipError_t launch_bg_finish(float *acc, long n, float count, hipStream_t st);
 hipError_t launch_place(const GridDev &g, const GridDev &prev, int has_prev, const float *pdx,
                         const float *pdy, int H, int W, float *origins, hipStream_t st);
 
