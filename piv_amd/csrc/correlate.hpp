@@ -8,8 +8,7 @@
 //     windows (consecutive windows share most of their pixels: one XCD's L2 serves them).
 //
 // Per window (correlate.rs:99-139 restated for the GPU):
-//   1. placement from the previous pass's validated field (multipass.rs:207-231) and the
-//      signal mask from the f64 summed-area tables (multipass.rs:233-241);
+//   1. origins and the signal mask come from the placement kernel (one thread per window);
 //   2. fused ingest: raw u8/u16 samples are read straight from HBM and the window is sampled
 //      bilinearly in raw units (the correlation coefficient is invariant to the decode scale
 //      1/255 or 1/65535, which only rescales the energy threshold);
@@ -19,7 +18,8 @@
 //   5. Hermitian inverse: row IFFT of rows 0..N/2, then column-pair IFFTs (two real
 //      columns per complex transform), scaled into an LDS correlation plane;
 //   6. find_peak (correlate.rs:215-303): first row-major argmax, footprint half width,
-//      highest local maximum outside it, 3-point Gaussian fit, peak ratio.
+//      highest local maximum outside it; the peak record goes to the finishing kernel
+//      (one thread per window), which does the 3-point fit, the peak ratio and assembly.
 #pragma once
 
 #include "fft.hpp"
@@ -98,6 +98,47 @@ __device__ __forceinline__ void load_column_raw(const void *__restrict__ img, in
     }
 }
 
+// Does the (N+1)^2 bilinear footprint of the window at (x, y) lie inside the frame?
+template <int N>
+__device__ __forceinline__ bool interior(float x, float y, int W, int H)
+{
+    return x >= 0.0f && y >= 0.0f && (int)x + N + 1 <= W && (int)y + N + 1 <= H;
+}
+
+// load_column_raw for a window whose footprint lies inside the frame: no clamping, and
+// for integer samples the integer path is taken by zeroing the weights (0 * (p1 - p0) + p0
+// is exactly p0 for finite samples), so each row is one address add, two loads and two
+// lerps.
+template <int N, int FMT>
+__device__ __forceinline__ void load_column_fast(const void *__restrict__ img, int pitch, float x,
+                                                 float y, int c, float (&col)[N])
+{
+    const int x0 = (int)floorf(x), y0 = (int)floorf(y);
+    float tx = x - (float)x0, ty = y - (float)y0;
+    const bool ints = tx < 1e-4f && ty < 1e-4f;
+    const uint32_t o = (uint32_t)y0 * (uint32_t)pitch + (uint32_t)(x0 + c);
+    if constexpr (FMT != FMT_F32) {
+        tx = ints ? 0.0f : tx;
+        ty = ints ? 0.0f : ty;
+    }
+    float hprev = 0.0f;
+#pragma unroll
+    for (int r = 0; r <= N; ++r) {
+        const uint32_t row = o + (uint32_t)r * (uint32_t)pitch;
+        const float p0 = raw_sample<FMT>(img, row);
+        const float p1 = raw_sample<FMT>(img, row + 1);
+        float h;
+        if constexpr (FMT == FMT_F32) h = ints ? p0 : __builtin_fmaf(tx, p1 - p0, p0);
+        else h = __builtin_fmaf(tx, p1 - p0, p0);
+        if (r > 0) {
+            if constexpr (FMT == FMT_F32) col[r - 1] = ints ? hprev : __builtin_fmaf(ty, h - hprev, hprev);
+            else col[r - 1] = __builtin_fmaf(ty, h - hprev, hprev);
+        }
+        hprev = h;
+        if ((r & 15) == 15) __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
 // energy threshold of correlate.rs:110 in raw sample units
 template <int FMT>
 __device__ __forceinline__ double energy_floor()
@@ -140,43 +181,30 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
     int *pki = s_pki + wl * N;
     const int ix = (int)(k % (uint32_t)g.nx), iy = (int)(k / (uint32_t)g.nx);
 
-    // ---- placement (multipass.rs:207-231) -------------------------------------------
+    // ---- origins and signal mask from the placement kernel ------------------------------
     float ax = 0.f, ay = 0.f, bx = 0.f, by = 0.f;
+    bool sig = false;
     if (active) {
-        if (pp.in_origins) {
-            ax = pp.in_origins[4 * k + 0];
-            ay = pp.in_origins[4 * k + 1];
-            bx = pp.in_origins[4 * k + 2];
-            by = pp.in_origins[4 * k + 3];
-        } else {
-            float px = 0.0f, py = 0.0f;
-            if (pp.has_prev)
-                sample_field(pp.prev, pp.prev_dx, pp.prev_dy, center(g.x0, ix, g.step, N),
-                             center(g.y0, iy, g.step, N), px, py);
-            const float max_x = (float)(pp.W - N > 1 ? pp.W - N - 1 : 0);
-            const float max_y = (float)(pp.H - N > 1 ? pp.H - N - 1 : 0);
-            place(g.x0 + ix * g.step, px, max_x, ax, bx);
-            place(g.y0 + iy * g.step, py, max_y, ay, by);
-        }
-    }
-
-    // ---- signal mask (multipass.rs:233-241) -------------------------------------------
-    bool sig = active;
-    if (active && pp.use_mask) {
-        float m, sd;
-        sat_stats(pp.sat_a, pp.sat_a + pp.sat_cells, pp.W, (int)roundf(ax), (int)roundf(ay), N,
-                  N, m, sd);
-        if (!(m >= pp.min_mean && sd >= pp.min_std)) sig = false;
-        sat_stats(pp.sat_b, pp.sat_b + pp.sat_cells, pp.W, (int)roundf(bx), (int)roundf(by), N,
-                  N, m, sd);
-        if (!(m >= pp.min_mean && sd >= pp.min_std)) sig = false;
+        const float4 o = reinterpret_cast<const float4 *>(pp.origins)[k];
+        ax = o.x;
+        ay = o.y;
+        bx = o.z;
+        by = o.w;
+        sig = pp.sig[k] != 0;
     }
 
     // ---- fused ingest + mean removal --------------------------------------------------
     float zr[N], zi[N];
+    // every window of the wave keeps its (N+1)^2 footprint inside the frame: no clamping
+    const bool fast = __all(!sig || (interior<N>(ax, ay, pp.W, pp.H) && interior<N>(bx, by, pp.W, pp.H)));
     if (sig) {
-        load_column_raw<N, FMT>(pp.img_a, pp.pitch, pp.H, pp.W, ax, ay, lane, zr);
-        load_column_raw<N, FMT>(pp.img_b, pp.pitch, pp.H, pp.W, bx, by, lane, zi);
+        if (fast) {
+            load_column_fast<N, FMT>(pp.img_a, pp.pitch, ax, ay, lane, zr);
+            load_column_fast<N, FMT>(pp.img_b, pp.pitch, bx, by, lane, zi);
+        } else {
+            load_column_raw<N, FMT>(pp.img_a, pp.pitch, pp.H, pp.W, ax, ay, lane, zr);
+            load_column_raw<N, FMT>(pp.img_b, pp.pitch, pp.H, pp.W, bx, by, lane, zi);
+        }
     } else {
 #pragma unroll
         for (int r = 0; r < N; ++r) {
@@ -526,50 +554,16 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
         }
     }
     if (lane == 0 && active) {
-        // (4) three-point Gaussian (parabolic fallback) sub-pixel fit
-        float sub[2];
-#pragma unroll
-        for (int axis = 0; axis < 2; ++axis) {
-            const float m = axis == 0 ? pr[pby * P + (pbx + N - 1) % N] : pr[((pby + N - 1) % N) * P + pbx];
-            const float c = best;
-            const float q = axis == 0 ? pr[pby * P + (pbx + 1) % N] : pr[((pby + 1) % N) * P + pbx];
-            float d;
-            if (m > 0.0f && c > 0.0f && q > 0.0f) {
-                const float lm = logf(m), lc = logf(c), lq = logf(q);
-                d = (lm - lq) / (2.0f * (lm - 2.0f * lc + lq));
-            } else {
-                d = (m - q) / (2.0f * (m - 2.0f * c + q));
-            }
-            sub[axis] = isfinite(d) ? clamp_rs(d, -0.5f, 0.5f) : 0.0f;
-        }
-        // (5) wrap to signed displacement, peak ratio
-        const float wx = pbx >= H2 ? (float)pbx - (float)N : (float)pbx;
-        const float wy = pby >= H2 ? (float)pby - (float)N : (float)pby;
-        float ppr = 1000.0f;
-        if (second > 0.0f) {
-            ppr = best / second;
-            ppr = (ppr > 1000.0f || ppr != ppr) ? 1000.0f : ppr;
-        }
-        // assembly (multipass.rs:238-259)
-        if (has_peak) {
-            pp.out_dx[k] = bx - ax + (wx + sub[0]);
-            pp.out_dy[k] = by - ay + (wy + sub[1]);
-            pp.out_ppr[k] = ppr;
-            pp.out_flags[k] = 0;
-            if (pp.out_peak) pp.out_peak[k] = bidx;
-        } else {
-            pp.out_dx[k] = NAN;
-            pp.out_dy[k] = NAN;
-            pp.out_ppr[k] = 0.0f;
-            pp.out_flags[k] = NO_SIGNAL;
-            if (pp.out_peak) pp.out_peak[k] = -1;
-        }
-        if (pp.out_origins) {
-            pp.out_origins[4 * k + 0] = ax;
-            pp.out_origins[4 * k + 1] = ay;
-            pp.out_origins[4 * k + 2] = bx;
-            pp.out_origins[4 * k + 3] = by;
-        }
+        PeakRec r;
+        r.best = best;
+        r.second = second;
+        r.mx = pr[pby * P + (pbx + N - 1) % N];
+        r.qx = pr[pby * P + (pbx + 1) % N];
+        r.my = pr[((pby + N - 1) % N) * P + pbx];
+        r.qy = pr[((pby + 1) % N) * P + pbx];
+        r.index = bidx;
+        r.has_peak = has_peak ? 1 : 0;
+        pp.peaks[k] = r;
     }
 }
 

@@ -449,8 +449,10 @@ piv_status run_batch(piv_analyzer *a, const piv_frame *const *fa, const piv_fram
     const size_t np = plans.size();
     // ---- workspace layout ----
     size_t off = 0;
-    std::vector<size_t> o_raw(np), o_val(np), o_snap(np), o_org(np), o_pk(np);
-    std::vector<uint8_t *> snap(np);
+    std::vector<size_t> o_raw(np), o_val(np), o_snap(np), o_org(np), o_sig(np), o_rec(np), o_pk(np);
+    std::vector<uint8_t *> snap(np), sigs(np);
+    std::vector<PeakRec *> recs(np);
+    std::vector<float *> orgs(np);
     for (size_t i = 0; i < np; ++i) {
         const size_t c = plans[i].cells * npairs;
         o_raw[i] = off;
@@ -459,9 +461,13 @@ piv_status run_batch(piv_analyzer *a, const piv_frame *const *fa, const piv_fram
         off = align_up(off + c * (3 * sizeof(float) + 1), 256);
         o_snap[i] = off;
         off = align_up(off + c, 256);
+        o_org[i] = off;
+        off = align_up(off + c * 4 * sizeof(float), 256);
+        o_sig[i] = off;
+        off = align_up(off + c, 256);
+        o_rec[i] = off;
+        off = align_up(off + c * sizeof(PeakRec), 256);
         if (traced) {
-            o_org[i] = off;
-            off = align_up(off + c * 4 * sizeof(float), 256);
             o_pk[i] = off;
             off = align_up(off + c * sizeof(int32_t), 256);
         }
@@ -485,8 +491,11 @@ piv_status run_batch(piv_analyzer *a, const piv_frame *const *fa, const piv_fram
         raw[i] = carve(o_raw[i]);
         val[i] = carve(o_val[i]);
         snap[i] = reinterpret_cast<uint8_t *>(base + o_snap[i]);
+        orgs[i] = reinterpret_cast<float *>(base + o_org[i]);
+        sigs[i] = reinterpret_cast<uint8_t *>(base + o_sig[i]);
+        recs[i] = reinterpret_cast<PeakRec *>(base + o_rec[i]);
         if (traced) {
-            origins[i] = reinterpret_cast<float *>(base + o_org[i]);
+            origins[i] = orgs[i];
             peaks[i] = reinterpret_cast<int32_t *>(base + o_pk[i]);
         }
     }
@@ -538,7 +547,9 @@ piv_status run_batch(piv_analyzer *a, const piv_frame *const *fa, const piv_fram
             d.out_dy = raw[i].dy + co;
             d.out_ppr = raw[i].ppr + co;
             d.out_flags = raw[i].flags + co;
-            d.out_origins = traced ? origins[i] + 4 * co : nullptr;
+            d.origins = orgs[i] + 4 * co;
+            d.sig = sigs[i] + co;
+            d.peaks = recs[i] + co;
             d.out_peak = traced ? peaks[i] + co : nullptr;
             d.H = H;
             d.W = W;
@@ -547,7 +558,6 @@ piv_status run_batch(piv_analyzer *a, const piv_frame *const *fa, const piv_fram
             d.grid = pl.grid;
             d.min_mean = a->cfg.min_mean;
             d.min_std = a->cfg.min_std;
-            d.win_begin = (uint32_t)(k * pl.cells);
             if (B->pitch != A->pitch)
                 return fail(PIV_ERR_INVALID_ARGUMENT, "frames of a pair must share a row pitch");
             pp[k] = d;
@@ -595,9 +605,18 @@ piv_status run_batch(piv_analyzer *a, const piv_frame *const *fa, const piv_fram
             const PassPlan &pl = plans[i];
             const uint32_t total = (uint32_t)(pl.cells * npairs);
             const double cbytes = (double)total * (2.0 * pl.n * pl.n * 4.0 + 49.0);
+            const PassPair *ppd = reinterpret_cast<PassPair *>(dd + o_pp[i]);
+            // placement reads the predictor field (8 B/vector of the previous grid, bilinear
+            // taps mostly cached) and writes origins + mask (17 B/window)
+            HIP_TRY(timed(a, "place", 0, (double)total * 17.0, total, [&] {
+                return launch_place_pass(ppd, (int)npairs, (uint32_t)pl.cells, st);
+            }));
             HIP_TRY(timed(a, "correlate", pl.n, cbytes, total, [&] {
-                return launch_correlate(pl.n, fa[0]->fmt, reinterpret_cast<PassPair *>(dd + o_pp[i]),
-                                        (int)npairs, (uint32_t)pl.cells, st);
+                return launch_correlate(pl.n, fa[0]->fmt, ppd, (int)npairs, (uint32_t)pl.cells, st);
+            }));
+            // the finishing kernel reads peak records + origins (48 B) and writes 13 B/window
+            HIP_TRY(timed(a, "finish", 0, (double)total * 61.0, total, [&] {
+                return launch_peak_finish(ppd, (int)npairs, (uint32_t)pl.cells, st);
             }));
             // validation reads the raw field (13 B/vector) and writes the validated one (13 B)
             HIP_TRY(timed(a, "validate", 0, (double)total * 26.0, total, [&] {
@@ -1081,13 +1100,14 @@ piv_status piv_correlate_windows(piv_analyzer *a, const piv_frame *fa, const piv
     }
     std::lock_guard<std::mutex> lk(a->mu);
     PIV_TRY(set_device(a->device));
-    const size_t in_bytes = align_up(count * 16, 256) + align_up(sizeof(PassPair), 256) + 256;
-    PIV_TRY(a->stage_in.reserve(in_bytes));
+    const size_t o_pp = align_up(count * 16, 256);
+    const size_t o_sig = o_pp + align_up(sizeof(PassPair), 256);
+    const size_t o_rec = o_sig + align_up(count, 256);
+    PIV_TRY(a->stage_in.reserve(o_rec + count * sizeof(PeakRec)));
     PIV_TRY(a->stage_out.reserve((size_t)count * 17 + 1024));
     char *din = a->stage_in.as<char>();
     float *d_org = reinterpret_cast<float *>(din);
-    auto *d_pp = reinterpret_cast<PassPair *>(din + align_up(count * 16, 256));
-    auto *d_begin = reinterpret_cast<uint32_t *>(din + align_up(count * 16, 256) + align_up(sizeof(PassPair), 256));
+    auto *d_pp = reinterpret_cast<PassPair *>(din + o_pp);
     float *o_dx = a->stage_out.as<float>(), *o_dy = o_dx + count, *o_ppr = o_dy + count;
     int32_t *o_pk = reinterpret_cast<int32_t *>(o_ppr + count);
     uint8_t *o_fl = reinterpret_cast<uint8_t *>(o_pk + count);
@@ -1098,7 +1118,10 @@ piv_status piv_correlate_windows(piv_analyzer *a, const piv_frame *fa, const piv
     d.sat_a = fa->has_sat ? fa->sat.as<double>() : nullptr;
     d.sat_b = fb->has_sat ? fb->sat.as<double>() : nullptr;
     d.sat_cells = fa->cells;
-    d.in_origins = d_org;
+    d.origins = d_org;
+    d.origins_given = 1;
+    d.sig = reinterpret_cast<uint8_t *>(din + o_sig);
+    d.peaks = reinterpret_cast<PeakRec *>(din + o_rec);
     d.out_dx = o_dx;
     d.out_dy = o_dy;
     d.out_ppr = o_ppr;
@@ -1112,11 +1135,9 @@ piv_status piv_correlate_windows(piv_analyzer *a, const piv_frame *fa, const piv
     d.min_mean = a->cfg.min_mean;
     d.min_std = a->cfg.min_std;
     hipStream_t st = a->stream;
-    uint32_t zero = 0;
     HIP_TRY(hipMemcpyAsync(d_org, origins, count * 16, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(d_pp, &d, sizeof d, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(d_begin, &zero, sizeof zero, hipMemcpyHostToDevice, st));
-    HIP_TRY(launch_correlate((int)window, fa->fmt, d_pp, 1, count, st));
+    HIP_TRY(launch_pass((int)window, fa->fmt, d_pp, 1, count, st));
     HIP_TRY(hipMemcpyAsync(dx, o_dx, count * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(dy, o_dy, count * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(ppr, o_ppr, count * 4, hipMemcpyDeviceToHost, st));

@@ -16,9 +16,18 @@ struct GridDev {
 constexpr uint8_t NO_SIGNAL = 1, LOW_PPR = 2, MEDIAN_OUTLIER = 4, REPLACED = 8, OUT_OF_RANGE = 16;
 constexpr uint8_t INVALID = NO_SIGNAL | LOW_PPR | MEDIAN_OUTLIER | OUT_OF_RANGE;
 
+// Correlation peak of one window (correlate.rs:215-303 before the sub-pixel fit): the
+// correlation kernel writes it, the finishing kernel turns it into the raw vector.
+struct PeakRec {
+    float best, second;       // highest value, highest local maximum outside the footprint
+    float mx, qx, my, qy;     // neighbours of the peak at -1 / +1 along x and along y
+    int32_t index;            // integer peak iy*n + ix
+    int32_t has_peak;         // 0: masked or flat window (NO_SIGNAL)
+};
+
 // One image pair's share of one correlation pass.  A launch covers many pairs (the whole
-// batch, or windows of one size from several serving requests); window k of the launch
-// belongs to the pair whose [win_begin, win_begin + nx*ny) range holds k.
+// batch, or windows of one size from several serving requests): block b of a launch with
+// bpp blocks per pair works on pair b / bpp.
 struct PassPair {
     const void *img_a;        // conditioned image (FMT_F32) or raw pixels (fused u8/u16/f32)
     const void *img_b;
@@ -26,10 +35,12 @@ struct PassPair {
     const double *sat_b;      //   sum at [0, cells), sum of squares at [cells, 2 cells); or null
     const float *prev_dx;     // previous pass's validated field (predictor), or null
     const float *prev_dy;
-    const float *in_origins;  // optional [nx*ny*4]: use these origins instead of placing
+    float *origins;           // [nx*ny*4] window origins (ax, ay, bx, by): written by the
+                              //   placement kernel, or given by the caller (origins_given)
+    uint8_t *sig;             // [nx*ny] 1 if the window passes the signal mask
+    PeakRec *peaks;           // [nx*ny] correlation peaks
     float *out_dx, *out_dy, *out_ppr;
     uint8_t *out_flags;
-    float *out_origins;       // optional [nx*ny*4] (traced runs)
     int32_t *out_peak;        // optional [nx*ny] integer peak iy*n+ix, -1 if no peak
     int64_t sat_cells;        // (H+1)*(W+1)
     int32_t H, W, pitch;      // pitch in pixels
@@ -37,9 +48,9 @@ struct PassPair {
     GridDev grid, prev;
     int32_t has_prev;
     int32_t use_mask;
+    int32_t origins_given;
     float min_mean, min_std;
-    uint32_t win_begin;
-    uint32_t pad_;
+    int32_t pad_;
 };
 
 // One image pair's validation job (piv-core/src/validate.rs:51-130).

@@ -462,6 +462,103 @@ __global__ void k_place(GridDev g, GridDev prev, int has_prev, const float *__re
 }
 
 // ---------------------------------------------------------------------------------------
+// Per-window kernels around the correlation kernel, one thread per window; block b of a
+// launch with bpp blocks per pair works on pair b / bpp (pair descriptors stay scalar).
+//   k_place_pass  : window placement (multipass.rs:207-231) and signal mask
+//                   (multipass.rs:233-241) -> origins, sig
+//   k_peak_finish : sub-pixel fit and peak ratio (correlate.rs:262-303) and assembly of
+//                   the raw vector (multipass.rs:238-259)
+// ---------------------------------------------------------------------------------------
+constexpr int WBLOCK = 256;
+
+__global__ __launch_bounds__(WBLOCK) void k_place_pass(const PassPair *__restrict__ pairs,
+                                                       uint32_t bpp)
+{
+    const uint32_t p = blockIdx.x / bpp, blk = blockIdx.x - p * bpp;
+    const PassPair &pp = pairs[p];
+    const GridDev g = pp.grid;
+    const uint32_t k = blk * WBLOCK + threadIdx.x;
+    if (k >= (uint32_t)(g.nx * g.ny)) return;
+    const int n = g.window;
+    float4 o;
+    if (pp.origins_given) {
+        o = reinterpret_cast<const float4 *>(pp.origins)[k];
+    } else {
+        const int ix = (int)(k % (uint32_t)g.nx), iy = (int)(k / (uint32_t)g.nx);
+        float px = 0.0f, py = 0.0f;
+        if (pp.has_prev)
+            sample_field(pp.prev, pp.prev_dx, pp.prev_dy, center(g.x0, ix, g.step, n),
+                         center(g.y0, iy, g.step, n), px, py);
+        const float max_x = (float)(pp.W - n > 1 ? pp.W - n - 1 : 0);
+        const float max_y = (float)(pp.H - n > 1 ? pp.H - n - 1 : 0);
+        place(g.x0 + ix * g.step, px, max_x, o.x, o.z);
+        place(g.y0 + iy * g.step, py, max_y, o.y, o.w);
+        reinterpret_cast<float4 *>(pp.origins)[k] = o;
+    }
+    bool sig = true;
+    if (pp.use_mask) {
+        float m, sd;
+        sat_stats(pp.sat_a, pp.sat_a + pp.sat_cells, pp.W, (int)roundf(o.x), (int)roundf(o.y), n,
+                  n, m, sd);
+        if (!(m >= pp.min_mean && sd >= pp.min_std)) sig = false;
+        sat_stats(pp.sat_b, pp.sat_b + pp.sat_cells, pp.W, (int)roundf(o.z), (int)roundf(o.w), n,
+                  n, m, sd);
+        if (!(m >= pp.min_mean && sd >= pp.min_std)) sig = false;
+    }
+    pp.sig[k] = sig ? 1 : 0;
+}
+
+// correlate.rs:262-276: three-point Gaussian fit, parabolic when a value is not positive,
+// clamped to +-0.5, non-finite -> 0
+__device__ __forceinline__ float subpixel(float m, float c, float q)
+{
+    float d;
+    if (m > 0.0f && c > 0.0f && q > 0.0f) {
+        const float lm = logf(m), lc = logf(c), lq = logf(q);
+        d = (lm - lq) / (2.0f * (lm - 2.0f * lc + lq));
+    } else {
+        d = (m - q) / (2.0f * (m - 2.0f * c + q));
+    }
+    return isfinite(d) ? clamp_rs(d, -0.5f, 0.5f) : 0.0f;
+}
+
+__global__ __launch_bounds__(WBLOCK) void k_peak_finish(const PassPair *__restrict__ pairs,
+                                                        uint32_t bpp)
+{
+    const uint32_t p = blockIdx.x / bpp, blk = blockIdx.x - p * bpp;
+    const PassPair &pp = pairs[p];
+    const GridDev g = pp.grid;
+    const uint32_t k = blk * WBLOCK + threadIdx.x;
+    if (k >= (uint32_t)(g.nx * g.ny)) return;
+    const PeakRec r = pp.peaks[k];
+    if (r.has_peak) {
+        const int n = g.window, h2 = n / 2;
+        const float4 o = reinterpret_cast<const float4 *>(pp.origins)[k];
+        const int pby = r.index / n, pbx = r.index - pby * n;
+        const float sx = subpixel(r.mx, r.best, r.qx), sy = subpixel(r.my, r.best, r.qy);
+        // correlate.rs:278-284: indices at or past n/2 wrap to negative displacements
+        const float wx = pbx >= h2 ? (float)pbx - (float)n : (float)pbx;
+        const float wy = pby >= h2 ? (float)pby - (float)n : (float)pby;
+        float ppr = 1000.0f;  // MAX_PPR (correlate.rs:12)
+        if (r.second > 0.0f) {
+            ppr = r.best / r.second;
+            ppr = (ppr > 1000.0f || ppr != ppr) ? 1000.0f : ppr;
+        }
+        pp.out_dx[k] = o.z - o.x + (wx + sx);
+        pp.out_dy[k] = o.w - o.y + (wy + sy);
+        pp.out_ppr[k] = ppr;
+        pp.out_flags[k] = 0;
+        if (pp.out_peak) pp.out_peak[k] = r.index;
+    } else {
+        pp.out_dx[k] = NAN;
+        pp.out_dy[k] = NAN;
+        pp.out_ppr[k] = 0.0f;
+        pp.out_flags[k] = NO_SIGNAL;
+        if (pp.out_peak) pp.out_peak[k] = -1;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
 // host-side launchers
 // ---------------------------------------------------------------------------------------
 
@@ -500,6 +597,33 @@ hipError_t launch_validate(const ValidatePair *jobs, int njobs, uint32_t max_cel
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_val_replace, dim3(blocks), dim3(VBLOCK), 0, st, jobs, bpp);
     return hipGetLastError();
+}
+
+hipError_t launch_place_pass(const PassPair *pairs, int npairs, uint32_t max_cells, hipStream_t st)
+{
+    const uint32_t bpp = (max_cells + WBLOCK - 1) / WBLOCK;
+    const uint32_t blocks = bpp * (uint32_t)npairs;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_place_pass, dim3(blocks), dim3(WBLOCK), 0, st, pairs, bpp);
+    return hipGetLastError();
+}
+
+hipError_t launch_peak_finish(const PassPair *pairs, int npairs, uint32_t max_cells, hipStream_t st)
+{
+    const uint32_t bpp = (max_cells + WBLOCK - 1) / WBLOCK;
+    const uint32_t blocks = bpp * (uint32_t)npairs;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_peak_finish, dim3(blocks), dim3(WBLOCK), 0, st, pairs, bpp);
+    return hipGetLastError();
+}
+
+hipError_t launch_pass(int n, int fmt, const PassPair *pairs, int npairs, uint32_t max_cells,
+                       hipStream_t st)
+{
+    hipError_t e = launch_place_pass(pairs, npairs, max_cells, st);
+    if (e == hipSuccess) e = launch_correlate(n, fmt, pairs, npairs, max_cells, st);
+    if (e == hipSuccess) e = launch_peak_finish(pairs, npairs, max_cells, st);
+    return e;
 }
 
 hipError_t launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t st)
