@@ -87,6 +87,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--latency-requests", type=int, default=50)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default correlation-kernel variants")
     ap.add_argument("--only-batched", action="store_true",
                     help="time the batched steps only (for profiler runs)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
@@ -138,6 +140,14 @@ def main():
     for _ in range(args.warmup):
         an.analyze_batch_device(fa, fb, dx, dy, pr, fl, summaries=False, sync=True)
     torch.cuda.synchronize()
+    tune = None
+    if not args.no_autotune:
+        # kernel variants are bit-identical: the autotuner only picks the fastest schedule
+        tune = {f'n{t["window"]}': {"chosen_occupancy": t["occupancy"][t["chosen"]],
+                                    "occupancy": t["occupancy"],
+                                    "ms": [round(m, 4) for m in t["ms"]]}
+                for t in an.autotune(fa, fb, repeats=3)}
+        an.analyze_batch_device(fa, fb, dx, dy, pr, fl, summaries=False, sync=True)
 
     an.profile(True)
     torch.cuda.synchronize()
@@ -170,7 +180,7 @@ def main():
 
     if args.only_batched:
         print(json.dumps({"fields_per_s": fields_per_s, "ms_per_step": ms_per_step,
-                          "kernel_avg_ms": kernels}), flush=True)
+                          "kernel_avg_ms": kernels, "autotune": tune}), flush=True)
         return
 
     # single-request latency (one pair per call, arrival to result on the host)
@@ -218,6 +228,7 @@ def main():
         "rms_px_error_vs_truth": round(rms, 5),
         "rms_vectors": nvec,
         "kernel_avg_ms": kernels,
+        "autotune": tune,
         "roofline": {"bound": "hbm", "kernel": f"k_correlate<{dom['window']}>",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,

@@ -222,6 +222,32 @@ piv_status piv_profile_enable(piv_analyzer *a, int enable);
 piv_status piv_profile_read(piv_analyzer *a, piv_kernel_stat *out, uint32_t capacity,
                             uint32_t *count);
 
+/* ---- autotuning (no counterpart in the reference) -------------------------------------- */
+
+/* The correlation kernel of each window size has launch variants (occupancy targets of the
+ * register allocator).  Variants differ only in scheduling: every variant gives bit-identical
+ * results.  piv_autotune runs the analyzer's pass plan on the caller's batch once untimed and
+ * `repeats` times timed per variant, keeps the fastest variant per window size for the frames'
+ * sample format on this analyzer, and reports one entry per distinct window size. */
+typedef struct {
+    uint32_t window;
+    int32_t fmt;             /* piv_pixel_format of the frames */
+    uint32_t chosen;         /* variant index now used by the analyzer */
+    uint32_t num_variants;
+    uint32_t occupancy[4];   /* waves per SIMD targeted by each variant */
+    float ms[4];             /* mean kernel time per launch of each variant */
+} piv_tune_entry;
+
+/* Number of variants of the correlation kernel for a window size (0 if unsupported). */
+uint32_t piv_kernel_variant_count(uint32_t window);
+/* Pin a variant for (window, fmt) on this analyzer (as piv_autotune would choose it). */
+piv_status piv_kernel_variant_set(piv_analyzer *a, uint32_t window, piv_pixel_format fmt,
+                                  uint32_t variant);
+
+piv_status piv_autotune(piv_analyzer *a, const piv_frame *const *frames_a,
+                        const piv_frame *const *frames_b, uint32_t npairs, uint32_t repeats,
+                        piv_tune_entry *out, uint32_t capacity, uint32_t *count);
+
 /* decode.rs:227-234 sample normalisation on the device (host in/out), for parity checks. */
 piv_status piv_normalize_pixels(int device, const void *pixels, piv_pixel_format fmt,
                                 size_t count, float *out);

@@ -342,6 +342,29 @@ class PivAnalyzer:
                      total_ms=float(arr[i].total_ms), bytes=float(arr[i].bytes))
                 for i in range(cnt.value)]
 
+    def set_variant(self, window, dtype, variant):
+        """Pin the correlation-kernel variant for (window, sample dtype)."""
+        _check(self._lib.piv_kernel_variant_set(self._h, window, _FMT[np.dtype(dtype)], variant))
+
+    def autotune(self, frames_a, frames_b, repeats=3):
+        """Time every correlation-kernel variant on this batch and keep the fastest per window
+        size (variants give bit-identical results).  Returns one dict per window size."""
+        n = len(frames_a)
+        A = (C.c_void_p * n)(*[f._h.value for f in frames_a])
+        B = (C.c_void_p * n)(*[f._h.value for f in frames_b])
+        arr = (L.piv_tune_entry * 16)()
+        cnt = C.c_uint32(0)
+        _check(self._lib.piv_autotune(self._h, A, B, n, repeats, C.cast(arr, C.c_void_p), 16,
+                                      C.byref(cnt)))
+        out = []
+        for i in range(min(cnt.value, 16)):
+            e = arr[i]
+            k = e.num_variants
+            out.append(dict(window=int(e.window), fmt=int(e.fmt), chosen=int(e.chosen),
+                            occupancy=[int(x) for x in e.occupancy[:k]],
+                            ms=[float(x) for x in e.ms[:k]]))
+        return out
+
     def analyze_batch_device(self, frames_a, frames_b, dx, dy, ppr, flags_out,
                              summaries=True, stream=None, sync=True):
         """Analyse len(frames_a) pairs; outputs are device tensors [npairs, ny, nx]."""
@@ -586,6 +609,11 @@ def normalize_pixels(pixels, device=0):
     out = np.empty(p.shape, np.float32)
     _check(L.load().piv_normalize_pixels(device, L.ptr(p), _FMT[p.dtype], p.size, L.ptr(out)))
     return out
+
+
+def kernel_variant_count(window):
+    """Number of launch variants of the correlation kernel for a window size."""
+    return int(L.load().piv_kernel_variant_count(window))
 
 
 def device_count():

@@ -59,6 +59,11 @@ class piv_kernel_stat(C.Structure):
                 ("units", C.c_uint64), ("total_ms", C.c_double), ("bytes", C.c_double)]
 
 
+class piv_tune_entry(C.Structure):
+    _fields_ = [("window", C.c_uint32), ("fmt", C.c_int32), ("chosen", C.c_uint32),
+                ("num_variants", C.c_uint32), ("occupancy", C.c_uint32 * 4), ("ms", C.c_float * 4)]
+
+
 # every entry point declared in include/piv.h, with its signature
 _SIGS = {
     "piv_last_error": ([], C.c_char_p),
@@ -101,6 +106,10 @@ _SIGS = {
     "piv_analyzer_synchronize": ([C.c_void_p], C.c_int),
     "piv_profile_enable": ([C.c_void_p, C.c_int], C.c_int),
     "piv_profile_read": ([C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)], C.c_int),
+    "piv_kernel_variant_count": ([C.c_uint32], C.c_uint32),
+    "piv_kernel_variant_set": ([C.c_void_p, C.c_uint32, C.c_int, C.c_uint32], C.c_int),
+    "piv_autotune": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
+                      C.c_uint32, C.POINTER(C.c_uint32)], C.c_int),
     "piv_normalize_pixels": ([C.c_int, C.c_void_p, C.c_int, C.c_size_t, C.c_void_p], C.c_int),
 }
 

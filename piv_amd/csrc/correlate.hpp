@@ -22,8 +22,11 @@
 //      (one thread per window), which does the 3-point fit, the peak ratio and assembly.
 #pragma once
 
+#include <type_traits>
+
 #include "fft.hpp"
 #include "piv_common.hpp"
+#include "variants.hpp"
 
 namespace piv {
 
@@ -35,8 +38,6 @@ struct KCfg {
     static constexpr int BLOCK = WPB * N;
     static constexpr int P = N + 1;   // odd LDS row stride: conflict-free rows and columns
     static constexpr int PL = N * P;
-    // minimum waves per SIMD the register allocator must allow
-    static constexpr int WAVES = N <= 16 ? 4 : (N <= 32 ? 3 : (N <= 64 ? 2 : 1));
 };
 
 // Sum over the N lanes of one window (all lanes receive the total).
@@ -121,12 +122,16 @@ __device__ __forceinline__ void load_column_fast(const void *__restrict__ img, i
         tx = ints ? 0.0f : tx;
         ty = ints ? 0.0f : ty;
     }
+    using T = std::conditional_t<FMT == FMT_U8, uint8_t, std::conditional_t<FMT == FMT_U16, uint16_t, float>>;
+    const __attribute__((address_space(1))) T *base = (const __attribute__((address_space(1))) T *)img;
     float hprev = 0.0f;
 #pragma unroll
     for (int r = 0; r <= N; ++r) {
-        const uint32_t row = o + (uint32_t)r * (uint32_t)pitch;
-        const float p0 = raw_sample<FMT>(img, row);
-        const float p1 = raw_sample<FMT>(img, row + 1);
+        // 64-bit pointer arithmetic on a 32-bit row offset: the +1 sample folds into the
+        // load's immediate offset
+        const __attribute__((address_space(1))) T *q = base + (o + (uint32_t)r * (uint32_t)pitch);
+        const float p0 = (float)q[0];
+        const float p1 = (float)q[1];
         float h;
         if constexpr (FMT == FMT_F32) h = ints ? p0 : __builtin_fmaf(tx, p1 - p0, p0);
         else h = __builtin_fmaf(tx, p1 - p0, p0);
@@ -148,8 +153,8 @@ __device__ __forceinline__ double energy_floor()
     else return 1e-12;
 }
 
-template <int N, int FMT>
-__global__ __launch_bounds__(KCfg<N>::BLOCK) __attribute__((amdgpu_waves_per_eu(KCfg<N>::WAVES)))
+template <int N, int FMT, int OCC>
+__global__ __launch_bounds__(KCfg<N>::BLOCK) __attribute__((amdgpu_waves_per_eu(OCC)))
 void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblocks)
 {
     using C = KCfg<N>;
@@ -515,6 +520,25 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
         int dyc = abs(lane - pby);
         dyc = dyc < N - dyc ? dyc : N - dyc;
         const bool row_far = dyc > exclude;
+        // Columns of this row outside the footprint as a bit mask (N <= 64): every column
+        // when the row is far, else all but the circular run pbx - exclude .. pbx + exclude.
+        using Mask = std::conditional_t<(N > 32), unsigned long long, unsigned int>;
+        constexpr int MB = 8 * (int)sizeof(Mask);
+        constexpr Mask full = N == MB ? ~Mask(0) : ((Mask(1) << (N % MB)) - 1);
+        Mask allowed = full;
+        if constexpr (N <= 64) {
+            if (!row_far) {
+                const int len = 2 * exclude + 1;
+                if (len >= N) {
+                    allowed = 0;
+                } else {
+                    const Mask run = (Mask(1) << len) - 1;
+                    const int s = (pbx - exclude + N) % N;
+                    const Mask ex = s == 0 ? run : (Mask)(((run << s) | (run >> (N - s))) & full);
+                    allowed = full & ~ex;
+                }
+            }
+        }
         // vertical max of (row above, row, row below) per column, sliding along x
         float vm_prev = fmaxf(fmaxf(pr[ru * P + N - 1], zr[N - 1]), pr[rd * P + N - 1]);
         float u1 = pr[ru * P + 0], d1 = pr[rd * P + 0];
@@ -524,19 +548,27 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
             const int xn = x + 1 < N ? x + 1 : 0;
             const float u2 = pr[ru * P + xn], d2 = pr[rd * P + xn];
             const float vm_next = fmaxf(fmaxf(u2, zr[xn]), d2);
-            int dxc = abs(x - pbx);
-            dxc = dxc < N - dxc ? dxc : N - dxc;
             const float v = zr[x];
+            bool far;
+            if constexpr (N <= 64) {
+                far = (allowed >> x) & 1;
+            } else {
+                int dxc = abs(x - pbx);
+                dxc = dxc < N - dxc ? dxc : N - dxc;
+                far = row_far || dxc > exclude;
+            }
             bool local;
             if constexpr (MAYBE_NAN) {
                 const float m0 = zr[(x + N - 1) % N], m2 = zr[xn];
                 const float u0 = pr[ru * P + (x + N - 1) % N], d0 = pr[rd * P + (x + N - 1) % N];
                 local = u0 <= v && u1 <= v && u2 <= v && m0 <= v && m2 <= v && d0 <= v &&
                         d1 <= v && d2 <= v;
+                if (local && far && v > cand) cand = v;
             } else {
+                // finite values: max(cand, v) is the strict-> update of the reference
                 local = fmaxf(fmaxf(vm_prev, vm_next), fmaxf(u1, d1)) <= v;
+                cand = (local && far) ? fmaxf(cand, v) : cand;
             }
-            if (local && (row_far || dxc > exclude) && v > cand) cand = v;
             vm_prev = vm_cur;
             vm_cur = vm_next;
             u1 = u2;
@@ -567,26 +599,39 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
     }
 }
 
-template <int N, int FMT>
+template <int N, int FMT, int OCC>
 hipError_t launch_corr_t(const PassPair *pairs, int npairs, uint32_t max_cells, hipStream_t st)
 {
     using C = KCfg<N>;
     const uint32_t bpp = (max_cells + C::WPB - 1) / C::WPB;
     const uint32_t blocks = bpp * (uint32_t)npairs;
     if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_correlate<N, FMT>), dim3(blocks), dim3(C::BLOCK), 0, st, pairs, bpp,
+    hipLaunchKernelGGL((k_correlate<N, FMT, OCC>), dim3(blocks), dim3(C::BLOCK), 0, st, pairs, bpp,
                        blocks);
     return hipGetLastError();
 }
 
-template <int N>
-hipError_t launch_corr_n(int fmt, const PassPair *pairs, int npairs, uint32_t max_cells,
+template <int N, int FMT>
+hipError_t launch_corr_v(int variant, const PassPair *pairs, int npairs, uint32_t max_cells,
                          hipStream_t st)
 {
+    constexpr VariantSet vs = corr_variants(N);
+    if (variant == 0) return launch_corr_t<N, FMT, vs.occ[0]>(pairs, npairs, max_cells, st);
+    if constexpr (vs.count > 1)
+        if (variant == 1) return launch_corr_t<N, FMT, vs.occ[1]>(pairs, npairs, max_cells, st);
+    if constexpr (vs.count > 2)
+        if (variant == 2) return launch_corr_t<N, FMT, vs.occ[2]>(pairs, npairs, max_cells, st);
+    return hipErrorInvalidValue;
+}
+
+template <int N>
+hipError_t launch_corr_n(int fmt, int variant, const PassPair *pairs, int npairs,
+                         uint32_t max_cells, hipStream_t st)
+{
     switch (fmt) {
-    case FMT_U8: return launch_corr_t<N, FMT_U8>(pairs, npairs, max_cells, st);
-    case FMT_U16: return launch_corr_t<N, FMT_U16>(pairs, npairs, max_cells, st);
-    default: return launch_corr_t<N, FMT_F32>(pairs, npairs, max_cells, st);
+    case FMT_U8: return launch_corr_v<N, FMT_U8>(variant, pairs, npairs, max_cells, st);
+    case FMT_U16: return launch_corr_v<N, FMT_U16>(variant, pairs, npairs, max_cells, st);
+    default: return launch_corr_v<N, FMT_F32>(variant, pairs, npairs, max_cells, st);
     }
 }
 

@@ -7,6 +7,7 @@
 // uploaded with a single host-to-device copy before the first launch.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -20,6 +21,7 @@
 #include "../../include/piv.h"
 #include "piv_device.hpp"
 #include "piv_kernels.hpp"
+#include "variants.hpp"
 
 using namespace piv;
 
@@ -214,6 +216,9 @@ struct piv_analyzer {
         uint64_t units;
         hipEvent_t e0, e1;
     };
+    // correlation kernel variant (variants.hpp) per (window size, sample format), set by
+    // piv_autotune; absent means variant 0
+    std::map<std::pair<int, int>, int> variants;
     // replay cache: the last call's descriptor bytes and the graph of its launch sequence
     std::vector<char> last_sig;
     hipGraphExec_t graph_exec = nullptr;
@@ -245,6 +250,12 @@ struct piv_analyzer {
 };
 
 namespace {
+
+int variant_of(const piv_analyzer *a, int n, int fmt)
+{
+    auto it = a->variants.find({n, fmt});
+    return it == a->variants.end() ? 0 : it->second;
+}
 
 // PivAnalyzer::new checks (multipass.rs:91-119)
 piv_status check_config(const piv_config &c)
@@ -594,7 +605,8 @@ piv_status run_batch(piv_analyzer *a, const piv_frame *const *fa, const piv_fram
         sig.insert(sig.end(), reinterpret_cast<const char *>(extra),
                    reinterpret_cast<const char *>(extra) + sizeof extra);
         for (const PassPlan &pl : plans) {
-            const uintptr_t pv[2] = {(uintptr_t)pl.n, (uintptr_t)pl.cells};
+            const uintptr_t pv[3] = {(uintptr_t)pl.n, (uintptr_t)pl.cells,
+                                     (uintptr_t)variant_of(a, pl.n, fa[0]->fmt)};
             sig.insert(sig.end(), reinterpret_cast<const char *>(pv),
                        reinterpret_cast<const char *>(pv) + sizeof pv);
         }
@@ -612,7 +624,8 @@ piv_status run_batch(piv_analyzer *a, const piv_frame *const *fa, const piv_fram
                 return launch_place_pass(ppd, (int)npairs, (uint32_t)pl.cells, st);
             }));
             HIP_TRY(timed(a, "correlate", pl.n, cbytes, total, [&] {
-                return launch_correlate(pl.n, fa[0]->fmt, ppd, (int)npairs, (uint32_t)pl.cells, st);
+                return launch_correlate(pl.n, fa[0]->fmt, variant_of(a, pl.n, fa[0]->fmt), ppd,
+                                        (int)npairs, (uint32_t)pl.cells, st);
             }));
             // the finishing kernel reads peak records + origins (48 B) and writes 13 B/window
             HIP_TRY(timed(a, "finish", 0, (double)total * 61.0, total, [&] {
@@ -1137,7 +1150,7 @@ piv_status piv_correlate_windows(piv_analyzer *a, const piv_frame *fa, const piv
     hipStream_t st = a->stream;
     HIP_TRY(hipMemcpyAsync(d_org, origins, count * 16, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(d_pp, &d, sizeof d, hipMemcpyHostToDevice, st));
-    HIP_TRY(launch_pass((int)window, fa->fmt, d_pp, 1, count, st));
+    HIP_TRY(launch_pass((int)window, fa->fmt, variant_of(a, (int)window, fa->fmt), d_pp, 1, count, st));
     HIP_TRY(hipMemcpyAsync(dx, o_dx, count * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(dy, o_dy, count * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(ppr, o_ppr, count * 4, hipMemcpyDeviceToHost, st));
@@ -1350,6 +1363,112 @@ piv_status piv_profile_read(piv_analyzer *a, piv_kernel_stat *out, uint32_t capa
     uint32_t i = 0;
     for (auto &kv : a->prof_stats) {
         if (out && i < capacity) out[i] = kv.second;
+        ++i;
+    }
+    *count = i;
+    return PIV_OK;
+}
+
+uint32_t piv_kernel_variant_count(uint32_t window)
+{
+    return correlate_supported((int)window) ? (uint32_t)corr_variants((int)window).count : 0u;
+}
+
+piv_status piv_kernel_variant_set(piv_analyzer *a, uint32_t window, piv_pixel_format fmt,
+                                  uint32_t variant)
+{
+    if (!a) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+    if (!correlate_supported((int)window))
+        return fail(PIV_ERR_UNSUPPORTED, "window size %u has no gfx950 kernel", window);
+    if ((int)fmt != FMT_U8 && (int)fmt != FMT_U16 && (int)fmt != FMT_F32)
+        return fail(PIV_ERR_INVALID_ARGUMENT, "unknown pixel format %d", (int)fmt);
+    if (variant >= (uint32_t)corr_variants((int)window).count)
+        return fail(PIV_ERR_INVALID_ARGUMENT, "window %u has %d variants", window,
+                    corr_variants((int)window).count);
+    std::lock_guard<std::mutex> lk(a->mu);
+    a->variants[{(int)window, (int)fmt}] = (int)variant;
+    return PIV_OK;
+}
+
+piv_status piv_autotune(piv_analyzer *a, const piv_frame *const *fa, const piv_frame *const *fb,
+                        uint32_t npairs, uint32_t repeats, piv_tune_entry *out, uint32_t capacity,
+                        uint32_t *count)
+{
+    if (!a || !fa || !fb || !count || npairs == 0) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+    if (repeats == 0) repeats = 1;
+    std::lock_guard<std::mutex> lk(a->mu);
+    PIV_TRY(set_device(a->device));
+    const int fmt = fa[0]->fmt;
+    std::vector<PassPlan> plans;
+    PIV_TRY(plan_passes(a, fa[0]->H, fa[0]->W, plans));
+    std::vector<int> sizes;
+    for (const PassPlan &pl : plans)
+        if (std::find(sizes.begin(), sizes.end(), pl.n) == sizes.end()) sizes.push_back(pl.n);
+    int maxv = 1;
+    for (int n : sizes) maxv = std::max(maxv, corr_variants(n).count);
+    HIP_TRY(hipStreamSynchronize(a->stream));
+    prof_harvest(a);
+    const bool saved_on = a->prof_on;
+    const auto saved_stats = a->prof_stats;
+    std::map<int, std::vector<double>> ms;
+    for (int n : sizes) ms[n].assign(corr_variants(n).count, 0.0);
+    piv_status s = PIV_OK;
+    for (int v = 0; v < maxv && s == PIV_OK; ++v) {
+        for (int n : sizes) a->variants[{n, fmt}] = std::min(v, corr_variants(n).count - 1);
+        std::vector<PassPlan> pl;
+        std::vector<FieldPtrs> raw, val;
+        std::vector<float *> org;
+        std::vector<int32_t *> pk;
+        uint32_t *sums = nullptr;
+        a->prof_on = false;  // one untimed run per variant (first launch of each kernel)
+        s = run_batch(a, fa, fb, npairs, BatchOut{}, false, pl, raw, val, org, pk, &sums);
+        if (s != PIV_OK) break;
+        if (hipStreamSynchronize(a->stream) != hipSuccess) {
+            s = fail(PIV_ERR_DEVICE, "autotune run failed");
+            break;
+        }
+        a->prof_stats.clear();
+        a->prof_on = true;
+        for (uint32_t r = 0; r < repeats && s == PIV_OK; ++r)
+            s = run_batch(a, fa, fb, npairs, BatchOut{}, false, pl, raw, val, org, pk, &sums);
+        if (s != PIV_OK) break;
+        if (hipStreamSynchronize(a->stream) != hipSuccess) {
+            s = fail(PIV_ERR_DEVICE, "autotune run failed");
+            break;
+        }
+        prof_harvest(a);
+        for (int n : sizes) {
+            if (v >= corr_variants(n).count) continue;
+            auto it = a->prof_stats.find({"correlate", n});
+            if (it != a->prof_stats.end() && it->second.launches)
+                ms[n][v] = it->second.total_ms / (double)it->second.launches;
+        }
+    }
+    a->prof_on = saved_on;
+    a->prof_stats = saved_stats;
+    if (s != PIV_OK) {
+        for (int n : sizes) a->variants.erase({n, fmt});
+        return s;
+    }
+    uint32_t i = 0;
+    for (int n : sizes) {
+        const VariantSet vs = corr_variants(n);
+        int best = 0;
+        for (int v = 1; v < vs.count; ++v)
+            if (ms[n][v] < ms[n][best]) best = v;
+        a->variants[{n, fmt}] = best;
+        if (out && i < capacity) {
+            piv_tune_entry &e = out[i];
+            e = piv_tune_entry{};
+            e.window = (uint32_t)n;
+            e.fmt = fmt;
+            e.chosen = (uint32_t)best;
+            e.num_variants = (uint32_t)vs.count;
+            for (int v = 0; v < vs.count && v < 4; ++v) {
+                e.occupancy[v] = (uint32_t)vs.occ[v];
+                e.ms[v] = (float)ms[n][v];
+            }
+        }
         ++i;
     }
     *count = i;

@@ -25,21 +25,21 @@
 
 namespace piv {
 
-hipError_t launch_correlate_8(int fmt, const PassPair *pairs, int npairs, uint32_t max_cells,
+hipError_t launch_correlate_8(int fmt, int variant, const PassPair *pairs, int npairs, uint32_t max_cells,
                               hipStream_t st);
-hipError_t launch_correlate_12(int fmt, const PassPair *pairs, int npairs, uint32_t max_cells,
+hipError_t launch_correlate_12(int fmt, int variant, const PassPair *pairs, int npairs, uint32_t max_cells,
                               hipStream_t st);
-hipError_t launch_correlate_16(int fmt, const PassPair *pairs, int npairs, uint32_t max_cells,
+hipError_t launch_correlate_16(int fmt, int variant, const PassPair *pairs, int npairs, uint32_t max_cells,
                               hipStream_t st);
-hipError_t launch_correlate_24(int fmt, const PassPair *pairs, int npairs, uint32_t max_cells,
+hipError_t launch_correlate_24(int fmt, int variant, const PassPair *pairs, int npairs, uint32_t max_cells,
                               hipStream_t st);
-hipError_t launch_correlate_32(int fmt, const PassPair *pairs, int npairs, uint32_t max_cells,
+hipError_t launch_correlate_32(int fmt, int variant, const PassPair *pairs, int npairs, uint32_t max_cells,
                               hipStream_t st);
-hipError_t launch_correlate_48(int fmt, const PassPair *pairs, int npairs, uint32_t max_cells,
+hipError_t launch_correlate_48(int fmt, int variant, const PassPair *pairs, int npairs, uint32_t max_cells,
                               hipStream_t st);
-hipError_t launch_correlate_64(int fmt, const PassPair *pairs, int npairs, uint32_t max_cells,
+hipError_t launch_correlate_64(int fmt, int variant, const PassPair *pairs, int npairs, uint32_t max_cells,
                               hipStream_t st);
-hipError_t launch_correlate_96(int fmt, const PassPair *pairs, int npairs, uint32_t max_cells,
+hipError_t launch_correlate_96(int fmt, int variant, const PassPair *pairs, int npairs, uint32_t max_cells,
                               hipStream_t st);
 
 // ---------------------------------------------------------------------------------------
@@ -570,18 +570,18 @@ bool correlate_supported(int n)
     }
 }
 
-hipError_t launch_correlate(int n, int fmt, const PassPair *pairs, int npairs,
+hipError_t launch_correlate(int n, int fmt, int variant, const PassPair *pairs, int npairs,
                             uint32_t max_cells, hipStream_t st)
 {
     switch (n) {
-    case 8: return launch_correlate_8(fmt, pairs, npairs, max_cells, st);
-    case 12: return launch_correlate_12(fmt, pairs, npairs, max_cells, st);
-    case 16: return launch_correlate_16(fmt, pairs, npairs, max_cells, st);
-    case 24: return launch_correlate_24(fmt, pairs, npairs, max_cells, st);
-    case 32: return launch_correlate_32(fmt, pairs, npairs, max_cells, st);
-    case 48: return launch_correlate_48(fmt, pairs, npairs, max_cells, st);
-    case 64: return launch_correlate_64(fmt, pairs, npairs, max_cells, st);
-    case 96: return launch_correlate_96(fmt, pairs, npairs, max_cells, st);
+    case 8: return launch_correlate_8(fmt, variant, pairs, npairs, max_cells, st);
+    case 12: return launch_correlate_12(fmt, variant, pairs, npairs, max_cells, st);
+    case 16: return launch_correlate_16(fmt, variant, pairs, npairs, max_cells, st);
+    case 24: return launch_correlate_24(fmt, variant, pairs, npairs, max_cells, st);
+    case 32: return launch_correlate_32(fmt, variant, pairs, npairs, max_cells, st);
+    case 48: return launch_correlate_48(fmt, variant, pairs, npairs, max_cells, st);
+    case 64: return launch_correlate_64(fmt, variant, pairs, npairs, max_cells, st);
+    case 96: return launch_correlate_96(fmt, variant, pairs, npairs, max_cells, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -617,11 +617,11 @@ hipError_t launch_peak_finish(const PassPair *pairs, int npairs, uint32_t max_ce
     return hipGetLastError();
 }
 
-hipError_t launch_pass(int n, int fmt, const PassPair *pairs, int npairs, uint32_t max_cells,
-                       hipStream_t st)
+hipError_t launch_pass(int n, int fmt, int variant, const PassPair *pairs, int npairs,
+                       uint32_t max_cells, hipStream_t st)
 {
     hipError_t e = launch_place_pass(pairs, npairs, max_cells, st);
-    if (e == hipSuccess) e = launch_correlate(n, fmt, pairs, npairs, max_cells, st);
+    if (e == hipSuccess) e = launch_correlate(n, fmt, variant, pairs, npairs, max_cells, st);
     if (e == hipSuccess) e = launch_peak_finish(pairs, npairs, max_cells, st);
     return e;
 }

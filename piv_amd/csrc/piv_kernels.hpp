@@ -8,14 +8,15 @@
 namespace piv {
 
 bool correlate_supported(int n);
-hipError_t launch_correlate(int n, int fmt, const PassPair *pairs, int npairs,
+// variant: index into corr_variants(n) (variants.hpp); 0 is the default
+hipError_t launch_correlate(int n, int fmt, int variant, const PassPair *pairs, int npairs,
                             uint32_t max_cells, hipStream_t st);
 // placement + mask (one thread per window), and sub-pixel fit + assembly of the raw field
 hipError_t launch_place_pass(const PassPair *pairs, int npairs, uint32_t max_cells, hipStream_t st);
 hipError_t launch_peak_finish(const PassPair *pairs, int npairs, uint32_t max_cells, hipStream_t st);
 // placement + mask, correlation, sub-pixel fit and assembly of one pass (three launches)
-hipError_t launch_pass(int n, int fmt, const PassPair *pairs, int npairs, uint32_t max_cells,
-                       hipStream_t st);
+hipError_t launch_pass(int n, int fmt, int variant, const PassPair *pairs, int npairs,
+                       uint32_t max_cells, hipStream_t st);
 // both validation launches of one pass; summaries must be zeroed beforehand
 hipError_t launch_validate(const ValidatePair *jobs, int njobs, uint32_t max_cells,
                            hipStream_t st);

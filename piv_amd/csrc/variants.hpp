@@ -1,0 +1,29 @@
+// Launch variants of the correlation kernel, per window size.
+//
+// A variant fixes the occupancy target the register allocator must meet (waves per SIMD):
+// fewer waves leave more registers per lane, more waves hide more memory and LDS latency but
+// may spill.  Variants differ only in register allocation and instruction scheduling, so
+// every variant computes bit-identical results; the autotuner picks one per window size and
+// sample format by timing them on the caller's own batch.
+#pragma once
+
+namespace piv {
+
+constexpr int kMaxVariants = 3;
+
+struct VariantSet {
+    int count;
+    int occ[kMaxVariants];  // waves per SIMD; occ[0] is the default
+};
+
+constexpr VariantSet corr_variants(int n)
+{
+    switch (n) {
+    case 8: case 12: case 16: return {3, {4, 6, 8}};
+    case 24: case 32: return {2, {3, 4}};
+    case 48: case 64: return {2, {2, 1}};
+    default: return {1, {1, 0, 0}};
+    }
+}
+
+}  // namespace piv

@@ -87,3 +87,12 @@ def test_no_device_is_a_device_error():
         pytest.skip("a GPU is present")
     with pytest.raises(piv_amd.DeviceError):
         piv_amd.PivAnalyzer(piv_amd.PivConfig(windows=[32]))
+
+
+def test_kernel_variant_counts():
+    """Every supported window size has at least one correlation-kernel variant; others none."""
+    import piv_amd
+    for n in (8, 12, 16, 24, 32, 48, 64, 96):
+        assert 1 <= piv_amd.kernel_variant_count(n) <= 4
+    for n in (0, 10, 40, 128):
+        assert piv_amd.kernel_variant_count(n) == 0

@@ -262,6 +262,36 @@ def test_batch_replay_matches_eager(piv):
         assert np.array_equal(r[0], tdx) and np.array_equal(r[3], tfl) and r[4] == tsum, k
 
 
+@pytest.mark.parametrize("fmt", [np.uint8, np.uint16, np.float32])
+def test_kernel_variants_bit_identical(piv, fmt):
+    """Every launch variant of every window size gives the default variant's field bitwise;
+    the autotuner then picks one of them and the result is unchanged."""
+    h, w = 208, 224
+    for n in (8, 12, 16, 24, 32, 48, 64, 96):
+        a, b = S.particle_frames(h, w, 40 + n, field=S.uniform_field(1.3, -0.6))
+        if fmt is np.uint16:
+            a, b = a.astype(np.uint16) * 16, b.astype(np.uint16) * 16
+        elif fmt is np.float32:
+            a, b = a.astype(np.float32) / 255, b.astype(np.float32) / 255
+        cfg = piv.PivConfig(windows=[n, n], final_iterations=1)
+        an = piv.PivAnalyzer(cfg, piv.PreprocessConfig.raw())
+        fa, fb = an.prepare(a), an.prepare(b)
+        ref = an.analyze(fa, fb)
+        assert np.isfinite(ref.field.dx).mean() > 0.5, n
+        for v in range(1, piv.kernel_variant_count(n)):
+            an.set_variant(n, fmt, v)
+            r = an.analyze(fa, fb)
+            assert np.array_equal(bits(r.field.dx), bits(ref.field.dx)), (n, v)
+            assert np.array_equal(bits(r.field.dy), bits(ref.field.dy)), (n, v)
+            assert np.array_equal(r.field.flags, ref.field.flags), (n, v)
+        tune = an.autotune([fa], [fb], repeats=2)
+        assert [t["window"] for t in tune] == [n]
+        assert 0 <= tune[0]["chosen"] < piv.kernel_variant_count(n)
+        assert all(m > 0 for m in tune[0]["ms"])
+        r = an.analyze(fa, fb)
+        assert np.array_equal(bits(r.field.dx), bits(ref.field.dx))
+
+
 def test_deterministic(piv):
     a, b = S.particle_frames(512, 512, 5, field=S.uniform_field(3.3, 1.1))
     an = piv.PivAnalyzer(piv.PivConfig(windows=[64, 32], final_iterations=2),
