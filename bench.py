@@ -77,6 +77,177 @@ def cpu_baseline(pair, seconds):
                       f"{cores} OpenMP threads"}
 
 
+# Mixed serving workload: four streams (frame size, sample format, passes, particle model,
+# share of the arrivals); Poisson arrivals at --serve-rate pairs/s in total.
+SERVE_STREAMS = [
+    dict(name="512x512 u8 [32]", h=512, w=512, dtype="u8", bits=8, windows=[32],
+         density=0.02, diameter_range=None, rate=0.40),
+    dict(name="1024x1024 u8 [64,32,16]", h=1024, w=1024, dtype="u8", bits=8,
+         windows=[64, 32, 16], density=0.02, diameter_range=None, rate=0.30),
+    dict(name="1280x800 u16 [48,24]", h=800, w=1280, dtype="u16", bits=12, windows=[48, 24],
+         density=0.015, diameter_range=(2.0, 3.0), rate=0.20),
+    dict(name="2048x2048 u16 [64,32,16]", h=2048, w=2048, dtype="u16", bits=12,
+         windows=[64, 32, 16], density=0.03, diameter_range=None, rate=0.10),
+]
+
+
+def gather_bytes(h, w, s_in, windows):
+    """Window-gather model bytes of one pair: 2 H W s_in + sum_pass N (2 n^2 4 + 49)."""
+    import piv_amd
+
+    b = 2 * h * w * s_in
+    for n in windows:
+        b += len(piv_amd.Grid.new(h, w, n, 0.5)) * (2 * n * n * 4 + 49)
+    return b
+
+
+def serve_bench(rate, seconds, backlog, seed=77):
+    """Continuous-batching server on the mixed workload: p50/p99 latency (arrival to result
+    on the host) under Poisson arrivals at `rate` pairs/s, then the saturated rate with a
+    backlog of `backlog` requests (host staging and PCIe upload included)."""
+    import threading
+
+    import piv_amd
+    from piv_amd import synthetic as S
+
+    # the load generator and the result collector are Python threads: short GIL slices
+    sys.setswitchinterval(2e-4)
+    srv = piv_amd.PivServer(hbm_budget=16 << 30, max_batch_pairs=64, max_inflight=2)
+    streams = []
+    for k, sp in enumerate(SERVE_STREAMS):
+        dt = np.uint8 if sp["dtype"] == "u8" else np.uint16
+        cfg = piv_amd.PivConfig(windows=sp["windows"], overlap=0.5, final_iterations=1)
+        sid = srv.add_stream(cfg, sp["h"], sp["w"], dt, piv_amd.PreprocessConfig.raw())
+        field = S.lamb_oseen_shear(sp["h"], sp["w"], vmax=6.0, shear=0.02)
+        pairs = [S.particle_frames(sp["h"], sp["w"], seed + 10 * k + j, density=sp["density"],
+                                   diameter_range=sp["diameter_range"], field=field,
+                                   bits=sp["bits"], container=dt,
+                                   amp_scale=16.0 if sp["bits"] == 12 else 1.0)
+                 for j in range(2)]
+        gb = gather_bytes(sp["h"], sp["w"], 1 if dt is np.uint8 else 2, sp["windows"])
+        streams.append((sid, pairs, gb, sp))
+    share = np.array([sp["rate"] for sp in SERVE_STREAMS], np.float64)
+    share /= share.sum()
+
+    lat, per_stream, lock = [], {s[0]: [] for s in streams}, threading.Lock()
+    done_count = [0]
+    stop = threading.Event()
+
+    def poller():
+        while not stop.is_set() or done_count[0] < submitted[0]:
+            for r in srv.poll(1024, wait_ms=20, copy=False):
+                with lock:
+                    done_count[0] += 1
+                    if r.error is None:
+                        lat.append(r.latency_ms)
+                        per_stream[r.stream_id].append(r.latency_ms)
+
+    # warm-up: every stream's kernels, slots and graphs
+    submitted = [0]
+    th = threading.Thread(target=poller, daemon=True)
+    th.start()
+    for sid, pairs, _, _ in streams:
+        for a, b in pairs:
+            srv.submit(sid, a, b)
+            submitted[0] += 1
+    while done_count[0] < submitted[0]:
+        time.sleep(0.005)
+    with lock:
+        lat.clear()
+        for v in per_stream.values():
+            v.clear()
+
+    # Poisson arrivals
+    rng = np.random.default_rng(seed)
+    t_end = time.perf_counter() + seconds
+    t_next = time.perf_counter()
+    base_done = done_count[0]
+    rejected = 0
+    n_sub = 0
+    while True:
+        t_next += rng.exponential(1.0 / rate)
+        if t_next > t_end:
+            break
+        dt_ = t_next - time.perf_counter()
+        if dt_ > 0:
+            time.sleep(dt_)
+        k = int(rng.choice(len(streams), p=share))
+        sid, pairs, _, _ = streams[k]
+        a, b = pairs[n_sub % 2]
+        try:
+            srv.submit(sid, a, b)
+            submitted[0] += 1
+            n_sub += 1
+        except piv_amd.AdmissionError:
+            rejected += 1
+    while done_count[0] < submitted[0]:
+        time.sleep(0.005)
+    with lock:
+        L = np.array(lat)
+        ps = {streams[i][3]["name"]: (round(float(np.percentile(per_stream[s[0]], 50)), 3),
+                                      round(float(np.percentile(per_stream[s[0]], 99)), 3))
+              for i, s in enumerate(streams) if per_stream[s[0]]}
+    poisson = {"rate_pairs_per_s": rate, "seconds": seconds, "requests": int(L.size),
+               "rejected": rejected,
+               "p50_ms": round(float(np.percentile(L, 50)), 3),
+               "p99_ms": round(float(np.percentile(L, 99)), 3),
+               "per_stream_p50_p99_ms": ps}
+
+    def saturate(device_frames):
+        kinds = rng.choice(len(streams), size=backlog, p=share)
+        total_bytes = 0
+        t0 = time.perf_counter()
+        for i, k in enumerate(kinds):
+            sid, pairs, gb, _ = streams[k]
+            a, b = (dev_pairs[k] if device_frames else pairs)[i % 2]
+            while True:
+                try:
+                    if device_frames:
+                        srv.submit_device(sid, a, b)
+                    else:
+                        srv.submit(sid, a, b)
+                    break
+                except piv_amd.AdmissionError:
+                    time.sleep(0.0005)
+            submitted[0] += 1
+            total_bytes += gb
+        while done_count[0] < submitted[0]:
+            time.sleep(0.0005)
+        el = time.perf_counter() - t0
+        gbs = total_bytes / el / 1e9
+        st = srv.stats()
+        return {"requests": int(backlog), "pairs_per_s": round(backlog / el, 1),
+                "gather_model_GBps": round(gbs, 1),
+                "hbm_roofline_frac": round(gbs / HBM_PEAK_GBS, 4)}, st
+
+    # saturation: a backlog submitted as fast as the host stages it, then the same backlog
+    # with frames already in HBM (read in place by the server)
+    import torch
+
+    dev_pairs = [[(torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()) for a, b in s[1]]
+                 for s in streams]
+    torch.cuda.synchronize()
+    b0 = srv.stats()
+    sat_host, b1 = saturate(False)
+    sat_host["note"] = "host frames: staging copy and PCIe upload included"
+    sat_dev, b2 = saturate(True)
+    sat_dev["note"] = "frames resident in HBM (piv_submit_device)"
+    sat_host["mean_batch_pairs"] = round((b1["batched_pairs"] - b0["batched_pairs"]) /
+                                         max(b1["batches"] - b0["batches"], 1), 2)
+    sat_dev["mean_batch_pairs"] = round((b2["batched_pairs"] - b1["batched_pairs"]) /
+                                        max(b2["batches"] - b1["batches"], 1), 2)
+    stop.set()
+    th.join()
+    st = srv.stats()
+    srv.close()
+    return {"streams": [sp["name"] for sp in SERVE_STREAMS],
+            "share": [round(float(x), 3) for x in share],
+            "poisson": poisson,
+            "saturated_host_frames": sat_host,
+            "saturated_device_frames": sat_dev,
+            "batches": int(st["batches"]), "hbm_peak_bytes": int(st["hbm_peak"])}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +258,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--latency-requests", type=int, default=50)
+    ap.add_argument("--serve-seconds", type=float, default=6.0,
+                    help="Poisson phase of the serving measurement (0 skips serving)")
+    ap.add_argument("--serve-rate", type=float, default=500.0)
+    ap.add_argument("--serve-backlog", type=int, default=400)
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default correlation-kernel variants")
     ap.add_argument("--only-batched", action="store_true",
@@ -197,6 +372,10 @@ def main():
                             sync=True)
     rms, nvec = truth_rms(dx[0].cpu().numpy(), dy[0].cpu().numpy(), fl[0].cpu().numpy(), g, field)
 
+    serving = None
+    if args.serve_seconds > 0:
+        serving = serve_bench(args.serve_rate, args.serve_seconds, args.serve_backlog)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(uniq[0], args.cpu_seconds)
@@ -233,6 +412,7 @@ def main():
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": round(avg_ms, 5)},
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -222,6 +222,65 @@ piv_status piv_profile_enable(piv_analyzer *a, int enable);
 piv_status piv_profile_read(piv_analyzer *a, piv_kernel_stat *out, uint32_t capacity,
                             uint32_t *count);
 
+/* ---- serving (no counterpart in the reference, which analyses one pair per call:
+ *      piv-cli/src/analyze.rs:428-441) ------------------------------------------------------
+ * A server runs a device-resident request loop with continuous batching.  Each stream is one
+ * frame source with its own frame size, sample format, pass schedule, conditioning and
+ * background (kept in HBM).  Pairs submitted while others are on the device join the next
+ * batch as soon as a compute lane is free; a batch runs one launch per kernel and window size
+ * per pass step for all its requests, whatever their streams.  Every result is the one
+ * piv_analyze_pair gives for the same pair and configuration (pairs are independent). */
+typedef struct piv_server piv_server;
+
+typedef struct {
+    size_t hbm_budget;         /* device bytes requests may hold at once (0: 4 GiB) */
+    uint32_t max_batch_pairs;  /* pairs per batch (0: 64) */
+    uint32_t max_inflight;     /* batches on the device at once, one HIP stream each (0: 2) */
+} piv_server_config;
+
+typedef struct {
+    uint64_t request_id;
+    uint32_t stream_id;
+    piv_status status;
+    piv_grid grid;                   /* final pass grid: ny * nx values below */
+    const float *dx, *dy, *peak_ratio;
+    const uint8_t *flags;            /* host memory owned by the server until release */
+    piv_summary summary;
+    double submit_ms, done_ms;       /* server clock: submission and result on the host */
+    uint32_t batch_pairs;            /* pairs of the batch it ran in */
+} piv_result;
+
+typedef struct {
+    uint64_t submitted, completed, rejected, batches, batched_pairs;
+    size_t hbm_allocated, hbm_peak;  /* request slots (cached free slots included) */
+    size_t queued;
+} piv_server_stats;
+
+piv_status piv_server_create(int device, const piv_server_config *cfg, piv_server **out);
+/* Finishes the batches on the device, then frees everything (unpolled results included). */
+void piv_server_destroy(piv_server *s);
+/* background: host f32 [height * width] or null; it is uploaded once and stays in HBM. */
+piv_status piv_server_add_stream(piv_server *s, const piv_config *cfg, uint32_t height,
+                                 uint32_t width, piv_pixel_format fmt, const float *background,
+                                 uint32_t *stream_id);
+/* Copies the pair (row_stride bytes per row, 0 = packed) and queues it.  PIV_ERR_ADMISSION:
+ * the HBM budget or the queue is full and nothing was queued (retry after releasing results). */
+piv_status piv_submit(piv_server *s, uint32_t stream_id, const void *pixels_a,
+                      const void *pixels_b, size_t row_stride, uint64_t *request_id);
+/* As piv_submit for frames already in device memory on the server's GPU (complete when
+ * called): they are read in place, so keep them unchanged until the result is released. */
+piv_status piv_submit_device(piv_server *s, uint32_t stream_id, const void *pixels_a,
+                             const void *pixels_b, size_t row_stride, uint64_t *request_id);
+/* Up to `capacity` finished results, oldest first; waits up to wait_ms (-1: forever) for one. */
+piv_status piv_poll(piv_server *s, piv_result *out, uint32_t capacity, uint32_t *count,
+                    int wait_ms);
+/* Returns a polled result's memory (and its request slot) to the server. */
+piv_status piv_result_release(piv_server *s, uint64_t request_id);
+/* paused != 0: admitted requests wait and no batch is launched (batches on the device
+ * finish); 0 resumes.  Used to drain or to hand a burst to the device as full batches. */
+piv_status piv_server_pause(piv_server *s, int paused);
+piv_status piv_server_stats_read(piv_server *s, piv_server_stats *out);
+
 /* ---- autotuning (no counterpart in the reference) -------------------------------------- */
 
 /* The correlation kernel of each window size has launch variants (occupancy targets of the

@@ -70,8 +70,17 @@ class InvalidArgument(PivError):
     status = L.ERR_INVALID_ARGUMENT
 
 
+class AdmissionError(PivError):
+    """Serving: the request was refused by the HBM budget or a full queue."""
+    status = L.ERR_ADMISSION
+
+
+class OutOfMemory(PivError):
+    status = L.ERR_OOM
+
+
 _ERRORS = {c.status: c for c in (ConfigError, ShapeMismatch, ImageTooSmall, DeviceError,
-                                 Unsupported, InvalidArgument)}
+                                 Unsupported, InvalidArgument, AdmissionError, OutOfMemory)}
 
 
 def _check(rc):
@@ -609,6 +618,128 @@ def normalize_pixels(pixels, device=0):
     out = np.empty(p.shape, np.float32)
     _check(L.load().piv_normalize_pixels(device, L.ptr(p), _FMT[p.dtype], p.size, L.ptr(out)))
     return out
+
+
+@dataclass
+class ServedResult:
+    """One finished request of a PivServer (arrays are copies owned by Python)."""
+    request_id: int
+    stream_id: int
+    field: Optional["VectorField"]
+    summary: Optional["ValidationSummary"]
+    submit_ms: float
+    done_ms: float
+    batch_pairs: int
+    error: Optional[str] = None
+
+    @property
+    def latency_ms(self):
+        return self.done_ms - self.submit_ms
+
+
+class PivServer:
+    """Continuous-batching request loop on one GPU (include/piv.h, serving).  Streams have
+    their own frame size, sample format, pass schedule, conditioning and background; every
+    result equals PivAnalyzer.analyze on the same pair and configuration."""
+
+    def __init__(self, device=0, hbm_budget=0, max_batch_pairs=0, max_inflight=0):
+        self._lib = L.load()
+        cfg = L.piv_server_config(hbm_budget, max_batch_pairs, max_inflight)
+        h = C.c_void_p()
+        _check(self._lib.piv_server_create(device, C.byref(cfg), C.byref(h)))
+        self._h = h
+        self._streams = {}
+
+    def add_stream(self, cfg: "PivConfig", height, width, dtype,
+                   preprocess: "PreprocessConfig" = None, background=None):
+        pre = preprocess if preprocess is not None else PreprocessConfig()
+        c = cfg.to_c(pre)
+        bg = None
+        if background is not None:
+            bg = np.ascontiguousarray(background, np.float32)
+            if bg.shape != (height, width):
+                raise ShapeMismatch(f"background {bg.shape} vs stream {(height, width)}")
+        sid = C.c_uint32()
+        _check(self._lib.piv_server_add_stream(self._h, C.byref(c), height, width,
+                                               _FMT[np.dtype(dtype)],
+                                               None if bg is None else L.ptr(bg), C.byref(sid)))
+        self._streams[sid.value] = (height, width, np.dtype(dtype), bg)
+        return sid.value
+
+    def submit(self, stream_id, frame_a, frame_b):
+        h, w, dt, _ = self._streams[stream_id]
+        a = np.ascontiguousarray(frame_a, dt)
+        b = np.ascontiguousarray(frame_b, dt)
+        if a.shape != (h, w) or b.shape != (h, w):
+            raise ShapeMismatch(f"frames {a.shape}/{b.shape} vs stream {(h, w)}")
+        rid = C.c_uint64()
+        _check(self._lib.piv_submit(self._h, stream_id, L.ptr(a), L.ptr(b), 0, C.byref(rid)))
+        return rid.value
+
+    def submit_device(self, stream_id, frame_a, frame_b):
+        """Frames already in device memory (DeviceArray or torch tensor, [h, w] of the stream's
+        dtype), read in place: keep them unchanged until the result is polled."""
+        h, w, dt, _ = self._streams[stream_id]
+        for f in (frame_a, frame_b):
+            shp = tuple(f.shape)
+            if shp != (h, w):
+                raise ShapeMismatch(f"frame {shp} vs stream {(h, w)}")
+        rid = C.c_uint64()
+        _check(self._lib.piv_submit_device(self._h, stream_id, L.ptr(frame_a), L.ptr(frame_b), 0,
+                                           C.byref(rid)))
+        return rid.value
+
+    def poll(self, max_results=64, wait_ms=0, copy=True):
+        """Finished requests, oldest first, released after reading.  copy=False skips the
+        field arrays (field is None): ids, status, timings and summaries only."""
+        if getattr(self, "_poll_cap", 0) < max_results:
+            self._poll_arr = (L.piv_result * max_results)()
+            self._poll_cap = max_results
+        arr = self._poll_arr
+        cnt = C.c_uint32()
+        _check(self._lib.piv_poll(self._h, arr, max_results, C.byref(cnt), int(wait_ms)))
+        out = []
+        release = self._lib.piv_result_release
+        for i in range(cnt.value):
+            r = arr[i]
+            field, summ, err = None, None, None
+            if r.status == L.OK:
+                summ = ValidationSummary._from_c(r.summary)
+                if copy:
+                    g = Grid._from_c(r.grid)
+                    n = g.nx * g.ny
+                    shp = (g.ny, g.nx)
+                    field = VectorField(
+                        g, np.ctypeslib.as_array(r.dx, (n,)).reshape(shp).copy(),
+                        np.ctypeslib.as_array(r.dy, (n,)).reshape(shp).copy(),
+                        np.ctypeslib.as_array(r.peak_ratio, (n,)).reshape(shp).copy(),
+                        np.ctypeslib.as_array(r.flags, (n,)).reshape(shp).copy())
+            else:
+                err = f"status {r.status}"
+            out.append(ServedResult(int(r.request_id), int(r.stream_id), field, summ,
+                                    float(r.submit_ms), float(r.done_ms), int(r.batch_pairs), err))
+            _check(release(self._h, r.request_id))
+        return out
+
+    def pause(self, paused=True):
+        """Hold (True) or release (False) the launch of admitted requests."""
+        _check(self._lib.piv_server_pause(self._h, 1 if paused else 0))
+
+    def stats(self):
+        s = L.piv_server_stats()
+        _check(self._lib.piv_server_stats_read(self._h, C.byref(s)))
+        return {k: int(getattr(s, k)) for k, _ in L.piv_server_stats._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.piv_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def kernel_variant_count(window):

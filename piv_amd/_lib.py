@@ -59,6 +59,25 @@ class piv_kernel_stat(C.Structure):
                 ("units", C.c_uint64), ("total_ms", C.c_double), ("bytes", C.c_double)]
 
 
+class piv_server_config(C.Structure):
+    _fields_ = [("hbm_budget", C.c_size_t), ("max_batch_pairs", C.c_uint32),
+                ("max_inflight", C.c_uint32)]
+
+
+class piv_result(C.Structure):
+    _fields_ = [("request_id", C.c_uint64), ("stream_id", C.c_uint32), ("status", C.c_int),
+                ("grid", piv_grid), ("dx", C.POINTER(C.c_float)), ("dy", C.POINTER(C.c_float)),
+                ("peak_ratio", C.POINTER(C.c_float)), ("flags", C.POINTER(C.c_uint8)),
+                ("summary", piv_summary), ("submit_ms", C.c_double), ("done_ms", C.c_double),
+                ("batch_pairs", C.c_uint32)]
+
+
+class piv_server_stats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("rejected", C.c_uint64),
+                ("batches", C.c_uint64), ("batched_pairs", C.c_uint64),
+                ("hbm_allocated", C.c_size_t), ("hbm_peak", C.c_size_t), ("queued", C.c_size_t)]
+
+
 class piv_tune_entry(C.Structure):
     _fields_ = [("window", C.c_uint32), ("fmt", C.c_int32), ("chosen", C.c_uint32),
                 ("num_variants", C.c_uint32), ("occupancy", C.c_uint32 * 4), ("ms", C.c_float * 4)]
@@ -106,6 +125,19 @@ _SIGS = {
     "piv_analyzer_synchronize": ([C.c_void_p], C.c_int),
     "piv_profile_enable": ([C.c_void_p, C.c_int], C.c_int),
     "piv_profile_read": ([C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)], C.c_int),
+    "piv_server_create": ([C.c_int, C.POINTER(piv_server_config), C.POINTER(C.c_void_p)], C.c_int),
+    "piv_server_destroy": ([C.c_void_p], None),
+    "piv_server_add_stream": ([C.c_void_p, C.POINTER(piv_config), C.c_uint32, C.c_uint32, C.c_int,
+                               C.c_void_p, C.POINTER(C.c_uint32)], C.c_int),
+    "piv_submit": ([C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_size_t,
+                    C.POINTER(C.c_uint64)], C.c_int),
+    "piv_submit_device": ([C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_size_t,
+                           C.POINTER(C.c_uint64)], C.c_int),
+    "piv_poll": ([C.c_void_p, C.POINTER(piv_result), C.c_uint32, C.POINTER(C.c_uint32), C.c_int],
+                 C.c_int),
+    "piv_result_release": ([C.c_void_p, C.c_uint64], C.c_int),
+    "piv_server_pause": ([C.c_void_p, C.c_int], C.c_int),
+    "piv_server_stats_read": ([C.c_void_p, C.POINTER(piv_server_stats)], C.c_int),
     "piv_kernel_variant_count": ([C.c_uint32], C.c_uint32),
     "piv_kernel_variant_set": ([C.c_void_p, C.c_uint32, C.c_int, C.c_uint32], C.c_int),
     "piv_autotune": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,

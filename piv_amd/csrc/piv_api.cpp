@@ -5,27 +5,11 @@
 // in the call, then one k_validate launch validates every pair's field; the validated field
 // stays in HBM as the next pass's predictor.  All per-pair descriptors of a call are
 // uploaded with a single host-to-device copy before the first launch.
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <cmath>
-#include <cstdarg>
-#include <cstdio>
-#include <cstring>
-#include <limits>
-#include <map>
-#include <mutex>
-#include <string>
-#include <vector>
-
-#include "../../include/piv.h"
-#include "piv_device.hpp"
-#include "piv_kernels.hpp"
-#include "variants.hpp"
+#include "piv_internal.hpp"
 
 using namespace piv;
 
-namespace {
+namespace pivi {
 
 thread_local std::string g_err;
 
@@ -39,70 +23,6 @@ piv_status fail(piv_status st, const char *fmt, ...)
     g_err = buf;
     return st;
 }
-
-#define HIP_TRY(expr)                                                                         \
-    do {                                                                                      \
-        hipError_t e_ = (expr);                                                               \
-        if (e_ != hipSuccess)                                                                 \
-            return fail(e_ == hipErrorOutOfMemory ? PIV_ERR_OOM : PIV_ERR_DEVICE, "%s: %s (%s:%d)", \
-                        #expr, hipGetErrorString(e_), __FILE__, __LINE__);                    \
-    } while (0)
-
-#define PIV_TRY(expr)                                                                         \
-    do {                                                                                      \
-        piv_status s_ = (expr);                                                               \
-        if (s_ != PIV_OK) return s_;                                                          \
-    } while (0)
-
-// Device buffer that grows on demand (contents not preserved).
-struct DevBuf {
-    void *p = nullptr;
-    size_t cap = 0;
-    piv_status reserve(size_t bytes)
-    {
-        if (bytes <= cap) return PIV_OK;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-        size_t want = bytes < 256 ? 256 : bytes;
-        HIP_TRY(hipMalloc(&p, want));
-        cap = want;
-        return PIV_OK;
-    }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-    template <class T> T *as() const { return static_cast<T *>(p); }
-};
-
-struct HostBuf {
-    void *p = nullptr;
-    size_t cap = 0;
-    piv_status reserve(size_t bytes)
-    {
-        if (bytes <= cap) return PIV_OK;
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        cap = 0;
-        size_t want = bytes < 4096 ? 4096 : bytes;
-        HIP_TRY(hipHostMalloc(&p, want, hipHostMallocDefault));
-        cap = want;
-        return PIV_OK;
-    }
-    void release()
-    {
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-};
-
-size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
-
-int bytes_per_px(int fmt) { return fmt == FMT_U8 ? 1 : (fmt == FMT_U16 ? 2 : 4); }
 
 // grid.rs:20-44
 piv_status grid_new(uint32_t h, uint32_t w, uint32_t window, float overlap, GridDev &g)
@@ -149,107 +69,11 @@ int gauss_taps(float sigma, std::vector<float> &k)
     return r;
 }
 
-}  // namespace
+}  // namespace pivi
 
-struct piv_frame {
-    int device = 0;
-    int H = 0, W = 0;
-    int fmt = FMT_F32;         // format the correlation kernel reads `img` in
-    const void *img = nullptr; // device
-    int pitch = 0;             // pixels
-    DevBuf raw;                // owned copy of the input pixels (host input)
-    DevBuf image;              // owned conditioned f32 image
-    DevBuf base;               // owned background-subtracted base (f32), when materialised
-    DevBuf sat;                // owned f64 SAT of base (sum, then sum of squares)
-    int64_t cells = 0;
-    bool has_sat = false;
-    int in_fmt = FMT_F32;      // original sample format
-    const void *raw_ptr = nullptr;  // device pointer of the original samples
-    int raw_pitch = 0;
-    bool has_bg = false;
-    ~piv_frame()
-    {
-        raw.release();
-        image.release();
-        base.release();
-        sat.release();
-    }
-};
+using namespace pivi;
 
-struct piv_background {
-    piv_analyzer *an = nullptr;
-    int mode = 0;
-    int H = 0, W = 0;
-    uint64_t count = 0;
-    bool finished = false;
-    DevBuf acc;   // estimate [H*W] f32
-    DevBuf stage; // upload of host samples
-    ~piv_background()
-    {
-        acc.release();
-        stage.release();
-    }
-};
-
-struct piv_analyzer {
-    piv_config cfg{};
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::mutex mu;
-    std::vector<int> passes;
-    std::vector<float> taps;
-    int taps_r = 0;
-    DevBuf d_taps;
-    // workspace
-    DevBuf fields;       // per-pass raw + validated fields of the current call
-    DevBuf desc;         // descriptors
-    HostBuf h_desc;
-    DevBuf summaries;    // u32 [npass][npairs][5]
-    HostBuf h_summaries;
-    DevBuf prep_tmp, prep_tmp2, prep_sat;  // conditioning scratch
-    DevBuf stage_in, stage_out;            // stage-level entry points
-    // kernel timing (piv_profile_*)
-    struct ProfRec {
-        std::string name;
-        int window;
-        double bytes;
-        uint64_t units;
-        hipEvent_t e0, e1;
-    };
-    // correlation kernel variant (variants.hpp) per (window size, sample format), set by
-    // piv_autotune; absent means variant 0
-    std::map<std::pair<int, int>, int> variants;
-    // replay cache: the last call's descriptor bytes and the graph of its launch sequence
-    std::vector<char> last_sig;
-    hipGraphExec_t graph_exec = nullptr;
-    bool prof_on = false;
-    std::vector<ProfRec> prof_pending;
-    std::vector<hipEvent_t> prof_pool;
-    std::map<std::pair<std::string, int>, piv_kernel_stat> prof_stats;
-    ~piv_analyzer()
-    {
-        for (auto &r : prof_pending) {
-            (void)hipEventDestroy(r.e0);
-            (void)hipEventDestroy(r.e1);
-        }
-        for (auto e : prof_pool) (void)hipEventDestroy(e);
-        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-        fields.release();
-        desc.release();
-        h_desc.release();
-        summaries.release();
-        h_summaries.release();
-        prep_tmp.release();
-        prep_tmp2.release();
-        prep_sat.release();
-        stage_in.release();
-        stage_out.release();
-        d_taps.release();
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-};
-
-namespace {
+namespace pivi {
 
 int variant_of(const piv_analyzer *a, int n, int fmt)
 {
@@ -404,12 +228,6 @@ piv_status prepare_frame(piv_analyzer *a, piv_frame *f, const float *bg_dev)
     f->pitch = W;
     return PIV_OK;
 }
-
-struct PassPlan {
-    GridDev grid;
-    int n;
-    size_t cells;  // per pair
-};
 
 piv_status plan_passes(piv_analyzer *a, int H, int W, std::vector<PassPlan> &plans)
 {
@@ -724,7 +542,7 @@ piv_status check_pixels_args(const void *pixels, int fmt, uint32_t h, uint32_t w
     return PIV_OK;
 }
 
-}  // namespace
+}  // namespace pivi
 
 // =========================================================================================
 // C ABI
