@@ -133,6 +133,13 @@ piv_status piv_frame_prepare(piv_analyzer *a, const void *pixels, piv_pixel_form
 piv_status piv_frame_prepare_device(piv_analyzer *a, const void *pixels, piv_pixel_format fmt,
                                     uint32_t height, uint32_t width, size_t row_stride,
                                     const float *background, piv_frame **out);
+/* `count` frames at pixels + i * frame_stride bytes (device memory, kept unchanged while the
+ * frames live), prepared together: one launch per conditioning kernel for the whole set where
+ * the configuration allows.  out[i] receives frame i (release each with piv_frame_release). */
+piv_status piv_frame_prepare_device_batch(piv_analyzer *a, const void *pixels,
+                                          piv_pixel_format fmt, uint32_t count, uint32_t height,
+                                          uint32_t width, size_t row_stride, size_t frame_stride,
+                                          const float *background, piv_frame **out);
 void piv_frame_release(piv_frame *f);
 /* PreparedFrame's public `image` (and the background-subtracted base behind `stats`), as
  * host float [h*w]; either pointer may be NULL. */

@@ -446,6 +446,39 @@ class PreparedFrame:
         f._keep = (arr, background)
         return f
 
+    @staticmethod
+    def from_device_batch(an: PivAnalyzer, arr, background=None):
+        """Wrap a device-resident stack of frames [count, h, w] (DeviceArray or row-major
+        torch tensor), prepared together (one launch per conditioning kernel where the
+        configuration allows).  Returns a list of frames; keep `arr` unchanged meanwhile."""
+        if isinstance(arr, DeviceArray):
+            if len(arr.shape) != 3 or arr.dtype not in _FMT:
+                raise InvalidArgument("expected a 3-D uint8/uint16/float32 DeviceArray")
+            fmt, shape, ptr_ = _FMT[arr.dtype], arr.shape, arr.device_ptr
+            stride = shape[2] * arr.dtype.itemsize
+            fstride = shape[1] * stride
+        else:
+            name = str(arr.dtype).replace("torch.", "")
+            fmt = {"uint8": L.PIXEL_U8, "uint16": L.PIXEL_U16, "int16": L.PIXEL_U16,
+                   "float32": L.PIXEL_F32}[name]
+            if arr.dim() != 3 or arr.stride(2) != 1:
+                raise InvalidArgument("expected a 3-D device tensor with unit column stride")
+            shape, ptr_ = tuple(arr.shape), arr.data_ptr()
+            stride = arr.stride(1) * arr.element_size()
+            fstride = arr.stride(0) * arr.element_size()
+        n = shape[0]
+        hs = (C.c_void_p * n)()
+        _check(an._lib.piv_frame_prepare_device_batch(
+            an._h, C.c_void_p(ptr_), fmt, n, shape[1], shape[2], stride, fstride,
+            None if background is None else L.ptr(background), hs))
+        out = []
+        for i in range(n):
+            f = PreparedFrame()
+            f._h, f._lib, f.shape, f._owner = C.c_void_p(hs[i]), an._lib, tuple(shape[1:]), an
+            f._keep = (arr, background)
+            out.append(f)
+        return out
+
     @property
     def height(self):
         return self.shape[0]

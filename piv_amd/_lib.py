@@ -96,6 +96,9 @@ _SIGS = {
     "piv_output_grid": ([C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(piv_grid)], C.c_int),
     "piv_frame_prepare": ([C.c_void_p, C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, C.c_size_t,
                            C.c_void_p, C.POINTER(C.c_void_p)], C.c_int),
+    "piv_frame_prepare_device_batch": ([C.c_void_p, C.c_void_p, C.c_int, C.c_uint32, C.c_uint32,
+                                        C.c_uint32, C.c_size_t, C.c_size_t, C.c_void_p,
+                                        C.c_void_p], C.c_int),
     "piv_frame_prepare_device": ([C.c_void_p, C.c_void_p, C.c_int, C.c_uint32, C.c_uint32,
                                   C.c_size_t, C.c_void_p, C.POINTER(C.c_void_p)], C.c_int),
     "piv_frame_release": ([C.c_void_p], None),

@@ -187,12 +187,13 @@ piv_status prepare_frame(piv_analyzer *a, piv_frame *f, const float *bg_dev)
     hipStream_t st = a->stream;
     if (cond || mask) {
         PIV_TRY(f->base.reserve(n * sizeof(float)));
-        HIP_TRY(launch_base(f->in_fmt, f->raw_ptr, f->raw_pitch, bg_dev, H, W, f->base.as<float>(), st));
+        f->base_ptr = f->base.as<float>();
+        HIP_TRY(launch_base(f->in_fmt, f->raw_ptr, f->raw_pitch, bg_dev, H, W, f->base_ptr, st));
     }
     if (mask) {
         PIV_TRY(f->sat.reserve((size_t)f->cells * 2 * sizeof(double)));
-        HIP_TRY(launch_sat(f->base.as<float>(), H, W, f->sat.as<double>(),
-                           f->sat.as<double>() + f->cells, st));
+        f->sat_ptr = f->sat.as<double>();
+        HIP_TRY(launch_sat(f->base_ptr, H, W, f->sat.as<double>(), f->sat.as<double>() + f->cells, st));
         f->has_sat = true;
     }
     if (!cond) {
@@ -202,7 +203,7 @@ piv_status prepare_frame(piv_analyzer *a, piv_frame *f, const float *bg_dev)
         return PIV_OK;
     }
     PIV_TRY(f->image.reserve(n * sizeof(float)));
-    const float *cur = f->base.as<float>();
+    const float *cur = f->base_ptr;
     if (c.smooth_sigma > 0.0f) {
         PIV_TRY(a->prep_tmp.reserve(n * sizeof(float)));
         float *dst = c.normalize_radius > 0 ? nullptr : f->image.as<float>();
@@ -226,6 +227,75 @@ piv_status prepare_frame(piv_analyzer *a, piv_frame *f, const float *bg_dev)
     f->fmt = FMT_F32;
     f->img = f->image.p;
     f->pitch = W;
+    return PIV_OK;
+}
+
+// Conditioning of `count` frames of one size and format (frames raw_frame_bytes apart): the
+// base image and SAT of every frame come from one launch per kernel into storage the frames
+// share.  Blur / local normalisation configurations condition frame by frame.
+piv_status prepare_frames_batch(piv_analyzer *a, piv_frame *const *fs, int count,
+                                const float *bg_dev, long raw_frame_bytes)
+{
+    if (count <= 0) return PIV_OK;
+    const piv_config &c = a->cfg;
+    piv_frame *f0 = fs[0];
+    const bool cond = c.smooth_sigma > 0.0f || c.normalize_radius > 0 || bg_dev != nullptr;
+    const bool mask = mask_needed(c, f0->in_fmt, bg_dev != nullptr);
+    if (cond) {
+        for (int i = 0; i < count; ++i) PIV_TRY(prepare_frame(a, fs[i], bg_dev));
+        return PIV_OK;
+    }
+    const int H = f0->H, W = f0->W;
+    const size_t n = (size_t)H * W;
+    const int64_t cells = (int64_t)(H + 1) * (W + 1);
+    std::shared_ptr<DevBuf> store;
+    if (mask && count >= 8 && sat_frames_supported(W)) {
+        // one pass over the raw samples per frame, one wave per frame; no base image is kept
+        // (piv_frame_download recomputes it)
+        store.reset(new DevBuf, [](DevBuf *b) {
+            b->release();
+            delete b;
+        });
+        PIV_TRY(store->reserve((size_t)count * cells * 2 * sizeof(double)));
+        double *sat = store->as<double>();
+        HIP_TRY(launch_sat_frames(f0->in_fmt, f0->raw_ptr, f0->raw_pitch, raw_frame_bytes, H, W, sat,
+                                  2 * cells, count, a->stream));
+        for (int i = 0; i < count; ++i) {
+            piv_frame *f = fs[i];
+            f->base_ptr = nullptr;
+            f->sat_ptr = sat + (size_t)i * 2 * cells;
+            f->has_sat = true;
+            f->batch = store;
+        }
+    } else if (mask) {
+        store.reset(new DevBuf, [](DevBuf *b) {
+            b->release();
+            delete b;
+        });
+        const size_t base_bytes = align_up((size_t)count * n * sizeof(float), 256);
+        PIV_TRY(store->reserve(base_bytes + (size_t)count * cells * 2 * sizeof(double)));
+        float *base = store->as<float>();
+        double *sat = reinterpret_cast<double *>(store->as<char>() + base_bytes);
+        hipStream_t st = a->stream;
+        HIP_TRY(launch_base(f0->in_fmt, f0->raw_ptr, f0->raw_pitch, nullptr, H, W, base, st, count,
+                            raw_frame_bytes));
+        HIP_TRY(launch_sat(base, H, W, sat, sat + cells, st, count, 2 * cells));
+        for (int i = 0; i < count; ++i) {
+            piv_frame *f = fs[i];
+            f->base_ptr = base + (size_t)i * n;
+            f->sat_ptr = sat + (size_t)i * 2 * cells;
+            f->has_sat = true;
+            f->batch = store;
+        }
+    }
+    for (int i = 0; i < count; ++i) {
+        piv_frame *f = fs[i];
+        f->has_bg = false;
+        f->cells = cells;
+        f->fmt = f->in_fmt;
+        f->img = f->raw_ptr;
+        f->pitch = f->raw_pitch;
+    }
     return PIV_OK;
 }
 
@@ -362,8 +432,8 @@ piv_status run_batch(piv_analyzer *a, const piv_frame *const *fa, const piv_fram
             d.img_a = A->img;
             d.img_b = B->img;
             d.use_mask = (A->has_sat && B->has_sat) ? 1 : 0;
-            d.sat_a = A->has_sat ? A->sat.as<double>() : nullptr;
-            d.sat_b = B->has_sat ? B->sat.as<double>() : nullptr;
+            d.sat_a = A->has_sat ? A->sat_ptr : nullptr;
+            d.sat_b = B->has_sat ? B->sat_ptr : nullptr;
             d.sat_cells = A->cells;
             if (i > 0) {
                 d.prev_dx = val[i - 1].dx + (size_t)k * plans[i - 1].cells;
@@ -709,6 +779,45 @@ piv_status piv_frame_prepare_device(piv_analyzer *a, const void *pixels, piv_pix
     return PIV_OK;
 }
 
+piv_status piv_frame_prepare_device_batch(piv_analyzer *a, const void *pixels,
+                                          piv_pixel_format fmt, uint32_t count, uint32_t height,
+                                          uint32_t width, size_t row_stride, size_t frame_stride,
+                                          const float *background, piv_frame **out)
+{
+    if (!a || !out) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+    if (count == 0) return PIV_OK;
+    PIV_TRY(check_pixels_args(pixels, fmt, height, width, row_stride));
+    if (frame_stride < row_stride * height)
+        return fail(PIV_ERR_INVALID_ARGUMENT, "frame_stride %zu < %zu", frame_stride, row_stride * height);
+    std::lock_guard<std::mutex> lk(a->mu);
+    PIV_TRY(set_device(a->device));
+    std::vector<piv_frame *> fs(count, nullptr);
+    piv_status s = PIV_OK;
+    for (uint32_t i = 0; i < count && s == PIV_OK; ++i) {
+        fs[i] = new (std::nothrow) piv_frame();
+        if (!fs[i]) {
+            s = fail(PIV_ERR_OOM, "host allocation failed");
+            break;
+        }
+        piv_frame *f = fs[i];
+        f->device = a->device;
+        f->H = (int)height;
+        f->W = (int)width;
+        f->in_fmt = (int)fmt;
+        f->raw_ptr = static_cast<const char *>(pixels) + (size_t)i * frame_stride;
+        f->raw_pitch = (int)(row_stride / bytes_per_px((int)fmt));
+    }
+    if (s == PIV_OK) s = prepare_frames_batch(a, fs.data(), (int)count, background, (long)frame_stride);
+    if (s == PIV_OK && hipStreamSynchronize(a->stream) != hipSuccess)
+        s = fail(PIV_ERR_DEVICE, "frame preparation failed");
+    if (s != PIV_OK) {
+        for (piv_frame *f : fs) delete f;
+        return s;
+    }
+    for (uint32_t i = 0; i < count; ++i) out[i] = fs[i];
+    return PIV_OK;
+}
+
 void piv_frame_release(piv_frame *f)
 {
     if (!f) return;
@@ -740,8 +849,8 @@ piv_status piv_frame_download(const piv_frame *f, float *image, float *base)
         }
     }
     if (base) {
-        if (f->base.p) {
-            HIP_TRY(hipMemcpy(base, f->base.p, n * sizeof(float), hipMemcpyDeviceToHost));
+        if (f->base_ptr) {
+            HIP_TRY(hipMemcpy(base, f->base_ptr, n * sizeof(float), hipMemcpyDeviceToHost));
         } else {
             float *p = nullptr;
             PIV_TRY(normalized(&p));
@@ -946,8 +1055,8 @@ piv_status piv_correlate_windows(piv_analyzer *a, const piv_frame *fa, const piv
     d.img_a = fa->img;
     d.img_b = fb->img;
     d.use_mask = (fa->has_sat && fb->has_sat) ? 1 : 0;
-    d.sat_a = fa->has_sat ? fa->sat.as<double>() : nullptr;
-    d.sat_b = fb->has_sat ? fb->sat.as<double>() : nullptr;
+    d.sat_a = fa->has_sat ? fa->sat_ptr : nullptr;
+    d.sat_b = fb->has_sat ? fb->sat_ptr : nullptr;
     d.sat_cells = fa->cells;
     d.origins = d_org;
     d.origins_given = 1;

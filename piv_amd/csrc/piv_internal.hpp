@@ -10,6 +10,7 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -112,6 +113,9 @@ struct piv_frame {
     DevBuf sat;                // owned f64 SAT of base (sum, then sum of squares)
     int64_t cells = 0;
     bool has_sat = false;
+    float *base_ptr = nullptr;        // background-subtracted base: own `base` or batch storage
+    const double *sat_ptr = nullptr;  // SAT (sum, then sum of squares): own `sat` or batch storage
+    std::shared_ptr<DevBuf> batch;    // storage shared by the frames of one batch preparation
     int in_fmt = FMT_F32;      // original sample format
     const void *raw_ptr = nullptr;  // device pointer of the original samples
     int raw_pitch = 0;
@@ -211,6 +215,8 @@ piv_status set_device(int dev);
 int variant_of(const piv_analyzer *a, int n, int fmt);
 piv_status check_config(const piv_config &c);
 piv_status prepare_frame(piv_analyzer *a, piv_frame *f, const float *bg_dev);
+piv_status prepare_frames_batch(piv_analyzer *a, piv_frame *const *fs, int count,
+                                const float *bg_dev, long raw_frame_bytes);
 piv_status plan_passes(piv_analyzer *a, int H, int W, std::vector<PassPlan> &plans);
 void to_summary(const uint32_t *c, piv_summary *s);
 piv_status frame_from_device(piv_analyzer *a, const void *pixels, int fmt, uint32_t h,

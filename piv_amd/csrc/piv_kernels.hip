@@ -305,11 +305,14 @@ __global__ void k_zero_u32(uint32_t *__restrict__ p, uint32_t n)
 
 // base = normalise(raw) [- background, clamped at 0]
 template <int FMT>
-__global__ void k_base(const void *__restrict__ raw, int pitch, const float *__restrict__ bg,
-                       int H, int W, float *__restrict__ out)
+__global__ void k_base(const void *__restrict__ raw, int pitch, long raw_frame_bytes,
+                       const float *__restrict__ bg, int H, int W, float *__restrict__ out)
 {
+    // blockIdx.y: frame of a batch (frames raw_frame_bytes apart, outputs H*W apart)
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (long)H * W) return;
+    raw = static_cast<const char *>(raw) + (long)blockIdx.y * raw_frame_bytes;
+    out += (long)blockIdx.y * H * W;
     const int y = (int)(i / W), x = (int)(i - (long)y * W);
     float v = load_pixel<FMT>(raw, (long)y * pitch + x);
     if (bg) {
@@ -319,45 +322,212 @@ __global__ void k_base(const void *__restrict__ raw, int pitch, const float *__r
     out[i] = v;
 }
 
-// SAT, step 1: running row sums in f64 (one row per thread), written at (y+1, x+1)
-__global__ void k_sat_rows(const float *__restrict__ img, int H, int W, double *__restrict__ sum,
-                           double *__restrict__ sq)
+// SAT (preprocess.rs:84-110), step 1: running row sums of v and v^2 in f64, in row order
+// (one row per lane, the reference's summation order), written at (y+1, x+1).  A wave owns 64
+// rows: tiles of 64 rows x 32 columns are loaded and stored coalesced through LDS and each
+// lane scans its row of the tile from LDS.
+constexpr int SAT_ROWS = 64, SAT_TW = 32;
+
+__global__ __launch_bounds__(SAT_ROWS) void k_sat_rows(const float *__restrict__ img, int H, int W,
+                                                       double *__restrict__ sum,
+                                                       double *__restrict__ sq, long sat_stride)
 {
-    const int y = blockIdx.x * blockDim.x + threadIdx.x;
+    // blockIdx.y: frame of a batch (images H*W apart, tables sat_stride apart)
+    img += (long)blockIdx.y * H * W;
+    sum += (long)blockIdx.y * sat_stride;
+    sq += (long)blockIdx.y * sat_stride;
+    __shared__ float tin[SAT_ROWS][SAT_TW + 1];
+    __shared__ double ts[SAT_ROWS][SAT_TW + 1], tq[SAT_ROWS][SAT_TW + 1];
+    const int lane = threadIdx.x, y0 = blockIdx.x * SAT_ROWS;
     const long s = (long)W + 1;
-    if (y == 0) {
-        for (int x = 0; x <= W; ++x) {
+    if (blockIdx.x == 0)
+        for (int x = lane; x <= W; x += SAT_ROWS) {
             sum[x] = 0.0;
             sq[x] = 0.0;
         }
+    if (y0 + lane < H) {
+        sum[(long)(y0 + lane + 1) * s] = 0.0;
+        sq[(long)(y0 + lane + 1) * s] = 0.0;
     }
-    if (y >= H) return;
+    const int sub = lane >> 5, c = lane & 31;  // tile transfers: 2 rows x 32 columns per step
     double rs = 0.0, rq = 0.0;
-    const float *row = img + (long)y * W;
-    double *ps = sum + (long)(y + 1) * s, *pq = sq + (long)(y + 1) * s;
-    ps[0] = 0.0;
-    pq[0] = 0.0;
-    for (int x = 0; x < W; ++x) {
-        const double v = (double)row[x];
-        rs += v;
-        rq += v * v;
-        ps[x + 1] = rs;
-        pq[x + 1] = rq;
+    for (int x0 = 0; x0 < W; x0 += SAT_TW) {
+        const int x = x0 + c;
+        for (int i = 0; i < SAT_ROWS / 2; ++i) {
+            const int r = 2 * i + sub, y = y0 + r;
+            tin[r][c] = (y < H && x < W) ? img[(long)y * W + x] : 0.0f;
+        }
+        __syncthreads();
+        // the whole tile row first (the LDS reads overlap), then the two dependent chains;
+        // columns past the frame edge hold 0, and x + 0 = x leaves the sums unchanged
+        float v[SAT_TW];
+#pragma unroll
+        for (int k = 0; k < SAT_TW; ++k) v[k] = tin[lane][k];
+#pragma unroll
+        for (int k = 0; k < SAT_TW; ++k) {
+            const double d = (double)v[k];
+            rs += d;
+            rq += d * d;
+            ts[lane][k] = rs;
+            tq[lane][k] = rq;
+        }
+        __syncthreads();
+        if (x < W)
+            for (int i = 0; i < SAT_ROWS / 2; ++i) {
+                const int r = 2 * i + sub, y = y0 + r;
+                if (y < H) {
+                    sum[(long)(y + 1) * s + x + 1] = ts[r][c];
+                    sq[(long)(y + 1) * s + x + 1] = tq[r][c];
+                }
+            }
+        __syncthreads();
     }
 }
 
-// SAT, step 2: t[y+1][x] = t[y][x] + rowsum[y][x], down each column (one column per thread)
-__global__ void k_sat_cols(int H, int W, double *__restrict__ sum, double *__restrict__ sq)
+// SAT, step 2: t[y+1][x] = t[y][x] + rowsum[y][x], down each column in row order (one column
+// per thread, coalesced across the wave).  Software-pipelined: the loads of the next 16 rows
+// are issued before the dependent adds and stores of the current 16.
+constexpr int SAT_CH = 16;
+
+__device__ __forceinline__ void sat_col_chunk(double *__restrict__ sum, double *__restrict__ sq,
+                                              long idx, long s, const double (&a)[SAT_CH],
+                                              const double (&b)[SAT_CH], double &cs, double &cq)
 {
+#pragma unroll
+    for (int k = 0; k < SAT_CH; ++k) {
+        cs = cs + a[k];
+        cq = cq + b[k];
+        sum[idx + k * s] = cs;
+        sq[idx + k * s] = cq;
+    }
+}
+
+__global__ void k_sat_cols(int H, int W, double *__restrict__ sum, double *__restrict__ sq,
+                           long sat_stride)
+{
+    sum += (long)blockIdx.y * sat_stride;
+    sq += (long)blockIdx.y * sat_stride;
     const int x = blockIdx.x * blockDim.x + threadIdx.x + 1;
     if (x > W) return;
     const long s = (long)W + 1;
     double cs = 0.0, cq = 0.0;
-    for (int y = 1; y <= H; ++y) {
-        cs = cs + sum[(long)y * s + x];
-        cq = cq + sq[(long)y * s + x];
-        sum[(long)y * s + x] = cs;
-        sq[(long)y * s + x] = cq;
+    const int full = H / SAT_CH;  // whole chunks of rows 1 .. full * SAT_CH
+    double a0[SAT_CH], b0[SAT_CH], a1[SAT_CH], b1[SAT_CH];
+    long idx = s + x;
+    if (full > 0) {
+#pragma unroll
+        for (int k = 0; k < SAT_CH; ++k) {
+            a0[k] = sum[idx + k * s];
+            b0[k] = sq[idx + k * s];
+        }
+    }
+    for (int c = 0; c < full; c += 2) {
+        const long nidx = idx + SAT_CH * s;
+        if (c + 1 < full) {
+#pragma unroll
+            for (int k = 0; k < SAT_CH; ++k) {
+                a1[k] = sum[nidx + k * s];
+                b1[k] = sq[nidx + k * s];
+            }
+        }
+        sat_col_chunk(sum, sq, idx, s, a0, b0, cs, cq);
+        if (c + 1 >= full) {
+            idx = nidx;
+            break;
+        }
+        const long n2 = nidx + SAT_CH * s;
+        if (c + 2 < full) {
+#pragma unroll
+            for (int k = 0; k < SAT_CH; ++k) {
+                a0[k] = sum[n2 + k * s];
+                b0[k] = sq[n2 + k * s];
+            }
+        }
+        sat_col_chunk(sum, sq, nidx, s, a1, b1, cs, cq);
+        idx = n2;
+    }
+    for (int y = full * SAT_CH + 1; y <= H; ++y, idx += s) {
+        cs = cs + sum[idx];
+        cq = cq + sq[idx];
+        sum[idx] = cs;
+        sq[idx] = cq;
+    }
+}
+
+// SAT of a batch of raw frames in one pass (no background, no blur): one wave per frame walks
+// down bands of 64 rows.  Per 32-column tile: each lane scans its row of the tile (the running
+// row sum continues across tiles), then each lane carries one column (lanes 0-31: sums,
+// 32-63: squares) down the band from the previous band's last row, in registers.  Same
+// additions in the same order as k_sat_rows + k_sat_cols; the raw samples are read once (the
+// next tile's loads are in flight while the current tile is scanned) and the table is
+// written once.  A lane reads the column carry back from the rows it wrote itself.
+constexpr int SATF_MAXW = 1 << 15;
+
+template <int FMT>
+__global__ __launch_bounds__(64) void k_sat_frames(const void *__restrict__ raw, int pitch,
+                                                   long raw_frame_bytes, int H, int W,
+                                                   double *__restrict__ sat_all, long sat_stride)
+{
+    __shared__ float tin[SAT_ROWS][SAT_TW + 1];
+    __shared__ double tv[2][SAT_ROWS][SAT_TW + 1];
+    const int lane = threadIdx.x;
+    raw = static_cast<const char *>(raw) + (long)blockIdx.x * raw_frame_bytes;
+    const long s = (long)W + 1, cells = s * ((long)H + 1);
+    const int sub = lane >> 5, c = lane & 31;
+    double *tab = sat_all + (long)blockIdx.x * sat_stride + sub * cells;  // this lane's table
+    for (int x = c; x <= W; x += 32) tab[x] = 0.0;
+    const int ntile = (W + SAT_TW - 1) / SAT_TW;
+    float pre[SAT_ROWS / 2];
+    auto fetch = [&](int y0, int x0) {
+        const int x = x0 + c;
+#pragma unroll
+        for (int i = 0; i < SAT_ROWS / 2; ++i) {
+            const int y = y0 + 2 * i + sub;
+            pre[i] = (y < H && x < W) ? load_pixel<FMT>(raw, (long)y * pitch + x) : 0.0f;
+        }
+    };
+    fetch(0, 0);
+    for (int y0 = 0; y0 < H; y0 += SAT_ROWS) {
+        const int rows = min(SAT_ROWS, H - y0);
+        for (int r = c; r < rows; r += 32) tab[(long)(y0 + r + 1) * s] = 0.0;
+        double rs = 0.0, rq = 0.0;
+        for (int t = 0; t < ntile; ++t) {
+            const int x0 = t * SAT_TW, x = x0 + c;
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < SAT_ROWS / 2; ++i) tin[2 * i + sub][c] = pre[i];
+            __syncthreads();
+            // next tile (or the next band's first) while this one is scanned
+            if (t + 1 < ntile) fetch(y0, x0 + SAT_TW);
+            else if (y0 + SAT_ROWS < H) fetch(y0 + SAT_ROWS, 0);
+            {
+                float v[SAT_TW];
+#pragma unroll
+                for (int k = 0; k < SAT_TW; ++k) v[k] = tin[lane][k];
+#pragma unroll
+                for (int k = 0; k < SAT_TW; ++k) {
+                    const double d = (double)v[k];
+                    rs += d;
+                    rq += d * d;
+                    tv[0][lane][k] = rs;
+                    tv[1][lane][k] = rq;
+                }
+            }
+            __syncthreads();
+            if (x < W) {
+                double col[SAT_ROWS];
+#pragma unroll
+                for (int r = 0; r < SAT_ROWS; ++r) col[r] = tv[sub][r][c];
+                double cv = y0 == 0 ? 0.0 : tab[(long)y0 * s + x + 1];
+#pragma unroll
+                for (int r = 0; r < SAT_ROWS; ++r) {
+                    if (r < rows) {
+                        cv = cv + col[r];
+                        tab[(long)(y0 + r + 1) * s + x + 1] = cv;
+                    }
+                }
+            }
+        }
     }
 }
 
@@ -636,23 +806,41 @@ hipError_t launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t st)
 static inline unsigned nblk(long n, int b) { return (unsigned)((n + b - 1) / b); }
 
 hipError_t launch_base(int fmt, const void *raw, int pitch, const float *bg, int H, int W,
-                       float *out, hipStream_t st)
+                       float *out, hipStream_t st, int nframes, long raw_frame_bytes)
 {
     const long n = (long)H * W;
+    const dim3 grid(nblk(n, 256), (unsigned)nframes);
     switch (fmt) {
-    case FMT_U8: hipLaunchKernelGGL(k_base<FMT_U8>, dim3(nblk(n, 256)), dim3(256), 0, st, raw, pitch, bg, H, W, out); break;
-    case FMT_U16: hipLaunchKernelGGL(k_base<FMT_U16>, dim3(nblk(n, 256)), dim3(256), 0, st, raw, pitch, bg, H, W, out); break;
-    default: hipLaunchKernelGGL(k_base<FMT_F32>, dim3(nblk(n, 256)), dim3(256), 0, st, raw, pitch, bg, H, W, out); break;
+    case FMT_U8: hipLaunchKernelGGL(k_base<FMT_U8>, grid, dim3(256), 0, st, raw, pitch, raw_frame_bytes, bg, H, W, out); break;
+    case FMT_U16: hipLaunchKernelGGL(k_base<FMT_U16>, grid, dim3(256), 0, st, raw, pitch, raw_frame_bytes, bg, H, W, out); break;
+    default: hipLaunchKernelGGL(k_base<FMT_F32>, grid, dim3(256), 0, st, raw, pitch, raw_frame_bytes, bg, H, W, out); break;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_sat(const float *img, int H, int W, double *sum, double *sq, hipStream_t st)
+hipError_t launch_sat(const float *img, int H, int W, double *sum, double *sq, hipStream_t st,
+                      int nframes, long sat_stride)
 {
-    hipLaunchKernelGGL(k_sat_rows, dim3(nblk(H, 64)), dim3(64), 0, st, img, H, W, sum, sq);
+    hipLaunchKernelGGL(k_sat_rows, dim3(nblk(H, SAT_ROWS), (unsigned)nframes), dim3(SAT_ROWS), 0, st,
+                       img, H, W, sum, sq, sat_stride);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_sat_cols, dim3(nblk(W, 64)), dim3(64), 0, st, H, W, sum, sq);
+    hipLaunchKernelGGL(k_sat_cols, dim3(nblk(W, 64), (unsigned)nframes), dim3(64), 0, st, H, W, sum,
+                       sq, sat_stride);
+    return hipGetLastError();
+}
+
+bool sat_frames_supported(int W) { return W <= SATF_MAXW; }
+
+hipError_t launch_sat_frames(int fmt, const void *raw, int pitch, long raw_frame_bytes, int H,
+                             int W, double *sat, long sat_stride, int nframes, hipStream_t st)
+{
+    if (W > SATF_MAXW) return hipErrorInvalidValue;
+    switch (fmt) {
+    case FMT_U8: hipLaunchKernelGGL(k_sat_frames<FMT_U8>, dim3(nframes), dim3(64), 0, st, raw, pitch, raw_frame_bytes, H, W, sat, sat_stride); break;
+    case FMT_U16: hipLaunchKernelGGL(k_sat_frames<FMT_U16>, dim3(nframes), dim3(64), 0, st, raw, pitch, raw_frame_bytes, H, W, sat, sat_stride); break;
+    default: hipLaunchKernelGGL(k_sat_frames<FMT_F32>, dim3(nframes), dim3(64), 0, st, raw, pitch, raw_frame_bytes, H, W, sat, sat_stride); break;
+    }
     return hipGetLastError();
 }
 

@@ -21,9 +21,16 @@ hipError_t launch_pass(int n, int fmt, int variant, const PassPair *pairs, int n
 hipError_t launch_validate(const ValidatePair *jobs, int njobs, uint32_t max_cells,
                            hipStream_t st);
 hipError_t launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t st);
+// nframes > 1: a batch of frames (raw frames raw_frame_bytes apart, base images H*W apart,
+// tables sat_stride doubles apart), one launch per kernel
 hipError_t launch_base(int fmt, const void *raw, int pitch, const float *bg, int H, int W,
-                       float *out, hipStream_t st);
-hipError_t launch_sat(const float *img, int H, int W, double *sum, double *sq, hipStream_t st);
+                       float *out, hipStream_t st, int nframes = 1, long raw_frame_bytes = 0);
+hipError_t launch_sat(const float *img, int H, int W, double *sum, double *sq, hipStream_t st,
+                      int nframes = 1, long sat_stride = 0);
+// SAT of a batch of raw frames (no background) in one pass, one wave per frame
+bool sat_frames_supported(int W);
+hipError_t launch_sat_frames(int fmt, const void *raw, int pitch, long raw_frame_bytes, int H,
+                             int W, double *sat, long sat_stride, int nframes, hipStream_t st);
 hipError_t launch_blur(const float *in, int H, int W, const float *taps, int r, float *tmp,
                        float *out, hipStream_t st);
 hipError_t launch_local_norm(const float *in, int H, int W, int radius, float eps,

@@ -304,8 +304,8 @@ void build_batch(Lane &L, BatchPlan &bp)
                 d.img_a = A->img;
                 d.img_b = Bf->img;
                 d.use_mask = (A->has_sat && Bf->has_sat) ? 1 : 0;
-                d.sat_a = A->has_sat ? A->sat.as<double>() : nullptr;
-                d.sat_b = Bf->has_sat ? Bf->sat.as<double>() : nullptr;
+                d.sat_a = A->has_sat ? A->sat_ptr : nullptr;
+                d.sat_b = Bf->has_sat ? Bf->sat_ptr : nullptr;
                 d.sat_cells = A->cells;
                 if (si > 0) {
                     const size_t pc = st->plans[si - 1].cells;

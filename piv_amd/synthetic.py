@@ -162,3 +162,57 @@ def particle_frames(h, w, seed, density=0.02, diameter=2.5, amp=(150.0, 255.0), 
     fa = fa + np.float32(noise * amp_scale) * normal(seed ^ 0x5A5A, h * w).reshape(h, w)
     fb = fb + np.float32(noise * amp_scale) * normal(seed ^ 0xA5A5, h * w).reshape(h, w)
     return quantize(fa, bits, container), quantize(fb, bits, container)
+
+
+def swirl_field(h, w, t, period=64.0, vmax=5.0):
+    """Time-varying swirl at frame time t: a Lamb-Oseen vortex whose centre circles the image
+    centre (radius h/8, one turn per `period` frames) and whose strength oscillates between
+    0.5 vmax and vmax."""
+    ang = 2.0 * math.pi * t / period
+    cx = (w - 1) / 2.0 + (h / 8.0) * math.cos(ang)
+    cy = (h - 1) / 2.0 + (h / 8.0) * math.sin(ang)
+    rc = min(h, w) / 6.0
+    rpk = 1.1209 * rc
+    peak = (1.0 - math.exp(-(rpk / rc) ** 2)) / rpk
+    gam = vmax * (0.75 + 0.25 * math.sin(ang)) / peak
+
+    def f(x, y):
+        xd = x.astype(np.float64) - cx
+        yd = y.astype(np.float64) - cy
+        r2 = xd * xd + yd * yd
+        with np.errstate(invalid="ignore", divide="ignore"):
+            ut = np.where(r2 > 0, gam * (1.0 - np.exp(-r2 / rc ** 2)) / np.maximum(r2, 1e-30), 0.0)
+        return (-ut * yd).astype(np.float32), (ut * xd).astype(np.float32)
+
+    return f
+
+
+def video_frames(h, w, count, seed, density=0.015, diameter_range=(2.0, 3.0), noise=3.0,
+                 bits=12, container=np.uint16, period=64.0, vmax=5.0):
+    """A high-speed video: `count` frames of one particle population, each frame advected from
+    the previous one by swirl_field at that frame's time (particles leaving the view are
+    re-seeded on the far side, so the density stays constant).  Returns the frames and the
+    field of each consecutive pair (pair k: frame k -> k+1 moves by swirl_field(t = k))."""
+    margin = 20.0
+    area = (h + 2 * margin) * (w + 2 * margin)
+    n = int(density * area)
+    u = uniform(seed, 4 * n).reshape(n, 4)
+    x = u[:, 0] * np.float32(w + 2 * margin) - np.float32(margin)
+    y = u[:, 1] * np.float32(h + 2 * margin) - np.float32(margin)
+    d = np.float32(diameter_range[0]) + np.float32(diameter_range[1] - diameter_range[0]) * u[:, 2]
+    s = (d / np.float32(4.0)).astype(np.float32)
+    scale = np.float32((1 << bits) - 1) / np.float32(255.0)
+    a = (np.float32(150.0) + np.float32(105.0) * u[:, 3]) * scale
+    frames, fields = [], []
+    for k in range(count):
+        f = render(h, w, x, y, s, a)
+        f = f + np.float32(noise) * scale * normal(seed ^ (0x9E37 + k), h * w).reshape(h, w)
+        frames.append(quantize(f, bits, container))
+        fld = swirl_field(h, w, float(k), period, vmax)
+        fields.append(fld)
+        x, y = displace(x, y, fld)
+        wrapx = np.float32(w + 2 * margin)
+        wrapy = np.float32(h + 2 * margin)
+        x = ((x + np.float32(margin)) % wrapx - np.float32(margin)).astype(np.float32)
+        y = ((y + np.float32(margin)) % wrapy - np.float32(margin)).astype(np.float32)
+    return frames, fields[:-1]

@@ -292,6 +292,43 @@ def test_kernel_variants_bit_identical(piv, fmt):
         assert np.array_equal(bits(r.field.dx), bits(ref.field.dx))
 
 
+@pytest.mark.parametrize("dtype,pre,n", [(np.uint16, "raw", 5), (np.uint16, "raw", 12),
+                                         (np.uint8, "raw", 5), (np.uint8, "default", 5),
+                                         (np.float32, "raw", 9)])
+def test_batch_prepared_frames_equal_single(piv, dtype, pre, n):
+    """Frames prepared as one batch (one launch per conditioning kernel, shared storage; from
+    8 frames on, the one-pass table kernel) give the single-frame preparation's analysis
+    bitwise, masks included."""
+    h, w = 160, 224
+    frames, _ = S.video_frames(h, w, n, 17, bits=12 if dtype is np.uint16 else 8,
+                               container=np.uint8 if dtype is np.uint8 else np.uint16)
+    if dtype is np.float32:
+        frames = [f.astype(np.float32) / np.float32(4095) for f in frames]
+    prep = piv.PreprocessConfig.raw() if pre == "raw" else piv.PreprocessConfig()
+    # a mask near the frame's own mean and spread rejects about half of the windows, so the
+    # tables decide outcomes (u8 frames keep the default mask, which needs no tables)
+    mask = piv.MaskConfig()
+    if dtype is not np.uint8:
+        norm = frames[0].astype(np.float32) / (np.float32(65535) if dtype is np.uint16 else 1)
+        mask = piv.MaskConfig(float(np.float32(norm.mean())), float(np.float32(0.9 * norm.std())))
+    cfg = piv.PivConfig(windows=[48, 24], final_iterations=1, mask=mask)
+    an = piv.PivAnalyzer(cfg, prep)
+    stack = piv.DeviceArray.from_numpy(np.stack(frames))
+    fs = piv.PreparedFrame.from_device_batch(an, stack)
+    assert len(fs) == n
+    for k in range(n - 1):
+        r = an.analyze(fs[k], fs[k + 1])
+        ref = an.analyze(an.prepare(frames[k]), an.prepare(frames[k + 1]))
+        assert np.array_equal(bits(r.field.dx), bits(ref.field.dx)), k
+        assert np.array_equal(r.field.flags, ref.field.flags), k
+        assert r.summary == ref.summary
+        if dtype is not np.uint8:
+            assert 0 < r.summary.no_signal < r.field.flags.size, r.summary
+    img, base = fs[2].download()
+    rimg, rbase = an.prepare(frames[2]).download()
+    assert np.array_equal(bits(img), bits(rimg))
+
+
 def test_deterministic(piv):
     a, b = S.particle_frames(512, 512, 5, field=S.uniform_field(3.3, 1.1))
     an = piv.PivAnalyzer(piv.PivConfig(windows=[64, 32], final_iterations=2),
