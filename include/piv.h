@@ -288,6 +288,21 @@ piv_status piv_result_release(piv_server *s, uint64_t request_id);
 piv_status piv_server_pause(piv_server *s, int paused);
 piv_status piv_server_stats_read(piv_server *s, piv_server_stats *out);
 
+/* ---- correlation precision (no counterpart in the reference, which is FP32) ----------- */
+
+typedef enum {
+    PIV_PRECISION_FP32 = 0, /* FFT correlation in FP32: the reference's arithmetic */
+    PIV_PRECISION_FP16 = 1, /* DFT-matrix GEMMs on the matrix cores, fp16 operands */
+    PIV_PRECISION_BF16 = 2  /* the same with bf16 operands */
+} piv_precision;
+
+/* FP16 / BF16: passes with 16, 32 or 64 px windows compute the correlation as dense DFT-matrix
+ * GEMMs (16-bit operands, FP32 accumulation, windows scaled to unit RMS), fit the peak and
+ * accumulate the displacement in FP64; other window sizes keep the FP32 FFT.  Results are no
+ * longer bit-comparable with the reference: integer peaks can move on near-ties and the
+ * sub-pixel displacement carries the 16-bit rounding (see the DFT-GEMM accuracy tests). */
+piv_status piv_analyzer_set_precision(piv_analyzer *a, piv_precision p);
+
 /* ---- autotuning (no counterpart in the reference) -------------------------------------- */
 
 /* The correlation kernel of each window size has launch variants (occupancy targets of the

@@ -351,6 +351,13 @@ class PivAnalyzer:
                      total_ms=float(arr[i].total_ms), bytes=float(arr[i].bytes))
                 for i in range(cnt.value)]
 
+    def set_precision(self, precision):
+        """'fp32' (FFT, the reference's arithmetic), 'fp16' or 'bf16' (DFT-matrix GEMMs on the
+        matrix cores for 16/32/64 px windows, FP64 peak fit)."""
+        code = {"fp32": 0, "fp16": 1, "bf16": 2}[precision]
+        _check(self._lib.piv_analyzer_set_precision(self._h, code))
+        self.precision = precision
+
     def set_variant(self, window, dtype, variant):
         """Pin the correlation-kernel variant for (window, sample dtype)."""
         _check(self._lib.piv_kernel_variant_set(self._h, window, _FMT[np.dtype(dtype)], variant))

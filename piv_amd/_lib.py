@@ -141,6 +141,7 @@ _SIGS = {
     "piv_result_release": ([C.c_void_p, C.c_uint64], C.c_int),
     "piv_server_pause": ([C.c_void_p, C.c_int], C.c_int),
     "piv_server_stats_read": ([C.c_void_p, C.POINTER(piv_server_stats)], C.c_int),
+    "piv_analyzer_set_precision": ([C.c_void_p, C.c_int], C.c_int),
     "piv_kernel_variant_count": ([C.c_uint32], C.c_uint32),
     "piv_kernel_variant_set": ([C.c_void_p, C.c_uint32, C.c_int, C.c_uint32], C.c_int),
     "piv_autotune": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,

@@ -153,6 +153,242 @@ __device__ __forceinline__ double energy_floor()
     else return 1e-12;
 }
 
+// Window pair of one lane's column: raw samples bilinearly sampled at the window origins
+// (zeros for a masked window), with the approximate window means removed (the DC bin is
+// cleared exactly later, which is what the reference's mean removal amounts to).
+template <int N, int FMT>
+__device__ __forceinline__ void ingest(const PassPair &pp, bool sig, float ax, float ay, float bx,
+                                       float by, int lane, float *red, float (&zr)[N],
+                                       float (&zi)[N])
+{
+    // every window of the wave keeps its (N+1)^2 footprint inside the frame: no clamping
+    const bool fast = __all(!sig || (interior<N>(ax, ay, pp.W, pp.H) && interior<N>(bx, by, pp.W, pp.H)));
+    if (sig) {
+        if (fast) {
+            load_column_fast<N, FMT>(pp.img_a, pp.pitch, ax, ay, lane, zr);
+            load_column_fast<N, FMT>(pp.img_b, pp.pitch, bx, by, lane, zi);
+        } else {
+            load_column_raw<N, FMT>(pp.img_a, pp.pitch, pp.H, pp.W, ax, ay, lane, zr);
+            load_column_raw<N, FMT>(pp.img_b, pp.pitch, pp.H, pp.W, bx, by, lane, zi);
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < N; ++r) {
+            zr[r] = 0.0f;
+            zi[r] = 0.0f;
+        }
+    }
+    {
+        // Approximate means only reduce the dynamic range of the transform: the DC bin is
+        // cleared exactly in the cross spectrum, which is what the reference's mean removal
+        // amounts to.
+        float sa = 0.0f, sb = 0.0f;
+#pragma unroll
+        for (int r = 0; r < N; ++r) {
+            sa += zr[r];
+            sb += zi[r];
+        }
+        constexpr float inv = 1.0f / (float)(N * N);
+        const float ma = window_sum<N>(sa, red, lane) * inv;
+        const float mb = window_sum<N>(sb, red, lane) * inv;
+#pragma unroll
+        for (int r = 0; r < N; ++r) {
+            zr[r] -= ma;
+            zi[r] -= mb;
+        }
+    }
+}
+
+// find_peak (correlate.rs:215-303) on a window's correlation plane in LDS (row stride N+1),
+// lane = row: first row-major argmax, footprint half width, highest local maximum outside
+// it; lane 0 of the window writes the peak record.
+template <int N, int FMT>
+__device__ __forceinline__ void find_peak_plane(const float *pr, int lane, int tid, float *pkv,
+                                                int *pki, bool active, bool has_peak,
+                                                PeakRec *rec)
+{
+    using C = KCfg<N>;
+    constexpr int P = C::P, H2 = N / 2;
+    float zr[N];
+    // ---- find_peak (correlate.rs:215-303), lane = row -----------------------------------
+    // NaN can only reach the plane from non-finite float input; the integer formats take
+    // the comparison shortcuts that assume finite values.
+    constexpr bool MAYBE_NAN = (FMT == FMT_F32);
+    // (1) first maximum in row-major order (strict >): row scan, then across rows
+#pragma unroll
+    for (int x = 0; x < N; ++x) zr[x] = pr[lane * P + x];
+    float best = -INFINITY;
+    int bidx = 0;
+    {
+        float rb = -INFINITY;
+        int ri = 0;
+#pragma unroll
+        for (int x = 0; x < N; ++x) {
+            const bool gt = zr[x] > rb;
+            rb = gt ? zr[x] : rb;
+            ri = gt ? x : ri;
+        }
+        ri += lane * N;
+        if constexpr (C::IN_WAVE) {
+            // lexicographic (value desc, index asc) maximum: same as the first maximum
+#pragma unroll
+            for (int m = 1; m < N; m <<= 1) {
+                const float ov = __shfl_xor(rb, m, N);
+                const int oi = __shfl_xor(ri, m, N);
+                const bool take = (ov > rb) || (ov == rb && oi < ri);
+                rb = take ? ov : rb;
+                ri = take ? oi : ri;
+            }
+            best = rb;
+            bidx = ri;
+        } else {
+            pkv[lane] = rb;
+            pki[lane] = ri;
+            __syncthreads();
+            for (int r = 0; r < N; ++r) {
+                const float v = pkv[r];
+                if (v > best) {
+                    best = v;
+                    bidx = pki[r];
+                }
+            }
+        }
+        if (!(best > -INFINITY)) bidx = 0;  // no maximum at all: (0, 0) as the reference
+    }
+    const int pby = bidx / N, pbx = bidx - pby * N;
+    // (2) footprint: half width at half maximum along +-x, +-y (k < N/2 while the value at
+    //     distance k is > best / 2).  Lane l of the window reads the row of the peak at
+    //     distance l to the right and the column at distance l downwards; distances to the
+    //     left/up are lanes N - l.  The walk lengths are runs of set bits in ballots.
+    int hw = 0;
+    {
+        const float half_best = 0.5f * best;
+        const float hv = pr[pby * P + (pbx + lane) % N];
+        const float vv = pr[((pby + lane) % N) * P + pbx];
+        if constexpr (C::IN_WAVE) {
+            const int base = tid - lane;
+            const unsigned long long full = N == 64 ? ~0ull : ((1ull << N) - 1);
+            const unsigned long long hm = (__ballot(hv > half_best) >> base) & full;
+            const unsigned long long vm = (__ballot(vv > half_best) >> base) & full;
+            // forward: lanes 0, 1, 2, ...; backward: lanes 0, N-1, N-2, ...
+            auto run_fwd = [&](unsigned long long m) {
+                const unsigned long long inv = ~m & full;
+                return inv ? __builtin_ctzll(inv) : N;
+            };
+            auto run_bwd = [&](unsigned long long m) {
+                if (!(m & 1ull)) return 0;
+                // bits N-1, N-2, ... : count leading ones of the N-bit field below bit 0
+                const unsigned long long rest = (m >> 1) << (64 - (N - 1)) ;
+                const unsigned long long inv = ~rest;
+                const int lead = N - 1 > 0 ? (inv ? __builtin_clzll(inv) : 64) : 0;
+                return 1 + (lead < N - 1 ? lead : N - 1);
+            };
+            const int k0 = min(run_fwd(vm), H2), k1 = min(run_bwd(vm), H2);
+            const int k2 = min(run_fwd(hm), H2), k3 = min(run_bwd(hm), H2);
+            hw = max(max(k0, k1), max(k2, k3));
+        } else {
+            const int sy[4] = {1, N - 1, 0, 0};
+            const int sx[4] = {0, 0, 1, N - 1};
+            (void)hv;
+            (void)vv;
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+                int kk = 0;
+                while (kk < H2 && pr[((pby + kk * sy[d]) % N) * P + (pbx + kk * sx[d]) % N] > half_best)
+                    ++kk;
+                hw = kk > hw ? kk : hw;
+            }
+        }
+    }
+    int exclude = (int)ceilf(1.5f * (float)hw);
+    exclude = exclude < 2 ? 2 : exclude;
+    // (3) highest local maximum outside the footprint (all 8 neighbours <= v)
+    float second = -INFINITY;
+    {
+        float cand = -INFINITY;
+        const int ru = (lane + N - 1) % N, rd = (lane + 1) % N;
+        int dyc = abs(lane - pby);
+        dyc = dyc < N - dyc ? dyc : N - dyc;
+        const bool row_far = dyc > exclude;
+        // Columns of this row outside the footprint as a bit mask (N <= 64): every column
+        // when the row is far, else all but the circular run pbx - exclude .. pbx + exclude.
+        using Mask = std::conditional_t<(N > 32), unsigned long long, unsigned int>;
+        constexpr int MB = 8 * (int)sizeof(Mask);
+        constexpr Mask full = N == MB ? ~Mask(0) : ((Mask(1) << (N % MB)) - 1);
+        Mask allowed = full;
+        if constexpr (N <= 64) {
+            if (!row_far) {
+                const int len = 2 * exclude + 1;
+                if (len >= N) {
+                    allowed = 0;
+                } else {
+                    const Mask run = (Mask(1) << len) - 1;
+                    const int s = (pbx - exclude + N) % N;
+                    const Mask ex = s == 0 ? run : (Mask)(((run << s) | (run >> (N - s))) & full);
+                    allowed = full & ~ex;
+                }
+            }
+        }
+        // vertical max of (row above, row, row below) per column, sliding along x
+        float vm_prev = fmaxf(fmaxf(pr[ru * P + N - 1], zr[N - 1]), pr[rd * P + N - 1]);
+        float u1 = pr[ru * P + 0], d1 = pr[rd * P + 0];
+        float vm_cur = fmaxf(fmaxf(u1, zr[0]), d1);
+#pragma unroll
+        for (int x = 0; x < N; ++x) {
+            const int xn = x + 1 < N ? x + 1 : 0;
+            const float u2 = pr[ru * P + xn], d2 = pr[rd * P + xn];
+            const float vm_next = fmaxf(fmaxf(u2, zr[xn]), d2);
+            const float v = zr[x];
+            bool far;
+            if constexpr (N <= 64) {
+                far = (allowed >> x) & 1;
+            } else {
+                int dxc = abs(x - pbx);
+                dxc = dxc < N - dxc ? dxc : N - dxc;
+                far = row_far || dxc > exclude;
+            }
+            bool local;
+            if constexpr (MAYBE_NAN) {
+                const float m0 = zr[(x + N - 1) % N], m2 = zr[xn];
+                const float u0 = pr[ru * P + (x + N - 1) % N], d0 = pr[rd * P + (x + N - 1) % N];
+                local = u0 <= v && u1 <= v && u2 <= v && m0 <= v && m2 <= v && d0 <= v &&
+                        d1 <= v && d2 <= v;
+                if (local && far && v > cand) cand = v;
+            } else {
+                // finite values: max(cand, v) is the strict-> update of the reference
+                local = fmaxf(fmaxf(vm_prev, vm_next), fmaxf(u1, d1)) <= v;
+                cand = (local && far) ? fmaxf(cand, v) : cand;
+            }
+            vm_prev = vm_cur;
+            vm_cur = vm_next;
+            u1 = u2;
+            d1 = d2;
+        }
+        if constexpr (C::IN_WAVE) {
+#pragma unroll
+            for (int m = 1; m < N; m <<= 1) cand = fmaxf(cand, __shfl_xor(cand, m, N));
+            second = cand;
+        } else {
+            __syncthreads();
+            pkv[lane] = cand;
+            __syncthreads();
+            for (int r = 0; r < N; ++r) second = pkv[r] > second ? pkv[r] : second;
+        }
+    }
+    if (lane == 0 && active) {
+        PeakRec r;
+        r.best = best;
+        r.second = second;
+        r.mx = pr[pby * P + (pbx + N - 1) % N];
+        r.qx = pr[pby * P + (pbx + 1) % N];
+        r.my = pr[((pby + N - 1) % N) * P + pbx];
+        r.qy = pr[((pby + 1) % N) * P + pbx];
+        r.index = bidx;
+        r.has_peak = has_peak ? 1 : 0;
+        *rec = r;
+    }
+}
+
 template <int N, int FMT, int OCC>
 __global__ __launch_bounds__(KCfg<N>::BLOCK) __attribute__((amdgpu_waves_per_eu(OCC)))
 void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblocks)
@@ -200,42 +436,7 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
 
     // ---- fused ingest + mean removal --------------------------------------------------
     float zr[N], zi[N];
-    // every window of the wave keeps its (N+1)^2 footprint inside the frame: no clamping
-    const bool fast = __all(!sig || (interior<N>(ax, ay, pp.W, pp.H) && interior<N>(bx, by, pp.W, pp.H)));
-    if (sig) {
-        if (fast) {
-            load_column_fast<N, FMT>(pp.img_a, pp.pitch, ax, ay, lane, zr);
-            load_column_fast<N, FMT>(pp.img_b, pp.pitch, bx, by, lane, zi);
-        } else {
-            load_column_raw<N, FMT>(pp.img_a, pp.pitch, pp.H, pp.W, ax, ay, lane, zr);
-            load_column_raw<N, FMT>(pp.img_b, pp.pitch, pp.H, pp.W, bx, by, lane, zi);
-        }
-    } else {
-#pragma unroll
-        for (int r = 0; r < N; ++r) {
-            zr[r] = 0.0f;
-            zi[r] = 0.0f;
-        }
-    }
-    {
-        // Approximate means only reduce the dynamic range of the transform: the DC bin is
-        // cleared exactly in the cross spectrum, which is what the reference's mean removal
-        // amounts to.
-        float sa = 0.0f, sb = 0.0f;
-#pragma unroll
-        for (int r = 0; r < N; ++r) {
-            sa += zr[r];
-            sb += zi[r];
-        }
-        constexpr float inv = 1.0f / (float)(N * N);
-        const float ma = window_sum<N>(sa, red, lane) * inv;
-        const float mb = window_sum<N>(sb, red, lane) * inv;
-#pragma unroll
-        for (int r = 0; r < N; ++r) {
-            zr[r] -= ma;
-            zi[r] -= mb;
-        }
-    }
+    ingest<N, FMT>(pp, sig, ax, ay, bx, by, lane, red, zr, zi);
 
     // ---- forward transform of z = a + i b ----------------------------------------------
     // Transposes go through LDS.  A window inside one wave uses a single float plane and
@@ -420,183 +621,7 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
     }
     __syncthreads();
 
-    // ---- find_peak (correlate.rs:215-303), lane = row -----------------------------------
-    // NaN can only reach the plane from non-finite float input; the integer formats take
-    // the comparison shortcuts that assume finite values.
-    constexpr bool MAYBE_NAN = (FMT == FMT_F32);
-    // (1) first maximum in row-major order (strict >): row scan, then across rows
-#pragma unroll
-    for (int x = 0; x < N; ++x) zr[x] = pr[lane * P + x];
-    float best = -INFINITY;
-    int bidx = 0;
-    {
-        float rb = -INFINITY;
-        int ri = 0;
-#pragma unroll
-        for (int x = 0; x < N; ++x) {
-            const bool gt = zr[x] > rb;
-            rb = gt ? zr[x] : rb;
-            ri = gt ? x : ri;
-        }
-        ri += lane * N;
-        if constexpr (C::IN_WAVE) {
-            // lexicographic (value desc, index asc) maximum: same as the first maximum
-#pragma unroll
-            for (int m = 1; m < N; m <<= 1) {
-                const float ov = __shfl_xor(rb, m, N);
-                const int oi = __shfl_xor(ri, m, N);
-                const bool take = (ov > rb) || (ov == rb && oi < ri);
-                rb = take ? ov : rb;
-                ri = take ? oi : ri;
-            }
-            best = rb;
-            bidx = ri;
-        } else {
-            pkv[lane] = rb;
-            pki[lane] = ri;
-            __syncthreads();
-            for (int r = 0; r < N; ++r) {
-                const float v = pkv[r];
-                if (v > best) {
-                    best = v;
-                    bidx = pki[r];
-                }
-            }
-        }
-        if (!(best > -INFINITY)) bidx = 0;  // no maximum at all: (0, 0) as the reference
-    }
-    const int pby = bidx / N, pbx = bidx - pby * N;
-    // (2) footprint: half width at half maximum along +-x, +-y (k < N/2 while the value at
-    //     distance k is > best / 2).  Lane l of the window reads the row of the peak at
-    //     distance l to the right and the column at distance l downwards; distances to the
-    //     left/up are lanes N - l.  The walk lengths are runs of set bits in ballots.
-    int hw = 0;
-    {
-        const float half_best = 0.5f * best;
-        const float hv = pr[pby * P + (pbx + lane) % N];
-        const float vv = pr[((pby + lane) % N) * P + pbx];
-        if constexpr (C::IN_WAVE) {
-            const int base = tid - lane;
-            const unsigned long long full = N == 64 ? ~0ull : ((1ull << N) - 1);
-            const unsigned long long hm = (__ballot(hv > half_best) >> base) & full;
-            const unsigned long long vm = (__ballot(vv > half_best) >> base) & full;
-            // forward: lanes 0, 1, 2, ...; backward: lanes 0, N-1, N-2, ...
-            auto run_fwd = [&](unsigned long long m) {
-                const unsigned long long inv = ~m & full;
-                return inv ? __builtin_ctzll(inv) : N;
-            };
-            auto run_bwd = [&](unsigned long long m) {
-                if (!(m & 1ull)) return 0;
-                // bits N-1, N-2, ... : count leading ones of the N-bit field below bit 0
-                const unsigned long long rest = (m >> 1) << (64 - (N - 1)) ;
-                const unsigned long long inv = ~rest;
-                const int lead = N - 1 > 0 ? (inv ? __builtin_clzll(inv) : 64) : 0;
-                return 1 + (lead < N - 1 ? lead : N - 1);
-            };
-            const int k0 = min(run_fwd(vm), H2), k1 = min(run_bwd(vm), H2);
-            const int k2 = min(run_fwd(hm), H2), k3 = min(run_bwd(hm), H2);
-            hw = max(max(k0, k1), max(k2, k3));
-        } else {
-            const int sy[4] = {1, N - 1, 0, 0};
-            const int sx[4] = {0, 0, 1, N - 1};
-            (void)hv;
-            (void)vv;
-#pragma unroll
-            for (int d = 0; d < 4; ++d) {
-                int kk = 0;
-                while (kk < H2 && pr[((pby + kk * sy[d]) % N) * P + (pbx + kk * sx[d]) % N] > half_best)
-                    ++kk;
-                hw = kk > hw ? kk : hw;
-            }
-        }
-    }
-    int exclude = (int)ceilf(1.5f * (float)hw);
-    exclude = exclude < 2 ? 2 : exclude;
-    // (3) highest local maximum outside the footprint (all 8 neighbours <= v)
-    float second = -INFINITY;
-    {
-        float cand = -INFINITY;
-        const int ru = (lane + N - 1) % N, rd = (lane + 1) % N;
-        int dyc = abs(lane - pby);
-        dyc = dyc < N - dyc ? dyc : N - dyc;
-        const bool row_far = dyc > exclude;
-        // Columns of this row outside the footprint as a bit mask (N <= 64): every column
-        // when the row is far, else all but the circular run pbx - exclude .. pbx + exclude.
-        using Mask = std::conditional_t<(N > 32), unsigned long long, unsigned int>;
-        constexpr int MB = 8 * (int)sizeof(Mask);
-        constexpr Mask full = N == MB ? ~Mask(0) : ((Mask(1) << (N % MB)) - 1);
-        Mask allowed = full;
-        if constexpr (N <= 64) {
-            if (!row_far) {
-                const int len = 2 * exclude + 1;
-                if (len >= N) {
-                    allowed = 0;
-                } else {
-                    const Mask run = (Mask(1) << len) - 1;
-                    const int s = (pbx - exclude + N) % N;
-                    const Mask ex = s == 0 ? run : (Mask)(((run << s) | (run >> (N - s))) & full);
-                    allowed = full & ~ex;
-                }
-            }
-        }
-        // vertical max of (row above, row, row below) per column, sliding along x
-        float vm_prev = fmaxf(fmaxf(pr[ru * P + N - 1], zr[N - 1]), pr[rd * P + N - 1]);
-        float u1 = pr[ru * P + 0], d1 = pr[rd * P + 0];
-        float vm_cur = fmaxf(fmaxf(u1, zr[0]), d1);
-#pragma unroll
-        for (int x = 0; x < N; ++x) {
-            const int xn = x + 1 < N ? x + 1 : 0;
-            const float u2 = pr[ru * P + xn], d2 = pr[rd * P + xn];
-            const float vm_next = fmaxf(fmaxf(u2, zr[xn]), d2);
-            const float v = zr[x];
-            bool far;
-            if constexpr (N <= 64) {
-                far = (allowed >> x) & 1;
-            } else {
-                int dxc = abs(x - pbx);
-                dxc = dxc < N - dxc ? dxc : N - dxc;
-                far = row_far || dxc > exclude;
-            }
-            bool local;
-            if constexpr (MAYBE_NAN) {
-                const float m0 = zr[(x + N - 1) % N], m2 = zr[xn];
-                const float u0 = pr[ru * P + (x + N - 1) % N], d0 = pr[rd * P + (x + N - 1) % N];
-                local = u0 <= v && u1 <= v && u2 <= v && m0 <= v && m2 <= v && d0 <= v &&
-                        d1 <= v && d2 <= v;
-                if (local && far && v > cand) cand = v;
-            } else {
-                // finite values: max(cand, v) is the strict-> update of the reference
-                local = fmaxf(fmaxf(vm_prev, vm_next), fmaxf(u1, d1)) <= v;
-                cand = (local && far) ? fmaxf(cand, v) : cand;
-            }
-            vm_prev = vm_cur;
-            vm_cur = vm_next;
-            u1 = u2;
-            d1 = d2;
-        }
-        if constexpr (C::IN_WAVE) {
-#pragma unroll
-            for (int m = 1; m < N; m <<= 1) cand = fmaxf(cand, __shfl_xor(cand, m, N));
-            second = cand;
-        } else {
-            __syncthreads();
-            pkv[lane] = cand;
-            __syncthreads();
-            for (int r = 0; r < N; ++r) second = pkv[r] > second ? pkv[r] : second;
-        }
-    }
-    if (lane == 0 && active) {
-        PeakRec r;
-        r.best = best;
-        r.second = second;
-        r.mx = pr[pby * P + (pbx + N - 1) % N];
-        r.qx = pr[pby * P + (pbx + 1) % N];
-        r.my = pr[((pby + N - 1) % N) * P + pbx];
-        r.qy = pr[((pby + 1) % N) * P + pbx];
-        r.index = bidx;
-        r.has_peak = has_peak ? 1 : 0;
-        pp.peaks[k] = r;
-    }
+    find_peak_plane<N, FMT>(pr, lane, tid, pkv, pki, active, has_peak, pp.peaks + k);
 }
 
 template <int N, int FMT, int OCC>

@@ -1,0 +1,10 @@
+// k_correlate_mf (DFT-GEMM, 16-bit MFMA) instantiations for 32 px windows.
+#include "correlate_mfma.hpp"
+
+namespace piv {
+hipError_t launch_correlate_mf_32(int fmt, int precision, const PassPair *pairs, int npairs,
+                                  uint32_t max_cells, hipStream_t st)
+{
+    return launch_corr_mf_n<32>(fmt, precision, pairs, npairs, max_cells, st);
+}
+}  // namespace piv

@@ -81,6 +81,11 @@ int variant_of(const piv_analyzer *a, int n, int fmt)
     return it == a->variants.end() ? 0 : it->second;
 }
 
+int mf_precision(const piv_analyzer *a, int n)
+{
+    return (a->precision != 0 && correlate_mf_supported(n)) ? a->precision : 0;
+}
+
 // PivAnalyzer::new checks (multipass.rs:91-119)
 piv_status check_config(const piv_config &c)
 {
@@ -493,8 +498,9 @@ piv_status run_batch(piv_analyzer *a, const piv_frame *const *fa, const piv_fram
         sig.insert(sig.end(), reinterpret_cast<const char *>(extra),
                    reinterpret_cast<const char *>(extra) + sizeof extra);
         for (const PassPlan &pl : plans) {
-            const uintptr_t pv[3] = {(uintptr_t)pl.n, (uintptr_t)pl.cells,
-                                     (uintptr_t)variant_of(a, pl.n, fa[0]->fmt)};
+            const uintptr_t pv[4] = {(uintptr_t)pl.n, (uintptr_t)pl.cells,
+                                     (uintptr_t)variant_of(a, pl.n, fa[0]->fmt),
+                                     (uintptr_t)mf_precision(a, pl.n)};
             sig.insert(sig.end(), reinterpret_cast<const char *>(pv),
                        reinterpret_cast<const char *>(pv) + sizeof pv);
         }
@@ -511,13 +517,16 @@ piv_status run_batch(piv_analyzer *a, const piv_frame *const *fa, const piv_fram
             HIP_TRY(timed(a, "place", 0, (double)total * 17.0, total, [&] {
                 return launch_place_pass(ppd, (int)npairs, (uint32_t)pl.cells, st);
             }));
-            HIP_TRY(timed(a, "correlate", pl.n, cbytes, total, [&] {
-                return launch_correlate(pl.n, fa[0]->fmt, variant_of(a, pl.n, fa[0]->fmt), ppd,
-                                        (int)npairs, (uint32_t)pl.cells, st);
+            const int mp = mf_precision(a, pl.n);
+            HIP_TRY(timed(a, mp ? "correlate_mf" : "correlate", pl.n, cbytes, total, [&] {
+                return mp ? launch_correlate_mf(pl.n, fa[0]->fmt, mp, ppd, (int)npairs,
+                                                (uint32_t)pl.cells, st)
+                          : launch_correlate(pl.n, fa[0]->fmt, variant_of(a, pl.n, fa[0]->fmt),
+                                             ppd, (int)npairs, (uint32_t)pl.cells, st);
             }));
             // the finishing kernel reads peak records + origins (48 B) and writes 13 B/window
             HIP_TRY(timed(a, "finish", 0, (double)total * 61.0, total, [&] {
-                return launch_peak_finish(ppd, (int)npairs, (uint32_t)pl.cells, st);
+                return launch_peak_finish(ppd, (int)npairs, (uint32_t)pl.cells, st, mp != 0);
             }));
             // validation reads the raw field (13 B/vector) and writes the validated one (13 B)
             HIP_TRY(timed(a, "validate", 0, (double)total * 26.0, total, [&] {
@@ -1077,7 +1086,13 @@ piv_status piv_correlate_windows(piv_analyzer *a, const piv_frame *fa, const piv
     hipStream_t st = a->stream;
     HIP_TRY(hipMemcpyAsync(d_org, origins, count * 16, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(d_pp, &d, sizeof d, hipMemcpyHostToDevice, st));
-    HIP_TRY(launch_pass((int)window, fa->fmt, variant_of(a, (int)window, fa->fmt), d_pp, 1, count, st));
+    if (const int mp = mf_precision(a, (int)window)) {
+        HIP_TRY(launch_place_pass(d_pp, 1, count, st));
+        HIP_TRY(launch_correlate_mf((int)window, fa->fmt, mp, d_pp, 1, count, st));
+        HIP_TRY(launch_peak_finish(d_pp, 1, count, st, true));
+    } else {
+        HIP_TRY(launch_pass((int)window, fa->fmt, variant_of(a, (int)window, fa->fmt), d_pp, 1, count, st));
+    }
     HIP_TRY(hipMemcpyAsync(dx, o_dx, count * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(dy, o_dy, count * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(ppr, o_ppr, count * 4, hipMemcpyDeviceToHost, st));
@@ -1293,6 +1308,15 @@ piv_status piv_profile_read(piv_analyzer *a, piv_kernel_stat *out, uint32_t capa
         ++i;
     }
     *count = i;
+    return PIV_OK;
+}
+
+piv_status piv_analyzer_set_precision(piv_analyzer *a, piv_precision p)
+{
+    if (!a) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+    if ((int)p < 0 || (int)p > 2) return fail(PIV_ERR_INVALID_ARGUMENT, "unknown precision %d", (int)p);
+    std::lock_guard<std::mutex> lk(a->mu);
+    a->precision = (int)p;
     return PIV_OK;
 }
 

@@ -172,6 +172,8 @@ struct piv_analyzer {
     // correlation kernel variant (variants.hpp) per (window size, sample format), set by
     // piv_autotune; absent means variant 0
     std::map<std::pair<int, int>, int> variants;
+    // correlation precision (piv_analyzer_set_precision): 0 FP32 FFT, 1 fp16 / 2 bf16 DFT-GEMM
+    int precision = 0;
     // replay cache: the last call's descriptor bytes and the graph of its launch sequence
     std::vector<char> last_sig;
     hipGraphExec_t graph_exec = nullptr;
@@ -213,6 +215,8 @@ struct PassPlan {
 
 piv_status set_device(int dev);
 int variant_of(const piv_analyzer *a, int n, int fmt);
+// DFT-GEMM precision for a pass of window n (0: the FP32 FFT kernel)
+int mf_precision(const piv_analyzer *a, int n);
 piv_status check_config(const piv_config &c);
 piv_status prepare_frame(piv_analyzer *a, piv_frame *f, const float *bg_dev);
 piv_status prepare_frames_batch(piv_analyzer *a, piv_frame *const *fs, int count,

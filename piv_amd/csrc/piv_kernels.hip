@@ -39,6 +39,12 @@ hipError_t launch_correlate_48(int fmt, int variant, const PassPair *pairs, int 
                               hipStream_t st);
 hipError_t launch_correlate_64(int fmt, int variant, const PassPair *pairs, int npairs, uint32_t max_cells,
                               hipStream_t st);
+hipError_t launch_correlate_mf_16(int fmt, int precision, const PassPair *pairs, int npairs,
+                                  uint32_t max_cells, hipStream_t st);
+hipError_t launch_correlate_mf_32(int fmt, int precision, const PassPair *pairs, int npairs,
+                                  uint32_t max_cells, hipStream_t st);
+hipError_t launch_correlate_mf_64(int fmt, int precision, const PassPair *pairs, int npairs,
+                                  uint32_t max_cells, hipStream_t st);
 hipError_t launch_correlate_96(int fmt, int variant, const PassPair *pairs, int npairs, uint32_t max_cells,
                               hipStream_t st);
 
@@ -692,6 +698,23 @@ __device__ __forceinline__ float subpixel(float m, float c, float q)
     return isfinite(d) ? clamp_rs(d, -0.5f, 0.5f) : 0.0f;
 }
 
+// the same fit in FP64 (reduced-precision passes)
+__device__ __forceinline__ double subpixel64(float m32, float c32, float q32)
+{
+    const double m = m32, c = c32, q = q32;
+    double d;
+    if (m > 0.0 && c > 0.0 && q > 0.0) {
+        const double lm = log(m), lc = log(c), lq = log(q);
+        d = (lm - lq) / (2.0 * (lm - 2.0 * lc + lq));
+    } else {
+        d = (m - q) / (2.0 * (m - 2.0 * c + q));
+    }
+    return isfinite(d) ? (d < -0.5 ? -0.5 : (d > 0.5 ? 0.5 : d)) : 0.0;
+}
+
+// F64 = true: the reduced-precision passes fit the peak and accumulate the displacement in
+// FP64 (only the final value is rounded to f32).
+template <bool F64>
 __global__ __launch_bounds__(WBLOCK) void k_peak_finish(const PassPair *__restrict__ pairs,
                                                         uint32_t bpp)
 {
@@ -705,7 +728,6 @@ __global__ __launch_bounds__(WBLOCK) void k_peak_finish(const PassPair *__restri
         const int n = g.window, h2 = n / 2;
         const float4 o = reinterpret_cast<const float4 *>(pp.origins)[k];
         const int pby = r.index / n, pbx = r.index - pby * n;
-        const float sx = subpixel(r.mx, r.best, r.qx), sy = subpixel(r.my, r.best, r.qy);
         // correlate.rs:278-284: indices at or past n/2 wrap to negative displacements
         const float wx = pbx >= h2 ? (float)pbx - (float)n : (float)pbx;
         const float wy = pby >= h2 ? (float)pby - (float)n : (float)pby;
@@ -714,8 +736,15 @@ __global__ __launch_bounds__(WBLOCK) void k_peak_finish(const PassPair *__restri
             ppr = r.best / r.second;
             ppr = (ppr > 1000.0f || ppr != ppr) ? 1000.0f : ppr;
         }
-        pp.out_dx[k] = o.z - o.x + (wx + sx);
-        pp.out_dy[k] = o.w - o.y + (wy + sy);
+        if constexpr (F64) {
+            const double sx = subpixel64(r.mx, r.best, r.qx), sy = subpixel64(r.my, r.best, r.qy);
+            pp.out_dx[k] = (float)(((double)o.z - (double)o.x) + ((double)wx + sx));
+            pp.out_dy[k] = (float)(((double)o.w - (double)o.y) + ((double)wy + sy));
+        } else {
+            const float sx = subpixel(r.mx, r.best, r.qx), sy = subpixel(r.my, r.best, r.qy);
+            pp.out_dx[k] = o.z - o.x + (wx + sx);
+            pp.out_dy[k] = o.w - o.y + (wy + sy);
+        }
         pp.out_ppr[k] = ppr;
         pp.out_flags[k] = 0;
         if (pp.out_peak) pp.out_peak[k] = r.index;
@@ -778,13 +807,28 @@ hipError_t launch_place_pass(const PassPair *pairs, int npairs, uint32_t max_cel
     return hipGetLastError();
 }
 
-hipError_t launch_peak_finish(const PassPair *pairs, int npairs, uint32_t max_cells, hipStream_t st)
+hipError_t launch_peak_finish(const PassPair *pairs, int npairs, uint32_t max_cells, hipStream_t st,
+                              bool f64)
 {
     const uint32_t bpp = (max_cells + WBLOCK - 1) / WBLOCK;
     const uint32_t blocks = bpp * (uint32_t)npairs;
     if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_peak_finish, dim3(blocks), dim3(WBLOCK), 0, st, pairs, bpp);
+    if (f64) hipLaunchKernelGGL(k_peak_finish<true>, dim3(blocks), dim3(WBLOCK), 0, st, pairs, bpp);
+    else hipLaunchKernelGGL(k_peak_finish<false>, dim3(blocks), dim3(WBLOCK), 0, st, pairs, bpp);
     return hipGetLastError();
+}
+
+bool correlate_mf_supported(int n) { return n == 16 || n == 32 || n == 64; }
+
+hipError_t launch_correlate_mf(int n, int fmt, int precision, const PassPair *pairs, int npairs,
+                               uint32_t max_cells, hipStream_t st)
+{
+    switch (n) {
+    case 16: return launch_correlate_mf_16(fmt, precision, pairs, npairs, max_cells, st);
+    case 32: return launch_correlate_mf_32(fmt, precision, pairs, npairs, max_cells, st);
+    case 64: return launch_correlate_mf_64(fmt, precision, pairs, npairs, max_cells, st);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_pass(int n, int fmt, int variant, const PassPair *pairs, int npairs,
