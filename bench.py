@@ -330,6 +330,62 @@ def video_bench(total_pairs=1999, distinct=32, batch=256, seed=31):
             "rms_px_error_vs_truth": round(rms, 4)}
 
 
+def precision_bench(an, fa, fb, dx, dy, pr, fl, g, field, uniq, steps):
+    """The same batched workload with the correlation as DFT-matrix GEMMs on the matrix cores
+    (fp16 and bf16 operands, FP32 accumulation, FP64 peak fit): fields/s, the DFT-GEMM kernels'
+    MFMA rate, and displacement error against the ground truth and against the FP64 CPU
+    oracle on one pair (vectors valid in both)."""
+    import torch
+
+    from oracle import oracle as O
+
+    P = len(fa)
+    ocfg = O.PivConfig(windows=list(WINDOWS), overlap=0.5, final_iterations=1)
+    ref, _ = O.analyze(O.prepare(uniq[0][0]), O.prepare(uniq[0][1]), ocfg, precision=64)
+    out = {}
+    for prec in ("fp16", "bf16"):
+        an.set_precision(prec)
+        an.analyze_batch_device(fa, fb, dx, dy, pr, fl, summaries=False, sync=True)
+        an.profile(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            an.analyze_batch_device(fa, fb, dx, dy, pr, fl, summaries=False, sync=False)
+        an.synchronize()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        stats = an.profile_read()
+        an.profile(False)
+        kern = {f'{s["name"]}{s["window"] or ""}': round(s["total_ms"] / s["launches"], 4)
+                for s in stats}
+        # nominal DFT-GEMM work: 11 (N/16)^3 MFMA 16x16x32 (16384 flop each) per window
+        mf = [s for s in stats if s["name"] == "correlate_mf"]
+        tflops = {}
+        for s in mf:
+            n = s["window"]
+            flop = s["units"] / s["launches"] * 11 * (n // 16) ** 3 * 16384
+            tflops[f"n{n}"] = round(flop / (s["total_ms"] / s["launches"] * 1e-3) / 1e12, 1)
+        an.analyze_batch_device(fa[:1], fb[:1], dx[:1], dy[:1], pr[:1], fl[:1], summaries=False,
+                                sync=True)
+        hdx, hdy, hfl = dx[0].cpu().numpy(), dy[0].cpu().numpy(), fl[0].cpu().numpy()
+        rms_t, nv = truth_rms(hdx, hdy, hfl, g, field)
+        both = ((hfl & 23) == 0) & ((ref.flags & 23) == 0)
+        d2 = (hdx - ref.dx)[both] ** 2 + (hdy - ref.dy)[both] ** 2
+        out[prec] = {"fields_per_s": round(P * steps / el, 1), "kernel_avg_ms": kern,
+                     "mfma_TFLOPs_nominal": tflops,
+                     "rms_px_vs_fp64_oracle": round(float(np.sqrt(d2.mean())), 5),
+                     "max_px_vs_fp64_oracle": round(float(np.sqrt(d2.max())), 4),
+                     "rms_px_vs_truth": round(rms_t, 5), "vectors": int(both.sum())}
+    an.set_precision("fp32")
+    # the FP32 path against the same oracle
+    an.analyze_batch_device(fa[:1], fb[:1], dx[:1], dy[:1], pr[:1], fl[:1], summaries=False, sync=True)
+    hdx, hdy, hfl = dx[0].cpu().numpy(), dy[0].cpu().numpy(), fl[0].cpu().numpy()
+    both = ((hfl & 23) == 0) & ((ref.flags & 23) == 0)
+    d2 = (hdx - ref.dx)[both] ** 2 + (hdy - ref.dy)[both] ** 2
+    out["fp32_rms_px_vs_fp64_oracle"] = float(np.sqrt(d2.mean()))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -344,6 +400,8 @@ def main():
                     help="Poisson phase of the serving measurement (0 skips serving)")
     ap.add_argument("--serve-rate", type=float, default=500.0)
     ap.add_argument("--no-video", action="store_true", help="skip the batched-video measurement")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the reduced-precision (DFT-GEMM) measurement")
     ap.add_argument("--serve-backlog", type=int, default=400)
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default correlation-kernel variants")
@@ -433,8 +491,8 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             tr = json.load(f)
-        key = f"correlate{dom['window']}_pairs{P}"
-        traffic = tr.get(key)
+        occ = (tune or {}).get(f"n{dom['window']}", {}).get("chosen_occupancy")
+        traffic = tr.get(f"correlate{dom['window']}_occ{occ}_pairs{P}")
 
     if args.only_batched:
         print(json.dumps({"fields_per_s": fields_per_s, "ms_per_step": ms_per_step,
@@ -454,6 +512,10 @@ def main():
     an.analyze_batch_device(fa[:1], fb[:1], dx[:1], dy[:1], pr[:1], fl[:1], summaries=False,
                             sync=True)
     rms, nvec = truth_rms(dx[0].cpu().numpy(), dy[0].cpu().numpy(), fl[0].cpu().numpy(), g, field)
+
+    precision = None
+    if not args.no_precision:
+        precision = precision_bench(an, fa, fb, dx, dy, pr, fl, g, field, uniq, args.steps)
 
     video = None
     if not args.no_video:
@@ -499,6 +561,7 @@ def main():
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": round(avg_ms, 5)},
+        "reduced_precision": precision,
         "video": video,
         "serving": serving,
         "cpu_baseline": cpu,

@@ -1,10 +1,11 @@
 // C-ABI implementation (include/piv.h): PivAnalyzer / PreparedFrame lifecycle, frame
 // conditioning and the multi-pass orchestration over the HIP kernels.
 //
-// Per pass (multipass.rs:152-192) one k_correlate launch covers every window of every pair
-// in the call, then one k_validate launch validates every pair's field; the validated field
-// stays in HBM as the next pass's predictor.  All per-pair descriptors of a call are
-// uploaded with a single host-to-device copy before the first launch.
+// Per pass (multipass.rs:152-192) each kernel is one launch for every window of every pair in
+// the call: placement + mask, correlation, sub-pixel fit + assembly, then the two validation
+// kernels; the validated field stays in HBM as the next pass's predictor.  All per-pair
+// descriptors of a call are uploaded with a single host-to-device copy before the first
+// launch, and a call whose descriptors equal the previous call's replays its captured graph.
 #include "piv_internal.hpp"
 
 using namespace piv;
