@@ -209,6 +209,16 @@ piv_status piv_copy_to_device(int device, void *dst, const void *src, size_t byt
 piv_status piv_copy_to_host(int device, void *dst, const void *src, size_t bytes);
 piv_status piv_analyzer_synchronize(piv_analyzer *a);
 
+/* ---- output -------------------------------------------------------------------------------
+ * NpzWriter::write (piv-io/src/write.rs:129-165): <dir>/pair_AAAAAA_BBBBBB.npz, deflated, with
+ * x, y (grid centres + ROI offset), dx, dy, u, v (= d / dt), u_mps, v_mps (only when
+ * m_per_px > 0), peak_ratio, flags, frames [a, b], t (pair midpoint, seconds).  Host arrays of
+ * grid->ny * grid->nx values. */
+piv_status piv_write_pair_npz(const char *dir, uint64_t frame_a, uint64_t frame_b, double time_s,
+                              double dt_s, double m_per_px, uint32_t roi_x, uint32_t roi_y,
+                              const piv_grid *grid, const float *dx, const float *dy,
+                              const float *peak_ratio, const uint8_t *flags);
+
 /* ---- measurement (no counterpart in the reference) ------------------------------------ */
 
 /* Per-kernel timing: when enabled, every kernel the analyzer launches is bracketed by HIP

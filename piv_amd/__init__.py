@@ -18,6 +18,7 @@ PivError subclasses carrying the library's message.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass, field
 from typing import List, Optional
 
@@ -780,6 +781,21 @@ class PivServer:
             self.close()
         except Exception:
             pass
+
+
+def write_pair_npz(directory, frame_a, frame_b, time_s, dt_s, field: "VectorField",
+                   m_per_px=None, roi=None):
+    """NpzWriter::write (write.rs:129-165): <directory>/pair_AAAAAA_BBBBBB.npz."""
+    g = field.grid
+    cg = L.piv_grid(g.window, g.step, g.nx, g.ny, g.x0, g.y0)
+    arrs = [np.ascontiguousarray(field.dx, np.float32), np.ascontiguousarray(field.dy, np.float32),
+            np.ascontiguousarray(field.peak_ratio, np.float32),
+            np.ascontiguousarray(field.flags, np.uint8)]
+    rx, ry = (roi[0], roi[1]) if roi is not None else (0, 0)
+    _check(L.load().piv_write_pair_npz(str(directory).encode(), frame_a, frame_b, time_s, dt_s,
+                                       m_per_px if m_per_px else 0.0, rx, ry, C.byref(cg),
+                                       *[L.ptr(x) for x in arrs]))
+    return os.path.join(str(directory), f"pair_{frame_a:06d}_{frame_b:06d}.npz")
 
 
 def kernel_variant_count(window):

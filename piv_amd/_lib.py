@@ -146,6 +146,9 @@ _SIGS = {
     "piv_kernel_variant_set": ([C.c_void_p, C.c_uint32, C.c_int, C.c_uint32], C.c_int),
     "piv_autotune": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
                       C.c_uint32, C.POINTER(C.c_uint32)], C.c_int),
+    "piv_write_pair_npz": ([C.c_char_p, C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_double,
+                            C.c_uint32, C.c_uint32, C.POINTER(piv_grid), C.c_void_p, C.c_void_p,
+                            C.c_void_p, C.c_void_p], C.c_int),
     "piv_normalize_pixels": ([C.c_int, C.c_void_p, C.c_int, C.c_size_t, C.c_void_p], C.c_int),
 }
 

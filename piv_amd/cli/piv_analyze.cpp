@@ -1,0 +1,433 @@
+// piv-analyze: the `piv analyze` host loop (piv-cli/src/analyze.rs:243-523) over the C ABI of
+// libpiv_amd, for frame stacks already decoded to a .npy file [frames, height, width] of u8,
+// u16 or f32 samples (the ffmpeg decode of the reference is outside this library).
+//
+// Same frame range, pairing, background sampling and output as the reference: pairs
+// (a, a + stride) for a in start, start + pair_step, ... < end - stride; background = min or
+// mean of N frames sampled at k (total - 1) / (N - 1) across the whole clip; dt = stride / fps;
+// pair time = capture time of the pair midpoint; one pair_AAAAAA_BBBBBB.npz per pair plus
+// run.json in the output directory.  The pairs are analysed in batches on the GPU (one launch
+// per kernel and pass for a whole batch).
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <fcntl.h>
+#include <unistd.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <filesystem>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/piv.h"
+
+namespace {
+
+[[noreturn]] void die(const std::string &msg)
+{
+    std::fprintf(stderr, "error: %s\n", msg.c_str());
+    std::exit(1);
+}
+
+void check(piv_status s, const char *what)
+{
+    if (s != PIV_OK) die(std::string(what) + ": " + piv_last_error());
+}
+
+struct Args {
+    std::string input, output;
+    bool force = false;
+    long start = -1, end = -1;
+    double start_s = -1, end_s = -1;
+    int stride = 1, pair_step = 1;
+    double fps = 0;  // capture rate; required for physical time (default 1 frame per second)
+    bool roi = false;
+    uint32_t rx = 0, ry = 0, rw = 0, rh = 0;
+    std::string background = "min:40";
+    float smooth_sigma = 1.0f;
+    uint32_t normalize_radius = 4;
+    std::vector<uint32_t> windows{96, 64};
+    float overlap = 0.5f;
+    uint32_t final_iterations = 2;
+    float min_mean = 0.04f, min_std = 0.015f;
+    float ppr = -1, median_threshold = -1, median_eps = -1;
+    bool no_replace = false;
+    std::string format = "npz";
+    double px_per_mm = 0;
+    uint32_t batch = 64;
+    std::string precision = "fp32";
+    int device = 0;
+};
+
+void usage()
+{
+    std::fprintf(stderr,
+                 "usage: piv-analyze INPUT.npy -o OUTPUT_DIR [--force]\n"
+                 "  frames:        --start N | --start-s S, --end N | --end-s S, --stride 1, --pair-step 1,\n"
+                 "                 --capture-fps FPS (default 1)\n"
+                 "  image:         --roi x,y,w,h, --background none|min:N|mean:N (min:40),\n"
+                 "                 --smooth-sigma 1, --normalize-radius 4\n"
+                 "  interrogation: --windows 96,64, --overlap 0.5, --final-iterations 2,\n"
+                 "                 --min-mean 0.04, --min-std 0.015\n"
+                 "  validation:    --ppr-threshold, --median-threshold, --median-eps, --no-replace\n"
+                 "  output:        --format npz (hdf5 needs libhdf5), --px-per-mm X\n"
+                 "  device:        --batch 64, --precision fp32|fp16|bf16, --device 0\n");
+}
+
+Args parse(int argc, char **argv)
+{
+    Args a;
+    auto need = [&](int &i) -> std::string {
+        if (i + 1 >= argc) die(std::string("missing value for ") + argv[i]);
+        return argv[++i];
+    };
+    for (int i = 1; i < argc; ++i) {
+        const std::string s = argv[i];
+        if (s == "-h" || s == "--help") {
+            usage();
+            std::exit(0);
+        } else if (s == "-o" || s == "--output") a.output = need(i);
+        else if (s == "--force") a.force = true;
+        else if (s == "--start") a.start = std::stol(need(i));
+        else if (s == "--end") a.end = std::stol(need(i));
+        else if (s == "--start-s") a.start_s = std::stod(need(i));
+        else if (s == "--end-s") a.end_s = std::stod(need(i));
+        else if (s == "--stride") a.stride = std::stoi(need(i));
+        else if (s == "--pair-step") a.pair_step = std::stoi(need(i));
+        else if (s == "--capture-fps") {
+            const std::string v = need(i);
+            const size_t sl = v.find('/');
+            a.fps = sl == std::string::npos ? std::stod(v)
+                                            : std::stod(v.substr(0, sl)) / std::stod(v.substr(sl + 1));
+        } else if (s == "--roi") {
+            const std::string v = need(i);
+            if (std::sscanf(v.c_str(), "%u,%u,%u,%u", &a.rx, &a.ry, &a.rw, &a.rh) != 4)
+                die("--roi expects x,y,w,h");
+            a.roi = true;
+        } else if (s == "--background") a.background = need(i);
+        else if (s == "--smooth-sigma") a.smooth_sigma = std::stof(need(i));
+        else if (s == "--normalize-radius") a.normalize_radius = (uint32_t)std::stoul(need(i));
+        else if (s == "--windows") {
+            a.windows.clear();
+            std::string v = need(i);
+            size_t p = 0;
+            while (p <= v.size()) {
+                const size_t c = v.find(',', p);
+                a.windows.push_back((uint32_t)std::stoul(v.substr(p, c - p)));
+                if (c == std::string::npos) break;
+                p = c + 1;
+            }
+        } else if (s == "--overlap") a.overlap = std::stof(need(i));
+        else if (s == "--final-iterations") a.final_iterations = (uint32_t)std::stoul(need(i));
+        else if (s == "--min-mean") a.min_mean = std::stof(need(i));
+        else if (s == "--min-std") a.min_std = std::stof(need(i));
+        else if (s == "--ppr-threshold") a.ppr = std::stof(need(i));
+        else if (s == "--median-threshold") a.median_threshold = std::stof(need(i));
+        else if (s == "--median-eps") a.median_eps = std::stof(need(i));
+        else if (s == "--no-replace") a.no_replace = true;
+        else if (s == "--format") a.format = need(i);
+        else if (s == "--px-per-mm") a.px_per_mm = std::stod(need(i));
+        else if (s == "--batch") a.batch = (uint32_t)std::stoul(need(i));
+        else if (s == "--precision") a.precision = need(i);
+        else if (s == "--device") a.device = std::stoi(need(i));
+        else if (!s.empty() && s[0] == '-') die("unknown option " + s);
+        else if (a.input.empty()) a.input = s;
+        else die("unexpected argument " + s);
+    }
+    if (a.input.empty() || a.output.empty()) {
+        usage();
+        std::exit(2);
+    }
+    return a;
+}
+
+// A C-order .npy array [frames][height][width], memory-mapped.
+struct Npy {
+    const unsigned char *base = nullptr;
+    size_t size = 0;
+    const unsigned char *data = nullptr;
+    piv_pixel_format fmt = PIV_PIXEL_U8;
+    size_t bpp = 1;
+    size_t frames = 0, height = 0, width = 0;
+};
+
+Npy open_npy(const std::string &path)
+{
+    Npy n;
+    const int fd = ::open(path.c_str(), O_RDONLY);
+    if (fd < 0) die("cannot open " + path);
+    struct stat st {};
+    if (fstat(fd, &st) != 0) die("cannot stat " + path);
+    n.size = (size_t)st.st_size;
+    void *m = mmap(nullptr, n.size, PROT_READ, MAP_PRIVATE, fd, 0);
+    ::close(fd);
+    if (m == MAP_FAILED) die("cannot map " + path);
+    n.base = static_cast<const unsigned char *>(m);
+    if (n.size < 10 || std::memcmp(n.base, "\x93NUMPY", 6) != 0) die(path + " is not a .npy file");
+    const int major = n.base[6];
+    size_t hl, off;
+    if (major == 1) {
+        hl = n.base[8] | (n.base[9] << 8);
+        off = 10;
+    } else {
+        hl = n.base[8] | (n.base[9] << 8) | ((size_t)n.base[10] << 16) | ((size_t)n.base[11] << 24);
+        off = 12;
+    }
+    const std::string h(reinterpret_cast<const char *>(n.base + off), hl);
+    n.data = n.base + off + hl;
+    auto field = [&](const char *key) {
+        const size_t p = h.find(key);
+        if (p == std::string::npos) die(std::string(".npy header without ") + key);
+        return p + std::strlen(key);
+    };
+    const size_t d = h.find('\'', field("'descr':")) + 1;
+    const std::string descr = h.substr(d, h.find('\'', d) - d);
+    if (descr == "|u1" || descr == "u1") n.fmt = PIV_PIXEL_U8, n.bpp = 1;
+    else if (descr == "<u2") n.fmt = PIV_PIXEL_U16, n.bpp = 2;
+    else if (descr == "<f4") n.fmt = PIV_PIXEL_F32, n.bpp = 4;
+    else die("unsupported sample type " + descr + " (u8, u16 or f32)");
+    if (h.find("True", field("'fortran_order':")) < h.find(',', field("'fortran_order':")))
+        die("Fortran-order arrays are not supported");
+    const size_t sp = h.find('(', field("'shape':")) + 1;
+    const std::string shp = h.substr(sp, h.find(')', sp) - sp);
+    std::vector<size_t> dims;
+    size_t p = 0;
+    while (p < shp.size()) {
+        const size_t c = shp.find(',', p);
+        const std::string t = shp.substr(p, c == std::string::npos ? std::string::npos : c - p);
+        if (t.find_first_not_of(' ') != std::string::npos) dims.push_back(std::stoul(t));
+        if (c == std::string::npos) break;
+        p = c + 1;
+    }
+    if (dims.size() != 3) die("expected a [frames, height, width] array");
+    n.frames = dims[0];
+    n.height = dims[1];
+    n.width = dims[2];
+    if (n.data + n.frames * n.height * n.width * n.bpp > n.base + n.size) die("truncated .npy file");
+    return n;
+}
+
+std::string json_str(const std::string &s)
+{
+    std::string o = "\"";
+    for (char c : s) {
+        if (c == '"' || c == '\\') o += '\\';
+        o += c;
+    }
+    return o + "\"";
+}
+
+}  // namespace
+
+int main(int argc, char **argv)
+{
+    const Args A = parse(argc, argv);
+    const auto t0 = std::chrono::steady_clock::now();
+    if (A.format == "hdf5")
+        die("this build has no HDF5 support (libhdf5 is not available); use --format npz");
+    if (A.format != "npz") die("unknown format " + A.format);
+    const Npy in = open_npy(A.input);
+    const double fps = A.fps > 0 ? A.fps : 1.0;
+
+    // frame range (analyze.rs:251-269)
+    auto to_frame = [&](double s) { return (long)std::max(0.0, std::round(s * fps)); };
+    const long total = (long)in.frames;
+    const long start = A.start >= 0 ? A.start : (A.start_s >= 0 ? to_frame(A.start_s) : 0);
+    long end = A.end >= 0 ? A.end : (A.end_s >= 0 ? to_frame(A.end_s) : total);
+    end = std::min(end, total);
+    if (A.stride < 1 || A.pair_step < 1) die("--stride and --pair-step must be >= 1");
+    if (!(start + A.stride < end))
+        die("frame range [" + std::to_string(start) + ", " + std::to_string(end) +
+            ") is too short for stride " + std::to_string(A.stride) + " (clip has " +
+            std::to_string(total) + " frames)");
+    std::vector<long> pair_starts;
+    for (long a = start; a < end - A.stride; a += A.pair_step) pair_starts.push_back(a);
+    const double dt = A.stride / fps;
+
+    // output directory
+    namespace fs = std::filesystem;
+    if (fs::exists(A.output)) {
+        if (!A.force) die(A.output + " exists; pass --force to overwrite");
+        fs::remove_all(A.output);
+    }
+    fs::create_directories(A.output);
+
+    // region of interest
+    uint32_t H = (uint32_t)in.height, W = (uint32_t)in.width, x0 = 0, y0 = 0;
+    if (A.roi) {
+        if (A.rw == 0 || A.rh == 0 || A.rx + A.rw > W || A.ry + A.rh > H) die("--roi outside the frame");
+        x0 = A.rx;
+        y0 = A.ry;
+        W = A.rw;
+        H = A.rh;
+    }
+    const size_t row_stride = in.width * in.bpp;
+    const size_t frame_bytes = in.height * row_stride;
+    const size_t roi_off = (size_t)y0 * row_stride + (size_t)x0 * in.bpp;
+
+    // analyzer (PivConfig / PreprocessConfig from the options)
+    piv_config cfg;
+    piv_config_default(&cfg);
+    if (A.windows.empty() || A.windows.size() > PIV_MAX_PASSES) die("--windows: 1 to 16 sizes");
+    for (size_t i = 0; i < A.windows.size(); ++i) cfg.windows[i] = A.windows[i];
+    cfg.num_windows = (uint32_t)A.windows.size();
+    cfg.overlap = A.overlap;
+    cfg.final_iterations = A.final_iterations;
+    cfg.min_mean = A.min_mean;
+    cfg.min_std = A.min_std;
+    if (A.ppr >= 0) cfg.ppr_threshold = A.ppr;
+    if (A.median_threshold >= 0) cfg.median_threshold = A.median_threshold;
+    if (A.median_eps >= 0) cfg.median_eps = A.median_eps;
+    cfg.replace = A.no_replace ? 0 : 1;
+    cfg.smooth_sigma = A.smooth_sigma;
+    cfg.normalize_radius = A.normalize_radius;
+    piv_analyzer *an = nullptr;
+    check(piv_analyzer_create(&cfg, A.device, &an), "analyzer");
+    if (A.precision != "fp32") {
+        const piv_precision p = A.precision == "fp16" ? PIV_PRECISION_FP16
+                                : A.precision == "bf16" ? PIV_PRECISION_BF16
+                                                        : (die("unknown precision " + A.precision), PIV_PRECISION_FP32);
+        check(piv_analyzer_set_precision(an, p), "precision");
+    }
+    piv_grid grid;
+    check(piv_output_grid(an, H, W, &grid), "grid");
+
+    // the clip in HBM
+    void *clip = nullptr;
+    check(piv_device_alloc(A.device, in.frames * frame_bytes, &clip), "device memory for the clip");
+    check(piv_copy_to_device(A.device, clip, in.data, in.frames * frame_bytes), "upload");
+    auto frame_ptr = [&](long f) {
+        return static_cast<const char *>(clip) + (size_t)f * frame_bytes + roi_off;
+    };
+
+    // background (analyze.rs:370-385): N frames sampled across the whole clip
+    const float *bg_dev = nullptr;
+    piv_background *bg = nullptr;
+    std::string bg_spec = "none";
+    if (A.background != "none") {
+        const size_t c = A.background.find(':');
+        const std::string mode = A.background.substr(0, c);
+        long n = c == std::string::npos ? 40 : std::stol(A.background.substr(c + 1));
+        if ((mode != "min" && mode != "mean") || n < 1) die("--background: none, min:N or mean:N");
+        n = std::min(n, total);
+        check(piv_background_create(an, mode == "min" ? PIV_BACKGROUND_MIN : PIV_BACKGROUND_MEAN, H, W, &bg),
+              "background");
+        for (long k = 0; k < n; ++k) {
+            const long idx = n > 1 ? k * (total - 1) / (n - 1) : 0;
+            check(piv_background_add(bg, frame_ptr(idx), in.fmt, row_stride, 1), "background frame");
+        }
+        check(piv_background_finish(bg, nullptr), "background");
+        check(piv_background_device(bg, &bg_dev), "background");
+        bg_spec = A.background;
+        std::fprintf(stderr, "background: %s of %ld frames sampled across the clip\n", mode.c_str(), n);
+    }
+
+    // pairs, in batches
+    const size_t cells = (size_t)grid.nx * grid.ny, B = std::max<uint32_t>(1, A.batch);
+    void *ddx, *ddy, *dpr, *dfl;
+    check(piv_device_alloc(A.device, B * cells * 4, &ddx), "outputs");
+    check(piv_device_alloc(A.device, B * cells * 4, &ddy), "outputs");
+    check(piv_device_alloc(A.device, B * cells * 4, &dpr), "outputs");
+    check(piv_device_alloc(A.device, B * cells, &dfl), "outputs");
+    std::vector<float> hdx(B * cells), hdy(B * cells), hpr(B * cells);
+    std::vector<uint8_t> hfl(B * cells);
+    std::vector<piv_summary> sums(B);
+    std::map<long, piv_frame *> held;  // prepared frames still needed
+    auto frame = [&](long f) {
+        auto it = held.find(f);
+        if (it != held.end()) return it->second;
+        piv_frame *pf = nullptr;
+        check(piv_frame_prepare_device(an, frame_ptr(f), in.fmt, H, W, row_stride, bg_dev, &pf), "frame");
+        held[f] = pf;
+        return pf;
+    };
+    const double m_per_px = A.px_per_mm > 0 ? 1e-3 / A.px_per_mm : 0.0;
+    uint64_t valid = 0, vectors = 0;
+    const auto t_loop = std::chrono::steady_clock::now();
+    for (size_t p0 = 0; p0 < pair_starts.size(); p0 += B) {
+        const size_t n = std::min(B, pair_starts.size() - p0);
+        std::vector<const piv_frame *> fa(n), fb(n);
+        for (size_t k = 0; k < n; ++k) {
+            fa[k] = frame(pair_starts[p0 + k]);
+            fb[k] = frame(pair_starts[p0 + k] + A.stride);
+        }
+        check(piv_analyze_batch_device(an, fa.data(), fb.data(), (uint32_t)n, (float *)ddx, (float *)ddy,
+                                       (float *)dpr, (uint8_t *)dfl, sums.data(), nullptr, 0),
+              "analysis");
+        check(piv_copy_to_host(A.device, hdx.data(), ddx, n * cells * 4), "download");
+        check(piv_copy_to_host(A.device, hdy.data(), ddy, n * cells * 4), "download");
+        check(piv_copy_to_host(A.device, hpr.data(), dpr, n * cells * 4), "download");
+        check(piv_copy_to_host(A.device, hfl.data(), dfl, n * cells), "download");
+        for (size_t k = 0; k < n; ++k) {
+            const long a = pair_starts[p0 + k];
+            const double time_s = (a + A.stride / 2.0) / fps;
+            check(piv_write_pair_npz(A.output.c_str(), (uint64_t)a, (uint64_t)(a + A.stride), time_s, dt,
+                                     m_per_px, x0, y0, &grid, hdx.data() + k * cells,
+                                     hdy.data() + k * cells, hpr.data() + k * cells,
+                                     hfl.data() + k * cells),
+                  "output");
+            for (size_t i = 0; i < cells; ++i) valid += (hfl[k * cells + i] & PIV_FLAG_INVALID) == 0;
+            vectors += cells;
+        }
+        // drop frames no later pair needs
+        const long keep_from = p0 + n < pair_starts.size() ? pair_starts[p0 + n] : end;
+        for (auto it = held.begin(); it != held.end();) {
+            if (it->first < keep_from) {
+                piv_frame_release(it->second);
+                it = held.erase(it);
+            } else {
+                ++it;
+            }
+        }
+    }
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_loop).count();
+    for (auto &kv : held) piv_frame_release(kv.second);
+
+    // run.json (write.rs:20-49 RunMetadata, the fields this host loop knows)
+    std::string ws;
+    for (size_t i = 0; i < A.windows.size(); ++i) ws += (i ? ", " : "") + std::to_string(A.windows[i]);
+    char buf[4096];
+    std::snprintf(
+        buf, sizeof buf,
+        "{\n  \"tool_version\": \"piv-amd %d\",\n  \"source_path\": %s,\n  \"source_size\": %zu,\n"
+        "  \"camera\": null,\n  \"frame_width\": %zu,\n  \"frame_height\": %zu,\n  \"roi\": %s,\n"
+        "  \"capture_fps\": %.17g,\n  \"start_frame\": %ld,\n  \"end_frame\": %ld,\n  \"stride\": %d,\n"
+        "  \"pair_step\": %d,\n  \"dt_s\": %.17g,\n  \"background\": %s,\n"
+        "  \"preprocess\": {\"smooth_sigma\": %.9g, \"normalize_radius\": %u, \"normalize_eps\": %.9g},\n"
+        "  \"piv\": {\"windows\": [%s], \"overlap\": %.9g, \"final_iterations\": %u,\n"
+        "          \"mask\": {\"min_mean\": %.9g, \"min_std\": %.9g},\n"
+        "          \"validation\": {\"ppr_threshold\": %.9g, \"median_threshold\": %.9g, "
+        "\"median_eps\": %.9g, \"replace\": %s}},\n"
+        "  \"scale\": %s,\n  \"precision\": %s\n}\n",
+        piv_abi_version(), json_str(A.input).c_str(), in.size, in.width, in.height,
+        A.roi ? ("{\"x\": " + std::to_string(A.rx) + ", \"y\": " + std::to_string(A.ry) + ", \"w\": " +
+                 std::to_string(A.rw) + ", \"h\": " + std::to_string(A.rh) + "}").c_str()
+              : "null",
+        fps, start, end, A.stride, A.pair_step, dt,
+        bg_spec == "none" ? "null" : json_str(bg_spec).c_str(), cfg.smooth_sigma, cfg.normalize_radius,
+        cfg.normalize_eps, ws.c_str(), cfg.overlap, cfg.final_iterations, cfg.min_mean, cfg.min_std,
+        cfg.ppr_threshold, cfg.median_threshold, cfg.median_eps, cfg.replace ? "true" : "false",
+        m_per_px > 0 ? ("{\"m_per_px\": " + std::to_string(m_per_px) + "}").c_str() : "null",
+        json_str(A.precision).c_str());
+    FILE *f = std::fopen((A.output + "/run.json").c_str(), "w");
+    if (!f || std::fputs(buf, f) < 0 || std::fclose(f) != 0) die("writing run.json failed");
+
+    const double all = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::fprintf(stderr, "analysed %zu pairs in %.3f s (%.1f pairs/s on the GPU loop, %.3f s total); "
+                         "%.1f %% of %llu vectors valid -> %s\n",
+                 pair_starts.size(), secs, pair_starts.size() / std::max(secs, 1e-9), all,
+                 vectors ? 100.0 * valid / vectors : 0.0, (unsigned long long)vectors, A.output.c_str());
+    piv_device_free(A.device, ddx);
+    piv_device_free(A.device, ddy);
+    piv_device_free(A.device, dpr);
+    piv_device_free(A.device, dfl);
+    piv_device_free(A.device, clip);
+    if (bg) piv_background_destroy(bg);
+    piv_analyzer_destroy(an);
+    munmap(const_cast<unsigned char *>(in.base), in.size);
+    return 0;
+}
