@@ -144,6 +144,29 @@ __device__ __forceinline__ void load_column_fast(const void *__restrict__ img, i
     }
 }
 
+// Both fractional parts below 1e-4: the reference copies samples without interpolation
+// (correlate.rs:186-188).
+__device__ __forceinline__ bool integer_origin(float x, float y)
+{
+    return x - floorf(x) < 1e-4f && y - floorf(y) < 1e-4f;
+}
+
+// load_column_fast for integer origins: one sample per row, no interpolation.
+template <int N, int FMT>
+__device__ __forceinline__ void load_column_int(const void *__restrict__ img, int pitch, float x,
+                                                float y, int c, float (&col)[N])
+{
+    const int x0 = (int)floorf(x), y0 = (int)floorf(y);
+    const uint32_t o = (uint32_t)y0 * (uint32_t)pitch + (uint32_t)(x0 + c);
+    using T = std::conditional_t<FMT == FMT_U8, uint8_t, std::conditional_t<FMT == FMT_U16, uint16_t, float>>;
+    const __attribute__((address_space(1))) T *base = (const __attribute__((address_space(1))) T *)img;
+#pragma unroll
+    for (int r = 0; r < N; ++r) {
+        col[r] = (float)base[o + (uint32_t)r * (uint32_t)pitch];
+        if ((r & 15) == 15) __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
 // energy threshold of correlate.rs:110 in raw sample units
 template <int FMT>
 __device__ __forceinline__ double energy_floor()
@@ -163,8 +186,13 @@ __device__ __forceinline__ void ingest(const PassPair &pp, bool sig, float ax, f
 {
     // every window of the wave keeps its (N+1)^2 footprint inside the frame: no clamping
     const bool fast = __all(!sig || (interior<N>(ax, ay, pp.W, pp.H) && interior<N>(bx, by, pp.W, pp.H)));
+    // every window of the wave at integer origins (a first pass without predictor)
+    const bool ints = __all(!sig || (integer_origin(ax, ay) && integer_origin(bx, by)));
     if (sig) {
-        if (fast) {
+        if (fast && ints) {
+            load_column_int<N, FMT>(pp.img_a, pp.pitch, ax, ay, lane, zr);
+            load_column_int<N, FMT>(pp.img_b, pp.pitch, bx, by, lane, zi);
+        } else if (fast) {
             load_column_fast<N, FMT>(pp.img_a, pp.pitch, ax, ay, lane, zr);
             load_column_fast<N, FMT>(pp.img_b, pp.pitch, bx, by, lane, zi);
         } else {
