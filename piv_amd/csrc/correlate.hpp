@@ -295,7 +295,7 @@ __device__ __forceinline__ void find_peak_plane(const float *pr, int lane, int t
         const float vv = pr[((pby + lane) % N) * P + pbx];
         if constexpr (C::IN_WAVE) {
             const int base = tid - lane;
-            const unsigned long long full = N == 64 ? ~0ull : ((1ull << N) - 1);
+            const unsigned long long full = N == 64 ? ~0ull : ((1ull << (N & 63)) - 1);
             const unsigned long long hm = (__ballot(hv > half_best) >> base) & full;
             const unsigned long long vm = (__ballot(vv > half_best) >> base) & full;
             // forward: lanes 0, 1, 2, ...; backward: lanes 0, N-1, N-2, ...
@@ -448,7 +448,6 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
     float *red = s_red + wl * N;
     float *pkv = s_pkv + wl * N;
     int *pki = s_pki + wl * N;
-    const int ix = (int)(k % (uint32_t)g.nx), iy = (int)(k / (uint32_t)g.nx);
 
     // ---- origins and signal mask from the placement kernel ------------------------------
     float ax = 0.f, ay = 0.f, bx = 0.f, by = 0.f;

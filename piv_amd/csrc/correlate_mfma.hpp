@@ -55,7 +55,10 @@ struct MfCfg {
     static_assert(N == 16 || N == 32 || N == 64, "DFT-GEMM window sizes");
     static constexpr int T = N / 16;           // tiles per side
     static constexpr int WPW = 64 / N;         // windows per wave (lane per column)
-    static constexpr int WAVES = 4;
+    // the DFT matrix as B operands: in registers for N <= 32 (2 T^2 x 4 VGPRs), in LDS shared
+    // by four waves for N = 64
+    static constexpr bool BREG = T <= 2;
+    static constexpr int WAVES = BREG ? 1 : 4;
     static constexpr int WPB = WAVES * WPW;    // windows per workgroup
     static constexpr int BLOCK = 64 * WAVES;
     static constexpr int P = N + 1, PL = N * P;
@@ -63,6 +66,27 @@ struct MfCfg {
     // multiplies the RMS by sqrt(N)); exact, so it only rescales the plane
     static constexpr float S = N == 16 ? 0.25f : (N == 32 ? 0.25f : 0.125f);
 };
+
+// B operands of tile (p, t) of C = F (forward DFT, F[x][k] = exp(-2 pi i x k / N)) for this
+// lane, k-slots (re, 4g+j) and (im, 4g+j) of rows x = 16p + 4g + j:  B_re = [Fr; -Fi],
+// B_im = [Fi; Fr].  The hardware sine and cosine take the angle in turns; x k mod N / N is
+// exact, and the 16-bit rounding of the entries dominates their error.
+template <int N, typename E>
+__device__ __forceinline__ void mf_bconst(int p, int t, int lane, typename Mfma<E>::v8 &br,
+                                          typename Mfma<E>::v8 &bi)
+{
+    const int g = lane >> 4, n = lane & 15, k = 16 * t + n;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int x = 16 * p + 4 * g + j;
+        const float turns = (float)((x * k) & (N - 1)) * (1.0f / (float)N);
+        const float fr = __builtin_amdgcn_cosf(turns), fi = -__builtin_amdgcn_sinf(turns);
+        br[j] = (E)fr;
+        br[4 + j] = (E)(-fi);
+        bi[j] = (E)fi;
+        bi[4 + j] = (E)fr;
+    }
+}
 
 template <typename E>
 __device__ __forceinline__ typename Mfma<E>::v8 mf_pack(fv4 re, fv4 im, float s)
@@ -76,21 +100,46 @@ __device__ __forceinline__ typename Mfma<E>::v8 mf_pack(fv4 re, fv4 im, float s)
     return v;
 }
 
-// One output tile of a stage D = Z^T C:  (re, im) = sum_p A[p] * B[p][t]  (A: one row of tiles).
-template <int T, typename E>
-__device__ __forceinline__ void mf_tile(const typename Mfma<E>::v8 (&A)[T],
-                                        const typename Mfma<E>::v8 *bre,
-                                        const typename Mfma<E>::v8 *bim, int t, int lane,
-                                        fv4 &re, fv4 &im)
+// B operand source: registers (small N) or LDS
+template <int T, typename E, bool REG>
+struct MfB {
+    using V8 = typename Mfma<E>::v8;
+    const V8 (*rre)[T];
+    const V8 (*rim)[T];
+    const V8 *lre, *lim;
+    int lane;
+    __device__ __forceinline__ V8 re(int p, int t) const
+    {
+        if constexpr (REG) return rre[p][t];
+        else return lre[(p * T + t) * 64 + lane];
+    }
+    __device__ __forceinline__ V8 im(int p, int t) const
+    {
+        if constexpr (REG) return rim[p][t];
+        else return lim[(p * T + t) * 64 + lane];
+    }
+};
+
+// Output column t of a stage D = Z^T C for every row of tiles q:
+//   (re[q], im[q]) = sum_p A[q][p] * B[p][t]   (each B operand read once per column)
+template <int T, typename E, class BS>
+__device__ __forceinline__ void mf_col(const typename Mfma<E>::v8 (&A)[T][T], const BS &b, int t,
+                                       fv4 (&re)[T], fv4 (&im)[T])
 {
-    fv4 r = {0.f, 0.f, 0.f, 0.f}, i = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int q = 0; q < T; ++q) {
+        re[q] = fv4{0.f, 0.f, 0.f, 0.f};
+        im[q] = fv4{0.f, 0.f, 0.f, 0.f};
+    }
 #pragma unroll
     for (int p = 0; p < T; ++p) {
-        r = Mfma<E>::mma(A[p], bre[(p * T + t) * 64 + lane], r);
-        i = Mfma<E>::mma(A[p], bim[(p * T + t) * 64 + lane], i);
+        const auto br = b.re(p, t), bi = b.im(p, t);
+#pragma unroll
+        for (int q = 0; q < T; ++q) {
+            re[q] = Mfma<E>::mma(A[q][p], br, re[q]);
+            im[q] = Mfma<E>::mma(A[q][p], bi, im[q]);
+        }
     }
-    re = r;
-    im = i;
 }
 
 template <int N, int FMT, typename E>
@@ -105,26 +154,22 @@ __global__ __launch_bounds__(MfCfg<N>::BLOCK) void k_correlate_mf(const PassPair
     __shared__ float s_red[WPB * N];
     __shared__ float s_pkv[WPB * N];
     __shared__ int s_pki[WPB * N];
-    __shared__ V8 s_bre[T * T * 64], s_bim[T * T * 64];  // DFT matrix as B operands
+    __shared__ V8 s_bre[M::BREG ? 1 : T * T * 64], s_bim[M::BREG ? 1 : T * T * 64];
 
     const int tid = threadIdx.x, wlane = tid & 63;
-    // B operands of C = F (forward DFT, F[x][k] = exp(-2 pi i x k / N)), k-slots (re, 4g+j)
-    // and (im, 4g+j) of rows 16p + 4g + j:  B_re = [Fr; -Fi], B_im = [Fi; Fr]
-    for (int e = tid; e < T * T * 64; e += M::BLOCK) {
-        const int pt = e >> 6, l = e & 63, p = pt / T, t = pt - p * T, g = l >> 4, n = l & 15;
-        V8 br, bi;
+    V8 r_bre[T][T], r_bim[T][T];  // unused (and removed) when the operands live in LDS
+    if constexpr (M::BREG) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int x = 16 * p + 4 * g + j, k = 16 * t + n, m = (x * k) % N;
-            const float fr = pivfft::Tw<N>::t.c[m], fi = -pivfft::Tw<N>::t.s[m];
-            br[j] = (E)fr;
-            br[4 + j] = (E)(-fi);
-            bi[j] = (E)fi;
-            bi[4 + j] = (E)fr;
+        for (int p = 0; p < T; ++p)
+#pragma unroll
+            for (int t = 0; t < T; ++t) mf_bconst<N, E>(p, t, wlane, r_bre[p][t], r_bim[p][t]);
+    } else {
+        for (int e = tid; e < T * T * 64; e += M::BLOCK) {
+            const int pt = e >> 6;
+            mf_bconst<N, E>(pt / T, pt % T, e & 63, s_bre[e], s_bim[e]);
         }
-        s_bre[e] = br;
-        s_bim[e] = bi;
     }
+    MfB<T, E, M::BREG> B{r_bre, r_bim, s_bre, s_bim, wlane};
 
     // ---- XCD-aware block order (as k_correlate) ----
     const uint32_t L = blockIdx.x;
@@ -182,30 +227,32 @@ __global__ __launch_bounds__(MfCfg<N>::BLOCK) void k_correlate_mf(const PassPair
         float *pw = s_pl + (wave * M::WPW + w) * PL;
         const E *pa = reinterpret_cast<const E *>(pw), *pb = pa + N * N;
         V8 A2a[T][T], A2b[T][T];
-        // stage 1: DFT along y of each real window (imaginary k-slots zero); the result tile
+        // stage 1: DFT along y of each real window (imaginary k-slots zero); result tile
         // (q, t) is the stage-2 operand A2[t][q]
 #pragma unroll
         for (int img = 0; img < 2; ++img) {
             const E *pl = img ? pb : pa;
+            V8 A1[T][T];
 #pragma unroll
-            for (int q = 0; q < T; ++q) {
-                V8 A1[T];
+            for (int q = 0; q < T; ++q)
 #pragma unroll
                 for (int p = 0; p < T; ++p) {
                     const V4 h = *reinterpret_cast<const V4 *>(pl + (16 * q + nn) * N + 16 * p + 4 * gq);
-                    V8 a;
+                    V8 v;
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
-                        a[j] = h[j];
-                        a[4 + j] = (E)0.0f;
+                        v[j] = h[j];
+                        v[4 + j] = (E)0.0f;
                     }
-                    A1[p] = a;
+                    A1[q][p] = v;
                 }
 #pragma unroll
-                for (int t = 0; t < T; ++t) {
-                    fv4 re, im;
-                    mf_tile<T, E>(A1, s_bre, s_bim, t, wlane, re, im);
-                    const V8 v = mf_pack<E>(re, im, M::S);
+            for (int t = 0; t < T; ++t) {
+                fv4 re[T], im[T];
+                mf_col<T, E>(A1, B, t, re, im);
+#pragma unroll
+                for (int q = 0; q < T; ++q) {
+                    const V8 v = mf_pack<E>(re[q], im[q], M::S);
                     if (img) A2b[t][q] = v;
                     else A2a[t][q] = v;
                 }
@@ -214,16 +261,17 @@ __global__ __launch_bounds__(MfCfg<N>::BLOCK) void k_correlate_mf(const PassPair
         // stage 2: DFT along x -> spectra; cross spectrum conj(X) = F_A conj(F_B), DC cleared
         V8 A3[T][T];
 #pragma unroll
-        for (int t = 0; t < T; ++t)
+        for (int u = 0; u < T; ++u) {
+            fv4 ar[T], ai[T], br[T], bi[T];
+            mf_col<T, E>(A2a, B, u, ar, ai);
+            mf_col<T, E>(A2b, B, u, br, bi);
 #pragma unroll
-            for (int u = 0; u < T; ++u) {
-                fv4 ar, ai, br, bi, xr, xi;
-                mf_tile<T, E>(A2a[t], s_bre, s_bim, u, wlane, ar, ai);
-                mf_tile<T, E>(A2b[t], s_bre, s_bim, u, wlane, br, bi);
+            for (int t = 0; t < T; ++t) {
+                fv4 xr, xi;
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    xr[j] = __builtin_fmaf(ar[j], br[j], ai[j] * bi[j]);
-                    xi[j] = __builtin_fmaf(ai[j], br[j], -(ar[j] * bi[j]));
+                    xr[j] = __builtin_fmaf(ar[t][j], br[t][j], ai[t][j] * bi[t][j]);
+                    xi[j] = __builtin_fmaf(ai[t][j], br[t][j], -(ar[t][j] * bi[t][j]));
                 }
                 if (t == 0 && u == 0 && wlane == 0) {
                     xr[0] = 0.0f;
@@ -231,26 +279,32 @@ __global__ __launch_bounds__(MfCfg<N>::BLOCK) void k_correlate_mf(const PassPair
                 }
                 A3[u][t] = mf_pack<E>(xr, xi, M::S * M::S);
             }
+        }
         // stage 3: along k1; stage 4: along k2, real part only -> N^2 c[y][x] (scaled)
         V8 A4[T][T];
 #pragma unroll
-        for (int u = 0; u < T; ++u)
+        for (int v = 0; v < T; ++v) {
+            fv4 re[T], im[T];
+            mf_col<T, E>(A3, B, v, re, im);
 #pragma unroll
-            for (int v = 0; v < T; ++v) {
-                fv4 re, im;
-                mf_tile<T, E>(A3[u], s_bre, s_bim, v, wlane, re, im);
-                A4[v][u] = mf_pack<E>(re, im, M::S);
+            for (int u = 0; u < T; ++u) A4[v][u] = mf_pack<E>(re[u], im[u], M::S);
+        }
+#pragma unroll
+        for (int x = 0; x < T; ++x) {
+            fv4 c[T];
+#pragma unroll
+            for (int v = 0; v < T; ++v) c[v] = fv4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int u = 0; u < T; ++u) {
+                const V8 br = B.re(u, x);
+#pragma unroll
+                for (int v = 0; v < T; ++v) c[v] = Mfma<E>::mma(A4[v][u], br, c[v]);
             }
 #pragma unroll
-        for (int v = 0; v < T; ++v)
+            for (int v = 0; v < T; ++v)
 #pragma unroll
-            for (int x = 0; x < T; ++x) {
-                fv4 c = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                for (int u = 0; u < T; ++u) c = Mfma<E>::mma(A4[v][u], s_bre[(u * T + x) * 64 + wlane], c);
-#pragma unroll
-                for (int j = 0; j < 4; ++j) pw[(16 * v + 4 * gq + j) * P + 16 * x + nn] = c[j];
-            }
+                for (int j = 0; j < 4; ++j) pw[(16 * v + 4 * gq + j) * P + 16 * x + nn] = c[v][j];
+        }
     }
     __syncthreads();
     find_peak_plane<N, FMT>(pr, lane, wlane, s_pkv + wl * N, s_pki + wl * N, active, has_peak,
