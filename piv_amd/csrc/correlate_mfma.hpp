@@ -62,8 +62,9 @@ struct MfCfg {
     static constexpr int WPB = WAVES * WPW;    // windows per workgroup
     static constexpr int BLOCK = 64 * WAVES;
     static constexpr int P = N + 1, PL = N * P;
-    // power-of-two stage scale keeping 16-bit intermediates near unit RMS (a DFT of length N
-    // multiplies the RMS by sqrt(N)); exact, so it only rescales the plane
+    // power-of-two scale of each DFT stage keeping 16-bit intermediates near unit RMS (a DFT
+    // of length N multiplies the RMS by sqrt(N), a cross spectrum squares it); exact, so it
+    // only rescales the plane
     static constexpr float S = N == 16 ? 0.25f : (N == 32 ? 0.25f : 0.125f);
 };
 
@@ -75,12 +76,15 @@ template <int N, typename E>
 __device__ __forceinline__ void mf_bconst(int p, int t, int lane, typename Mfma<E>::v8 &br,
                                           typename Mfma<E>::v8 &bi)
 {
+    // the stage scale S (a power of two) is folded into the matrix: exact, and the stage
+    // results come out ready to pack
+    constexpr float S = MfCfg<N>::S;
     const int g = lane >> 4, n = lane & 15, k = 16 * t + n;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int x = 16 * p + 4 * g + j;
         const float turns = (float)((x * k) & (N - 1)) * (1.0f / (float)N);
-        const float fr = __builtin_amdgcn_cosf(turns), fi = -__builtin_amdgcn_sinf(turns);
+        const float fr = S * __builtin_amdgcn_cosf(turns), fi = -S * __builtin_amdgcn_sinf(turns);
         br[j] = (E)fr;
         br[4 + j] = (E)(-fi);
         bi[j] = (E)fi;
@@ -89,13 +93,13 @@ __device__ __forceinline__ void mf_bconst(int p, int t, int lane, typename Mfma<
 }
 
 template <typename E>
-__device__ __forceinline__ typename Mfma<E>::v8 mf_pack(fv4 re, fv4 im, float s)
+__device__ __forceinline__ typename Mfma<E>::v8 mf_pack(fv4 re, fv4 im)
 {
     typename Mfma<E>::v8 v;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        v[j] = (E)(re[j] * s);
-        v[4 + j] = (E)(im[j] * s);
+        v[j] = (E)re[j];
+        v[4 + j] = (E)im[j];
     }
     return v;
 }
@@ -252,7 +256,7 @@ __global__ __launch_bounds__(MfCfg<N>::BLOCK) void k_correlate_mf(const PassPair
                 mf_col<T, E>(A1, B, t, re, im);
 #pragma unroll
                 for (int q = 0; q < T; ++q) {
-                    const V8 v = mf_pack<E>(re[q], im[q], M::S);
+                    const V8 v = mf_pack<E>(re[q], im[q]);
                     if (img) A2b[t][q] = v;
                     else A2a[t][q] = v;
                 }
@@ -277,7 +281,7 @@ __global__ __launch_bounds__(MfCfg<N>::BLOCK) void k_correlate_mf(const PassPair
                     xr[0] = 0.0f;
                     xi[0] = 0.0f;
                 }
-                A3[u][t] = mf_pack<E>(xr, xi, M::S * M::S);
+                A3[u][t] = mf_pack<E>(xr, xi);
             }
         }
         // stage 3: along k1; stage 4: along k2, real part only -> N^2 c[y][x] (scaled)
@@ -287,7 +291,7 @@ __global__ __launch_bounds__(MfCfg<N>::BLOCK) void k_correlate_mf(const PassPair
             fv4 re[T], im[T];
             mf_col<T, E>(A3, B, v, re, im);
 #pragma unroll
-            for (int u = 0; u < T; ++u) A4[v][u] = mf_pack<E>(re[u], im[u], M::S);
+            for (int u = 0; u < T; ++u) A4[v][u] = mf_pack<E>(re[u], im[u]);
         }
 #pragma unroll
         for (int x = 0; x < T; ++x) {
