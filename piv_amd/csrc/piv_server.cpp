@@ -81,11 +81,13 @@ struct Lane {
     std::vector<size_t> sum_index;  // per request index of its final summary in `sums`
     size_t out_bytes = 0, sum_count = 0;
     std::map<std::vector<int64_t>, hipGraphExec_t> graphs;
+    std::map<std::vector<int64_t>, int> seen;  // launch structures run eagerly so far
     bool busy = false;
     void drop_graphs()
     {
         for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
         graphs.clear();
+        seen.clear();
     }
 };
 
@@ -263,6 +265,7 @@ void build_batch(Lane &L, BatchPlan &bp)
             const std::pair<int, int> kk{r->st->plans[si].n, r->slot->fa->fmt};
             if (std::find(keys.begin(), keys.end(), kk) == keys.end()) keys.push_back(kk);
         }
+        std::sort(keys.begin(), keys.end());
         for (auto &kk : keys) {
             BatchPlan::Group g{kk.first, kk.second, 0, -1, 0, doff};
             for (Request *r : reqs) {
@@ -421,9 +424,14 @@ piv_status launch_batch(piv_server *s, Lane &L)
     for (Request *r : L.batch) HIP_TRY(hipStreamWaitEvent(st, r->slot->uploaded, 0));
     std::memcpy(L.h_desc.p, bp.host.data(), bp.host.size());
     HIP_TRY(hipMemcpyAsync(L.desc.p, L.h_desc.p, bp.host.size(), hipMemcpyHostToDevice, st));
+    // A structure seen for the first time is launched eagerly (capturing costs more than the
+    // launches); one that recurs is captured once and replayed from then on.
     auto it = L.graphs.find(bp.key);
     if (it != L.graphs.end()) {
         HIP_TRY(hipGraphLaunch(it->second, st));
+    } else if (++L.seen[bp.key] < 2) {
+        if (L.seen.size() > 4096) L.seen.clear();
+        HIP_TRY(enqueue_batch(L, bp));
     } else {
         if (L.graphs.size() >= kMaxGraphsPerLane) L.drop_graphs();
         hipGraph_t graph = nullptr;

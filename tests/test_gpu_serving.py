@@ -103,8 +103,8 @@ def test_mixed_streams_equal_single_pair_analysis(piv):
 
 def test_repeated_structure_replays_graph(piv):
     """Same stream, one request at a time: every batch has the same launch structure, so the
-    second and later batches replay one captured graph; eager single-pair work runs between
-    the replays and every replay matches it."""
+    first batch runs eagerly, the second is captured and later ones replay that graph; eager
+    single-pair work runs between the replays and every replay matches it."""
     srv = piv.PivServer(max_batch_pairs=4, max_inflight=1)
     cfg = piv.PivConfig(windows=[64, 32, 16], final_iterations=1)
     sid = srv.add_stream(cfg, 256, 320, np.uint8, piv.PreprocessConfig.raw())
