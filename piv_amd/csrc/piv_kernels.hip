@@ -329,134 +329,136 @@ __global__ void k_base(const void *__restrict__ raw, int pitch, long raw_frame_b
 }
 
 // SAT (preprocess.rs:84-110), step 1: running row sums of v and v^2 in f64, in row order
-// (one row per lane, the reference's summation order), written at (y+1, x+1).  A wave owns 64
-// rows: tiles of 64 rows x 32 columns are loaded and stored coalesced through LDS and each
-// lane scans its row of the tile from LDS.
-constexpr int SAT_ROWS = 64, SAT_TW = 32;
+// (one row per lane, the reference's summation order), written at (y+1, x+1).  A wave owns
+// SATR_ROWS rows and walks tiles of 64 columns: lane c loads column c of the next tile
+// (coalesced) while the current tile is scanned from LDS, then stores column c of the scanned
+// tile (coalesced).  Few rows per wave spread one frame over many CUs; the chain per row is
+// serial whatever the split, so the scan costs the same per tile.
+constexpr int SAT_ROWS = 64, SAT_TW = 32, SATR_ROWS = 16, SATR_TW = 64;
 
-__global__ __launch_bounds__(SAT_ROWS) void k_sat_rows(const float *__restrict__ img, int H, int W,
-                                                       double *__restrict__ sum,
-                                                       double *__restrict__ sq, long sat_stride)
+__global__ __launch_bounds__(64) void k_sat_rows(const float *__restrict__ img, int H, int W,
+                                                 double *__restrict__ sum,
+                                                 double *__restrict__ sq, long sat_stride)
 {
+    __shared__ float tin[SATR_ROWS][SATR_TW + 1];
+    __shared__ double ts[SATR_ROWS][SATR_TW + 1], tq[SATR_ROWS][SATR_TW + 1];
     // blockIdx.y: frame of a batch (images H*W apart, tables sat_stride apart)
     img += (long)blockIdx.y * H * W;
     sum += (long)blockIdx.y * sat_stride;
     sq += (long)blockIdx.y * sat_stride;
-    __shared__ float tin[SAT_ROWS][SAT_TW + 1];
-    __shared__ double ts[SAT_ROWS][SAT_TW + 1], tq[SAT_ROWS][SAT_TW + 1];
-    const int lane = threadIdx.x, y0 = blockIdx.x * SAT_ROWS;
+    const int lane = threadIdx.x, y0 = blockIdx.x * SATR_ROWS;
     const long s = (long)W + 1;
     if (blockIdx.x == 0)
-        for (int x = lane; x <= W; x += SAT_ROWS) {
+        for (int x = lane; x <= W; x += 64) {
             sum[x] = 0.0;
             sq[x] = 0.0;
         }
-    if (y0 + lane < H) {
+    if (lane < SATR_ROWS && y0 + lane < H) {
         sum[(long)(y0 + lane + 1) * s] = 0.0;
         sq[(long)(y0 + lane + 1) * s] = 0.0;
     }
-    const int sub = lane >> 5, c = lane & 31;  // tile transfers: 2 rows x 32 columns per step
+    const int rows = min(SATR_ROWS, H - y0), ntile = (W + SATR_TW - 1) / SATR_TW;
+    float pre[SATR_ROWS];
+    auto fetch = [&](int t) {
+        const int x = t * SATR_TW + lane;
+#pragma unroll
+        for (int r = 0; r < SATR_ROWS; ++r)
+            pre[r] = (r < rows && x < W) ? img[(long)(y0 + r) * W + x] : 0.0f;
+    };
+    fetch(0);
     double rs = 0.0, rq = 0.0;
-    for (int x0 = 0; x0 < W; x0 += SAT_TW) {
-        const int x = x0 + c;
-        for (int i = 0; i < SAT_ROWS / 2; ++i) {
-            const int r = 2 * i + sub, y = y0 + r;
-            tin[r][c] = (y < H && x < W) ? img[(long)y * W + x] : 0.0f;
-        }
+    for (int t = 0; t < ntile; ++t) {
         __syncthreads();
-        // the whole tile row first (the LDS reads overlap), then the two dependent chains;
-        // columns past the frame edge hold 0, and x + 0 = x leaves the sums unchanged
-        float v[SAT_TW];
 #pragma unroll
-        for (int k = 0; k < SAT_TW; ++k) v[k] = tin[lane][k];
-#pragma unroll
-        for (int k = 0; k < SAT_TW; ++k) {
-            const double d = (double)v[k];
-            rs += d;
-            rq += d * d;
-            ts[lane][k] = rs;
-            tq[lane][k] = rq;
-        }
+        for (int r = 0; r < SATR_ROWS; ++r) tin[r][lane] = pre[r];
         __syncthreads();
-        if (x < W)
-            for (int i = 0; i < SAT_ROWS / 2; ++i) {
-                const int r = 2 * i + sub, y = y0 + r;
-                if (y < H) {
-                    sum[(long)(y + 1) * s + x + 1] = ts[r][c];
-                    sq[(long)(y + 1) * s + x + 1] = tq[r][c];
-                }
+        if (t + 1 < ntile) fetch(t + 1);
+        if (lane < SATR_ROWS) {
+            // columns past the frame edge hold 0 and are never stored
+            float v[SATR_TW];
+#pragma unroll
+            for (int k = 0; k < SATR_TW; ++k) v[k] = tin[lane][k];
+#pragma unroll
+            for (int k = 0; k < SATR_TW; ++k) {
+                const double d = (double)v[k];
+                rs += d;
+                rq += d * d;
+                ts[lane][k] = rs;
+                tq[lane][k] = rq;
             }
+        }
         __syncthreads();
+        const int x = t * SATR_TW + lane;
+        if (x < W)
+            for (int r = 0; r < rows; ++r) {
+                sum[(long)(y0 + r + 1) * s + x + 1] = ts[r][lane];
+                sq[(long)(y0 + r + 1) * s + x + 1] = tq[r][lane];
+            }
     }
 }
 
-// SAT, step 2: t[y+1][x] = t[y][x] + rowsum[y][x], down each column in row order (one column
-// per thread, coalesced across the wave).  Software-pipelined: the loads of the next 16 rows
-// are issued before the dependent adds and stores of the current 16.
-constexpr int SAT_CH = 16;
+// SAT, step 2: t[y+1][x] = t[y][x] + rowsum[y][x], down each column in row order.  A wave owns
+// a strip of SATC_W columns and walks chunks of SATC_R rows: all 64 lanes load the next chunk
+// (4 rows per instruction, SATC_W * 8 contiguous bytes each) while lanes 0-15 carry the sums and
+// lanes 16-31 the squares of the current chunk down their column in LDS, one add per row.
+constexpr int SATC_W = 16, SATC_R = 64, SATC_PER = SATC_R * SATC_W / 64;
 
-__device__ __forceinline__ void sat_col_chunk(double *__restrict__ sum, double *__restrict__ sq,
-                                              long idx, long s, const double (&a)[SAT_CH],
-                                              const double (&b)[SAT_CH], double &cs, double &cq)
+__global__ __launch_bounds__(64) void k_sat_cols(int H, int W, double *__restrict__ sum,
+                                                 double *__restrict__ sq, long sat_stride)
 {
-#pragma unroll
-    for (int k = 0; k < SAT_CH; ++k) {
-        cs = cs + a[k];
-        cq = cq + b[k];
-        sum[idx + k * s] = cs;
-        sq[idx + k * s] = cq;
-    }
-}
-
-__global__ void k_sat_cols(int H, int W, double *__restrict__ sum, double *__restrict__ sq,
-                           long sat_stride)
-{
+    __shared__ double lt[2][SATC_R][SATC_W + 1];
     sum += (long)blockIdx.y * sat_stride;
     sq += (long)blockIdx.y * sat_stride;
-    const int x = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    if (x > W) return;
+    const int lane = threadIdx.x, c = lane % SATC_W, g = lane / SATC_W;
+    const int x = blockIdx.x * SATC_W + c + 1;
+    const bool okx = x <= W;
     const long s = (long)W + 1;
-    double cs = 0.0, cq = 0.0;
-    const int full = H / SAT_CH;  // whole chunks of rows 1 .. full * SAT_CH
-    double a0[SAT_CH], b0[SAT_CH], a1[SAT_CH], b1[SAT_CH];
-    long idx = s + x;
-    if (full > 0) {
+    const int nch = (H + SATC_R - 1) / SATC_R;
+    double a[SATC_PER], b[SATC_PER];
+    auto fetch = [&](int k) {
 #pragma unroll
-        for (int k = 0; k < SAT_CH; ++k) {
-            a0[k] = sum[idx + k * s];
-            b0[k] = sq[idx + k * s];
+        for (int j = 0; j < SATC_PER; ++j) {
+            const int y = 1 + k * SATC_R + g + 4 * j;
+            const bool ok = okx && y <= H;
+            a[j] = ok ? sum[(long)y * s + x] : 0.0;
+            b[j] = ok ? sq[(long)y * s + x] : 0.0;
         }
-    }
-    for (int c = 0; c < full; c += 2) {
-        const long nidx = idx + SAT_CH * s;
-        if (c + 1 < full) {
+    };
+    fetch(0);
+    // lanes 0-15: sums of column c, lanes 16-31: squares; rows past H hold 0 and are never stored
+    double carry = 0.0;
+    double(*chain)[SATC_W + 1] = lt[g & 1];
+    for (int k = 0; k < nch; ++k) {
+        __syncthreads();
 #pragma unroll
-            for (int k = 0; k < SAT_CH; ++k) {
-                a1[k] = sum[nidx + k * s];
-                b1[k] = sq[nidx + k * s];
+        for (int j = 0; j < SATC_PER; ++j) {
+            lt[0][g + 4 * j][c] = a[j];
+            lt[1][g + 4 * j][c] = b[j];
+        }
+        __syncthreads();
+        if (k + 1 < nch) fetch(k + 1);
+        if (g < 2) {
+#pragma unroll
+            for (int r0 = 0; r0 < SATC_R; r0 += 16) {
+                double v[16];
+#pragma unroll
+                for (int r = 0; r < 16; ++r) v[r] = chain[r0 + r][c];
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    carry = carry + v[r];
+                    chain[r0 + r][c] = carry;
+                }
             }
         }
-        sat_col_chunk(sum, sq, idx, s, a0, b0, cs, cq);
-        if (c + 1 >= full) {
-            idx = nidx;
-            break;
-        }
-        const long n2 = nidx + SAT_CH * s;
-        if (c + 2 < full) {
+        __syncthreads();
 #pragma unroll
-            for (int k = 0; k < SAT_CH; ++k) {
-                a0[k] = sum[n2 + k * s];
-                b0[k] = sq[n2 + k * s];
+        for (int j = 0; j < SATC_PER; ++j) {
+            const int y = 1 + k * SATC_R + g + 4 * j;
+            if (okx && y <= H) {
+                sum[(long)y * s + x] = lt[0][g + 4 * j][c];
+                sq[(long)y * s + x] = lt[1][g + 4 * j][c];
             }
         }
-        sat_col_chunk(sum, sq, nidx, s, a1, b1, cs, cq);
-        idx = n2;
-    }
-    for (int y = full * SAT_CH + 1; y <= H; ++y, idx += s) {
-        cs = cs + sum[idx];
-        cq = cq + sq[idx];
-        sum[idx] = cs;
-        sq[idx] = cq;
     }
 }
 
@@ -865,12 +867,12 @@ hipError_t launch_base(int fmt, const void *raw, int pitch, const float *bg, int
 hipError_t launch_sat(const float *img, int H, int W, double *sum, double *sq, hipStream_t st,
                       int nframes, long sat_stride)
 {
-    hipLaunchKernelGGL(k_sat_rows, dim3(nblk(H, SAT_ROWS), (unsigned)nframes), dim3(SAT_ROWS), 0, st,
+    hipLaunchKernelGGL(k_sat_rows, dim3(nblk(H, SATR_ROWS), (unsigned)nframes), dim3(64), 0, st,
                        img, H, W, sum, sq, sat_stride);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_sat_cols, dim3(nblk(W, 64), (unsigned)nframes), dim3(64), 0, st, H, W, sum,
-                       sq, sat_stride);
+    hipLaunchKernelGGL(k_sat_cols, dim3(nblk(W, SATC_W), (unsigned)nframes), dim3(64), 0, st, H, W,
+                       sum, sq, sat_stride);
     return hipGetLastError();
 }
 
