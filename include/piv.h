@@ -343,6 +343,74 @@ piv_status piv_autotune(piv_analyzer *a, const piv_frame *const *frames_a,
 piv_status piv_normalize_pixels(int device, const void *pixels, piv_pixel_format fmt,
                                 size_t count, float *out);
 
+/* ---- flow statistics of validated fields ----------------------------------------------- */
+
+/* FlowStats (flowstats.rs:9-25) plus the neighbour noise estimate (flowstats.rs:103-137). */
+typedef struct {
+    float vorticity_mean;     /* 1/s for px/s input; + = counter-clockwise on screen; NaN if none */
+    float vorticity_abs_mean; /* mean |vorticity|, NaN if none */
+    float ke_mean;            /* 1/2 <u^2 + v^2> over usable vectors (px^2/s^2), NaN if none */
+    float ke_sum;             /* 1/2 sum (u^2 + v^2) step^2 (px^4/s^2), 0 if none */
+    float coverage;           /* usable vectors / grid cells */
+    float noise_px;           /* neighbour_noise of dx, dy (px); NaN with < 20 residuals */
+    uint32_t n_usable, n_vorticity;
+} piv_flow_stats;
+
+/* Replaces the per-pair statistics of `piv analyze` (analyze.rs:483-489): usable vectors
+ * (flow.rs:13-17: finite and accepted or REPLACED), pair_stats (flow.rs:20-34: velocity
+ * = dx * f32(1 / dt_s), grid step as the spacing) and neighbour_noise of the raw displacements
+ * (flowstats.rs:106-137), for `npairs` validated fields of one grid, device-resident
+ * [npairs][ny][nx] (dx, dy in px, flags).  `out` is host memory [npairs].  Runs on the
+ * analyzer's stream and device; profiled as "flowstats". */
+piv_status piv_flow_stats_device(piv_analyzer *a, const piv_grid *grid, uint32_t npairs,
+                                 const float *dx, const float *dy, const uint8_t *flags,
+                                 double dt_s, piv_flow_stats *out);
+
+/* ---- time-series filters of a run (smooth.rs) ------------------------------------------ */
+/* Series are device-resident [nt][ny][nx] (pair, row, column); masks are u8 0/1.  Each call
+ * runs on its own stream of `device` and returns when the results are written. */
+
+typedef enum { PIV_TEMPORAL_MEAN = 0, PIV_TEMPORAL_MEDIAN = 1 } piv_temporal_kernel;
+
+/* smooth.rs:44-130 temporal(): moving mean or median over `window` (odd, <= 63) consecutive
+ * pairs of usable vectors, outputs only where `gate`, NaN below max(min_support, 1) samples;
+ * leave_out: offsets skipped (each non-zero one replaced by the next further out). */
+piv_status piv_smooth_temporal_device(int device, uint32_t nt, uint32_t ny, uint32_t nx,
+                                      const float *dx, const float *dy, const uint8_t *usable,
+                                      const uint8_t *gate, piv_temporal_kernel kernel,
+                                      uint32_t window, uint32_t min_support,
+                                      const int32_t *leave_out, uint32_t n_leave, float *out_dx,
+                                      float *out_dy, uint16_t *support);
+
+/* smooth.rs:137-208 spatial(): per-pair Gaussian normalized convolution, sigma in cells
+ * (> 0, radius ceil(2.5 sigma) <= 32); leave_out >= 0 skips that Chebyshev radius, < 0 none. */
+piv_status piv_smooth_spatial_device(int device, uint32_t nt, uint32_t ny, uint32_t nx,
+                                     const float *dx, const float *dy, const uint8_t *gate,
+                                     float sigma, float min_weight, int32_t leave_out,
+                                     float *out_dx, float *out_dy, uint16_t *support);
+
+/* smooth.rs:303-310 mask(): NaN where not usable. */
+piv_status piv_mask_device(int device, size_t n, const float *values, const uint8_t *usable,
+                           float *out);
+
+/* smooth.rs:213-232 cross_validation_errors(): |raw - prediction| of the usable vectors in
+ * index order into out[0 .. *count) (device); *count on the host. */
+piv_status piv_cross_validation_errors_device(int device, size_t n, const float *raw_dx,
+                                              const float *raw_dy, const uint8_t *usable,
+                                              const float *pred_dx, const float *pred_dy,
+                                              float *out, uint64_t *count);
+
+/* smooth.rs:236-249 median_error(): median of the finite errors where keep (NULL: all);
+ * NaN and 0 when there are none.  Results on the host. */
+piv_status piv_median_error_device(int device, const float *errors, size_t n, const uint8_t *keep,
+                                   float *median, uint64_t *count);
+
+/* smooth.rs:252-274 pair_to_pair_change() and 277-300 spatial_roughness() (host result). */
+piv_status piv_pair_to_pair_change_device(int device, uint32_t nt, uint32_t ny, uint32_t nx,
+                                          const float *dx, const float *dy, float *out);
+piv_status piv_spatial_roughness_device(int device, uint32_t nt, uint32_t ny, uint32_t nx,
+                                        const float *dx, const float *dy, float *out);
+
 #ifdef __cplusplus
 }
 #endif

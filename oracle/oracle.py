@@ -54,6 +54,16 @@ class Peak64(C.Structure):
                 ("ppr", C.c_double), ("iy", C.c_int32), ("ix", C.c_int32)]
 
 
+class FlowStat(C.Structure):
+    _fields_ = [("vorticity_mean", C.c_float), ("vorticity_abs_mean", C.c_float),
+                ("ke_mean", C.c_float), ("ke_sum", C.c_float), ("coverage", C.c_float),
+                ("n_usable", C.c_int64), ("n_vorticity", C.c_int64)]
+
+    def as_dict(self):
+        return {n: (int(getattr(self, n)) if n.startswith("n_") else float(getattr(self, n)))
+                for n, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -110,6 +120,28 @@ def lib():
                                   C.c_float, C.POINTER(Summary)]
         L.or_median.argtypes = [f32, C.c_int32]
         L.or_median.restype = C.c_float
+        u16 = P(np.uint16, flags="C")
+        L.or_flow_usable.argtypes = [C.c_int64, f32, u8, u8]
+        L.or_vorticity.argtypes = [C.c_int32, C.c_int32, f32, f32, u8, C.c_float, f32]
+        L.or_flow_stats.argtypes = [C.c_int32, C.c_int32, f32, f32, u8, C.c_float,
+                                    C.POINTER(FlowStat)]
+        L.or_pair_stats.argtypes = [C.c_int32, C.c_int32, f32, f32, u8, C.c_int32, C.c_double,
+                                    C.POINTER(FlowStat)]
+        L.or_neighbour_noise.argtypes = [C.c_int32, C.c_int32, f32, f32, u8]
+        L.or_neighbour_noise.restype = C.c_float
+        L.or_temporal.argtypes = [C.c_int32, C.c_int32, C.c_int32, f32, f32, u8, u8, C.c_int32,
+                                  C.c_int32, C.c_int32, i32, C.c_int32, f32, f32, u16]
+        L.or_spatial.argtypes = [C.c_int32, C.c_int32, C.c_int32, f32, f32, u8, C.c_float,
+                                 C.c_float, C.c_int32, f32, f32, u16]
+        L.or_cross_validation_errors.argtypes = [C.c_int64, f32, f32, u8, f32, f32, f32]
+        L.or_cross_validation_errors.restype = C.c_int64
+        L.or_median_error.argtypes = [f32, C.c_int64, C.c_void_p, C.POINTER(C.c_int64)]
+        L.or_median_error.restype = C.c_float
+        L.or_pair_to_pair_change.argtypes = [C.c_int32, C.c_int32, C.c_int32, f32, f32]
+        L.or_pair_to_pair_change.restype = C.c_float
+        L.or_spatial_roughness.argtypes = [C.c_int32, C.c_int32, C.c_int32, f32, f32]
+        L.or_spatial_roughness.restype = C.c_float
+        L.or_mask.argtypes = [C.c_int64, f32, u8, f32]
     return _lib
 
 
@@ -388,3 +420,107 @@ def analyze(pa: Prepared, pb: Prepared, cfg: PivConfig, precision=32, threads=0,
             traces.append(PassTrace(g.as_tuple(), origins, raw, val))
         prev_grid, prev = g, val
     return (prev, summary, traces) if traced else (prev, summary)
+
+
+# ---- post-analysis stages (oracle/post_oracle.c) ----------------------------------------
+def _u8(a):
+    return np.ascontiguousarray(a, dtype=np.uint8)
+
+
+def flow_usable(dx, flags):
+    """flow.rs:13-17"""
+    dx, fl = _f32(dx), _u8(flags)
+    out = np.empty(dx.shape, np.uint8)
+    lib().or_flow_usable(dx.size, dx, fl, out)
+    return out.astype(bool)
+
+
+def vorticity(u, v, usable, step):
+    """flowstats.rs:48-66"""
+    u, v = _f32(u), _f32(v)
+    w = np.empty(u.shape, np.float32)
+    lib().or_vorticity(u.shape[0], u.shape[1], u, v, _u8(usable), step, w)
+    return w
+
+
+def flow_stats(u, v, usable, step):
+    """flowstats.rs:69-101"""
+    u, v = _f32(u), _f32(v)
+    s = FlowStat()
+    lib().or_flow_stats(u.shape[0], u.shape[1], u, v, _u8(usable), step, C.byref(s))
+    return s.as_dict()
+
+
+def pair_stats(dx, dy, usable, step, dt_s):
+    """flow.rs:20-34"""
+    dx, dy = _f32(dx), _f32(dy)
+    s = FlowStat()
+    lib().or_pair_stats(dx.shape[0], dx.shape[1], dx, dy, _u8(usable), int(step), dt_s,
+                        C.byref(s))
+    return s.as_dict()
+
+
+def neighbour_noise(u, v, usable):
+    """flowstats.rs:106-137"""
+    u, v = _f32(u), _f32(v)
+    return float(lib().or_neighbour_noise(u.shape[0], u.shape[1], u, v, _u8(usable)))
+
+
+def temporal(dx, dy, usable, gate, kernel, window, min_support, leave_out=()):
+    """smooth.rs:44-130; kernel "mean" or "median" -> (dx, dy, support)"""
+    dx, dy = _f32(dx), _f32(dy)
+    nt, ny, nx = dx.shape
+    lo = np.ascontiguousarray(list(leave_out) or [0], dtype=np.int32)
+    ox, oy = np.empty_like(dx), np.empty_like(dy)
+    sup = np.empty(dx.shape, np.uint16)
+    lib().or_temporal(nt, ny, nx, dx, dy, _u8(usable), _u8(gate), 0 if kernel == "mean" else 1,
+                      window, min_support, lo, len(leave_out), ox, oy, sup)
+    return ox, oy, sup
+
+
+def spatial(dx, dy, gate, sigma, min_weight, leave_out=None):
+    """smooth.rs:137-208 -> (dx, dy, support)"""
+    dx, dy = _f32(dx), _f32(dy)
+    nt, ny, nx = dx.shape
+    ox, oy = np.empty_like(dx), np.empty_like(dy)
+    sup = np.empty(dx.shape, np.uint16)
+    lib().or_spatial(nt, ny, nx, dx, dy, _u8(gate), sigma, min_weight,
+                     -1 if leave_out is None else int(leave_out), ox, oy, sup)
+    return ox, oy, sup
+
+
+def cross_validation_errors(raw_dx, raw_dy, usable, pred_dx, pred_dy):
+    """smooth.rs:213-232"""
+    a = [_f32(x) for x in (raw_dx, raw_dy, pred_dx, pred_dy)]
+    out = np.empty(a[0].size, np.float32)
+    m = lib().or_cross_validation_errors(a[0].size, a[0], a[1], _u8(usable), a[2], a[3], out)
+    return out[:m]
+
+
+def median_error(errors, keep=None):
+    """smooth.rs:236-249 -> (median, count)"""
+    e = _f32(errors)
+    k = None if keep is None else _u8(keep)
+    n = C.c_int64()
+    r = lib().or_median_error(e, e.size, _ptr(k), C.byref(n))
+    return float(r), int(n.value)
+
+
+def pair_to_pair_change(dx, dy):
+    """smooth.rs:252-274"""
+    dx, dy = _f32(dx), _f32(dy)
+    return float(lib().or_pair_to_pair_change(*dx.shape, dx, dy))
+
+
+def spatial_roughness(dx, dy):
+    """smooth.rs:277-300"""
+    dx, dy = _f32(dx), _f32(dy)
+    return float(lib().or_spatial_roughness(*dx.shape, dx, dy))
+
+
+def mask(values, usable):
+    """smooth.rs:303-310"""
+    v = _f32(values)
+    out = np.empty_like(v)
+    lib().or_mask(v.size, v, _u8(usable), out)
+    return out

@@ -123,6 +123,37 @@ void or_validate(int32_t ny, int32_t nx, float *dx, float *dy, const float *ppr,
 /* median helper (validate.rs:163-171), exposed for tests; sorts vals in place */
 float or_median(float *vals, int32_t n);
 
+/* ---- post-analysis stages (post_oracle.c) ---------------------------------------------- */
+typedef struct {
+    float vorticity_mean, vorticity_abs_mean, ke_mean, ke_sum, coverage;
+    int64_t n_usable, n_vorticity;
+} or_flowstat;
+
+void or_flow_usable(int64_t n, const float *dx, const uint8_t *flags, uint8_t *out);
+void or_vorticity(int32_t ny, int32_t nx, const float *u, const float *v, const uint8_t *usable,
+                  float step, float *w);
+void or_flow_stats(int32_t ny, int32_t nx, const float *u, const float *v, const uint8_t *usable,
+                   float step, or_flowstat *out);
+void or_pair_stats(int32_t ny, int32_t nx, const float *dx, const float *dy,
+                   const uint8_t *usable, int32_t step, double dt_s, or_flowstat *out);
+float or_neighbour_noise(int32_t ny, int32_t nx, const float *u, const float *v,
+                         const uint8_t *usable);
+float or_post_median(float *v, int64_t n);
+void or_temporal(int32_t nt, int32_t ny, int32_t nx, const float *dx, const float *dy,
+                 const uint8_t *usable, const uint8_t *gate, int32_t kernel, int32_t window,
+                 int32_t min_support, const int32_t *leave_out, int32_t n_leave, float *out_dx,
+                 float *out_dy, uint16_t *support);
+void or_spatial(int32_t nt, int32_t ny, int32_t nx, const float *dx, const float *dy,
+                const uint8_t *gate, float sigma, float min_weight, int32_t leave_out,
+                float *out_dx, float *out_dy, uint16_t *support);
+int64_t or_cross_validation_errors(int64_t n, const float *raw_dx, const float *raw_dy,
+                                   const uint8_t *usable, const float *pred_dx,
+                                   const float *pred_dy, float *out);
+float or_median_error(const float *errors, int64_t n, const uint8_t *keep, int64_t *count);
+float or_pair_to_pair_change(int32_t nt, int32_t ny, int32_t nx, const float *dx, const float *dy);
+float or_spatial_roughness(int32_t nt, int32_t ny, int32_t nx, const float *dx, const float *dy);
+void or_mask(int64_t n, const float *values, const uint8_t *usable, float *out);
+
 #ifdef __cplusplus
 }
 #endif

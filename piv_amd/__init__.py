@@ -30,7 +30,20 @@ __all__ = ["PivConfig", "MaskConfig", "ValidationConfig", "PreprocessConfig", "P
            "PreparedFrame", "Grid", "VectorField", "PairResult", "PassTrace",
            "ValidationSummary", "PivError", "ConfigError", "ShapeMismatch", "ImageTooSmall",
            "DeviceError", "flags", "validate", "place_windows", "correlate_windows",
-           "normalize_pixels", "BackgroundEstimator", "DeviceArray"]
+           "normalize_pixels", "BackgroundEstimator", "DeviceArray", "FlowStats"]
+
+
+@dataclass
+class FlowStats:
+    """FlowStats (flowstats.rs:9-25) plus the neighbour noise estimate (px)."""
+    vorticity_mean: float
+    vorticity_abs_mean: float
+    ke_mean: float
+    ke_sum: float
+    coverage: float
+    noise_px: float
+    n_usable: int
+    n_vorticity: int
 
 
 class flags:  # field.rs:6-20
@@ -362,6 +375,23 @@ class PivAnalyzer:
     def set_variant(self, window, dtype, variant):
         """Pin the correlation-kernel variant for (window, sample dtype)."""
         _check(self._lib.piv_kernel_variant_set(self._h, window, _FMT[np.dtype(dtype)], variant))
+
+    def flow_stats_device(self, grid, dx, dy, flags_, dt_s):
+        """Per-pair statistics of validated fields on the device, as `piv analyze` gathers them
+        (analyze.rs:483-489): usable vectors (flow.rs:13-17), pair_stats (flow.rs:20-34) and
+        neighbour_noise (flowstats.rs:106-137).  dx, dy, flags_: device arrays (torch tensors or
+        DeviceArray) [npairs, ny, nx] or [ny, nx] on this analyzer's GPU; returns one FlowStats
+        per pair."""
+        shape = tuple(dx.shape)
+        n = 1 if len(shape) == 2 else shape[0]
+        if shape[-2:] != (grid.ny, grid.nx):
+            raise ShapeMismatch(f"fields {shape} vs grid {grid.ny}x{grid.nx}")
+        out = (L.piv_flow_stats * max(n, 1))()
+        _check(self._lib.piv_flow_stats_device(self._h, C.byref(grid._to_c()), n, L.ptr(dx),
+                                               L.ptr(dy), L.ptr(flags_), float(dt_s), out))
+        return [FlowStats(**{k: (int(getattr(o, k)) if k.startswith("n_") else
+                                 float(getattr(o, k))) for k, _ in o._fields_})
+                for o in out[:n]]
 
     def autotune(self, frames_a, frames_b, repeats=3):
         """Time every correlation-kernel variant on this batch and keep the fastest per window

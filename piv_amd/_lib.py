@@ -83,6 +83,12 @@ class piv_tune_entry(C.Structure):
                 ("num_variants", C.c_uint32), ("occupancy", C.c_uint32 * 4), ("ms", C.c_float * 4)]
 
 
+class piv_flow_stats(C.Structure):
+    _fields_ = [("vorticity_mean", C.c_float), ("vorticity_abs_mean", C.c_float),
+                ("ke_mean", C.c_float), ("ke_sum", C.c_float), ("coverage", C.c_float),
+                ("noise_px", C.c_float), ("n_usable", C.c_uint32), ("n_vorticity", C.c_uint32)]
+
+
 # every entry point declared in include/piv.h, with its signature
 _SIGS = {
     "piv_last_error": ([], C.c_char_p),
@@ -150,6 +156,26 @@ _SIGS = {
                             C.c_uint32, C.c_uint32, C.POINTER(piv_grid), C.c_void_p, C.c_void_p,
                             C.c_void_p, C.c_void_p], C.c_int),
     "piv_normalize_pixels": ([C.c_int, C.c_void_p, C.c_int, C.c_size_t, C.c_void_p], C.c_int),
+    "piv_flow_stats_device": ([C.c_void_p, C.POINTER(piv_grid), C.c_uint32, C.c_void_p,
+                               C.c_void_p, C.c_void_p, C.c_double, C.POINTER(piv_flow_stats)],
+                              C.c_int),
+    "piv_smooth_temporal_device": ([C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
+                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_uint32,
+                                    C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+                                    C.c_void_p], C.c_int),
+    "piv_smooth_spatial_device": ([C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
+                                   C.c_void_p, C.c_void_p, C.c_float, C.c_float, C.c_int32,
+                                   C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "piv_mask_device": ([C.c_int, C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "piv_cross_validation_errors_device": ([C.c_int, C.c_size_t, C.c_void_p, C.c_void_p,
+                                            C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                            C.POINTER(C.c_uint64)], C.c_int),
+    "piv_median_error_device": ([C.c_int, C.c_void_p, C.c_size_t, C.c_void_p,
+                                 C.POINTER(C.c_float), C.POINTER(C.c_uint64)], C.c_int),
+    "piv_pair_to_pair_change_device": ([C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
+                                        C.c_void_p, C.POINTER(C.c_float)], C.c_int),
+    "piv_spatial_roughness_device": ([C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
+                                      C.c_void_p, C.POINTER(C.c_float)], C.c_int),
 }
 
 _lib = None

@@ -143,23 +143,6 @@ hipError_t prof_event(piv_analyzer *a, hipEvent_t *e)
     return hipEventCreate(e);
 }
 
-// Launch `fn` bracketed by timing events when profiling is on.
-template <class F>
-hipError_t timed(piv_analyzer *a, const char *name, int window, double bytes, uint64_t units, F &&fn)
-{
-    if (!a->prof_on) return fn();
-    piv_analyzer::ProfRec r{name, window, bytes, units, nullptr, nullptr};
-    hipError_t e = prof_event(a, &r.e0);
-    if (e == hipSuccess) e = prof_event(a, &r.e1);
-    if (e == hipSuccess) e = hipEventRecord(r.e0, a->stream);
-    if (e != hipSuccess) return e;
-    e = fn();
-    if (e != hipSuccess) return e;
-    e = hipEventRecord(r.e1, a->stream);
-    if (e == hipSuccess) a->prof_pending.push_back(r);
-    return e;
-}
-
 // Fold completed timing events into the counters (the stream must be idle).
 void prof_harvest(piv_analyzer *a)
 {

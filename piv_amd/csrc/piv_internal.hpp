@@ -161,6 +161,7 @@ struct piv_analyzer {
     HostBuf h_summaries;
     DevBuf prep_tmp, prep_tmp2, prep_sat;  // conditioning scratch
     DevBuf stage_in, stage_out;            // stage-level entry points
+    DevBuf post_part, post_keys, post_out; // flow statistics (piv_flow_stats_device)
     // kernel timing (piv_profile_*)
     struct ProfRec {
         std::string name;
@@ -199,6 +200,9 @@ struct piv_analyzer {
         prep_sat.release();
         stage_in.release();
         stage_out.release();
+        post_part.release();
+        post_keys.release();
+        post_out.release();
         d_taps.release();
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -229,4 +233,28 @@ piv_status frame_from_device(piv_analyzer *a, const void *pixels, int fmt, uint3
 piv_status check_pixels_args(const void *pixels, int fmt, uint32_t h, uint32_t w,
                              size_t row_stride);
 
+}  // namespace pivi
+
+namespace pivi {
+// Timing events from the analyzer's pool (piv_profile_*).
+hipError_t prof_event(piv_analyzer *a, hipEvent_t *e);
+// Fold completed timing events into the counters (the stream must be idle).
+void prof_harvest(piv_analyzer *a);
+
+// Launch `fn` bracketed by timing events when profiling is on.
+template <class F>
+hipError_t timed(piv_analyzer *a, const char *name, int window, double bytes, uint64_t units, F &&fn)
+{
+    if (!a->prof_on) return fn();
+    piv_analyzer::ProfRec r{name, window, bytes, units, nullptr, nullptr};
+    hipError_t e = prof_event(a, &r.e0);
+    if (e == hipSuccess) e = prof_event(a, &r.e1);
+    if (e == hipSuccess) e = hipEventRecord(r.e0, a->stream);
+    if (e != hipSuccess) return e;
+    e = fn();
+    if (e != hipSuccess) return e;
+    e = hipEventRecord(r.e1, a->stream);
+    if (e == hipSuccess) a->prof_pending.push_back(r);
+    return e;
+}
 }  // namespace pivi
