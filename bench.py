@@ -386,6 +386,60 @@ def precision_bench(an, fa, fb, dx, dy, pr, fl, g, field, uniq, steps):
     return out
 
 
+def post_bench(an, g, dx, dy, fl, repeats=20, seed=3):
+    """Post-analysis stages on the device: the flow statistics of the step's validated fields
+    (one launch pair for the whole batch), and the temporal / spatial filters on a video-length
+    series (1999 pairs of the 65 x 105 grid of the 1280x800 48 -> 24 px video workload)."""
+    import torch
+    from piv_amd import smooth
+
+    P = int(dx.shape[0])
+    cells = P * g.ny * g.nx
+    dt = 1.0 / 240.0
+    an.flow_stats_device(g, dx, dy, fl, dt)
+    an.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        st = an.flow_stats_device(g, dx, dy, fl, dt)
+    t1 = time.perf_counter()
+    prof = [s for s in an.profile_read() if s["name"] == "flowstats"]
+    an.profile(False)
+    k_ms = prof[0]["total_ms"] / prof[0]["launches"]
+    gbps = 9.0 * cells / (k_ms * 1e-3) / 1e9  # dx, dy, flags read once
+    noise = [s.noise_px for s in st if math.isfinite(s.noise_px)]
+    flow = {"pairs": P, "grid": [g.ny, g.nx], "pairs_per_s": round(P * repeats / (t1 - t0), 1),
+            "kernel_ms": round(k_ms, 4), "achieved_GBps": round(gbps, 1),
+            "frac": round(gbps / HBM_PEAK_GBS, 4),
+            "noise_px_median": round(float(np.median(noise)), 5) if noise else None,
+            "vorticity_mean_first": st[0].vorticity_mean}
+
+    nt, ny, nx = 1999, 65, 105
+    rng = np.random.default_rng(seed)
+    dev = torch.device("cuda:0")
+    du = torch.from_numpy(rng.standard_normal((nt, ny, nx)).astype(np.float32)).to(dev)
+    dv = torch.from_numpy(rng.standard_normal((nt, ny, nx)).astype(np.float32)).to(dev)
+    ok = torch.from_numpy((rng.random((nt, ny, nx)) > 0.1).astype(np.uint8)).to(dev)
+    gate = torch.ones((nt, ny, nx), dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+    n = nt * ny * nx
+    series = {"series": [nt, ny, nx]}
+    runs = [("temporal_mean9", lambda: smooth.temporal(du, dv, ok, gate, "mean", 9, 1)),
+            ("temporal_median7", lambda: smooth.temporal(du, dv, ok, gate, "median", 7, 1)),
+            ("spatial_sigma1.5", lambda: smooth.spatial(du, dv, gate, 1.5, 0.25))]
+    for name, fn in runs:
+        fn()
+        reps = 5
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        sec = (time.perf_counter() - t0) / reps
+        # each input (dx, dy f32, usable, gate u8) read once and each output (dx, dy f32,
+        # support u16) written once
+        series[name] = {"ms": round(sec * 1e3, 3), "values_per_s": round(n / sec, 1),
+                        "achieved_GBps": round(20.0 * n / sec / 1e9, 1)}
+    return {"flow_stats": flow, "smoothing": series}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -400,6 +454,8 @@ def main():
                     help="Poisson phase of the serving measurement (0 skips serving)")
     ap.add_argument("--serve-rate", type=float, default=500.0)
     ap.add_argument("--no-video", action="store_true", help="skip the batched-video measurement")
+    ap.add_argument("--no-post", action="store_true",
+                    help="skip the flow-statistics and smoothing measurement")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the reduced-precision (DFT-GEMM) measurement")
     ap.add_argument("--serve-backlog", type=int, default=400)
@@ -521,6 +577,10 @@ def main():
     if not args.no_video:
         video = video_bench()
 
+    post = None
+    if not args.no_post:
+        post = post_bench(an, g, dx, dy, fl)
+
     serving = None
     if args.serve_seconds > 0:
         serving = serve_bench(args.serve_rate, args.serve_seconds, args.serve_backlog)
@@ -563,6 +623,7 @@ def main():
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": round(avg_ms, 5)},
         "reduced_precision": precision,
         "video": video,
+        "post": post,
         "serving": serving,
         "cpu_baseline": cpu,
     }

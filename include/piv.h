@@ -356,8 +356,8 @@ typedef struct {
     uint32_t n_usable, n_vorticity;
 } piv_flow_stats;
 
-/* Replaces the per-pair statistics of `piv analyze` (analyze.rs:483-489): usable vectors
- * (flow.rs:13-17: finite and accepted or REPLACED), pair_stats (flow.rs:20-34: velocity
+/* Replaces the per-pair statistics of `piv analyze` (piv-cli/src/analyze.rs:483-489): usable
+ * vectors (piv-io/src/flow.rs:13-17: finite and accepted or REPLACED), pair_stats (:20-34: velocity
  * = dx * f32(1 / dt_s), grid step as the spacing) and neighbour_noise of the raw displacements
  * (flowstats.rs:106-137), for `npairs` validated fields of one grid, device-resident
  * [npairs][ny][nx] (dx, dy in px, flags).  `out` is host memory [npairs].  Runs on the
@@ -365,6 +365,11 @@ typedef struct {
 piv_status piv_flow_stats_device(piv_analyzer *a, const piv_grid *grid, uint32_t npairs,
                                  const float *dx, const float *dy, const uint8_t *flags,
                                  double dt_s, piv_flow_stats *out);
+
+/* piv-io/src/flow.rs:133-145 write_stats_npz: `dir`/stats.npz with t (f64 [npairs], pair midpoint time)
+ * and vorticity_mean, vorticity_abs_mean, ke_mean, ke_sum, coverage (f32 [npairs]), deflated. */
+piv_status piv_write_stats_npz(const char *dir, uint32_t npairs, const double *t,
+                               const piv_flow_stats *stats);
 
 /* ---- time-series filters of a run (smooth.rs) ------------------------------------------ */
 /* Series are device-resident [nt][ny][nx] (pair, row, column); masks are u8 0/1.  Each call

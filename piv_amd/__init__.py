@@ -828,6 +828,20 @@ def write_pair_npz(directory, frame_a, frame_b, time_s, dt_s, field: "VectorFiel
     return os.path.join(str(directory), f"pair_{frame_a:06d}_{frame_b:06d}.npz")
 
 
+def write_stats_npz(directory, t, stats):
+    """piv-io/src/flow.rs:133-145: `directory`/stats.npz with t (f64) and the FlowStats columns
+    (f32) of a run, one entry per pair.  Returns the path."""
+    t = np.ascontiguousarray(t, np.float64)
+    if len(stats) != t.size:
+        raise ShapeMismatch(f"{t.size} times vs {len(stats)} statistics")
+    arr = (L.piv_flow_stats * max(len(stats), 1))()
+    for i, s in enumerate(stats):
+        for k, _ in L.piv_flow_stats._fields_:
+            setattr(arr[i], k, getattr(s, k))
+    _check(L.load().piv_write_stats_npz(str(directory).encode(), t.size, L.ptr(t), arr))
+    return os.path.join(str(directory), "stats.npz")
+
+
 def kernel_variant_count(window):
     """Number of launch variants of the correlation kernel for a window size."""
     return int(L.load().piv_kernel_variant_count(window))

@@ -13,6 +13,7 @@
 #include <fcntl.h>
 #include <unistd.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -347,6 +348,8 @@ int main(int argc, char **argv)
     };
     const double m_per_px = A.px_per_mm > 0 ? 1e-3 / A.px_per_mm : 0.0;
     uint64_t valid = 0, vectors = 0;
+    std::vector<double> series_t;
+    std::vector<piv_flow_stats> series_stats;
     const auto t_loop = std::chrono::steady_clock::now();
     for (size_t p0 = 0; p0 < pair_starts.size(); p0 += B) {
         const size_t n = std::min(B, pair_starts.size() - p0);
@@ -358,6 +361,11 @@ int main(int argc, char **argv)
         check(piv_analyze_batch_device(an, fa.data(), fb.data(), (uint32_t)n, (float *)ddx, (float *)ddy,
                                        (float *)dpr, (uint8_t *)dfl, sums.data(), nullptr, 0),
               "analysis");
+        // analyze.rs:483-489: per-pair flow statistics from the device-resident fields
+        series_stats.resize(p0 + n);
+        check(piv_flow_stats_device(an, &grid, (uint32_t)n, (const float *)ddx, (const float *)ddy,
+                                    (const uint8_t *)dfl, dt, series_stats.data() + p0),
+              "flow statistics");
         check(piv_copy_to_host(A.device, hdx.data(), ddx, n * cells * 4), "download");
         check(piv_copy_to_host(A.device, hdy.data(), ddy, n * cells * 4), "download");
         check(piv_copy_to_host(A.device, hpr.data(), dpr, n * cells * 4), "download");
@@ -365,6 +373,7 @@ int main(int argc, char **argv)
         for (size_t k = 0; k < n; ++k) {
             const long a = pair_starts[p0 + k];
             const double time_s = (a + A.stride / 2.0) / fps;
+            series_t.push_back(time_s);
             check(piv_write_pair_npz(A.output.c_str(), (uint64_t)a, (uint64_t)(a + A.stride), time_s, dt,
                                      m_per_px, x0, y0, &grid, hdx.data() + k * cells,
                                      hdy.data() + k * cells, hpr.data() + k * cells,
@@ -386,6 +395,19 @@ int main(int argc, char **argv)
     }
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_loop).count();
     for (auto &kv : held) piv_frame_release(kv.second);
+
+    // analyze.rs:549: the run's flow statistics next to the pair archives
+    check(piv_write_stats_npz(A.output.c_str(), (uint32_t)series_t.size(), series_t.data(),
+                              series_stats.data()),
+          "stats.npz");
+    std::vector<float> noise;
+    for (const auto &s : series_stats)
+        if (std::isfinite(s.noise_px)) noise.push_back(s.noise_px);
+    if (!noise.empty()) {
+        std::nth_element(noise.begin(), noise.begin() + noise.size() / 2, noise.end());
+        std::fprintf(stderr, "flow: neighbour noise median %.4f px over %zu pairs\n",
+                     noise[noise.size() / 2], noise.size());
+    }
 
     // run.json (write.rs:20-49 RunMetadata, the fields this host loop knows)
     std::string ws;

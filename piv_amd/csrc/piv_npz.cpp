@@ -205,4 +205,27 @@ piv_status piv_write_pair_npz(const char *dir, uint64_t frame_a, uint64_t frame_
     return write_npz(std::string(dir) + name, arrays);
 }
 
+piv_status piv_write_stats_npz(const char *dir, uint32_t npairs, const double *t,
+                               const piv_flow_stats *stats)
+{
+    if (!dir || (npairs && (!t || !stats))) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+    // flow.rs:133-145 write_stats_npz / :147-156 columns: t (f64), then the f32 columns
+    const size_t n = npairs;
+    std::vector<float> col[5];
+    for (auto &c : col) c.resize(n);
+    for (size_t i = 0; i < n; ++i) {
+        col[0][i] = stats[i].vorticity_mean;
+        col[1][i] = stats[i].vorticity_abs_mean;
+        col[2][i] = stats[i].ke_mean;
+        col[3][i] = stats[i].ke_sum;
+        col[4][i] = stats[i].coverage;
+    }
+    static const char *names[5] = {"vorticity_mean", "vorticity_abs_mean", "ke_mean", "ke_sum",
+                                   "coverage"};
+    std::vector<NpyArray> arrays;
+    arrays.push_back({"t", "<f8", {n}, t, n * 8});
+    for (int c = 0; c < 5; ++c) arrays.push_back({names[c], "<f4", {n}, col[c].data(), n * 4});
+    return write_npz(std::string(dir) + "/stats.npz", arrays);
+}
+
 }  // extern "C"

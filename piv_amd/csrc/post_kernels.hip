@@ -76,33 +76,65 @@ __device__ __forceinline__ void sort8(uint32_t (&k)[8])
     cswap(k[1], k[2]); cswap(k[3], k[4]); cswap(k[5], k[6]);
 }
 
-// The k-th smallest key (k < number of keys below NO_KEY) of keys[0..n), radix-256 from the
-// top byte; the whole workgroup calls it.  Counts are order-independent, so the result is
-// exact and deterministic.
-__device__ uint32_t block_select(const uint32_t *__restrict__ keys, uint32_t n, uint32_t k,
-                                 uint32_t *hist /* LDS [256] */, uint32_t *sh /* LDS [2] */)
+// The k-th smallest key (k < number of keys below NO_KEY) of keys[0..n): three radix passes
+// over bits 31-21, 20-10 and 9-0, from the top; the whole workgroup calls it.  Counts are
+// order-independent, so the result is exact and deterministic.
+constexpr int SEL_BINS = 2048;
+
+// Inclusive prefix sum of v over the workgroup (blockDim.x a multiple of 64, at most 1024);
+// wsum: LDS [16].
+__device__ uint32_t block_scan(uint32_t v, uint32_t *wsum)
 {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(v, d, 64);
+        if (lane >= d) v += o;
+    }
+    if (lane == 63) wsum[wv] = v;
+    __syncthreads();
+    uint32_t before = 0;
+    for (int q = 0; q < wv; ++q) before += wsum[q];
+    __syncthreads();
+    return v + before;
+}
+
+__device__ uint32_t block_select(const uint32_t *__restrict__ keys, uint32_t n, uint32_t k,
+                                 uint32_t *hist /* LDS [SEL_BINS] */, uint32_t *sh /* LDS [18] */)
+{
+    constexpr int shifts[3] = {21, 10, 0};
+    constexpr uint32_t widths[3] = {11, 11, 10};
     uint32_t prefix = 0, mask = 0;
-    for (int shift = 24; shift >= 0; shift -= 8) {
-        for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) hist[i] = 0;
+    for (int pass = 0; pass < 3; ++pass) {
+        const int shift = shifts[pass];
+        const uint32_t nb = 1u << widths[pass], bm = nb - 1;
+        for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) hist[i] = 0;
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
             const uint32_t key = keys[i];
-            if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
+            if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & bm], 1u);
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t cum = 0, b = 0;
-            for (; b < 255; ++b) {
-                if (cum + hist[b] > k) break;
-                cum += hist[b];
+        // the bin holding the k-th key: block-wide prefix sum over the bins
+        const uint32_t per = (nb + blockDim.x - 1) / blockDim.x, b0 = threadIdx.x * per;
+        uint32_t local = 0;
+        for (uint32_t j = 0; j < per; ++j) local += b0 + j < nb ? hist[b0 + j] : 0u;
+        const uint32_t incl = block_scan(local, sh + 2), excl = incl - local;
+        if (excl <= k && k < incl) {
+            uint32_t cum = excl;
+            for (uint32_t j = 0; j < per; ++j) {
+                const uint32_t h = hist[b0 + j];
+                if (cum + h > k) {
+                    sh[0] = b0 + j;
+                    sh[1] = k - cum;
+                    break;
+                }
+                cum += h;
             }
-            sh[0] = b;
-            sh[1] = k - cum;
         }
         __syncthreads();
         prefix |= sh[0] << shift;
-        mask |= 255u << shift;
+        mask |= bm << shift;
         k = sh[1];
         __syncthreads();
     }
@@ -160,20 +192,23 @@ __global__ __launch_bounds__(FLOW_BLOCK) void k_flow_cells(const float *__restri
     uint32_t nw = 0, nu = 0, nres = 0;
     if (i < cells) {
         const int iy = i / nx, ix = i - iy * nx;
-        auto ok = [&](int j) { return usable_cell(dx[j], flags[j]); };
+        // the 3x3 neighbourhood (q = 3 (dy + 1) + dx + 1), loaded together
+        float ndx[9], ndy[9];
+        bool nok[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            const int jy = iy + q / 3 - 1, jx = ix + q % 3 - 1;
+            const bool in = jy >= 0 && jy < ny && jx >= 0 && jx < nx;
+            const int j = in ? jy * nx + jx : i;
+            ndx[q] = dx[j];
+            ndy[q] = dy[j];
+            nok[q] = in && usable_cell(ndx[q], flags[j]);
+        }
         // flowstats.rs:31-66 on u = dx * k along y and v = dy * k along x
-        auto hu = [&](int j, float &val) {
-            val = dx[j] * k;
-            return ok(j) && finite(val);
-        };
-        auto hv = [&](int j, float &val) {
-            val = dy[j] * k;
-            return ok(j) && finite(val);
-        };
-        float uc, ul = 0.0f, uh = 0.0f, vc, vl = 0.0f, vh = 0.0f;
-        const bool cu = hu(i, uc), cv = hv(i, vc);
-        const bool lu = iy >= 1 && hu(i - nx, ul), hu_ = iy + 1 < ny && hu(i + nx, uh);
-        const bool lv = ix >= 1 && hv(i - 1, vl), hv_ = ix + 1 < nx && hv(i + 1, vh);
+        const float ul = ndx[1] * k, uc = ndx[4] * k, uh = ndx[7] * k;
+        const float vl = ndy[3] * k, vc = ndy[4] * k, vh = ndy[5] * k;
+        const bool cu = nok[4] && finite(uc), lu = nok[1] && finite(ul), hu_ = nok[7] && finite(uh);
+        const bool cv = nok[4] && finite(vc), lv = nok[3] && finite(vl), hv_ = nok[5] && finite(vh);
         bool da = false, db = false;
         float a = 0.0f, b = 0.0f;
         if (cu) {
@@ -195,42 +230,32 @@ __global__ __launch_bounds__(FLOW_BLOCK) void k_flow_cells(const float *__restri
             }
         }
         // flowstats.rs:83-89: kinetic energy of usable, finite velocities
-        const bool oki = ok(i);
-        if (oki && finite(uc) && finite(vc)) {
+        if (nok[4] && finite(uc) && finite(vc)) {
             e = 0.5 * ((double)uc * (double)uc + (double)vc * (double)vc);
             nu = 1;
         }
-        // flowstats.rs:106-130: residual to the median of the usable 3x3 neighbours, px units
-        const float *comp[2] = {dx, dy};
-        const int y0 = iy > 0 ? iy - 1 : 0, y1 = iy + 2 < ny ? iy + 2 : ny;
-        const int x0 = ix > 0 ? ix - 1 : 0, x1 = ix + 2 < nx ? ix + 2 : nx;
+        // flowstats.rs:106-130: residual to the median (element m/2 in total order) of the
+        // usable finite neighbours, px units; absent neighbours sort last as NO_KEY
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
-            const float *ac = comp[c];
+            const float *nv = c == 0 ? ndx : ndy;
             uint32_t key = NO_KEY;
-            const float av = ac[i];
-            if (oki && finite(av)) {
+            if (nok[4] && finite(nv[4])) {
                 uint32_t nb[8];
                 int m = 0;
 #pragma unroll
-                for (int q = 0; q < 8; ++q) nb[q] = NO_KEY;
-                for (int jy = y0; jy < y1; ++jy)
-                    for (int jx = x0; jx < x1; ++jx) {
-                        const int j = jy * nx + jx;
-                        if (j == i || !ok(j)) continue;
-                        const float x = ac[j];
-                        if (!finite(x)) continue;
-                        const uint32_t kk = total_key(x);
-#pragma unroll
-                        for (int q = 0; q < 8; ++q) nb[q] = (q == m) ? kk : nb[q];
-                        ++m;
-                    }
+                for (int q = 0, t = 0; q < 9; ++q) {
+                    if (q == 4) continue;
+                    const bool use = nok[q] && finite(nv[q]);
+                    nb[t++] = use ? total_key(nv[q]) : NO_KEY;
+                    m += use;
+                }
                 if (m >= 5) {
                     sort8(nb);
                     uint32_t med = nb[0];
 #pragma unroll
                     for (int q = 1; q < 8; ++q) med = (q == m / 2) ? nb[q] : med;
-                    key = total_key(fabsf(av - key_value(med)));
+                    key = total_key(fabsf(nv[4] - key_value(med)));
                     ++nres;
                 }
             }
@@ -279,22 +304,31 @@ __global__ __launch_bounds__(FINISH_BLOCK) void k_flow_finish(const FlowPartial 
                                                               int cells, float step,
                                                               FlowStatsDev *__restrict__ out)
 {
-    __shared__ uint32_t hist[256];
-    __shared__ uint32_t sh[2];
+    __shared__ uint32_t hist[SEL_BINS];
+    __shared__ uint32_t sh[18];
     __shared__ uint32_t s_nres;
     const int p = blockIdx.x;
     part += (long)p * nblk;
+    // the block partials, loaded in parallel, summed in block order by one thread
+    __shared__ FlowPartial s_part[FINISH_BLOCK];
+    double ws = 0.0, wa = 0.0, e = 0.0;
+    uint32_t nw = 0, nu = 0, nr = 0;
+    for (int b0 = 0; b0 < nblk; b0 += FINISH_BLOCK) {
+        const int cnt = min(FINISH_BLOCK, nblk - b0);
+        __syncthreads();
+        if ((int)threadIdx.x < cnt) s_part[threadIdx.x] = part[b0 + threadIdx.x];
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int b = 0; b < cnt; ++b) {
+                ws += s_part[b].ws;
+                wa += s_part[b].wa;
+                e += s_part[b].e;
+                nw += s_part[b].nw;
+                nu += s_part[b].nu;
+                nr += s_part[b].nres;
+            }
+    }
     if (threadIdx.x == 0) {
-        double ws = 0.0, wa = 0.0, e = 0.0;
-        uint32_t nw = 0, nu = 0, nr = 0;
-        for (int b = 0; b < nblk; ++b) {
-            ws += part[b].ws;
-            wa += part[b].wa;
-            e += part[b].e;
-            nw += part[b].nw;
-            nu += part[b].nu;
-            nr += part[b].nres;
-        }
         // flowstats.rs:90-100
         FlowStatsDev r;
         r.vorticity_mean = nw > 0 ? (float)(ws / (double)nw) : __builtin_nanf("");
@@ -309,10 +343,10 @@ __global__ __launch_bounds__(FINISH_BLOCK) void k_flow_finish(const FlowPartial 
         s_nres = nr;
     }
     __syncthreads();
-    const uint32_t nr = s_nres;
-    if (nr >= 20) {  // flowstats.rs:131-136
+    const uint32_t nres = s_nres;
+    if (nres >= 20) {  // flowstats.rs:131-136
         const float mad = key_value(block_select(resid + 2l * p * cells, 2u * (uint32_t)cells,
-                                                 nr / 2, hist, sh));
+                                                 nres / 2, hist, sh));
         if (threadIdx.x == 0)
             out[p].noise_px = 1.4826f * mad / sqrtf(1.0f + 3.14159265358979323846f / 16.0f);
     }
@@ -574,8 +608,8 @@ __global__ __launch_bounds__(FINISH_BLOCK) void k_median_keys(const uint32_t *__
                                                               uint32_t n, float *__restrict__ med,
                                                               uint64_t *__restrict__ count)
 {
-    __shared__ uint32_t hist[256];
-    __shared__ uint32_t sh[2];
+    __shared__ uint32_t hist[SEL_BINS];
+    __shared__ uint32_t sh[18];
     const uint32_t m = block_count_keys(keys, n, sh);
     float r = __builtin_nanf("");
     if (m > 0) r = block_median(keys, n, m, hist, sh);

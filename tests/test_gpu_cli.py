@@ -32,6 +32,12 @@ def bits(a):
     return np.ascontiguousarray(a, np.float32).view(np.uint32)
 
 
+def run_step(d):
+    """grid step of a pair archive from its x coordinates"""
+    x = d["x"]
+    return int(round(float(x[1] - x[0]))) if x.size > 1 else 1
+
+
 def _run(args):
     p = subprocess.run([CLI] + args, capture_output=True, text=True, timeout=120)
     return p.returncode, p.stdout + p.stderr
@@ -54,7 +60,7 @@ def test_cli_pairs_background_roi_npz(piv, tmp_path, dtype, bits_):
     assert rc == 0, log
     # pairs (a, a + stride) for a in range(start, end - stride, pair_step)
     starts = list(range(start, len(frames) - stride, step))
-    names = sorted(f for f in os.listdir(out) if f.endswith(".npz"))
+    names = sorted(f for f in os.listdir(out) if f.startswith("pair_") and f.endswith(".npz"))
     assert names == [f"pair_{a:06d}_{a + stride:06d}.npz" for a in starts]
     run = json.loads((out / "run.json").read_text())
     assert run["stride"] == stride and run["pair_step"] == step and run["start_frame"] == start
@@ -83,6 +89,17 @@ def test_cli_pairs_background_roi_npz(piv, tmp_path, dtype, bits_):
         assert list(d["frames"]) == [a, a + stride]
         assert float(d["t"]) == (a + stride / 2.0) / fps
         assert ((r.field.flags & 23) == 0).mean() > 0.5
+    # stats.npz (analyze.rs:483-489, 549): pair_stats of every pair's field, in pair order
+    from oracle import oracle as O
+    st = np.load(out / "stats.npz")
+    assert st["t"].shape == (len(starts),)
+    for i, a in enumerate(starts):
+        d = np.load(out / f"pair_{a:06d}_{a + stride:06d}.npz")
+        assert float(st["t"][i]) == float(d["t"])
+        ok = O.flow_usable(d["dx"], d["flags"])
+        want = O.pair_stats(d["dx"], d["dy"], ok, run_step(d), stride / fps)
+        for k in ("vorticity_mean", "vorticity_abs_mean", "ke_mean", "ke_sum", "coverage"):
+            assert st[k][i] == pytest.approx(want[k], rel=1e-6, nan_ok=True), (a, k)
 
 
 def test_cli_errors(piv, tmp_path):

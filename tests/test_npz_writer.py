@@ -68,3 +68,22 @@ def test_bad_arguments_are_errors(piv, tmp_path):
         piv.write_pair_npz(tmp_path, 0, 1, 0.5, 0.0, f)
     with pytest.raises(piv.InvalidArgument):
         piv.write_pair_npz(tmp_path / "missing" / "dir", 0, 1, 0.5, 1.0, f)
+
+
+def test_stats_archive_layout(piv, tmp_path):
+    """piv-io/src/flow.rs:133-156: t (f64) first, then the five f32 columns in that order."""
+    stats = [piv.FlowStats(0.5 * i, 1.0 + i, 2.0, 3.0e6 * i, 0.9, 0.1, 100, 90) for i in range(4)]
+    stats[2].vorticity_mean = float("nan")
+    t = np.array([0.1, 0.2, 0.3, 0.4])
+    path = piv.write_stats_npz(tmp_path, t, stats)
+    import zipfile
+    with zipfile.ZipFile(path) as z:
+        assert [i.filename for i in z.infolist()] == [
+            k + ".npy" for k in ("t", "vorticity_mean", "vorticity_abs_mean", "ke_mean", "ke_sum",
+                                 "coverage")]
+    d = np.load(path)
+    assert d["t"].dtype == np.float64 and np.array_equal(d["t"], t)
+    for k in ("vorticity_mean", "vorticity_abs_mean", "ke_mean", "ke_sum", "coverage"):
+        assert d[k].dtype == np.float32 and d[k].shape == (4,)
+        assert np.array_equal(d[k], np.array([getattr(s, k) for s in stats], np.float32),
+                              equal_nan=True)
