@@ -9,9 +9,10 @@
 //                  per-block f64 partial sums in a fixed tree order;
 //   k_flow_finish  one workgroup per pair: partials summed in block order, then the median
 //                  absolute residual by an exact radix select over the residual keys;
-//   k_temporal     one thread per (t, y, x): moving mean (f32, offset order) or median;
+//   k_temporal     one thread per (t, y, x): moving mean (f32, offset order) or median (a
+//                  bitonic network over register keys);
 //   k_spatial      one thread per (t, y, x): normalized Gaussian convolution, f32 sums in
-//                  the reference's (row, column) order;
+//                  the reference's (row, column) order, the tile and its halo staged in LDS;
 //   metrics        keys of hypot distances (total-order keys) + one-workgroup radix select;
 //                  stream compaction of the cross-validation errors in index order.
 // Arithmetic follows the reference operation by operation (no contraction: -ffp-contract=off),
@@ -353,7 +354,36 @@ __global__ __launch_bounds__(FINISH_BLOCK) void k_flow_finish(const FlowPartial 
 }
 
 // ---- smoothing ---------------------------------------------------------------------------
-template <bool MEDIAN>
+// Bitonic sorting network over N register keys (N a power of two), fully unrolled.
+template <int N>
+__device__ __forceinline__ void bitonic(uint32_t (&k)[N])
+{
+#pragma unroll
+    for (int size = 2; size <= N; size <<= 1)
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1)
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                const int j = i ^ stride;
+                if (j > i) {
+                    if ((i & size) == 0) cswap(k[i], k[j]);
+                    else cswap(k[j], k[i]);
+                }
+            }
+}
+
+template <int N>
+__device__ __forceinline__ uint32_t pick(const uint32_t (&k)[N], int idx)
+{
+    uint32_t r = k[0];
+#pragma unroll
+    for (int q = 1; q < N; ++q) r = q == idx ? k[q] : r;
+    return r;
+}
+
+// NK: 0 for the mean; else the median over at most NK samples, sorted in registers (absent
+// samples are NO_KEY and sort last)
+template <int NK>
 __global__ __launch_bounds__(256) void k_temporal(TemporalArgs args, const float *__restrict__ dx,
                                                   const float *__restrict__ dy,
                                                   const uint8_t *__restrict__ usable,
@@ -375,7 +405,7 @@ __global__ __launch_bounds__(256) void k_temporal(TemporalArgs args, const float
     }
     float ru = nan, rv = nan;
     int m = 0;
-    if constexpr (!MEDIAN) {
+    if constexpr (NK == 0) {
         // smooth.rs:303-305 mean: f32 sum from -0.0 in push (offset) order
         float su = -0.0f, sv = -0.0f;
         for (int q = 0; q < args.noff; ++q) {
@@ -390,33 +420,25 @@ __global__ __launch_bounds__(256) void k_temporal(TemporalArgs args, const float
             rv = sv / (float)m;
         }
     } else {
-        // smooth.rs:312-320 median: insertion sort by total_cmp key
-        uint32_t ku[TEMPORAL_MAXW], kv[TEMPORAL_MAXW];
-        for (int q = 0; q < args.noff; ++q) {
-            const int s = t + args.off[q];
-            if (s < 0 || s >= args.nt || !usable[(long)s * plane + p]) continue;
-            uint32_t a = total_key(dx[(long)s * plane + p]), b = total_key(dy[(long)s * plane + p]);
-            int j = m - 1;
-            while (j >= 0 && ku[j] > a) {
-                ku[j + 1] = ku[j];
-                --j;
-            }
-            ku[j + 1] = a;
-            j = m - 1;
-            while (j >= 0 && kv[j] > b) {
-                kv[j + 1] = kv[j];
-                --j;
-            }
-            kv[j + 1] = b;
-            ++m;
+        // smooth.rs:312-320 median: total_cmp order, middle or mean of the two middles
+        uint32_t ku[NK], kv[NK];
+#pragma unroll
+        for (int q = 0; q < NK; ++q) {
+            const int s = q < args.noff ? t + args.off[q] : -1;
+            const bool use = s >= 0 && s < args.nt && usable[(long)s * plane + p];
+            ku[q] = use ? total_key(dx[(long)s * plane + p]) : NO_KEY;
+            kv[q] = use ? total_key(dy[(long)s * plane + p]) : NO_KEY;
+            m += use;
         }
         if (m >= args.need) {
+            bitonic(ku);
+            bitonic(kv);
             if (m % 2 == 1) {
-                ru = key_value(ku[m / 2]);
-                rv = key_value(kv[m / 2]);
+                ru = key_value(pick(ku, m / 2));
+                rv = key_value(pick(kv, m / 2));
             } else {
-                ru = 0.5f * (key_value(ku[m / 2 - 1]) + key_value(ku[m / 2]));
-                rv = 0.5f * (key_value(kv[m / 2 - 1]) + key_value(kv[m / 2]));
+                ru = 0.5f * (key_value(pick(ku, m / 2 - 1)) + key_value(pick(ku, m / 2)));
+                rv = 0.5f * (key_value(pick(kv, m / 2 - 1)) + key_value(pick(kv, m / 2)));
             }
         }
     }
@@ -463,6 +485,79 @@ __global__ __launch_bounds__(256) void k_spatial(SpatialArgs args, const float *
             const float a = u[(long)yy * args.nx + xx], b = v[(long)yy * args.nx + xx];
             if (finite(a) && finite(b)) {
                 const float wt = s_w[(j + r) * kw + (i + r)];
+                su += wt * a;
+                sv += wt * b;
+                sw += wt;
+                ++n;
+            }
+        }
+    }
+    sup[o] = (uint16_t)n;
+    if (sw >= args.min_full && sw > 0.0f) {
+        odx[o] = su / sw;
+        ody[o] = sv / sw;
+    } else {
+        odx[o] = nan;
+        ody[o] = nan;
+    }
+}
+
+// k_spatial with a 16x16 tile of outputs of one pair and its halo staged in LDS.  Cells
+// outside the grid are staged as NaN: the reference skips them before the finiteness test, and
+// a non-finite value is skipped the same way, so the sums see the same terms in the same order.
+constexpr int SP_TILE = 16;
+
+__host__ __device__ constexpr size_t spatial_tile_floats(int r)
+{
+    return (size_t)(2 * r + 1) * (2 * r + 1) + 2 * (size_t)(SP_TILE + 2 * r) * (SP_TILE + 2 * r);
+}
+
+__global__ __launch_bounds__(256) void k_spatial_tiled(SpatialArgs args, const float *__restrict__ w,
+                                                       const float *__restrict__ dx,
+                                                       const float *__restrict__ dy,
+                                                       const uint8_t *__restrict__ gate,
+                                                       float *__restrict__ odx,
+                                                       float *__restrict__ ody,
+                                                       uint16_t *__restrict__ sup, int tiles_x,
+                                                       int tiles)
+{
+    extern __shared__ float sm[];
+    const int r = args.r, kw = 2 * r + 1, T = SP_TILE + 2 * r;
+    float *s_w = sm, *tu = sm + kw * kw, *tv = tu + T * T;
+    const int t = blockIdx.x / tiles, tile = blockIdx.x - t * tiles;
+    const int ty0 = (tile / tiles_x) * SP_TILE, tx0 = (tile % tiles_x) * SP_TILE;
+    const long plane = (long)args.ny * args.nx;
+    const float *u = dx + (long)t * plane, *v = dy + (long)t * plane;
+    const float nan = __builtin_nanf("");
+    for (int q = threadIdx.x; q < kw * kw; q += 256) s_w[q] = w[q];
+    for (int q = threadIdx.x; q < T * T; q += 256) {
+        const int yy = ty0 - r + q / T, xx = tx0 - r + q % T;
+        const bool in = yy >= 0 && yy < args.ny && xx >= 0 && xx < args.nx;
+        tu[q] = in ? u[(long)yy * args.nx + xx] : nan;
+        tv[q] = in ? v[(long)yy * args.nx + xx] : nan;
+    }
+    __syncthreads();
+    const int ly = threadIdx.x / SP_TILE, lx = threadIdx.x % SP_TILE;
+    const int y = ty0 + ly, x = tx0 + lx;
+    if (y >= args.ny || x >= args.nx) return;
+    const long o = (long)t * plane + (long)y * args.nx + x;
+    if (!gate[o] || args.none) {
+        odx[o] = nan;
+        ody[o] = nan;
+        sup[o] = 0;
+        return;
+    }
+    float su = 0.0f, sv = 0.0f, sw = 0.0f;
+    uint32_t n = 0;
+    for (int j = -r; j <= r; ++j) {
+        const float *ru = tu + (ly + r + j) * T + lx + r, *rv = tv + (ly + r + j) * T + lx + r;
+        const float *rw = s_w + (j + r) * kw + r;
+        const bool row_in = args.leave >= 0 && (j < 0 ? -j : j) <= args.leave;
+        for (int i = -r; i <= r; ++i) {
+            if (row_in && (i < 0 ? -i : i) <= args.leave) continue;
+            const float a = ru[i], b = rv[i];
+            if (finite(a) && finite(b)) {
+                const float wt = rw[i];
                 su += wt * a;
                 sv += wt * b;
                 sw += wt;
@@ -649,12 +744,17 @@ hipError_t launch_temporal(const TemporalArgs &args, const float *dx, const floa
 {
     const long n = (long)args.nt * args.ny * args.nx;
     if (n == 0) return hipSuccess;
+    const dim3 g(nblocks(n, 256)), b(256);
     if (args.kernel == 0)
-        hipLaunchKernelGGL(k_temporal<false>, dim3(nblocks(n, 256)), dim3(256), 0, st, args, dx,
-                           dy, usable, gate, odx, ody, sup);
+        hipLaunchKernelGGL(k_temporal<0>, g, b, 0, st, args, dx, dy, usable, gate, odx, ody, sup);
+    else if (args.noff <= 8)
+        hipLaunchKernelGGL(k_temporal<8>, g, b, 0, st, args, dx, dy, usable, gate, odx, ody, sup);
+    else if (args.noff <= 16)
+        hipLaunchKernelGGL(k_temporal<16>, g, b, 0, st, args, dx, dy, usable, gate, odx, ody, sup);
+    else if (args.noff <= 32)
+        hipLaunchKernelGGL(k_temporal<32>, g, b, 0, st, args, dx, dy, usable, gate, odx, ody, sup);
     else
-        hipLaunchKernelGGL(k_temporal<true>, dim3(nblocks(n, 256)), dim3(256), 0, st, args, dx, dy,
-                           usable, gate, odx, ody, sup);
+        hipLaunchKernelGGL(k_temporal<64>, g, b, 0, st, args, dx, dy, usable, gate, odx, ody, sup);
     return hipGetLastError();
 }
 
@@ -664,8 +764,16 @@ hipError_t launch_spatial(const SpatialArgs &args, const float *w, const float *
 {
     const long n = (long)args.nt * args.ny * args.nx;
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_spatial, dim3(nblocks(n, 256)), dim3(256), 0, st, args, w, dx, dy, gate,
-                       odx, ody, sup);
+    const size_t lds = spatial_tile_floats(args.r) * sizeof(float);
+    if (lds <= 64 * 1024) {
+        const int tiles_x = (args.nx + SP_TILE - 1) / SP_TILE, tiles_y = (args.ny + SP_TILE - 1) / SP_TILE;
+        const long blocks = (long)tiles_x * tiles_y * args.nt;
+        hipLaunchKernelGGL(k_spatial_tiled, dim3((unsigned)blocks), dim3(256), lds, st, args, w, dx,
+                           dy, gate, odx, ody, sup, tiles_x, tiles_x * tiles_y);
+    } else {
+        hipLaunchKernelGGL(k_spatial, dim3(nblocks(n, 256)), dim3(256), 0, st, args, w, dx, dy,
+                           gate, odx, ody, sup);
+    }
     return hipGetLastError();
 }
 

@@ -179,8 +179,10 @@ def test_temporal_video_sized_series_on_device(piv):
 
 @pytest.mark.parametrize("sigma,min_weight,leave_out", [(1.0, 0.25, None), (1.5, 0.5, 1),
                                                         (0.7, 0.0, None), (3.0, 0.3, 2),
-                                                        (0.5, 0.0, 1)])
+                                                        (0.5, 0.0, 1), (11.9, 0.2, 3),
+                                                        (12.6, 0.2, None)])
 def test_spatial_matches_oracle(piv, sigma, min_weight, leave_out):
+    """radius 2 .. 30 take the LDS-tiled kernel, radius 32 the untiled one"""
     from piv_amd import smooth
 
     rng = np.random.default_rng(int(sigma * 100))
