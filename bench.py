@@ -420,12 +420,15 @@ def post_bench(an, g, dx, dy, fl, repeats=20, seed=3):
     dv = torch.from_numpy(rng.standard_normal((nt, ny, nx)).astype(np.float32)).to(dev)
     ok = torch.from_numpy((rng.random((nt, ny, nx)) > 0.1).astype(np.uint8)).to(dev)
     gate = torch.ones((nt, ny, nx), dtype=torch.uint8, device=dev)
+    out = (torch.empty_like(du), torch.empty_like(dv),
+           torch.empty((nt, ny, nx), dtype=torch.int16, device=dev))  # u16 support
     torch.cuda.synchronize()
     n = nt * ny * nx
-    series = {"series": [nt, ny, nx]}
-    runs = [("temporal_mean9", lambda: smooth.temporal(du, dv, ok, gate, "mean", 9, 1)),
-            ("temporal_median7", lambda: smooth.temporal(du, dv, ok, gate, "median", 7, 1)),
-            ("spatial_sigma1.5", lambda: smooth.spatial(du, dv, gate, 1.5, 0.25))]
+    series = {"series": [nt, ny, nx], "note": "whole C-ABI call: stream set-up, kernel, sync"}
+    runs = [("temporal_mean9", lambda: smooth.temporal(du, dv, ok, gate, "mean", 9, 1, out=out)),
+            ("temporal_median7",
+             lambda: smooth.temporal(du, dv, ok, gate, "median", 7, 1, out=out)),
+            ("spatial_sigma1.5", lambda: smooth.spatial(du, dv, gate, 1.5, 0.25, out=out))]
     for name, fn in runs:
         fn()
         reps = 5
@@ -456,6 +459,8 @@ def main():
     ap.add_argument("--no-video", action="store_true", help="skip the batched-video measurement")
     ap.add_argument("--no-post", action="store_true",
                     help="skip the flow-statistics and smoothing measurement")
+    ap.add_argument("--only-post", action="store_true",
+                    help="analyse one batch, then measure only the post-analysis stages")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the reduced-precision (DFT-GEMM) measurement")
     ap.add_argument("--serve-backlog", type=int, default=400)
@@ -512,6 +517,10 @@ def main():
     for _ in range(args.warmup):
         an.analyze_batch_device(fa, fb, dx, dy, pr, fl, summaries=False, sync=True)
     torch.cuda.synchronize()
+    if args.only_post:
+        an.analyze_batch_device(fa, fb, dx, dy, pr, fl, summaries=False, sync=True)
+        print(json.dumps(post_bench(an, g, dx, dy, fl)), flush=True)
+        return
     tune = None
     if not args.no_autotune:
         # kernel variants are bit-identical: the autotuner only picks the fastest schedule

@@ -52,32 +52,40 @@ def _result(host, arrs):
     return [a.numpy() for a in arrs] if host else arrs
 
 
+def _outputs(shape, device, out):
+    """caller's (dx, dy, support) device buffers, or new ones"""
+    if out is not None:
+        return out
+    return (DeviceArray(shape, np.float32, device), DeviceArray(shape, np.float32, device),
+            DeviceArray(shape, np.uint16, device))
+
+
 def temporal(dx, dy, usable, gate, kernel=MEAN, window=5, min_support=1, leave_out=(),
-             device=0) -> Smoothed:
-    """smooth.rs:44-130: moving mean or median over `window` (odd) consecutive pairs."""
+             device=0, out=None) -> Smoothed:
+    """smooth.rs:44-130: moving mean or median over `window` (odd) consecutive pairs.
+    out: optional (dx, dy, support) device buffers [nt, ny, nx] (f32, f32, u16) to write."""
     nt, ny, nx = _shape3(dx)
     host, (ddx, ddy, dok, dg) = _device([dx, dy, usable, gate], device,
                                         [np.float32, np.float32, np.uint8, np.uint8])
-    odx, ody = DeviceArray((nt, ny, nx), np.float32, device), DeviceArray((nt, ny, nx), np.float32, device)
-    sup = DeviceArray((nt, ny, nx), np.uint16, device)
+    odx, ody, sup = _outputs((nt, ny, nx), device, out)
     lo = np.ascontiguousarray(list(leave_out) or [0], np.int32)
     _check(L.load().piv_smooth_temporal_device(
         device, nt, ny, nx, L.ptr(ddx), L.ptr(ddy), L.ptr(dok), L.ptr(dg),
         0 if kernel == MEAN else 1, window, min_support, L.ptr(lo), len(leave_out),
         L.ptr(odx), L.ptr(ody), L.ptr(sup)))
-    return Smoothed(*_result(host, [odx, ody, sup]))
+    return Smoothed(*_result(host and out is None, [odx, ody, sup]))
 
 
-def spatial(dx, dy, gate, sigma, min_weight, leave_out=None, device=0) -> Smoothed:
-    """smooth.rs:137-208: Gaussian low-pass of each pair's field by normalized convolution."""
+def spatial(dx, dy, gate, sigma, min_weight, leave_out=None, device=0, out=None) -> Smoothed:
+    """smooth.rs:137-208: Gaussian low-pass of each pair's field by normalized convolution.
+    out: optional (dx, dy, support) device buffers to write."""
     nt, ny, nx = _shape3(dx)
     host, (ddx, ddy, dg) = _device([dx, dy, gate], device, [np.float32, np.float32, np.uint8])
-    odx, ody = DeviceArray((nt, ny, nx), np.float32, device), DeviceArray((nt, ny, nx), np.float32, device)
-    sup = DeviceArray((nt, ny, nx), np.uint16, device)
+    odx, ody, sup = _outputs((nt, ny, nx), device, out)
     _check(L.load().piv_smooth_spatial_device(
         device, nt, ny, nx, L.ptr(ddx), L.ptr(ddy), L.ptr(dg), float(sigma), float(min_weight),
         -1 if leave_out is None else int(leave_out), L.ptr(odx), L.ptr(ody), L.ptr(sup)))
-    return Smoothed(*_result(host, [odx, ody, sup]))
+    return Smoothed(*_result(host and out is None, [odx, ody, sup]))
 
 
 def mask(values, usable, device=0):
