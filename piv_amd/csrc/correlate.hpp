@@ -227,6 +227,23 @@ __device__ __forceinline__ void ingest(const PassPair &pp, bool sig, float ax, f
     }
 }
 
+// IEEE max of finite values without the operand canonicalisation the compiler adds to fmaxf
+// when it cannot prove an operand canonical (values loaded from LDS); used only where every
+// value is finite.
+__device__ __forceinline__ float max3_finite(float a, float b, float c)
+{
+    float r;
+    asm("v_max3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
+__device__ __forceinline__ float max_finite(float a, float b)
+{
+    float r;
+    asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
 // find_peak (correlate.rs:215-303) on a window's correlation plane in LDS (row stride N+1),
 // lane = row: first row-major argmax, footprint half width, highest local maximum outside
 // it; lane 0 of the window writes the peak record.
@@ -242,6 +259,10 @@ __device__ __forceinline__ void find_peak_plane(const float *pr, int lane, int t
     // NaN can only reach the plane from non-finite float input; the integer formats take
     // the comparison shortcuts that assume finite values.
     constexpr bool MAYBE_NAN = (FMT == FMT_F32);
+    // The raw-max form of the 3x3 scan saves VALU at N <= 32; at N = 64 (one or two waves per
+    // SIMD) its extra scalar and LDS instructions cost more than it saves.  Both forms give
+    // identical results.
+    constexpr bool RAW_MAX = !MAYBE_NAN && N <= 32;
     // (1) first maximum in row-major order (strict >): row scan, then across rows
 #pragma unroll
     for (int x = 0; x < N; ++x) zr[x] = pr[lane * P + x];
@@ -358,15 +379,23 @@ __device__ __forceinline__ void find_peak_plane(const float *pr, int lane, int t
             }
         }
         // vertical max of (row above, row, row below) per column, sliding along x
-        float vm_prev = fmaxf(fmaxf(pr[ru * P + N - 1], zr[N - 1]), pr[rd * P + N - 1]);
         float u1 = pr[ru * P + 0], d1 = pr[rd * P + 0];
-        float vm_cur = fmaxf(fmaxf(u1, zr[0]), d1);
+        float vm_prev, vm_cur;
+        if constexpr (!RAW_MAX) {
+            vm_prev = fmaxf(fmaxf(pr[ru * P + N - 1], zr[N - 1]), pr[rd * P + N - 1]);
+            vm_cur = fmaxf(fmaxf(u1, zr[0]), d1);
+        } else {
+            vm_prev = max3_finite(pr[ru * P + N - 1], zr[N - 1], pr[rd * P + N - 1]);
+            vm_cur = max3_finite(u1, zr[0], d1);
+        }
 #pragma unroll
         for (int x = 0; x < N; ++x) {
             const int xn = x + 1 < N ? x + 1 : 0;
             const float u2 = pr[ru * P + xn], d2 = pr[rd * P + xn];
-            const float vm_next = fmaxf(fmaxf(u2, zr[xn]), d2);
             const float v = zr[x];
+            float vm_next;
+            if constexpr (!RAW_MAX) vm_next = fmaxf(fmaxf(u2, zr[xn]), d2);
+            else vm_next = max3_finite(u2, zr[xn], d2);
             bool far;
             if constexpr (N <= 64) {
                 far = (allowed >> x) & 1;
@@ -382,10 +411,14 @@ __device__ __forceinline__ void find_peak_plane(const float *pr, int lane, int t
                 local = u0 <= v && u1 <= v && u2 <= v && m0 <= v && m2 <= v && d0 <= v &&
                         d1 <= v && d2 <= v;
                 if (local && far && v > cand) cand = v;
-            } else {
+            } else if constexpr (!RAW_MAX) {
                 // finite values: max(cand, v) is the strict-> update of the reference
                 local = fmaxf(fmaxf(vm_prev, vm_next), fmaxf(u1, d1)) <= v;
                 cand = (local && far) ? fmaxf(cand, v) : cand;
+            } else {
+                // the 3x3 maximum (v included) is <= v exactly when every neighbour is
+                local = max3_finite(vm_prev, vm_cur, vm_next) <= v;
+                cand = (local && far) ? max_finite(cand, v) : cand;
             }
             vm_prev = vm_cur;
             vm_cur = vm_next;
@@ -394,7 +427,11 @@ __device__ __forceinline__ void find_peak_plane(const float *pr, int lane, int t
         }
         if constexpr (C::IN_WAVE) {
 #pragma unroll
-            for (int m = 1; m < N; m <<= 1) cand = fmaxf(cand, __shfl_xor(cand, m, N));
+            for (int m = 1; m < N; m <<= 1) {
+                const float o = __shfl_xor(cand, m, N);
+                if constexpr (!RAW_MAX) cand = fmaxf(cand, o);
+                else cand = max_finite(cand, o);
+            }
             second = cand;
         } else {
             __syncthreads();
