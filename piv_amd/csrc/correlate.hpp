@@ -16,7 +16,8 @@
 //   4. cross spectrum conj(FA) FB from Z[k] and Z[-k], with the DC bin cleared (the mean
 //      removal of load_window) and the window energies from Parseval;
 //   5. Hermitian inverse: row IFFT of rows 0..N/2, then column-pair IFFTs (two real
-//      columns per complex transform), scaled into an LDS correlation plane;
+//      columns per complex transform, split over two lanes for in-wave windows of N >= 32),
+//      scaled into an LDS correlation plane;
 //   6. find_peak (correlate.rs:215-303): first row-major argmax, footprint half width,
 //      highest local maximum outside it; the peak record goes to the finishing kernel
 //      (one thread per window), which does the 3-point fit, the peak ratio and assembly.
@@ -465,6 +466,11 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
     __shared__ float s_red[WPB * N];
     __shared__ float s_pkv[WPB * N];
     __shared__ int s_pki[WPB * N];
+    // inverse column pass split over two lanes per column pair (in-wave windows, N >= 32; at
+    // N = 16 the shorter transform saves less than the twiddles cost): entry (p, n) = 1 for the
+    // even half (p = 0), e^{+2 pi i n / N} for the odd half (p = 1), n < N / 2
+    constexpr bool SPLIT_INV = C::IN_WAVE && N >= 32;
+    __shared__ float2 s_tw[SPLIT_INV ? N : 1];
 
     // ---- XCD-aware block order: XCD x (blocks L = x, x+8, ...) gets a contiguous range ----
     const uint32_t L = blockIdx.x;
@@ -475,6 +481,13 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
     const int tid = threadIdx.x;
     const int wl = tid / N;
     const int lane = tid - wl * N;
+    if constexpr (SPLIT_INV) {
+        if (tid < N) {
+            const int hp = tid / H2, n = tid - hp * H2;
+            s_tw[tid] = hp ? make_float2(pivfft::Tw<N>::t.c[n], pivfft::Tw<N>::t.s[n])
+                           : make_float2(1.0f, 0.0f);
+        }
+    }
     const PassPair &pp = pairs[p];
     const GridDev g = pp.grid;
     const uint32_t cells = (uint32_t)(g.nx * g.ny);
@@ -668,12 +681,39 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
             if ((ky & 7) == 7) __builtin_amdgcn_sched_barrier(0);
         }
     }
-    pivfft::fft<N, 1>(zr, zi);
-    {
-        // correlate.rs:134-137: corr = C / (n^2 sqrt(ea eb)), the 1/4 undoes X' = 4 X
-        const float denom = (float)((double)(N * N) * sqrt(ea * eb));
-        const float scale = 0.25f / denom;
-        const int c0 = 2 * jj, c1 = 2 * jj + 1;
+    // correlate.rs:134-137: corr = C / (n^2 sqrt(ea eb)), the 1/4 undoes X' = 4 X
+    const float denom = (float)((double)(N * N) * sqrt(ea * eb));
+    const float scale = 0.25f / denom;
+    const int c0 = 2 * jj, c1 = 2 * jj + 1;
+    if constexpr (SPLIT_INV) {
+        // Column pair jj by two lanes (decimation in frequency): lane half hp transforms
+        // V[n] + (-1)^hp V[n + N/2], the odd half twiddled by e^{+2 pi i n / N}, with an N/2-point
+        // inverse, and owns rows y = 2k + hp of the two columns.
+        const int hp = lane / H2;
+        const float sg = hp ? -1.0f : 1.0f;
+        float ur[H2], ui[H2];
+#pragma unroll
+        for (int n = 0; n < H2; ++n) {
+            const float ar = __builtin_fmaf(sg, zr[n + H2], zr[n]);
+            const float ai = __builtin_fmaf(sg, zi[n + H2], zi[n]);
+            if (n == 0) {
+                ur[n] = ar;
+                ui[n] = ai;
+            } else {
+                const float2 w = s_tw[hp * H2 + n];
+                ur[n] = __builtin_fmaf(ar, w.x, -(ai * w.y));
+                ui[n] = __builtin_fmaf(ar, w.y, ai * w.x);
+            }
+        }
+        pivfft::fft<H2, 1>(ur, ui);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < H2; ++k) {
+            pr[(2 * k + hp) * P + c0] = ur[k] * scale;
+            pr[(2 * k + hp) * P + c1] = ui[k] * scale;
+        }
+    } else {
+        pivfft::fft<N, 1>(zr, zi);
         __syncthreads();
         if (lane < H2) {
 #pragma unroll
