@@ -260,10 +260,8 @@ __device__ __forceinline__ void find_peak_plane(const float *pr, int lane, int t
     // NaN can only reach the plane from non-finite float input; the integer formats take
     // the comparison shortcuts that assume finite values.
     constexpr bool MAYBE_NAN = (FMT == FMT_F32);
-    // The raw-max form of the 3x3 scan saves VALU at N <= 32; at N = 64 (one or two waves per
-    // SIMD) its extra scalar and LDS instructions cost more than it saves.  Both forms give
-    // identical results.
-    constexpr bool RAW_MAX = !MAYBE_NAN && N <= 32;
+    // finite planes take the raw-max form of the 3x3 scan (no operand canonicalisation)
+    constexpr bool RAW_MAX = !MAYBE_NAN;
     // (1) first maximum in row-major order (strict >): row scan, then across rows
 #pragma unroll
     for (int x = 0; x < N; ++x) zr[x] = pr[lane * P + x];
@@ -417,9 +415,11 @@ __device__ __forceinline__ void find_peak_plane(const float *pr, int lane, int t
                 local = fmaxf(fmaxf(vm_prev, vm_next), fmaxf(u1, d1)) <= v;
                 cand = (local && far) ? fmaxf(cand, v) : cand;
             } else {
-                // the 3x3 maximum (v included) is <= v exactly when every neighbour is
+                // the 3x3 maximum (v included) is <= v exactly when every neighbour is; the max
+                // is taken unconditionally so the update is a select, not a branch
                 local = max3_finite(vm_prev, vm_cur, vm_next) <= v;
-                cand = (local && far) ? max_finite(cand, v) : cand;
+                const float up = max_finite(cand, v);
+                cand = (local && far) ? up : cand;
             }
             vm_prev = vm_cur;
             vm_cur = vm_next;
