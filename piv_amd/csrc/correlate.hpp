@@ -95,7 +95,8 @@ __device__ __forceinline__ void load_column_raw(const void *__restrict__ img, in
         const float h = ints ? p0 : __builtin_fmaf(tx, p1 - p0, p0);
         if (r > 0) col[r - 1] = ints ? hprev : __builtin_fmaf(ty, h - hprev, hprev);
         hprev = h;
-        // bound the loads in flight (and their registers) to 16 rows at a time
+        // bound the loads in flight (and their registers) to 16 rows at a time (the clamped
+        // path, with two addresses per row)
         if ((r & 15) == 15) __builtin_amdgcn_sched_barrier(0);
     }
 }
@@ -141,7 +142,7 @@ __device__ __forceinline__ void load_column_fast(const void *__restrict__ img, i
             else col[r - 1] = __builtin_fmaf(ty, h - hprev, hprev);
         }
         hprev = h;
-        if ((r & 15) == 15) __builtin_amdgcn_sched_barrier(0);
+        if ((r & 31) == 31) __builtin_amdgcn_sched_barrier(0);
     }
 }
 
@@ -164,7 +165,7 @@ __device__ __forceinline__ void load_column_int(const void *__restrict__ img, in
 #pragma unroll
     for (int r = 0; r < N; ++r) {
         col[r] = (float)base[o + (uint32_t)r * (uint32_t)pitch];
-        if ((r & 15) == 15) __builtin_amdgcn_sched_barrier(0);
+        if ((r & 31) == 31) __builtin_amdgcn_sched_barrier(0);
     }
 }
 
