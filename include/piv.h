@@ -339,6 +339,13 @@ piv_status piv_autotune(piv_analyzer *a, const piv_frame *const *frames_a,
                         const piv_frame *const *frames_b, uint32_t npairs, uint32_t repeats,
                         piv_tune_entry *out, uint32_t capacity, uint32_t *count);
 
+/* Frame source channel selection on decoded clips (piv-io/src/decode.rs:12-26,124-133): one
+ * plane of `count` pixels of interleaved samples, device to device: out[i] = pixels[i *
+ * channels + plane].  The green plane of an RGB clip is ffmpeg's `format=gbrp,extractplanes=g`
+ * of an RGB source (channels 3 or 4, plane 1). */
+piv_status piv_extract_plane_device(int device, const void *pixels, piv_pixel_format fmt,
+                                    size_t count, uint32_t channels, uint32_t plane, void *out);
+
 /* decode.rs:227-234 sample normalisation on the device (host in/out), for parity checks. */
 piv_status piv_normalize_pixels(int device, const void *pixels, piv_pixel_format fmt,
                                 size_t count, float *out);

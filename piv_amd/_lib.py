@@ -159,6 +159,8 @@ _SIGS = {
     "piv_flow_stats_device": ([C.c_void_p, C.POINTER(piv_grid), C.c_uint32, C.c_void_p,
                                C.c_void_p, C.c_void_p, C.c_double, C.POINTER(piv_flow_stats)],
                               C.c_int),
+    "piv_extract_plane_device": ([C.c_int, C.c_void_p, C.c_int, C.c_size_t, C.c_uint32,
+                                  C.c_uint32, C.c_void_p], C.c_int),
     "piv_write_stats_npz": ([C.c_char_p, C.c_uint32, C.c_void_p, C.c_void_p], C.c_int),
     "piv_smooth_temporal_device": ([C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_uint32,

@@ -61,6 +61,7 @@ struct Args {
     double px_per_mm = 0;
     uint32_t batch = 64;
     std::string precision = "fp32";
+    std::string channel = "luma";
     int device = 0;
 };
 
@@ -70,7 +71,8 @@ void usage()
                  "usage: piv-analyze INPUT.npy -o OUTPUT_DIR [--force]\n"
                  "  frames:        --start N | --start-s S, --end N | --end-s S, --stride 1, --pair-step 1,\n"
                  "                 --capture-fps FPS (default 1)\n"
-                 "  image:         --roi x,y,w,h, --background none|min:N|mean:N (min:40),\n"
+                 "  image:         --channel luma|green|yplane, --roi x,y,w,h,\n"
+                 "                 --background none|min:N|mean:N (min:40),\n"
                  "                 --smooth-sigma 1, --normalize-radius 4\n"
                  "  interrogation: --windows 96,64, --overlap 0.5, --final-iterations 2,\n"
                  "                 --min-mean 0.04, --min-std 0.015\n"
@@ -110,6 +112,7 @@ Args parse(int argc, char **argv)
                 die("--roi expects x,y,w,h");
             a.roi = true;
         } else if (s == "--background") a.background = need(i);
+        else if (s == "--channel") a.channel = need(i);
         else if (s == "--smooth-sigma") a.smooth_sigma = std::stof(need(i));
         else if (s == "--normalize-radius") a.normalize_radius = (uint32_t)std::stoul(need(i));
         else if (s == "--windows") {
@@ -146,14 +149,15 @@ Args parse(int argc, char **argv)
     return a;
 }
 
-// A C-order .npy array [frames][height][width], memory-mapped.
+// A C-order .npy array [frames][height][width] (one plane) or [frames][height][width][3|4]
+// (interleaved RGB / RGBA samples), memory-mapped.
 struct Npy {
     const unsigned char *base = nullptr;
     size_t size = 0;
     const unsigned char *data = nullptr;
     piv_pixel_format fmt = PIV_PIXEL_U8;
     size_t bpp = 1;
-    size_t frames = 0, height = 0, width = 0;
+    size_t frames = 0, height = 0, width = 0, channels = 1;
 };
 
 Npy open_npy(const std::string &path)
@@ -204,11 +208,14 @@ Npy open_npy(const std::string &path)
         if (c == std::string::npos) break;
         p = c + 1;
     }
-    if (dims.size() != 3) die("expected a [frames, height, width] array");
+    if (dims.size() == 4 && (dims[3] == 3 || dims[3] == 4)) n.channels = dims[3];
+    else if (dims.size() != 3)
+        die("expected a [frames, height, width] or [frames, height, width, 3|4] array");
     n.frames = dims[0];
     n.height = dims[1];
     n.width = dims[2];
-    if (n.data + n.frames * n.height * n.width * n.bpp > n.base + n.size) die("truncated .npy file");
+    if (n.data + n.frames * n.height * n.width * n.channels * n.bpp > n.base + n.size)
+        die("truncated .npy file");
     return n;
 }
 
@@ -233,6 +240,18 @@ int main(int argc, char **argv)
     if (A.format != "npz") die("unknown format " + A.format);
     const Npy in = open_npy(A.input);
     const double fps = A.fps > 0 ? A.fps : 1.0;
+    // decode.rs:12-26,114-133 channel selection.  A one-plane clip already is the plane ffmpeg's
+    // filters would have produced (its gray conversion, or the stored Y plane); an RGB clip
+    // gives the green plane (gbrp + extractplanes=g is a plain copy for RGB sources).  ffmpeg's
+    // gray conversion of RGB, and a Y plane of RGB, are not reproduced.
+    if (A.channel != "luma" && A.channel != "green" && A.channel != "yplane")
+        die("unknown channel " + A.channel + " (luma, green or yplane)");
+    if (in.channels == 1 && A.channel == "green")
+        die("--channel green needs a colour clip [frames, height, width, 3|4]; this clip has one "
+            "plane");
+    if (in.channels > 1 && A.channel != "green")
+        die("--channel " + A.channel + " of an RGB clip is an ffmpeg conversion this tool does not "
+            "reproduce; decode the clip to that plane, or use --channel green");
 
     // frame range (analyze.rs:251-269)
     auto to_frame = [&](double s) { return (long)std::max(0.0, std::round(s * fps)); };
@@ -297,10 +316,21 @@ int main(int argc, char **argv)
     piv_grid grid;
     check(piv_output_grid(an, H, W, &grid), "grid");
 
-    // the clip in HBM
+    // the clip in HBM (one plane)
     void *clip = nullptr;
     check(piv_device_alloc(A.device, in.frames * frame_bytes, &clip), "device memory for the clip");
-    check(piv_copy_to_device(A.device, clip, in.data, in.frames * frame_bytes), "upload");
+    if (in.channels == 1) {
+        check(piv_copy_to_device(A.device, clip, in.data, in.frames * frame_bytes), "upload");
+    } else {
+        void *rgb = nullptr;
+        const size_t bytes = in.frames * frame_bytes * in.channels;
+        check(piv_device_alloc(A.device, bytes, &rgb), "device memory for the colour clip");
+        check(piv_copy_to_device(A.device, rgb, in.data, bytes), "upload");
+        check(piv_extract_plane_device(A.device, rgb, in.fmt, in.frames * in.height * in.width,
+                                       (uint32_t)in.channels, 1, clip),
+              "green plane");
+        piv_device_free(A.device, rgb);
+    }
     auto frame_ptr = [&](long f) {
         return static_cast<const char *>(clip) + (size_t)f * frame_bytes + roi_off;
     };
@@ -417,6 +447,7 @@ int main(int argc, char **argv)
         buf, sizeof buf,
         "{\n  \"tool_version\": \"piv-amd %d\",\n  \"source_path\": %s,\n  \"source_size\": %zu,\n"
         "  \"camera\": null,\n  \"frame_width\": %zu,\n  \"frame_height\": %zu,\n  \"roi\": %s,\n"
+        "  \"channel\": %s,\n"
         "  \"capture_fps\": %.17g,\n  \"start_frame\": %ld,\n  \"end_frame\": %ld,\n  \"stride\": %d,\n"
         "  \"pair_step\": %d,\n  \"dt_s\": %.17g,\n  \"background\": %s,\n"
         "  \"preprocess\": {\"smooth_sigma\": %.9g, \"normalize_radius\": %u, \"normalize_eps\": %.9g},\n"
@@ -429,7 +460,7 @@ int main(int argc, char **argv)
         A.roi ? ("{\"x\": " + std::to_string(A.rx) + ", \"y\": " + std::to_string(A.ry) + ", \"w\": " +
                  std::to_string(A.rw) + ", \"h\": " + std::to_string(A.rh) + "}").c_str()
               : "null",
-        fps, start, end, A.stride, A.pair_step, dt,
+        json_str(A.channel).c_str(), fps, start, end, A.stride, A.pair_step, dt,
         bg_spec == "none" ? "null" : json_str(bg_spec).c_str(), cfg.smooth_sigma, cfg.normalize_radius,
         cfg.normalize_eps, ws.c_str(), cfg.overlap, cfg.final_iterations, cfg.min_mean, cfg.min_std,
         cfg.ppr_threshold, cfg.median_threshold, cfg.median_eps, cfg.replace ? "true" : "false",

@@ -1410,6 +1410,26 @@ piv_status piv_autotune(piv_analyzer *a, const piv_frame *const *fa, const piv_f
     return PIV_OK;
 }
 
+piv_status piv_extract_plane_device(int device, const void *pixels, piv_pixel_format fmt,
+                                    size_t count, uint32_t channels, uint32_t plane, void *out)
+{
+    if (!pixels || !out) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+    if ((int)fmt != FMT_U8 && (int)fmt != FMT_U16 && (int)fmt != FMT_F32)
+        return fail(PIV_ERR_INVALID_ARGUMENT, "unknown pixel format %d", (int)fmt);
+    if (channels < 1 || channels > 4 || plane >= channels)
+        return fail(PIV_ERR_INVALID_ARGUMENT, "plane %u of %u channels", plane, channels);
+    if (count == 0) return PIV_OK;
+    PIV_TRY(set_device(device));
+    hipStream_t st = nullptr;
+    HIP_TRY(hipStreamCreate(&st));
+    hipError_t e = launch_extract_plane((int)fmt, pixels, (long)count, (int)channels, (int)plane,
+                                        out, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+    HIP_TRY(e);
+    return PIV_OK;
+}
+
 piv_status piv_normalize_pixels(int device, const void *pixels, piv_pixel_format fmt,
                                 size_t count, float *out)
 {

@@ -929,6 +929,32 @@ hipError_t launch_bg_finish(float *acc, long n, float count, hipStream_t st)
     return hipGetLastError();
 }
 
+// One plane of interleaved samples (decode.rs:124-133: the green plane of an RGB source is
+// ffmpeg's `format=gbrp,extractplanes=g`, a plain copy of every pixel's second sample).
+template <typename T>
+__global__ void k_extract_plane(const T *__restrict__ src, long count, int channels, int plane,
+                                T *__restrict__ dst)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) dst[i] = src[i * channels + plane];
+}
+
+hipError_t launch_extract_plane(int fmt, const void *src, long count, int channels, int plane,
+                                void *dst, hipStream_t st)
+{
+    const dim3 g(nblk(count, 256)), b(256);
+    if (fmt == FMT_U8)
+        hipLaunchKernelGGL(k_extract_plane<uint8_t>, g, b, 0, st, static_cast<const uint8_t *>(src),
+                           count, channels, plane, static_cast<uint8_t *>(dst));
+    else if (fmt == FMT_U16)
+        hipLaunchKernelGGL(k_extract_plane<uint16_t>, g, b, 0, st, static_cast<const uint16_t *>(src),
+                           count, channels, plane, static_cast<uint16_t *>(dst));
+    else
+        hipLaunchKernelGGL(k_extract_plane<float>, g, b, 0, st, static_cast<const float *>(src),
+                           count, channels, plane, static_cast<float *>(dst));
+    return hipGetLastError();
+}
+
 hipError_t launch_normalize(int fmt, const void *raw, long count, float *out, hipStream_t st)
 {
     if (fmt == FMT_U8)

@@ -41,6 +41,9 @@ hipError_t launch_blur(const float *in, int H, int W, const float *taps, int r, 
 hipError_t launch_local_norm(const float *in, int H, int W, int radius, float eps,
                              const double *sum, const double *sq, float *out, hipStream_t st);
 hipError_t launch_normalize(int fmt, const void *raw, long count, float *out, hipStream_t st);
+// dst[i] = src[i * channels + plane] for `count` pixels of u8 / u16 / f32 samples
+hipError_t launch_extract_plane(int fmt, const void *src, long count, int channels, int plane,
+                                void *dst, hipStream_t st);
 hipError_t launch_bg_add(int fmt, const void *raw, int pitch, int H, int W, int mode, int first,
                          float *acc, hipStream_t st);
 hipError_t launch_bg_finish(float *acc, long n, float count, hipStream_t st);

@@ -102,6 +102,38 @@ def test_cli_pairs_background_roi_npz(piv, tmp_path, dtype, bits_):
             assert st[k][i] == pytest.approx(want[k], rel=1e-6, nan_ok=True), (a, k)
 
 
+def test_cli_green_plane_of_rgb_clip(piv, tmp_path):
+    """decode.rs:124-133: --channel green of an RGB(A) clip analyses the green plane, exactly
+    as the one-plane clip of those samples."""
+    frames, _ = S.video_frames(128, 160, 5, 31, bits=8, container=np.uint8)
+    gray = np.stack(frames)
+    rng = np.random.default_rng(2)
+    for c in (3, 4):
+        rgb = rng.integers(0, 256, size=gray.shape + (c,), dtype=np.uint8)
+        rgb[..., 1] = gray
+        np.save(tmp_path / f"rgb{c}.npy", rgb)
+    np.save(tmp_path / "gray.npy", gray)
+    common = ["--windows", "32", "--final-iterations", "1", "--min-mean", "0", "--min-std", "0",
+              "--background", "none"]
+    rc, log = _run([str(tmp_path / "gray.npy"), "-o", str(tmp_path / "g")] + common)
+    assert rc == 0, log
+    for c in (3, 4):
+        out = tmp_path / f"c{c}"
+        rc, log = _run([str(tmp_path / f"rgb{c}.npy"), "-o", str(out), "--channel", "green"] + common)
+        assert rc == 0, log
+        assert json.loads((out / "run.json").read_text())["channel"] == "green"
+        for a in range(len(frames) - 1):
+            name = f"pair_{a:06d}_{a + 1:06d}.npz"
+            d, e = np.load(out / name), np.load(tmp_path / "g" / name)
+            for k in ("dx", "dy", "peak_ratio"):
+                assert np.array_equal(bits(d[k]), bits(e[k])), (c, a, k)
+            assert np.array_equal(d["flags"], e["flags"])
+    rc, log = _run([str(tmp_path / "gray.npy"), "-o", str(tmp_path / "x"), "--channel", "green"])
+    assert rc != 0 and "colour clip" in log
+    rc, log = _run([str(tmp_path / "rgb3.npy"), "-o", str(tmp_path / "y"), "--channel", "luma"])
+    assert rc != 0 and "ffmpeg" in log
+
+
 def test_cli_errors(piv, tmp_path):
     clip = np.zeros((3, 64, 64), np.uint8)
     src = tmp_path / "c.npy"
