@@ -174,7 +174,8 @@ piv_status prepare_frame(piv_analyzer *a, piv_frame *f, const float *bg_dev)
     f->has_bg = bg_dev != nullptr;
     f->cells = (int64_t)(H + 1) * (W + 1);
     hipStream_t st = a->stream;
-    if (cond || mask) {
+    f->base_ptr = nullptr;
+    if (cond) {
         PIV_TRY(f->base.reserve(n * sizeof(float)));
         f->base_ptr = f->base.as<float>();
         HIP_TRY(launch_base(f->in_fmt, f->raw_ptr, f->raw_pitch, bg_dev, H, W, f->base_ptr, st));
@@ -182,7 +183,11 @@ piv_status prepare_frame(piv_analyzer *a, piv_frame *f, const float *bg_dev)
     if (mask) {
         PIV_TRY(f->sat.reserve((size_t)f->cells * 2 * sizeof(double)));
         f->sat_ptr = f->sat.as<double>();
-        HIP_TRY(launch_sat(f->base_ptr, H, W, f->sat.as<double>(), f->sat.as<double>() + f->cells, st));
+        double *sum = f->sat.as<double>(), *sq = sum + f->cells;
+        // without conditioning the base image is the normalised raw frame: the table is summed
+        // from the samples and no base image is kept (piv_frame_download recomputes it)
+        if (cond) HIP_TRY(launch_sat(f->base_ptr, H, W, sum, sq, st));
+        else HIP_TRY(launch_sat_raw(f->in_fmt, f->raw_ptr, f->raw_pitch, 0, H, W, sum, sq, st));
         f->has_sat = true;
     }
     if (!cond) {
@@ -235,43 +240,22 @@ piv_status prepare_frames_batch(piv_analyzer *a, piv_frame *const *fs, int count
         return PIV_OK;
     }
     const int H = f0->H, W = f0->W;
-    const size_t n = (size_t)H * W;
     const int64_t cells = (int64_t)(H + 1) * (W + 1);
     std::shared_ptr<DevBuf> store;
-    if (mask && count >= 8 && sat_frames_supported(W)) {
-        // one pass over the raw samples per frame, one wave per frame; no base image is kept
-        // (piv_frame_download recomputes it)
+    if (mask) {
+        // one pass over the raw samples per frame; no base image is kept (piv_frame_download
+        // recomputes it)
         store.reset(new DevBuf, [](DevBuf *b) {
             b->release();
             delete b;
         });
         PIV_TRY(store->reserve((size_t)count * cells * 2 * sizeof(double)));
         double *sat = store->as<double>();
-        HIP_TRY(launch_sat_frames(f0->in_fmt, f0->raw_ptr, f0->raw_pitch, raw_frame_bytes, H, W, sat,
-                                  2 * cells, count, a->stream));
+        HIP_TRY(launch_sat_raw(f0->in_fmt, f0->raw_ptr, f0->raw_pitch, raw_frame_bytes, H, W, sat,
+                               sat + cells, a->stream, count, 2 * cells));
         for (int i = 0; i < count; ++i) {
             piv_frame *f = fs[i];
             f->base_ptr = nullptr;
-            f->sat_ptr = sat + (size_t)i * 2 * cells;
-            f->has_sat = true;
-            f->batch = store;
-        }
-    } else if (mask) {
-        store.reset(new DevBuf, [](DevBuf *b) {
-            b->release();
-            delete b;
-        });
-        const size_t base_bytes = align_up((size_t)count * n * sizeof(float), 256);
-        PIV_TRY(store->reserve(base_bytes + (size_t)count * cells * 2 * sizeof(double)));
-        float *base = store->as<float>();
-        double *sat = reinterpret_cast<double *>(store->as<char>() + base_bytes);
-        hipStream_t st = a->stream;
-        HIP_TRY(launch_base(f0->in_fmt, f0->raw_ptr, f0->raw_pitch, nullptr, H, W, base, st, count,
-                            raw_frame_bytes));
-        HIP_TRY(launch_sat(base, H, W, sat, sat + cells, st, count, 2 * cells));
-        for (int i = 0; i < count; ++i) {
-            piv_frame *f = fs[i];
-            f->base_ptr = base + (size_t)i * n;
             f->sat_ptr = sat + (size_t)i * 2 * cells;
             f->has_sat = true;
             f->batch = store;

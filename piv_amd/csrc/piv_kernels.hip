@@ -334,16 +334,18 @@ __global__ void k_base(const void *__restrict__ raw, int pitch, long raw_frame_b
 // (coalesced) while the current tile is scanned from LDS, then stores column c of the scanned
 // tile (coalesced).  Few rows per wave spread one frame over many CUs; the chain per row is
 // serial whatever the split, so the scan costs the same per tile.
-constexpr int SAT_ROWS = 64, SAT_TW = 32, SATR_ROWS = 16, SATR_TW = 64;
+constexpr int SATR_ROWS = 16, SATR_TW = 64;
 
-__global__ __launch_bounds__(64) void k_sat_rows(const float *__restrict__ img, int H, int W,
+template <int FMT>
+__global__ __launch_bounds__(64) void k_sat_rows(const void *__restrict__ img, int pitch,
+                                                 long frame_bytes, int H, int W,
                                                  double *__restrict__ sum,
                                                  double *__restrict__ sq, long sat_stride)
 {
     __shared__ float tin[SATR_ROWS][SATR_TW + 1];
     __shared__ double ts[SATR_ROWS][SATR_TW + 1], tq[SATR_ROWS][SATR_TW + 1];
-    // blockIdx.y: frame of a batch (images H*W apart, tables sat_stride apart)
-    img += (long)blockIdx.y * H * W;
+    // blockIdx.y: frame of a batch (images frame_bytes apart, tables sat_stride apart)
+    img = static_cast<const char *>(img) + (long)blockIdx.y * frame_bytes;
     sum += (long)blockIdx.y * sat_stride;
     sq += (long)blockIdx.y * sat_stride;
     const int lane = threadIdx.x, y0 = blockIdx.x * SATR_ROWS;
@@ -363,7 +365,7 @@ __global__ __launch_bounds__(64) void k_sat_rows(const float *__restrict__ img, 
         const int x = t * SATR_TW + lane;
 #pragma unroll
         for (int r = 0; r < SATR_ROWS; ++r)
-            pre[r] = (r < rows && x < W) ? img[(long)(y0 + r) * W + x] : 0.0f;
+            pre[r] = (r < rows && x < W) ? load_pixel<FMT>(img, (long)(y0 + r) * pitch + x) : 0.0f;
     };
     fetch(0);
     double rs = 0.0, rq = 0.0;
@@ -457,83 +459,6 @@ __global__ __launch_bounds__(64) void k_sat_cols(int H, int W, double *__restric
             if (okx && y <= H) {
                 sum[(long)y * s + x] = lt[0][g + 4 * j][c];
                 sq[(long)y * s + x] = lt[1][g + 4 * j][c];
-            }
-        }
-    }
-}
-
-// SAT of a batch of raw frames in one pass (no background, no blur): one wave per frame walks
-// down bands of 64 rows.  Per 32-column tile: each lane scans its row of the tile (the running
-// row sum continues across tiles), then each lane carries one column (lanes 0-31: sums,
-// 32-63: squares) down the band from the previous band's last row, in registers.  Same
-// additions in the same order as k_sat_rows + k_sat_cols; the raw samples are read once (the
-// next tile's loads are in flight while the current tile is scanned) and the table is
-// written once.  A lane reads the column carry back from the rows it wrote itself.
-constexpr int SATF_MAXW = 1 << 15;
-
-template <int FMT>
-__global__ __launch_bounds__(64) void k_sat_frames(const void *__restrict__ raw, int pitch,
-                                                   long raw_frame_bytes, int H, int W,
-                                                   double *__restrict__ sat_all, long sat_stride)
-{
-    __shared__ float tin[SAT_ROWS][SAT_TW + 1];
-    __shared__ double tv[2][SAT_ROWS][SAT_TW + 1];
-    const int lane = threadIdx.x;
-    raw = static_cast<const char *>(raw) + (long)blockIdx.x * raw_frame_bytes;
-    const long s = (long)W + 1, cells = s * ((long)H + 1);
-    const int sub = lane >> 5, c = lane & 31;
-    double *tab = sat_all + (long)blockIdx.x * sat_stride + sub * cells;  // this lane's table
-    for (int x = c; x <= W; x += 32) tab[x] = 0.0;
-    const int ntile = (W + SAT_TW - 1) / SAT_TW;
-    float pre[SAT_ROWS / 2];
-    auto fetch = [&](int y0, int x0) {
-        const int x = x0 + c;
-#pragma unroll
-        for (int i = 0; i < SAT_ROWS / 2; ++i) {
-            const int y = y0 + 2 * i + sub;
-            pre[i] = (y < H && x < W) ? load_pixel<FMT>(raw, (long)y * pitch + x) : 0.0f;
-        }
-    };
-    fetch(0, 0);
-    for (int y0 = 0; y0 < H; y0 += SAT_ROWS) {
-        const int rows = min(SAT_ROWS, H - y0);
-        for (int r = c; r < rows; r += 32) tab[(long)(y0 + r + 1) * s] = 0.0;
-        double rs = 0.0, rq = 0.0;
-        for (int t = 0; t < ntile; ++t) {
-            const int x0 = t * SAT_TW, x = x0 + c;
-            __syncthreads();
-#pragma unroll
-            for (int i = 0; i < SAT_ROWS / 2; ++i) tin[2 * i + sub][c] = pre[i];
-            __syncthreads();
-            // next tile (or the next band's first) while this one is scanned
-            if (t + 1 < ntile) fetch(y0, x0 + SAT_TW);
-            else if (y0 + SAT_ROWS < H) fetch(y0 + SAT_ROWS, 0);
-            {
-                float v[SAT_TW];
-#pragma unroll
-                for (int k = 0; k < SAT_TW; ++k) v[k] = tin[lane][k];
-#pragma unroll
-                for (int k = 0; k < SAT_TW; ++k) {
-                    const double d = (double)v[k];
-                    rs += d;
-                    rq += d * d;
-                    tv[0][lane][k] = rs;
-                    tv[1][lane][k] = rq;
-                }
-            }
-            __syncthreads();
-            if (x < W) {
-                double col[SAT_ROWS];
-#pragma unroll
-                for (int r = 0; r < SAT_ROWS; ++r) col[r] = tv[sub][r][c];
-                double cv = y0 == 0 ? 0.0 : tab[(long)y0 * s + x + 1];
-#pragma unroll
-                for (int r = 0; r < SAT_ROWS; ++r) {
-                    if (r < rows) {
-                        cv = cv + col[r];
-                        tab[(long)(y0 + r + 1) * s + x + 1] = cv;
-                    }
-                }
             }
         }
     }
@@ -850,44 +775,55 @@ hipError_t launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t st)
 }
 
 static inline unsigned nblk(long n, int b) { return (unsigned)((n + b - 1) / b); }
+constexpr int MAX_GRID_Y = 65535;  // frames per launch (gridDim.y)
 
 hipError_t launch_base(int fmt, const void *raw, int pitch, const float *bg, int H, int W,
                        float *out, hipStream_t st, int nframes, long raw_frame_bytes)
 {
     const long n = (long)H * W;
-    const dim3 grid(nblk(n, 256), (unsigned)nframes);
-    switch (fmt) {
-    case FMT_U8: hipLaunchKernelGGL(k_base<FMT_U8>, grid, dim3(256), 0, st, raw, pitch, raw_frame_bytes, bg, H, W, out); break;
-    case FMT_U16: hipLaunchKernelGGL(k_base<FMT_U16>, grid, dim3(256), 0, st, raw, pitch, raw_frame_bytes, bg, H, W, out); break;
-    default: hipLaunchKernelGGL(k_base<FMT_F32>, grid, dim3(256), 0, st, raw, pitch, raw_frame_bytes, bg, H, W, out); break;
+    for (int f0 = 0; f0 < nframes; f0 += MAX_GRID_Y) {
+        const dim3 grid(nblk(n, 256), (unsigned)min(MAX_GRID_Y, nframes - f0));
+        const void *r = static_cast<const char *>(raw) + (long)f0 * raw_frame_bytes;
+        float *o = out + (long)f0 * n;
+        switch (fmt) {
+        case FMT_U8: hipLaunchKernelGGL(k_base<FMT_U8>, grid, dim3(256), 0, st, r, pitch, raw_frame_bytes, bg, H, W, o); break;
+        case FMT_U16: hipLaunchKernelGGL(k_base<FMT_U16>, grid, dim3(256), 0, st, r, pitch, raw_frame_bytes, bg, H, W, o); break;
+        default: hipLaunchKernelGGL(k_base<FMT_F32>, grid, dim3(256), 0, st, r, pitch, raw_frame_bytes, bg, H, W, o); break;
+        }
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
     }
-    return hipGetLastError();
+    return hipSuccess;
+}
+
+hipError_t launch_sat_raw(int fmt, const void *raw, int pitch, long raw_frame_bytes, int H, int W,
+                          double *sum, double *sq, hipStream_t st, int nframes, long sat_stride)
+{
+    for (int f0 = 0; f0 < nframes; f0 += MAX_GRID_Y) {
+        const unsigned nf = (unsigned)min(MAX_GRID_Y, nframes - f0);
+        const void *r = static_cast<const char *>(raw) + (long)f0 * raw_frame_bytes;
+        double *su = sum + (long)f0 * sat_stride, *sqq = sq + (long)f0 * sat_stride;
+        const dim3 grid(nblk(H, SATR_ROWS), nf);
+        switch (fmt) {
+        case FMT_U8: hipLaunchKernelGGL(k_sat_rows<FMT_U8>, grid, dim3(64), 0, st, r, pitch, raw_frame_bytes, H, W, su, sqq, sat_stride); break;
+        case FMT_U16: hipLaunchKernelGGL(k_sat_rows<FMT_U16>, grid, dim3(64), 0, st, r, pitch, raw_frame_bytes, H, W, su, sqq, sat_stride); break;
+        default: hipLaunchKernelGGL(k_sat_rows<FMT_F32>, grid, dim3(64), 0, st, r, pitch, raw_frame_bytes, H, W, su, sqq, sat_stride); break;
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_sat_cols, dim3(nblk(W, SATC_W), nf), dim3(64), 0, st, H, W, su, sqq,
+                           sat_stride);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_sat(const float *img, int H, int W, double *sum, double *sq, hipStream_t st,
                       int nframes, long sat_stride)
 {
-    hipLaunchKernelGGL(k_sat_rows, dim3(nblk(H, SATR_ROWS), (unsigned)nframes), dim3(64), 0, st,
-                       img, H, W, sum, sq, sat_stride);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_sat_cols, dim3(nblk(W, SATC_W), (unsigned)nframes), dim3(64), 0, st, H, W,
-                       sum, sq, sat_stride);
-    return hipGetLastError();
-}
-
-bool sat_frames_supported(int W) { return W <= SATF_MAXW; }
-
-hipError_t launch_sat_frames(int fmt, const void *raw, int pitch, long raw_frame_bytes, int H,
-                             int W, double *sat, long sat_stride, int nframes, hipStream_t st)
-{
-    if (W > SATF_MAXW) return hipErrorInvalidValue;
-    switch (fmt) {
-    case FMT_U8: hipLaunchKernelGGL(k_sat_frames<FMT_U8>, dim3(nframes), dim3(64), 0, st, raw, pitch, raw_frame_bytes, H, W, sat, sat_stride); break;
-    case FMT_U16: hipLaunchKernelGGL(k_sat_frames<FMT_U16>, dim3(nframes), dim3(64), 0, st, raw, pitch, raw_frame_bytes, H, W, sat, sat_stride); break;
-    default: hipLaunchKernelGGL(k_sat_frames<FMT_F32>, dim3(nframes), dim3(64), 0, st, raw, pitch, raw_frame_bytes, H, W, sat, sat_stride); break;
-    }
-    return hipGetLastError();
+    return launch_sat_raw(FMT_F32, img, W, (long)H * W * (long)sizeof(float), H, W, sum, sq, st,
+                          nframes, sat_stride);
 }
 
 hipError_t launch_blur(const float *in, int H, int W, const float *taps, int r, float *tmp,

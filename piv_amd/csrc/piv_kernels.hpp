@@ -32,10 +32,11 @@ hipError_t launch_base(int fmt, const void *raw, int pitch, const float *bg, int
                        float *out, hipStream_t st, int nframes = 1, long raw_frame_bytes = 0);
 hipError_t launch_sat(const float *img, int H, int W, double *sum, double *sq, hipStream_t st,
                       int nframes = 1, long sat_stride = 0);
-// SAT of a batch of raw frames (no background) in one pass, one wave per frame
-bool sat_frames_supported(int W);
-hipError_t launch_sat_frames(int fmt, const void *raw, int pitch, long raw_frame_bytes, int H,
-                             int W, double *sat, long sat_stride, int nframes, hipStream_t st);
+// SAT straight from u8 / u16 / f32 samples (no background), a batch of frames raw_frame_bytes
+// apart: the same additions in the same order as launch_sat on the normalised image
+hipError_t launch_sat_raw(int fmt, const void *raw, int pitch, long raw_frame_bytes, int H, int W,
+                          double *sum, double *sq, hipStream_t st, int nframes = 1,
+                          long sat_stride = 0);
 hipError_t launch_blur(const float *in, int H, int W, const float *taps, int r, float *tmp,
                        float *out, hipStream_t st);
 hipError_t launch_local_norm(const float *in, int H, int W, int radius, float eps,

@@ -296,9 +296,9 @@ def test_kernel_variants_bit_identical(piv, fmt):
                                          (np.uint8, "raw", 5), (np.uint8, "default", 5),
                                          (np.float32, "raw", 9)])
 def test_batch_prepared_frames_equal_single(piv, dtype, pre, n):
-    """Frames prepared as one batch (one launch per conditioning kernel, shared storage; from
-    8 frames on, the one-pass table kernel) give the single-frame preparation's analysis
-    bitwise, masks included."""
+    """Frames prepared as one batch (one launch per kernel, tables summed from the samples
+    into storage the frames share) give the single-frame preparation's analysis bitwise, masks
+    included."""
     h, w = 160, 224
     frames, _ = S.video_frames(h, w, n, 17, bits=12 if dtype is np.uint16 else 8,
                                container=np.uint8 if dtype is np.uint8 else np.uint16)
