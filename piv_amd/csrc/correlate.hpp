@@ -33,12 +33,18 @@ namespace piv {
 
 template <int N>
 struct KCfg {
-    // windows per workgroup: one wave when N divides 64, three waves when N divides 192
+    // N divides 64: windows tile a wave exactly, and butterfly reductions over a window's
+    // lanes (power-of-two widths) apply
     static constexpr bool IN_WAVE = (64 % N == 0);
-    static constexpr int WPB = IN_WAVE ? 64 / N : ((192 % N == 0 && N < 48) ? 192 / N : 1);
+    // windows per workgroup: as many whole windows as one wave holds (N <= 64; 2 at N = 24,
+    // 5 at N = 12, one at N = 48, the rest of the wave idle), else one window of N lanes
+    static constexpr int WPB = N <= 64 ? 64 / N : 1;
     static constexpr int BLOCK = WPB * N;
     static constexpr int P = N + 1;   // odd LDS row stride: conflict-free rows and columns
     static constexpr int PL = N * P;
+    // every window inside one one-wave workgroup: cross-lane exchanges may use shuffles and
+    // ballots, and workgroup barriers cost nothing
+    static constexpr bool ONE_WAVE = N <= 64;
 };
 
 // Sum over the N lanes of one window (all lanes receive the total).
@@ -314,7 +320,7 @@ __device__ __forceinline__ void find_peak_plane(const float *pr, int lane, int t
         const float half_best = 0.5f * best;
         const float hv = pr[pby * P + (pbx + lane) % N];
         const float vv = pr[((pby + lane) % N) * P + pbx];
-        if constexpr (C::IN_WAVE) {
+        if constexpr (C::ONE_WAVE) {
             const int base = tid - lane;
             const unsigned long long full = N == 64 ? ~0ull : ((1ull << (N & 63)) - 1);
             const unsigned long long hm = (__ballot(hv > half_best) >> base) & full;
@@ -463,14 +469,14 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
     using C = KCfg<N>;
     constexpr int WPB = C::WPB, P = C::P, PL = C::PL, H2 = N / 2;
     __shared__ float s_re[WPB * PL];
-    __shared__ float s_im[C::IN_WAVE ? 1 : WPB * PL];
+    __shared__ float s_im[C::ONE_WAVE ? 1 : WPB * PL];
     __shared__ float s_red[WPB * N];
     __shared__ float s_pkv[WPB * N];
     __shared__ int s_pki[WPB * N];
-    // inverse column pass split over two lanes per column pair (in-wave windows, N >= 32; at
+    // inverse column pass split over two lanes per column pair (one-wave windows, N >= 24; at
     // N = 16 the shorter transform saves less than the twiddles cost): entry (p, n) = 1 for the
     // even half (p = 0), e^{+2 pi i n / N} for the odd half (p = 1), n < N / 2
-    constexpr bool SPLIT_INV = C::IN_WAVE && N >= 32;
+    constexpr bool SPLIT_INV = C::ONE_WAVE && N >= 24;
     __shared__ float2 s_tw[SPLIT_INV ? N : 1];
 
     // ---- XCD-aware block order: XCD x (blocks L = x, x+8, ...) gets a contiguous range ----
@@ -495,7 +501,7 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
     const uint32_t k = blk * WPB + wl;
     const bool active = k < cells;
     float *pr = s_re + wl * PL;
-    float *pim = C::IN_WAVE ? s_im : s_im + wl * PL;
+    float *pim = C::ONE_WAVE ? s_im : s_im + wl * PL;
     float *red = s_red + wl * N;
     float *pkv = s_pkv + wl * N;
     int *pki = s_pki + wl * N;
@@ -521,7 +527,7 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
     // moves real and imaginary parts in two passes (the barriers of a one-wave workgroup cost
     // nothing), halving the LDS footprint, which bounds occupancy at N = 32 and 64.
     pivfft::fft<N, -1>(zr, zi);  // along rows r; lane = column
-    if constexpr (C::IN_WAVE) {
+    if constexpr (C::ONE_WAVE) {
 #pragma unroll
         for (int r = 0; r < N; ++r) pr[r * P + lane] = zr[r];
         __syncthreads();
@@ -554,7 +560,7 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
     float ea2 = 0.0f, eb2 = 0.0f;
     {
         const int mrow = (N - lane) % N;
-        if constexpr (C::IN_WAVE) {
+        if constexpr (C::ONE_WAVE) {
             // The partner lane (row -ky) supplies Z[-ky][-kx] by shuffle.  Columns x and N-x
             // are done together so both originals are read before either is overwritten.
             const int src = (tid - lane) + mrow;  // partner lane in this wave
@@ -625,7 +631,7 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
     //   V[ky] = Y[ky][2j] + i Y[ky][2j+1]  ->  C[.][2j] + i C[.][2j+1].
     pivfft::fft<N, 1>(zr, zi);  // lane ky (<= N/2) holds Y[ky][x]
     const int jj = lane < H2 ? lane : lane - H2;
-    if constexpr (C::IN_WAVE) {
+    if constexpr (C::ONE_WAVE) {
         // Row owners write V's real parts (virtual rows ky and N-ky of the half plane), keep
         // V's imaginary parts in zi, then the column-pair lanes gather them in two passes.
         constexpr int HW = H2 + 1;  // odd row stride of the half-width plane

@@ -20,10 +20,11 @@ constexpr VariantSet corr_variants(int n)
 {
     switch (n) {
     case 8: case 16: return {3, {4, 6, 8}};
-    case 12: return {1, {4, 0, 0}};  // LDS bounds the 192-thread workgroups at 4 waves
-    case 24: return {1, {3, 0, 0}};
+    case 12: return {3, {6, 4, 8}};
+    case 24: return {3, {4, 3, 5}};
     case 32: return {2, {3, 4}};
-    case 48: case 64: return {2, {2, 1}};
+    case 48: return {3, {2, 3, 4}};
+    case 64: return {2, {2, 1}};
     default: return {1, {1, 0, 0}};
     }
 }
