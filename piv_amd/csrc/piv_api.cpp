@@ -186,8 +186,10 @@ piv_status prepare_frame(piv_analyzer *a, piv_frame *f, const float *bg_dev)
         double *sum = f->sat.as<double>(), *sq = sum + f->cells;
         // without conditioning the base image is the normalised raw frame: the table is summed
         // from the samples and no base image is kept (piv_frame_download recomputes it)
-        if (cond) HIP_TRY(launch_sat(f->base_ptr, H, W, sum, sq, st));
-        else HIP_TRY(launch_sat_raw(f->in_fmt, f->raw_ptr, f->raw_pitch, 0, H, W, sum, sq, st));
+        PIV_TRY(a->sat_carry.reserve(sat_carry_bytes(H, W, 1)));
+        double *carry = a->sat_carry.as<double>();
+        if (cond) HIP_TRY(launch_sat(f->base_ptr, H, W, sum, sq, carry, st));
+        else HIP_TRY(launch_sat_raw(f->in_fmt, f->raw_ptr, f->raw_pitch, 0, H, W, sum, sq, carry, st));
         f->has_sat = true;
     }
     if (!cond) {
@@ -212,7 +214,8 @@ piv_status prepare_frame(piv_analyzer *a, piv_frame *f, const float *bg_dev)
     if (c.normalize_radius > 0) {
         PIV_TRY(a->prep_sat.reserve((size_t)f->cells * 2 * sizeof(double)));
         double *s = a->prep_sat.as<double>();
-        HIP_TRY(launch_sat(cur, H, W, s, s + f->cells, st));
+        PIV_TRY(a->sat_carry.reserve(sat_carry_bytes(H, W, 1)));
+        HIP_TRY(launch_sat(cur, H, W, s, s + f->cells, a->sat_carry.as<double>(), st));
         HIP_TRY(launch_local_norm(cur, H, W, (int)c.normalize_radius, c.normalize_eps, s,
                                   s + f->cells, f->image.as<float>(), st));
     } else if (cur != f->image.as<float>()) {
@@ -251,8 +254,9 @@ piv_status prepare_frames_batch(piv_analyzer *a, piv_frame *const *fs, int count
         });
         PIV_TRY(store->reserve((size_t)count * cells * 2 * sizeof(double)));
         double *sat = store->as<double>();
+        PIV_TRY(a->sat_carry.reserve(sat_carry_bytes(H, W, count)));
         HIP_TRY(launch_sat_raw(f0->in_fmt, f0->raw_ptr, f0->raw_pitch, raw_frame_bytes, H, W, sat,
-                               sat + cells, a->stream, count, 2 * cells));
+                               sat + cells, a->sat_carry.as<double>(), a->stream, count, 2 * cells));
         for (int i = 0; i < count; ++i) {
             piv_frame *f = fs[i];
             f->base_ptr = nullptr;

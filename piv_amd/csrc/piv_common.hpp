@@ -7,6 +7,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 
 #include "piv_device.hpp"
 
@@ -90,27 +91,42 @@ __device__ __forceinline__ void sat_stats(const double *__restrict__ sum,
     sd = (float)sqrt(var > 0.0 ? var : 0.0);
 }
 
+// A sample as stored (u8 / u16 widened to 32 bits, f32 as is).  A kernel that loads many
+// samples ahead keeps them raw, so the loads stay independent of the arithmetic until the
+// values are used.
+template <int FMT>
+using raw_t = std::conditional_t<FMT == FMT_F32, float, uint32_t>;
+
+template <int FMT>
+__device__ __forceinline__ raw_t<FMT> load_raw(const void *__restrict__ base, long idx)
+{
+    if constexpr (FMT == FMT_U8) return static_cast<const uint8_t *>(base)[idx];
+    else if constexpr (FMT == FMT_U16) return static_cast<const uint16_t *>(base)[idx];
+    else return static_cast<const float *>(base)[idx];
+}
+
 // decode.rs:227-234: b / 255 and v / 65535 as IEEE quotients.  q = v * (1/D) is off by one
 // ulp for some v; one fma residual step r = v - q*D, q += r * (1/D) gives the correctly
 // rounded quotient for every u8 and u16 value (checked exhaustively in the tests).
 template <int FMT>
+__device__ __forceinline__ float normalise_raw(raw_t<FMT> raw)
+{
+    if constexpr (FMT == FMT_F32) {
+        return raw;
+    } else {
+        constexpr float D = FMT == FMT_U8 ? 255.0f : 65535.0f;
+        constexpr float inv = 1.0f / D;
+        const float v = (float)raw;
+        const float q = v * inv;
+        const float r = __builtin_fmaf(-q, D, v);
+        return __builtin_fmaf(r, inv, q);
+    }
+}
+
+template <int FMT>
 __device__ __forceinline__ float load_pixel(const void *__restrict__ base, long idx)
 {
-    if constexpr (FMT == FMT_U8) {
-        const float v = (float)static_cast<const uint8_t *>(base)[idx];
-        constexpr float inv = 1.0f / 255.0f;
-        const float q = v * inv;
-        const float r = __builtin_fmaf(-q, 255.0f, v);
-        return __builtin_fmaf(r, inv, q);
-    } else if constexpr (FMT == FMT_U16) {
-        const float v = (float)static_cast<const uint16_t *>(base)[idx];
-        constexpr float inv = 1.0f / 65535.0f;
-        const float q = v * inv;
-        const float r = __builtin_fmaf(-q, 65535.0f, v);
-        return __builtin_fmaf(r, inv, q);
-    } else {
-        return static_cast<const float *>(base)[idx];
-    }
+    return normalise_raw<FMT>(load_raw<FMT>(base, idx));
 }
 
 // correlate.rs:182-202 load_window, one column per lane: col[r] = window(r, c).

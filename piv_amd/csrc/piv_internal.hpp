@@ -160,6 +160,7 @@ struct piv_analyzer {
     DevBuf summaries;    // u32 [npass][npairs][5]
     HostBuf h_summaries;
     DevBuf prep_tmp, prep_tmp2, prep_sat;  // conditioning scratch
+    DevBuf sat_carry;                      // row carries of the SAT passes
     DevBuf stage_in, stage_out;            // stage-level entry points
     DevBuf post_part, post_keys, post_out; // flow statistics (piv_flow_stats_device)
     // kernel timing (piv_profile_*)
@@ -198,6 +199,7 @@ struct piv_analyzer {
         prep_tmp.release();
         prep_tmp2.release();
         prep_sat.release();
+        sat_carry.release();
         stage_in.release();
         stage_out.release();
         post_part.release();

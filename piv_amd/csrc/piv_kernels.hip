@@ -328,137 +328,146 @@ __global__ void k_base(const void *__restrict__ raw, int pitch, long raw_frame_b
     out[i] = v;
 }
 
-// SAT (preprocess.rs:84-110), step 1: running row sums of v and v^2 in f64, in row order
-// (one row per lane, the reference's summation order), written at (y+1, x+1).  A wave owns
-// SATR_ROWS rows and walks tiles of 64 columns: lane c loads column c of the next tile
-// (coalesced) while the current tile is scanned from LDS, then stores column c of the scanned
-// tile (coalesced).  Few rows per wave spread one frame over many CUs; the chain per row is
-// serial whatever the split, so the scan costs the same per tile.
-constexpr int SATR_ROWS = 16, SATR_TW = 64;
+// SAT (preprocess.rs:84-110): per row, running sums rs, rq of v and v^2 in f64 in column
+// order; t[y+1][x+1] = t[y][x+1] + rs (the same for rq), down each column in row order.
+// Two passes, the same additions in the same order:
+//   k_sat_carry:  each row's (rs, rq) after every SAT_SW-th column, the carries into the
+//                 column strips;
+//   k_sat_strips: one wave per strip of SAT_SW columns walks down the frame in bands of
+//                 SAT_BAND rows, continues each row's sums from its carry across the strip,
+//                 then carries each column down the band.
+// The samples are read twice and the table written once; the f64 chains are serial per row
+// and per column whatever the split, so the parallelism comes from rows x strips x frames.
+// In both kernels lanes 0-31 run the sums and lanes 32-63 the squares of 32 rows (or
+// columns), so every lane carries one chain.
+constexpr int SAT_SW = 32, SAT_BAND = 32, SATK_TW = 64;
 
 template <int FMT>
-__global__ __launch_bounds__(64) void k_sat_rows(const void *__restrict__ img, int pitch,
-                                                 long frame_bytes, int H, int W,
-                                                 double *__restrict__ sum,
-                                                 double *__restrict__ sq, long sat_stride)
+__global__ __launch_bounds__(64) void k_sat_carry(const void *__restrict__ img, int pitch,
+                                                  long frame_bytes, int H, int W,
+                                                  double *__restrict__ carry, int nb)
 {
-    __shared__ float tin[SATR_ROWS][SATR_TW + 1];
-    __shared__ double ts[SATR_ROWS][SATR_TW + 1], tq[SATR_ROWS][SATR_TW + 1];
-    // blockIdx.y: frame of a batch (images frame_bytes apart, tables sat_stride apart)
+    // carry: per frame [nb boundaries][2: sum, square][H rows]; boundary b holds the running
+    // sums through column SAT_SW * (b + 1) - 1
+    __shared__ float tin[SAT_BAND][SATK_TW + 1];
     img = static_cast<const char *>(img) + (long)blockIdx.y * frame_bytes;
-    sum += (long)blockIdx.y * sat_stride;
-    sq += (long)blockIdx.y * sat_stride;
-    const int lane = threadIdx.x, y0 = blockIdx.x * SATR_ROWS;
-    const long s = (long)W + 1;
-    if (blockIdx.x == 0)
-        for (int x = lane; x <= W; x += 64) {
-            sum[x] = 0.0;
-            sq[x] = 0.0;
-        }
-    if (lane < SATR_ROWS && y0 + lane < H) {
-        sum[(long)(y0 + lane + 1) * s] = 0.0;
-        sq[(long)(y0 + lane + 1) * s] = 0.0;
-    }
-    const int rows = min(SATR_ROWS, H - y0), ntile = (W + SATR_TW - 1) / SATR_TW;
-    float pre[SATR_ROWS];
+    carry += (long)blockIdx.y * nb * 2 * H;
+    const int lane = threadIdx.x, r = lane & 31, sq = lane >> 5, y0 = blockIdx.x * SAT_BAND;
+    const int rows = min(SAT_BAND, H - y0);
+    const int ntile = (nb * SAT_SW + SATK_TW - 1) / SATK_TW;
+    // raw samples from clamped (in-frame) addresses: all of a tile's loads are in flight at
+    // once; samples outside the frame are zeroed when the tile is normalised into LDS
+    raw_t<FMT> pre[SAT_BAND];
+    int xt = 0;  // first column of the fetched tile
     auto fetch = [&](int t) {
-        const int x = t * SATR_TW + lane;
+        xt = t * SATK_TW;
+        const int xc = min(xt + lane, W - 1);
 #pragma unroll
-        for (int r = 0; r < SATR_ROWS; ++r)
-            pre[r] = (r < rows && x < W) ? load_pixel<FMT>(img, (long)(y0 + r) * pitch + x) : 0.0f;
+        for (int i = 0; i < SAT_BAND; ++i)
+            pre[i] = load_raw<FMT>(img, (long)min(y0 + i, H - 1) * pitch + xc);
     };
-    fetch(0);
-    double rs = 0.0, rq = 0.0;
+    if (ntile > 0) fetch(0);
+    double acc = 0.0;
     for (int t = 0; t < ntile; ++t) {
+        const bool okx = xt + lane < W;
         __syncthreads();
 #pragma unroll
-        for (int r = 0; r < SATR_ROWS; ++r) tin[r][lane] = pre[r];
+        for (int i = 0; i < SAT_BAND; ++i)
+            tin[i][lane] = (i < rows && okx) ? normalise_raw<FMT>(pre[i]) : 0.0f;
         __syncthreads();
         if (t + 1 < ntile) fetch(t + 1);
-        if (lane < SATR_ROWS) {
-            // columns past the frame edge hold 0 and are never stored
-            float v[SATR_TW];
 #pragma unroll
-            for (int k = 0; k < SATR_TW; ++k) v[k] = tin[lane][k];
+        for (int h = 0; h < SATK_TW / SAT_SW; ++h) {
+            float v[SAT_SW];
 #pragma unroll
-            for (int k = 0; k < SATR_TW; ++k) {
-                const double d = (double)v[k];
-                rs += d;
-                rq += d * d;
-                ts[lane][k] = rs;
-                tq[lane][k] = rq;
+            for (int k = 0; k < SAT_SW; ++k) v[k] = tin[r][h * SAT_SW + k];
+#pragma unroll
+            for (int k = 0; k < SAT_SW; ++k) {
+                const double d = (double)v[k], dd = d * d;
+                acc += sq ? dd : d;
             }
+            const int b = t * (SATK_TW / SAT_SW) + h;
+            if (b < nb && r < rows) carry[((long)b * 2 + sq) * H + y0 + r] = acc;
         }
-        __syncthreads();
-        const int x = t * SATR_TW + lane;
-        if (x < W)
-            for (int r = 0; r < rows; ++r) {
-                sum[(long)(y0 + r + 1) * s + x + 1] = ts[r][lane];
-                sq[(long)(y0 + r + 1) * s + x + 1] = tq[r][lane];
-            }
     }
 }
 
-// SAT, step 2: t[y+1][x] = t[y][x] + rowsum[y][x], down each column in row order.  A wave owns
-// a strip of SATC_W columns and walks chunks of SATC_R rows: all 64 lanes load the next chunk
-// (4 rows per instruction, SATC_W * 8 contiguous bytes each) while lanes 0-15 carry the sums and
-// lanes 16-31 the squares of the current chunk down their column in LDS, one add per row.
-constexpr int SATC_W = 16, SATC_R = 64, SATC_PER = SATC_R * SATC_W / 64;
-
-__global__ __launch_bounds__(64) void k_sat_cols(int H, int W, double *__restrict__ sum,
-                                                 double *__restrict__ sq, long sat_stride)
+template <int FMT>
+__global__ __launch_bounds__(64) void k_sat_strips(const void *__restrict__ img, int pitch,
+                                                   long frame_bytes, int H, int W,
+                                                   const double *__restrict__ carry, int nb,
+                                                   double *__restrict__ sum,
+                                                   double *__restrict__ sumsq, long sat_stride)
 {
-    __shared__ double lt[2][SATC_R][SATC_W + 1];
-    sum += (long)blockIdx.y * sat_stride;
-    sq += (long)blockIdx.y * sat_stride;
-    const int lane = threadIdx.x, c = lane % SATC_W, g = lane / SATC_W;
-    const int x = blockIdx.x * SATC_W + c + 1;
-    const bool okx = x <= W;
+    __shared__ float tin[SAT_BAND][SAT_SW + 1];
+    __shared__ double tv[2][SAT_BAND][SAT_SW + 1];
+    img = static_cast<const char *>(img) + (long)blockIdx.y * frame_bytes;
+    carry += (long)blockIdx.y * nb * 2 * H;
+    const int lane = threadIdx.x, c = lane & 31, sq = lane >> 5, strip = blockIdx.x;
+    // this lane's table: sums (lanes 0-31) or squares (32-63)
+    double *tab = (sq ? sumsq : sum) + (long)blockIdx.y * sat_stride;
     const long s = (long)W + 1;
-    const int nch = (H + SATC_R - 1) / SATC_R;
-    double a[SATC_PER], b[SATC_PER];
-    auto fetch = [&](int k) {
+    const int x = strip * SAT_SW + c;  // sample column of table column x + 1
+    if (x < W) tab[x + 1] = 0.0;       // row 0
+    if (strip == 0 && c == 0) tab[0] = 0.0;
+    const double *crow = strip > 0 ? carry + ((long)(strip - 1) * 2 + sq) * H : nullptr;
+    // raw samples from clamped addresses, zeroed outside the frame when normalised into LDS
+    raw_t<FMT> pre[SAT_BAND / 2];
+    const int xc = min(x, W - 1);
+    auto fetch = [&](int y0) {
 #pragma unroll
-        for (int j = 0; j < SATC_PER; ++j) {
-            const int y = 1 + k * SATC_R + g + 4 * j;
-            const bool ok = okx && y <= H;
-            a[j] = ok ? sum[(long)y * s + x] : 0.0;
-            b[j] = ok ? sq[(long)y * s + x] : 0.0;
-        }
+        for (int i = 0; i < SAT_BAND / 2; ++i)
+            pre[i] = load_raw<FMT>(img, (long)min(y0 + 2 * i + sq, H - 1) * pitch + xc);
+    };
+    auto fetch_carry = [&](int y) {
+        if (!crow) return 0.0;
+        const double v = crow[min(y, H - 1)];
+        return y < H ? v : 0.0;
     };
     fetch(0);
-    // lanes 0-15: sums of column c, lanes 16-31: squares; rows past H hold 0 and are never stored
-    double carry = 0.0;
-    double(*chain)[SATC_W + 1] = lt[g & 1];
-    for (int k = 0; k < nch; ++k) {
+    double ci = fetch_carry(c);  // carry of row y0 + c into this strip
+    double cv = 0.0;                               // column chain, carried across bands
+    for (int y0 = 0; y0 < H; y0 += SAT_BAND) {
+        const int rows = min(SAT_BAND, H - y0);
+        if (strip == 0)
+            for (int r = c; r < rows; r += 32) tab[(long)(y0 + r + 1) * s] = 0.0;  // column 0
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < SATC_PER; ++j) {
-            lt[0][g + 4 * j][c] = a[j];
-            lt[1][g + 4 * j][c] = b[j];
+        for (int i = 0; i < SAT_BAND / 2; ++i)
+            tin[2 * i + sq][c] = (y0 + 2 * i + sq < H && x < W) ? normalise_raw<FMT>(pre[i]) : 0.0f;
+        __syncthreads();
+        double acc = ci;
+        if (y0 + SAT_BAND < H) {
+            fetch(y0 + SAT_BAND);
+            ci = fetch_carry(y0 + SAT_BAND + c);
         }
-        __syncthreads();
-        if (k + 1 < nch) fetch(k + 1);
-        if (g < 2) {
+        // lane (c, sq): row y0 + c across the strip, from its carry
+        {
+            float v[SAT_SW];
 #pragma unroll
-            for (int r0 = 0; r0 < SATC_R; r0 += 16) {
-                double v[16];
+            for (int k = 0; k < SAT_SW; ++k) v[k] = tin[c][k];
 #pragma unroll
-                for (int r = 0; r < 16; ++r) v[r] = chain[r0 + r][c];
-#pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    carry = carry + v[r];
-                    chain[r0 + r][c] = carry;
-                }
+            for (int k = 0; k < SAT_SW; ++k) {
+                const double d = (double)v[k], dd = d * d;
+                acc += sq ? dd : d;
+                tv[sq][c][k] = acc;
             }
         }
         __syncthreads();
+        // lane (c, sq): column x down the band
+        if (x < W) {
 #pragma unroll
-        for (int j = 0; j < SATC_PER; ++j) {
-            const int y = 1 + k * SATC_R + g + 4 * j;
-            if (okx && y <= H) {
-                sum[(long)y * s + x] = lt[0][g + 4 * j][c];
-                sq[(long)y * s + x] = lt[1][g + 4 * j][c];
+            for (int r0 = 0; r0 < SAT_BAND; r0 += 8) {
+                double col[8];
+#pragma unroll
+                for (int r = 0; r < 8; ++r) col[r] = tv[sq][r0 + r][c];
+#pragma unroll
+                for (int r = 0; r < 8; ++r) {
+                    if (r0 + r < rows) {
+                        cv = cv + col[r];
+                        tab[(long)(y0 + r0 + r + 1) * s + x + 1] = cv;
+                    }
+                }
             }
         }
     }
@@ -796,34 +805,49 @@ hipError_t launch_base(int fmt, const void *raw, int pitch, const float *bg, int
     return hipSuccess;
 }
 
-hipError_t launch_sat_raw(int fmt, const void *raw, int pitch, long raw_frame_bytes, int H, int W,
-                          double *sum, double *sq, hipStream_t st, int nframes, long sat_stride)
+int sat_boundaries(int W) { return (W + SAT_SW - 1) / SAT_SW - 1; }
+
+size_t sat_carry_bytes(int H, int W, int nframes)
 {
+    return (size_t)nframes * (size_t)sat_boundaries(W) * 2 * (size_t)H * sizeof(double);
+}
+
+hipError_t launch_sat_raw(int fmt, const void *raw, int pitch, long raw_frame_bytes, int H, int W,
+                          double *sum, double *sq, double *carry, hipStream_t st, int nframes,
+                          long sat_stride)
+{
+    const int nb = sat_boundaries(W);
     for (int f0 = 0; f0 < nframes; f0 += MAX_GRID_Y) {
         const unsigned nf = (unsigned)min(MAX_GRID_Y, nframes - f0);
         const void *r = static_cast<const char *>(raw) + (long)f0 * raw_frame_bytes;
         double *su = sum + (long)f0 * sat_stride, *sqq = sq + (long)f0 * sat_stride;
-        const dim3 grid(nblk(H, SATR_ROWS), nf);
-        switch (fmt) {
-        case FMT_U8: hipLaunchKernelGGL(k_sat_rows<FMT_U8>, grid, dim3(64), 0, st, r, pitch, raw_frame_bytes, H, W, su, sqq, sat_stride); break;
-        case FMT_U16: hipLaunchKernelGGL(k_sat_rows<FMT_U16>, grid, dim3(64), 0, st, r, pitch, raw_frame_bytes, H, W, su, sqq, sat_stride); break;
-        default: hipLaunchKernelGGL(k_sat_rows<FMT_F32>, grid, dim3(64), 0, st, r, pitch, raw_frame_bytes, H, W, su, sqq, sat_stride); break;
+        if (nb > 0) {
+            const dim3 grid(nblk(H, SAT_BAND), nf);
+            switch (fmt) {
+            case FMT_U8: hipLaunchKernelGGL(k_sat_carry<FMT_U8>, grid, dim3(64), 0, st, r, pitch, raw_frame_bytes, H, W, carry, nb); break;
+            case FMT_U16: hipLaunchKernelGGL(k_sat_carry<FMT_U16>, grid, dim3(64), 0, st, r, pitch, raw_frame_bytes, H, W, carry, nb); break;
+            default: hipLaunchKernelGGL(k_sat_carry<FMT_F32>, grid, dim3(64), 0, st, r, pitch, raw_frame_bytes, H, W, carry, nb); break;
+            }
+            const hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
         }
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_sat_cols, dim3(nblk(W, SATC_W), nf), dim3(64), 0, st, H, W, su, sqq,
-                           sat_stride);
-        e = hipGetLastError();
+        const dim3 grid(nblk(W, SAT_SW), nf);
+        switch (fmt) {
+        case FMT_U8: hipLaunchKernelGGL(k_sat_strips<FMT_U8>, grid, dim3(64), 0, st, r, pitch, raw_frame_bytes, H, W, carry, nb, su, sqq, sat_stride); break;
+        case FMT_U16: hipLaunchKernelGGL(k_sat_strips<FMT_U16>, grid, dim3(64), 0, st, r, pitch, raw_frame_bytes, H, W, carry, nb, su, sqq, sat_stride); break;
+        default: hipLaunchKernelGGL(k_sat_strips<FMT_F32>, grid, dim3(64), 0, st, r, pitch, raw_frame_bytes, H, W, carry, nb, su, sqq, sat_stride); break;
+        }
+        const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
 }
 
-hipError_t launch_sat(const float *img, int H, int W, double *sum, double *sq, hipStream_t st,
-                      int nframes, long sat_stride)
+hipError_t launch_sat(const float *img, int H, int W, double *sum, double *sq, double *carry,
+                      hipStream_t st, int nframes, long sat_stride)
 {
-    return launch_sat_raw(FMT_F32, img, W, (long)H * W * (long)sizeof(float), H, W, sum, sq, st,
-                          nframes, sat_stride);
+    return launch_sat_raw(FMT_F32, img, W, (long)H * W * (long)sizeof(float), H, W, sum, sq, carry,
+                          st, nframes, sat_stride);
 }
 
 hipError_t launch_blur(const float *in, int H, int W, const float *taps, int r, float *tmp,

@@ -30,13 +30,15 @@ hipError_t launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t st);
 // tables sat_stride doubles apart), one launch per kernel
 hipError_t launch_base(int fmt, const void *raw, int pitch, const float *bg, int H, int W,
                        float *out, hipStream_t st, int nframes = 1, long raw_frame_bytes = 0);
-hipError_t launch_sat(const float *img, int H, int W, double *sum, double *sq, hipStream_t st,
-                      int nframes = 1, long sat_stride = 0);
+// SAT of normalised images; `carry` is scratch of sat_carry_bytes(H, W, nframes)
+hipError_t launch_sat(const float *img, int H, int W, double *sum, double *sq, double *carry,
+                      hipStream_t st, int nframes = 1, long sat_stride = 0);
 // SAT straight from u8 / u16 / f32 samples (no background), a batch of frames raw_frame_bytes
 // apart: the same additions in the same order as launch_sat on the normalised image
 hipError_t launch_sat_raw(int fmt, const void *raw, int pitch, long raw_frame_bytes, int H, int W,
-                          double *sum, double *sq, hipStream_t st, int nframes = 1,
+                          double *sum, double *sq, double *carry, hipStream_t st, int nframes = 1,
                           long sat_stride = 0);
+size_t sat_carry_bytes(int H, int W, int nframes);
 hipError_t launch_blur(const float *in, int H, int W, const float *taps, int r, float *tmp,
                        float *out, hipStream_t st);
 hipError_t launch_local_norm(const float *in, int H, int W, int radius, float eps,

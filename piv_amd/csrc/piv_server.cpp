@@ -651,6 +651,17 @@ piv_status piv_server_add_stream(piv_server *s, const piv_config *cfg, uint32_t 
     st->W = (int)width;
     st->fmt = (int)fmt;
     r = plan_passes(st->an, st->H, st->W, st->plans);
+    // the conditioning scratch at its final size, so that concurrent submissions on this
+    // stream never grow it (its users are ordered on the analyzer's stream)
+    if (r == PIV_OK) {
+        const size_t n = (size_t)height * width, cells = (size_t)(height + 1) * (width + 1);
+        const piv_config &c = st->an->cfg;
+        r = st->an->sat_carry.reserve(sat_carry_bytes(st->H, st->W, 1));
+        if (r == PIV_OK && c.smooth_sigma > 0.0f) r = st->an->prep_tmp.reserve(n * sizeof(float));
+        if (r == PIV_OK && c.smooth_sigma > 0.0f && c.normalize_radius > 0)
+            r = st->an->prep_tmp2.reserve(n * sizeof(float));
+        if (r == PIV_OK && c.normalize_radius > 0) r = st->an->prep_sat.reserve(cells * 2 * sizeof(double));
+    }
     if (r == PIV_OK && background) {
         const size_t n = (size_t)height * width;
         r = st->bg.reserve(n * 4);
