@@ -172,7 +172,7 @@ piv_status prepare_frame(piv_analyzer *a, piv_frame *f, const float *bg_dev)
     const bool cond = c.smooth_sigma > 0.0f || c.normalize_radius > 0 || bg_dev != nullptr;
     const bool mask = mask_needed(c, f->in_fmt, bg_dev != nullptr);
     f->has_bg = bg_dev != nullptr;
-    f->cells = (int64_t)(H + 1) * (W + 1);
+    f->cells = sat_table_elems(H, W);
     hipStream_t st = a->stream;
     f->base_ptr = nullptr;
     if (cond) {
@@ -182,8 +182,8 @@ piv_status prepare_frame(piv_analyzer *a, piv_frame *f, const float *bg_dev)
     }
     if (mask) {
         PIV_TRY(f->sat.reserve((size_t)f->cells * 2 * sizeof(double)));
-        f->sat_ptr = f->sat.as<double>();
-        double *sum = f->sat.as<double>(), *sq = sum + f->cells;
+        double *sum = f->sat.as<double>() + SAT_LEAD, *sq = sum + f->cells;
+        f->sat_ptr = sum;
         // without conditioning the base image is the normalised raw frame: the table is summed
         // from the samples and no base image is kept (piv_frame_download recomputes it)
         PIV_TRY(a->sat_carry.reserve(sat_carry_bytes(H, W, 1)));
@@ -213,7 +213,7 @@ piv_status prepare_frame(piv_analyzer *a, piv_frame *f, const float *bg_dev)
     }
     if (c.normalize_radius > 0) {
         PIV_TRY(a->prep_sat.reserve((size_t)f->cells * 2 * sizeof(double)));
-        double *s = a->prep_sat.as<double>();
+        double *s = a->prep_sat.as<double>() + SAT_LEAD;
         PIV_TRY(a->sat_carry.reserve(sat_carry_bytes(H, W, 1)));
         HIP_TRY(launch_sat(cur, H, W, s, s + f->cells, a->sat_carry.as<double>(), st));
         HIP_TRY(launch_local_norm(cur, H, W, (int)c.normalize_radius, c.normalize_eps, s,
@@ -243,7 +243,7 @@ piv_status prepare_frames_batch(piv_analyzer *a, piv_frame *const *fs, int count
         return PIV_OK;
     }
     const int H = f0->H, W = f0->W;
-    const int64_t cells = (int64_t)(H + 1) * (W + 1);
+    const int64_t cells = sat_table_elems(H, W);
     std::shared_ptr<DevBuf> store;
     if (mask) {
         // one pass over the raw samples per frame; no base image is kept (piv_frame_download
@@ -253,7 +253,7 @@ piv_status prepare_frames_batch(piv_analyzer *a, piv_frame *const *fs, int count
             delete b;
         });
         PIV_TRY(store->reserve((size_t)count * cells * 2 * sizeof(double)));
-        double *sat = store->as<double>();
+        double *sat = store->as<double>() + SAT_LEAD;  // entry (0, 0) of frame 0's sums
         PIV_TRY(a->sat_carry.reserve(sat_carry_bytes(H, W, count)));
         HIP_TRY(launch_sat_raw(f0->in_fmt, f0->raw_ptr, f0->raw_pitch, raw_frame_bytes, H, W, sat,
                                sat + cells, a->sat_carry.as<double>(), a->stream, count, 2 * cells));

@@ -81,7 +81,7 @@ __device__ __forceinline__ void sat_stats(const double *__restrict__ sum,
                                           const double *__restrict__ sq, int W, int x, int y,
                                           int bw, int bh, float &mean, float &sd)
 {
-    const long s = (long)W + 1;
+    const long s = (long)sat_pitch(W);
     const double n = (double)((long)bw * bh);
     const long i11 = (long)(y + bh) * s + (x + bw), i01 = (long)y * s + (x + bw);
     const long i10 = (long)(y + bh) * s + x, i00 = (long)y * s + x;

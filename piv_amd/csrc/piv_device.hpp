@@ -25,6 +25,21 @@ struct PeakRec {
     int32_t has_peak;         // 0: masked or flat window (NO_SIGNAL)
 };
 
+// Summed-area table layout: a table of (H+1) x (W+1) f64 entries with a row pitch of
+// sat_pitch(W) entries (a multiple of 16, 128 B) and entry (0, 0) SAT_LEAD entries past a
+// 128 B boundary, so that table column 1 -- the first column of every 32-column strip the
+// table kernels write -- starts a 128 B line in every row.  A frame's sum and sum-of-squares
+// tables are sat_table_elems(H, W) entries apart.
+constexpr int SAT_ALIGN = 16, SAT_LEAD = SAT_ALIGN - 1;
+__host__ __device__ constexpr int64_t sat_pitch(int W)
+{
+    return ((int64_t)W + 1 + SAT_ALIGN - 1) / SAT_ALIGN * SAT_ALIGN;
+}
+__host__ __device__ constexpr int64_t sat_table_elems(int H, int W)
+{
+    return SAT_ALIGN + ((int64_t)H + 1) * sat_pitch(W);
+}
+
 // One image pair's share of one correlation pass.  A launch covers many pairs (the whole
 // batch, or windows of one size from several serving requests): block b of a launch with
 // bpp blocks per pair works on pair b / bpp.
@@ -32,7 +47,7 @@ struct PassPair {
     const void *img_a;        // conditioned image (FMT_F32) or raw pixels (fused u8/u16/f32)
     const void *img_b;
     const double *sat_a;      // f64 summed-area tables of the background-subtracted frame:
-    const double *sat_b;      //   sum at [0, cells), sum of squares at [cells, 2 cells); or null
+    const double *sat_b;      //   entry (0, 0) of the sums; the squares sat_cells further on
     const float *prev_dx;     // previous pass's validated field (predictor), or null
     const float *prev_dy;
     float *origins;           // [nx*ny*4] window origins (ax, ay, bx, by): written by the
@@ -42,7 +57,7 @@ struct PassPair {
     float *out_dx, *out_dy, *out_ppr;
     uint8_t *out_flags;
     int32_t *out_peak;        // optional [nx*ny] integer peak iy*n+ix, -1 if no peak
-    int64_t sat_cells;        // (H+1)*(W+1)
+    int64_t sat_cells;        // sat_table_elems(H, W)
     int32_t H, W, pitch;      // pitch in pixels
     int32_t fmt;
     GridDev grid, prev;

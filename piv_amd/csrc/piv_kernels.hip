@@ -406,7 +406,7 @@ __global__ __launch_bounds__(64) void k_sat_strips(const void *__restrict__ img,
     const int lane = threadIdx.x, c = lane & 31, sq = lane >> 5, strip = blockIdx.x;
     // this lane's table: sums (lanes 0-31) or squares (32-63)
     double *tab = (sq ? sumsq : sum) + (long)blockIdx.y * sat_stride;
-    const long s = (long)W + 1;
+    const long s = (long)sat_pitch(W);
     const int x = strip * SAT_SW + c;  // sample column of table column x + 1
     if (x < W) tab[x + 1] = 0.0;       // row 0
     if (strip == 0 && c == 0) tab[0] = 0.0;

@@ -654,7 +654,7 @@ piv_status piv_server_add_stream(piv_server *s, const piv_config *cfg, uint32_t 
     // the conditioning scratch at its final size, so that concurrent submissions on this
     // stream never grow it (its users are ordered on the analyzer's stream)
     if (r == PIV_OK) {
-        const size_t n = (size_t)height * width, cells = (size_t)(height + 1) * (width + 1);
+        const size_t n = (size_t)height * width, cells = (size_t)sat_table_elems(st->H, st->W);
         const piv_config &c = st->an->cfg;
         r = st->an->sat_carry.reserve(sat_carry_bytes(st->H, st->W, 1));
         if (r == PIV_OK && c.smooth_sigma > 0.0f) r = st->an->prep_tmp.reserve(n * sizeof(float));
@@ -685,7 +685,7 @@ piv_status piv_server_add_stream(piv_server *s, const piv_config *cfg, uint32_t 
     }
     // conditioned frames hold their own image, base and SAT buffers
     const size_t n = (size_t)height * width;
-    const size_t cells = (size_t)(height + 1) * (width + 1);
+    const size_t cells = (size_t)sat_table_elems(st->H, st->W);
     const piv_config &c = st->an->cfg;
     if (c.smooth_sigma > 0.0f || c.normalize_radius > 0 || background) dev += 2 * 2 * n * 4;
     dev += 2 * cells * 16;  // SAT (upper bound; only when the mask is needed)
