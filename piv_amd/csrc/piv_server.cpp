@@ -656,7 +656,7 @@ piv_status piv_server_add_stream(piv_server *s, const piv_config *cfg, uint32_t 
     if (r == PIV_OK) {
         const size_t n = (size_t)height * width, cells = (size_t)sat_table_elems(st->H, st->W);
         const piv_config &c = st->an->cfg;
-        r = st->an->sat_carry.reserve(sat_carry_bytes(st->H, st->W, 1));
+        r = st->an->sat_carry.reserve(sat_carry_bytes(st->H, st->W, 2));
         if (r == PIV_OK && c.smooth_sigma > 0.0f) r = st->an->prep_tmp.reserve(n * sizeof(float));
         if (r == PIV_OK && c.smooth_sigma > 0.0f && c.normalize_radius > 0)
             r = st->an->prep_tmp2.reserve(n * sizeof(float));
@@ -757,8 +757,7 @@ piv_status submit_impl(piv_server *s, uint32_t stream_id, const void *pa, const 
             sl->fa->raw_pitch = sl->fb->raw_pitch = st->W;
         }
         const float *bg = st->has_bg ? st->bg.as<float>() : nullptr;
-        if (r == PIV_OK) r = prepare_frame(st->an, sl->fa, bg);
-        if (r == PIV_OK) r = prepare_frame(st->an, sl->fb, bg);
+        if (r == PIV_OK) r = prepare_frame_pair(st->an, sl->fa, sl->fb, bg);
         if (r == PIV_OK && hipEventRecord(sl->uploaded, us) != hipSuccess)
             r = fail(PIV_ERR_DEVICE, "event record failed");
     }
