@@ -451,6 +451,20 @@ def post_bench(an, g, dx, dy, fl, repeats=20, seed=3):
     return {"flow_stats": flow, "smoothing": series}
 
 
+def copy_calibration(mib):
+    """One device-to-device copy of `mib` MiB: a kernel of known traffic for scaling the
+    profiler's byte counters."""
+    import torch
+
+    n = mib << 20
+    src = torch.ones(n // 4, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -467,6 +481,8 @@ def main():
     ap.add_argument("--no-video", action="store_true", help="skip the batched-video measurement")
     ap.add_argument("--no-post", action="store_true",
                     help="skip the flow-statistics and smoothing measurement")
+    ap.add_argument("--only-video", type=int, default=0, metavar="PAIRS",
+                    help="measure only the batched-video workload over this many pairs")
     ap.add_argument("--only-post", action="store_true",
                     help="analyse one batch, then measure only the post-analysis stages")
     ap.add_argument("--no-precision", action="store_true",
@@ -492,6 +508,12 @@ def main():
     import piv_amd
     from piv_amd import PivAnalyzer, PivConfig, PreparedFrame, PreprocessConfig
 
+    if args.only_video:
+        if args.copy_calib_mib:
+            copy_calibration(args.copy_calib_mib)
+        print(json.dumps(video_bench(total_pairs=args.only_video)), flush=True)
+        return
+
     dev = torch.device("cuda:0")
     P = args.pairs
     uniq, field = make_pairs(min(args.unique, P))
@@ -514,13 +536,7 @@ def main():
     fl = torch.empty((P, g.ny, g.nx), dtype=torch.uint8, device=dev)
 
     if args.copy_calib_mib:
-        n = args.copy_calib_mib << 20
-        src = torch.ones(n // 4, dtype=torch.float32, device=dev)
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+        copy_calibration(args.copy_calib_mib)
 
     for _ in range(args.warmup):
         an.analyze_batch_device(fa, fb, dx, dy, pr, fl, summaries=False, sync=True)
