@@ -140,7 +140,10 @@ hipError_t prof_event(piv_analyzer *a, hipEvent_t *e)
         a->prof_pool.pop_back();
         return hipSuccess;
     }
-    return hipEventCreate(e);
+    // timing only: read by hipEventElapsedTime after the stream is idle, so the record needs
+    // no system-scope fence (whose cache writeback and invalidation would otherwise stretch
+    // the gap between the kernels it separates)
+    return hipEventCreateWithFlags(e, hipEventDisableSystemFence);
 }
 
 // Fold completed timing events into the counters (the stream must be idle).
