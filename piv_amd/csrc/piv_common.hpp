@@ -76,6 +76,16 @@ __device__ __forceinline__ void place(int origin, float shift, float maxv, float
     b = a0 + shift;
 }
 
+// Workgroup id remapped so that each of the 8 XCDs (which take workgroups round-robin, each
+// with its own L2) works on one contiguous range of the grid: neighbouring workgroups that
+// read the same lines then share an L2.  A bijection of [0, gridDim.x).
+__device__ __forceinline__ uint32_t xcd_block()
+{
+    const uint32_t L = blockIdx.x, nb = gridDim.x;
+    const uint32_t q = nb >> 3, rem = nb & 7, xcd = L & 7;
+    return xcd * q + min(xcd, rem) + (L >> 3);
+}
+
 // preprocess.rs:112-124 IntegralImage::stats
 __device__ __forceinline__ void sat_stats(const double *__restrict__ sum,
                                           const double *__restrict__ sq, int W, int x, int y,

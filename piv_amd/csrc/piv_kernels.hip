@@ -586,7 +586,10 @@ constexpr int WBLOCK = 256;
 __global__ __launch_bounds__(WBLOCK) void k_place_pass(const PassPair *__restrict__ pairs,
                                                        uint32_t bpp)
 {
-    const uint32_t p = blockIdx.x / bpp, blk = blockIdx.x - p * bpp;
+    // window rows k and k + 2 read the same table rows (at 50 % overlap the bottom corners of
+    // one row of windows are the top corners of the row two below): keep them in one L2
+    const uint32_t b = xcd_block();
+    const uint32_t p = b / bpp, blk = b - p * bpp;
     const PassPair &pp = pairs[p];
     const GridDev g = pp.grid;
     const uint32_t k = blk * WBLOCK + threadIdx.x;
