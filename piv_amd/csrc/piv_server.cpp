@@ -651,8 +651,8 @@ piv_status piv_server_add_stream(piv_server *s, const piv_config *cfg, uint32_t 
     st->W = (int)width;
     st->fmt = (int)fmt;
     r = plan_passes(st->an, st->H, st->W, st->plans);
-    // the conditioning scratch at its final size, so that concurrent submissions on this
-    // stream never grow it (its users are ordered on the analyzer's stream)
+    // the conditioning scratch at its final size, so that no submission allocates (its users,
+    // serialised by upload_mu, are ordered on the analyzer's stream)
     if (r == PIV_OK) {
         const size_t n = (size_t)height * width, cells = (size_t)sat_table_elems(st->H, st->W);
         const piv_config &c = st->an->cfg;
