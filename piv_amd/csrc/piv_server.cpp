@@ -13,6 +13,7 @@
 //               hands it to piv_poll.
 // Up to max_inflight batches are on the device at once (one HIP stream per lane), so a batch
 // admitted while another runs overlaps with it.
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -743,15 +744,24 @@ piv_status submit_impl(piv_server *s, uint32_t stream_id, const void *pa, const 
             sl->fb->raw_ptr = pb;
             sl->fa->raw_pitch = sl->fb->raw_pitch = (int)(row_stride / bpp);
         } else {
-            // staged: the caller's buffers are free when this returns
-            for (int h = 0; h < st->H; ++h) {
-                std::memcpy(sl->h_raw_a + h * rowb, static_cast<const char *>(pa) + h * row_stride, rowb);
-                std::memcpy(sl->h_raw_b + h * rowb, static_cast<const char *>(pb) + h * row_stride, rowb);
-            }
-            const size_t fb = rowb * st->H;
-            if (hipMemcpyAsync(sl->raw_a, sl->h_raw_a, fb, hipMemcpyHostToDevice, us) != hipSuccess ||
-                hipMemcpyAsync(sl->raw_b, sl->h_raw_b, fb, hipMemcpyHostToDevice, us) != hipSuccess)
-                r = fail(PIV_ERR_DEVICE, "upload failed");
+            // staged: the caller's buffers are free when this returns.  Copied into pinned
+            // staging in chunks of about 1 MiB, each chunk's upload queued as soon as it is
+            // staged, so the DMA of one chunk overlaps the host copy of the next.
+            const int chunk_rows = (int)std::max<size_t>(1, ((size_t)1 << 20) / rowb);
+            const void *src[2] = {pa, pb};
+            char *stage[2] = {sl->h_raw_a, sl->h_raw_b};
+            char *dst[2] = {static_cast<char *>(sl->raw_a), static_cast<char *>(sl->raw_b)};
+            for (int i = 0; i < 2 && r == PIV_OK; ++i)
+                for (int h0 = 0; h0 < st->H && r == PIV_OK; h0 += chunk_rows) {
+                    const int h1 = std::min(st->H, h0 + chunk_rows);
+                    for (int h = h0; h < h1; ++h)
+                        std::memcpy(stage[i] + h * rowb,
+                                    static_cast<const char *>(src[i]) + h * row_stride, rowb);
+                    if (hipMemcpyAsync(dst[i] + h0 * rowb, stage[i] + h0 * rowb,
+                                       (size_t)(h1 - h0) * rowb, hipMemcpyHostToDevice,
+                                       us) != hipSuccess)
+                        r = fail(PIV_ERR_DEVICE, "upload failed");
+                }
             sl->fa->raw_ptr = sl->raw_a;
             sl->fb->raw_ptr = sl->raw_b;
             sl->fa->raw_pitch = sl->fb->raw_pitch = st->W;
