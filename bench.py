@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Throughput of the PIV per-pair path on one MI355X.
 
-Workload (the north-star metric's configuration): 1024x1024 uint8 particle-image pairs
+Workload (the headline metric's configuration): 1024x1024 uint8 particle-image pairs
 (d = 2.5 px, 0.02 particles/px, peaks U[150, 255] DN, noise sigma 3 DN), Lamb-Oseen vortex
 (max 6 px) plus 0.02 px/px shear, three passes 64 -> 32 -> 16 px at 50 % overlap
 (127 x 127 final vectors), FP32, RAW preprocessing, library-default mask and validation.

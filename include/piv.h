@@ -85,7 +85,7 @@ typedef struct {
     piv_summary summary; /* filled (final pass, as PairResult::summary) */
 } piv_field_out;
 
-/* PassTrace (multipass.rs:72-81) plus the integer peak index the north-star parity needs.
+/* PassTrace (multipass.rs:72-81) plus the integer peak index that peak parity checks need.
  * All arrays caller-owned, [nx*ny] (origins [nx*ny*4]); any may be NULL. */
 typedef struct {
     size_t capacity;

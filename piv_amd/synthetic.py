@@ -7,7 +7,7 @@ over the image plus a 20 px margin (:85-101).  Vectorised with numpy; float32 th
 like the reference, except that numpy's exp/log/cos may differ from libm by an ulp, so
 images agree with the Rust generator to float32 rounding, not bit for bit.
 
-Also provides the BASELINE workloads (particle diameter model, u8/u16 quantisation, noise,
+Also provides the benchmark workloads (particle diameter model, u8/u16 quantisation, noise,
 Lamb-Oseen vortex + shear, swirl) used by the benchmark and the tests.
 """
 from __future__ import annotations
@@ -141,7 +141,7 @@ def quantize(frame_dn, bits=8, container=np.uint8):
 
 def particle_frames(h, w, seed, density=0.02, diameter=2.5, amp=(150.0, 255.0), noise=3.0,
                     field=None, bits=8, container=np.uint8, amp_scale=1.0, diameter_range=None):
-    """The BASELINE particle model: Gaussian particles of image diameter d (sigma = d/4, e^-2
+    """The benchmark particle model: Gaussian particles of image diameter d (sigma = d/4, e^-2
     convention), peak intensities uniform in `amp` DN, additive N(0, noise) DN, rounded and
     clipped to `bits` in `container`.  Frame B has the particles moved by `field`."""
     margin = 20.0

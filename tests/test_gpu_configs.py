@@ -3,7 +3,7 @@
 For each full-size run the traced GPU analysis is replayed stage by stage through the oracle:
   - the GPU's window origins of every pass go into the oracle's correlation: integer peak
     indices must be identical (no exact ties occur in these seeded inputs), displacements within
-    1e-4 px (FP32 bar of the north star: 1e-3), NO_SIGNAL decisions identical;
+    1e-4 px (the FP32 parity bar is 1e-3 px), NO_SIGNAL decisions identical;
   - the GPU's raw field goes into the oracle's validation: flags and values bit-exact;
   - the final fields of an independent oracle run agree within 1e-3 px with identical flags.
 Accuracy against the generating displacement is checked where the reference's own tests set
