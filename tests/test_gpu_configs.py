@@ -88,7 +88,7 @@ def truth_rms(field, grid, truth, margin):
     return float(np.sqrt(e2[keep].mean()))
 
 
-def test_config1_512_u8_single_pass(piv, oracle):
+def test_512_u8_single_pass(piv, oracle):
     """512x512 u8, uniform (2.3, -1.7), [32] at 50 % (31x31 vectors)."""
     truth = S.uniform_field(2.3, -1.7)
     a, b = S.particle_frames(512, 512, 101, field=truth)
@@ -102,7 +102,7 @@ def test_config1_512_u8_single_pass(piv, oracle):
     assert truth_rms(res.field, res.field.grid, truth, 20) < 0.1
 
 
-def test_config2_1024_u8_three_pass(piv, oracle):
+def test_1024_u8_three_pass(piv, oracle):
     """1024x1024 u8, Lamb-Oseen vortex + shear, 64 -> 32 -> 16 (127x127 vectors)."""
     truth = S.lamb_oseen_shear(1024, 1024)
     a, b = S.particle_frames(1024, 1024, 2024, field=truth)
@@ -116,7 +116,7 @@ def test_config2_1024_u8_three_pass(piv, oracle):
     assert truth_rms(res.field, res.field.grid, truth, 32) < 0.15
 
 
-def test_config3_1280x800_u16_two_pass(piv, oracle):
+def test_1280x800_u16_two_pass(piv, oracle):
     """1280x800 u16 with 12-bit values, 48 -> 24 px (grids 52x32 and 105x65)."""
     swirl = S.lamb_oseen_shear(800, 1280, vmax=4.0, shear=0.0, rc=150.0)
     a, b = S.particle_frames(800, 1280, 77, density=0.015, diameter_range=(2.0, 3.0),
