@@ -341,6 +341,56 @@ def video_bench(total_pairs=1999, distinct=32, batch=256, seed=31):
             "rms_px_error_vs_truth": round(rms, 4)}
 
 
+def cli_bench(frames=400, batch=256, seed=41):
+    """piv-analyze (the host loop over the C ABI) end to end on a synthetic 1280x800 12-bit u16
+    clip saved as .npy on local disk: read, upload, preparation with the CLI's default
+    conditioning (blur sigma 1, local normalisation radius 4), analysis 48 -> 24 in batches,
+    flow statistics, one NPZ archive per pair, stats.npz and run.json.  The clip repeats 9
+    generated frames."""
+    import re
+    import shutil
+    import subprocess
+    import tempfile
+
+    from piv_amd import synthetic as S
+
+    h, w = 800, 1280
+    base, _ = S.video_frames(h, w, 9, seed)
+    clip = np.stack([base[i % len(base)] for i in range(frames)])
+    tmp = tempfile.mkdtemp(prefix="piv_cli_bench_")
+    try:
+        src = os.path.join(tmp, "clip.npy")
+        np.save(src, clip)
+        del clip
+        out = os.path.join(tmp, "out")
+        cli = os.path.join(ROOT, "piv_amd", "bin", "piv-analyze")
+        args = [cli, src, "-o", out, "--windows", "48,24", "--final-iterations", "1",
+                "--min-mean", "0", "--min-std", "0", "--background", "none",
+                "--capture-fps", "1000", "--batch", str(batch)]
+        t0 = time.perf_counter()
+        p = subprocess.run(args, capture_output=True, text=True)
+        el = time.perf_counter() - t0
+        if p.returncode != 0:
+            raise RuntimeError(f"piv-analyze failed: {p.stderr[-2000:]}")
+        m = re.search(r"analysed (\d+) pairs in ([0-9.]+) s \(([0-9.]+) pairs/s on the GPU loop, "
+                      r"([0-9.]+) s total\)", p.stderr)
+        lp = re.search(r"loop: (.*)", p.stderr)
+        files = os.listdir(out)
+        out_bytes = sum(os.path.getsize(os.path.join(out, f)) for f in files)
+        pairs = int(m.group(1))
+        return {"workload": f"piv-analyze on a {frames}-frame 1280x800 u16 .npy clip, windows 48->24, "
+                            f"the CLI's default conditioning (blur sigma 1, local normalisation "
+                            f"radius 4), batches of {batch} pairs, NPZ per pair",
+                "pairs": pairs, "process_seconds": round(el, 3),
+                "pairs_per_s_process": round(pairs / el, 1),
+                "loop_seconds": float(m.group(2)), "pairs_per_s_loop": float(m.group(3)),
+                "cli_total_seconds": float(m.group(4)),
+                "loop_phases": lp.group(1) if lp else None,
+                "output_files": len(files), "output_MB": round(out_bytes / 1e6, 1)}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 def precision_bench(an, fa, fb, dx, dy, pr, fl, g, field, uniq, steps):
     """The same batched workload with the correlation as DFT-matrix GEMMs on the matrix cores
     (fp16 and bf16 operands, FP32 accumulation, FP64 peak fit): fields/s, the DFT-GEMM kernels'
@@ -484,6 +534,8 @@ def main():
     ap.add_argument("--no-video", action="store_true", help="skip the batched-video measurement")
     ap.add_argument("--no-post", action="store_true",
                     help="skip the flow-statistics and smoothing measurement")
+    ap.add_argument("--only-cli", type=int, default=0, metavar="FRAMES",
+                    help="measure only piv-analyze end to end on a clip of this many frames")
     ap.add_argument("--only-video", type=int, default=0, metavar="PAIRS",
                     help="measure only the batched-video workload over this many pairs")
     ap.add_argument("--only-post", action="store_true",
@@ -510,6 +562,10 @@ def main():
 
     import piv_amd
     from piv_amd import PivAnalyzer, PivConfig, PreparedFrame, PreprocessConfig
+
+    if args.only_cli:
+        print(json.dumps(cli_bench(frames=args.only_cli)), flush=True)
+        return
 
     if args.only_video:
         if args.copy_calib_mib:

@@ -14,14 +14,20 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <filesystem>
+#include <deque>
 #include <map>
+#include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/piv.h"
@@ -364,54 +370,160 @@ int main(int argc, char **argv)
     check(piv_device_alloc(A.device, B * cells * 4, &ddy), "outputs");
     check(piv_device_alloc(A.device, B * cells * 4, &dpr), "outputs");
     check(piv_device_alloc(A.device, B * cells, &dfl), "outputs");
-    std::vector<float> hdx(B * cells), hdy(B * cells), hpr(B * cells);
-    std::vector<uint8_t> hfl(B * cells);
     std::vector<piv_summary> sums(B);
     std::map<long, piv_frame *> held;  // prepared frames still needed
-    auto frame = [&](long f) {
-        auto it = held.find(f);
-        if (it != held.end()) return it->second;
-        piv_frame *pf = nullptr;
-        check(piv_frame_prepare_device(an, frame_ptr(f), in.fmt, H, W, row_stride, bg_dev, &pf), "frame");
-        held[f] = pf;
-        return pf;
+    // The frames of a batch that no earlier batch prepared.  Evenly spaced runs of them are
+    // prepared together (one launch per kernel for the run, piv_frame_prepare_device_batch);
+    // the rest one by one.
+    auto prepare_frames = [&](std::vector<long> need) {
+        std::sort(need.begin(), need.end());
+        need.erase(std::unique(need.begin(), need.end()), need.end());
+        std::vector<long> miss;
+        for (long f : need)
+            if (!held.count(f)) miss.push_back(f);
+        for (size_t i = 0; i < miss.size();) {
+            size_t j = i + 1;  // [i, j): one evenly spaced run
+            if (j < miss.size()) {
+                const long d = miss[j] - miss[i];
+                while (j + 1 < miss.size() && miss[j + 1] - miss[j] == d) ++j;
+                ++j;
+            }
+            if (j - i >= 2) {
+                const size_t step = (size_t)(miss[i + 1] - miss[i]) * frame_bytes;
+                std::vector<piv_frame *> out(j - i);
+                check(piv_frame_prepare_device_batch(an, frame_ptr(miss[i]), in.fmt, (uint32_t)(j - i), H,
+                                                     W, row_stride, step, bg_dev, out.data()),
+                      "frames");
+                for (size_t k = i; k < j; ++k) held[miss[k]] = out[k - i];
+            } else {
+                piv_frame *pf = nullptr;
+                check(piv_frame_prepare_device(an, frame_ptr(miss[i]), in.fmt, H, W, row_stride, bg_dev, &pf),
+                      "frame");
+                held[miss[i]] = pf;
+            }
+            i = j;
+        }
     };
+
+    // Pair archives are compressed and written by a pool of writer threads while the GPU
+    // analyses the next batch; at most two batches of results wait for the writers.
+    struct HostBatch {
+        std::vector<float> dx, dy, ppr;
+        std::vector<uint8_t> flags;
+    };
+    struct WriteJob {
+        std::shared_ptr<HostBatch> hb;
+        size_t k;
+        long a;
+        double time_s;
+    };
+    std::deque<WriteJob> jobs;
+    std::mutex jmu;
+    std::condition_variable jcv, jdone;
+    bool closing = false;
+    std::string write_error;
     const double m_per_px = A.px_per_mm > 0 ? 1e-3 / A.px_per_mm : 0.0;
+    auto writer = [&]() {
+        for (;;) {
+            WriteJob j;
+            {
+                std::unique_lock<std::mutex> lk(jmu);
+                jcv.wait(lk, [&] { return closing || !jobs.empty(); });
+                if (jobs.empty()) return;
+                j = std::move(jobs.front());
+                jobs.pop_front();
+            }
+            const size_t o = j.k * cells;
+            const piv_status st = piv_write_pair_npz(A.output.c_str(), (uint64_t)j.a, (uint64_t)(j.a + A.stride),
+                                                     j.time_s, dt, m_per_px, x0, y0, &grid, j.hb->dx.data() + o,
+                                                     j.hb->dy.data() + o, j.hb->ppr.data() + o,
+                                                     j.hb->flags.data() + o);
+            std::lock_guard<std::mutex> lk(jmu);
+            if (st != PIV_OK && write_error.empty()) write_error = piv_last_error();
+            jdone.notify_all();
+        }
+    };
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const unsigned nwriters = std::min(16u, std::max(1u, hw - 1));
+    std::vector<std::thread> writers;
+    for (unsigned i = 0; i < nwriters; ++i) writers.emplace_back(writer);
+    auto stop_writers = [&]() {
+        {
+            std::lock_guard<std::mutex> lk(jmu);
+            closing = true;
+        }
+        jcv.notify_all();
+        for (auto &t : writers) t.join();
+        writers.clear();
+    };
+
     uint64_t valid = 0, vectors = 0;
+    // seconds of the loop per phase: preparation, analysis, flow statistics, download, and
+    // waiting for the writers
+    double ph_prep = 0, ph_an = 0, ph_stats = 0, ph_down = 0, ph_wait = 0;
+    auto lap = [](std::chrono::steady_clock::time_point &t) {
+        const auto now = std::chrono::steady_clock::now();
+        const double s = std::chrono::duration<double>(now - t).count();
+        t = now;
+        return s;
+    };
     std::vector<double> series_t;
     std::vector<piv_flow_stats> series_stats;
     const auto t_loop = std::chrono::steady_clock::now();
     for (size_t p0 = 0; p0 < pair_starts.size(); p0 += B) {
         const size_t n = std::min(B, pair_starts.size() - p0);
+        std::vector<long> need;
+        for (size_t k = 0; k < n; ++k) {
+            need.push_back(pair_starts[p0 + k]);
+            need.push_back(pair_starts[p0 + k] + A.stride);
+        }
+        auto tp = std::chrono::steady_clock::now();
+        prepare_frames(need);
+        ph_prep += lap(tp);
         std::vector<const piv_frame *> fa(n), fb(n);
         for (size_t k = 0; k < n; ++k) {
-            fa[k] = frame(pair_starts[p0 + k]);
-            fb[k] = frame(pair_starts[p0 + k] + A.stride);
+            fa[k] = held.at(pair_starts[p0 + k]);
+            fb[k] = held.at(pair_starts[p0 + k] + A.stride);
         }
         check(piv_analyze_batch_device(an, fa.data(), fb.data(), (uint32_t)n, (float *)ddx, (float *)ddy,
                                        (float *)dpr, (uint8_t *)dfl, sums.data(), nullptr, 0),
               "analysis");
+        ph_an += lap(tp);
         // analyze.rs:483-489: per-pair flow statistics from the device-resident fields
         series_stats.resize(p0 + n);
         check(piv_flow_stats_device(an, &grid, (uint32_t)n, (const float *)ddx, (const float *)ddy,
                                     (const uint8_t *)dfl, dt, series_stats.data() + p0),
               "flow statistics");
-        check(piv_copy_to_host(A.device, hdx.data(), ddx, n * cells * 4), "download");
-        check(piv_copy_to_host(A.device, hdy.data(), ddy, n * cells * 4), "download");
-        check(piv_copy_to_host(A.device, hpr.data(), dpr, n * cells * 4), "download");
-        check(piv_copy_to_host(A.device, hfl.data(), dfl, n * cells), "download");
-        for (size_t k = 0; k < n; ++k) {
-            const long a = pair_starts[p0 + k];
-            const double time_s = (a + A.stride / 2.0) / fps;
-            series_t.push_back(time_s);
-            check(piv_write_pair_npz(A.output.c_str(), (uint64_t)a, (uint64_t)(a + A.stride), time_s, dt,
-                                     m_per_px, x0, y0, &grid, hdx.data() + k * cells,
-                                     hdy.data() + k * cells, hpr.data() + k * cells,
-                                     hfl.data() + k * cells),
-                  "output");
-            for (size_t i = 0; i < cells; ++i) valid += (hfl[k * cells + i] & PIV_FLAG_INVALID) == 0;
-            vectors += cells;
+        ph_stats += lap(tp);
+        auto hb = std::make_shared<HostBatch>();
+        hb->dx.resize(n * cells);
+        hb->dy.resize(n * cells);
+        hb->ppr.resize(n * cells);
+        hb->flags.resize(n * cells);
+        check(piv_copy_to_host(A.device, hb->dx.data(), ddx, n * cells * 4), "download");
+        check(piv_copy_to_host(A.device, hb->dy.data(), ddy, n * cells * 4), "download");
+        check(piv_copy_to_host(A.device, hb->ppr.data(), dpr, n * cells * 4), "download");
+        check(piv_copy_to_host(A.device, hb->flags.data(), dfl, n * cells), "download");
+        for (size_t i = 0; i < n * cells; ++i) valid += (hb->flags[i] & PIV_FLAG_INVALID) == 0;
+        vectors += n * cells;
+        ph_down += lap(tp);
+        {
+            std::unique_lock<std::mutex> lk(jmu);
+            jdone.wait(lk, [&] { return jobs.size() <= B || !write_error.empty(); });
+            if (!write_error.empty()) {
+                lk.unlock();
+                stop_writers();
+                die("output: " + write_error);
+            }
+            for (size_t k = 0; k < n; ++k) {
+                const long a = pair_starts[p0 + k];
+                const double time_s = (a + A.stride / 2.0) / fps;
+                series_t.push_back(time_s);
+                jobs.push_back(WriteJob{hb, k, a, time_s});
+            }
         }
+        jcv.notify_all();
+        ph_wait += lap(tp);
         // drop frames no later pair needs
         const long keep_from = p0 + n < pair_starts.size() ? pair_starts[p0 + n] : end;
         for (auto it = held.begin(); it != held.end();) {
@@ -423,6 +535,15 @@ int main(int argc, char **argv)
             }
         }
     }
+    {
+        auto tw = std::chrono::steady_clock::now();
+        stop_writers();
+        ph_wait += lap(tw);
+    }
+    if (!write_error.empty()) die("output: " + write_error);
+    std::fprintf(stderr, "loop: preparation %.3f s, analysis %.3f s, flow statistics %.3f s, "
+                         "download %.3f s, waiting for %u writers %.3f s\n",
+                 ph_prep, ph_an, ph_stats, ph_down, nwriters, ph_wait);
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_loop).count();
     for (auto &kv : held) piv_frame_release(kv.second);
 

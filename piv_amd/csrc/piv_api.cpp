@@ -166,6 +166,15 @@ void prof_harvest(piv_analyzer *a)
     a->prof_pending.clear();
 }
 
+// A frame buffer of at least `bytes`: kept if large enough, else a fitting block from the
+// analyzer's pool of released frame memory, else a new allocation.
+static piv_status frame_reserve(piv_analyzer *a, DevBuf &b, size_t bytes)
+{
+    if (b.cap >= bytes) return PIV_OK;
+    if (a->pool->take(b, bytes)) return PIV_OK;
+    return b.reserve(bytes);
+}
+
 // PreparedFrame::new (preprocess.rs:137-153) on device samples.
 piv_status prepare_frame(piv_analyzer *a, piv_frame *f, const float *bg_dev)
 {
@@ -179,12 +188,12 @@ piv_status prepare_frame(piv_analyzer *a, piv_frame *f, const float *bg_dev)
     hipStream_t st = a->stream;
     f->base_ptr = nullptr;
     if (cond) {
-        PIV_TRY(f->base.reserve(n * sizeof(float)));
+        PIV_TRY(frame_reserve(a, f->base, n * sizeof(float)));
         f->base_ptr = f->base.as<float>();
         HIP_TRY(launch_base(f->in_fmt, f->raw_ptr, f->raw_pitch, bg_dev, H, W, f->base_ptr, st));
     }
     if (mask) {
-        PIV_TRY(f->sat.reserve((size_t)f->cells * 2 * sizeof(double)));
+        PIV_TRY(frame_reserve(a, f->sat, (size_t)f->cells * 2 * sizeof(double)));
         double *sum = f->sat.as<double>() + SAT_LEAD, *sq = sum + f->cells;
         f->sat_ptr = sum;
         // without conditioning the base image is the normalised raw frame: the table is summed
@@ -201,7 +210,7 @@ piv_status prepare_frame(piv_analyzer *a, piv_frame *f, const float *bg_dev)
         f->pitch = f->raw_pitch;
         return PIV_OK;
     }
-    PIV_TRY(f->image.reserve(n * sizeof(float)));
+    PIV_TRY(frame_reserve(a, f->image, n * sizeof(float)));
     const float *cur = f->base_ptr;
     if (c.smooth_sigma > 0.0f) {
         PIV_TRY(a->prep_tmp.reserve(n * sizeof(float)));
@@ -249,7 +258,7 @@ piv_status prepare_frame_pair(piv_analyzer *a, piv_frame *fa, piv_frame *fb, con
     piv_frame *const fs[2] = {fa, fb};
     for (int i = 0; i < 2; ++i) {
         piv_frame *f = fs[i];
-        PIV_TRY(f->sat.reserve((size_t)cells * 2 * sizeof(double)));
+        PIV_TRY(frame_reserve(a, f->sat, (size_t)cells * 2 * sizeof(double)));
         sum[i] = f->sat.as<double>() + SAT_LEAD;
         f->sat_ptr = sum[i];
         f->has_sat = true;
@@ -288,12 +297,26 @@ piv_status prepare_frames_batch(piv_analyzer *a, piv_frame *const *fs, int count
     std::shared_ptr<DevBuf> store;
     if (mask) {
         // one pass over the raw samples per frame; no base image is kept (piv_frame_download
-        // recomputes it)
-        store.reset(new DevBuf, [](DevBuf *b) {
+        // recomputes it).  The tables go into a block from the analyzer's batch pool when one
+        // of a fitting size is free.
+        const size_t need = (size_t)count * cells * 2 * sizeof(double);
+        std::weak_ptr<BlockPool> wp = a->pool;
+        const int dev = a->device;
+        store.reset(new DevBuf, [wp, dev](DevBuf *b) {
+            // the frames' last user is done before the block is reused (hipFree would
+            // synchronise the device as well)
+            if (b->p) {
+                int cur = dev;
+                (void)hipGetDevice(&cur);
+                if (cur != dev) (void)hipSetDevice(dev);
+                (void)hipDeviceSynchronize();
+                if (auto pool = wp.lock()) pool->give(*b);
+                if (cur != dev) (void)hipSetDevice(cur);
+            }
             b->release();
             delete b;
         });
-        PIV_TRY(store->reserve((size_t)count * cells * 2 * sizeof(double)));
+        PIV_TRY(frame_reserve(a, *store, need));
         double *sat = store->as<double>() + SAT_LEAD;  // entry (0, 0) of frame 0's sums
         PIV_TRY(a->sat_carry.reserve(sat_carry_bytes(H, W, count)));
         HIP_TRY(launch_sat_raw(f0->in_fmt, f0->raw_ptr, f0->raw_pitch, raw_frame_bytes, H, W, sat,
@@ -605,6 +628,7 @@ piv_status frame_from_device(piv_analyzer *a, const void *pixels, int fmt, uint3
     auto *f = new (std::nothrow) piv_frame();
     if (!f) return fail(PIV_ERR_OOM, "host allocation failed");
     f->device = a->device;
+    f->pool = a->pool;
     f->H = (int)h;
     f->W = (int)w;
     f->in_fmt = fmt;
@@ -753,11 +777,12 @@ piv_status piv_frame_prepare(piv_analyzer *a, const void *pixels, piv_pixel_form
     auto *f = new (std::nothrow) piv_frame();
     if (!f) return fail(PIV_ERR_OOM, "host allocation failed");
     f->device = a->device;
+    f->pool = a->pool;
     f->H = (int)height;
     f->W = (int)width;
     f->in_fmt = fmt;
     const size_t bpp = (size_t)bytes_per_px(fmt);
-    piv_status s = f->raw.reserve((size_t)height * width * bpp);
+    piv_status s = frame_reserve(a, f->raw, (size_t)height * width * bpp);
     DevBuf bg;
     if (s == PIV_OK) {
         hipError_t e = hipMemcpy2DAsync(f->raw.p, width * bpp, pixels, row_stride, width * bpp,
@@ -823,6 +848,7 @@ piv_status piv_frame_prepare_device_batch(piv_analyzer *a, const void *pixels,
         }
         piv_frame *f = fs[i];
         f->device = a->device;
+        f->pool = a->pool;
         f->H = (int)height;
         f->W = (int)width;
         f->in_fmt = (int)fmt;
@@ -844,6 +870,17 @@ void piv_frame_release(piv_frame *f)
 {
     if (!f) return;
     (void)hipSetDevice(f->device);
+    if (f->raw.p || f->image.p || f->base.p || f->sat.p) {
+        if (auto pool = f->pool.lock()) {
+            // the frame's last user is done before its memory is reused (hipFree would
+            // synchronise the device as well)
+            (void)hipDeviceSynchronize();
+            pool->give(f->raw);
+            pool->give(f->image);
+            pool->give(f->base);
+            pool->give(f->sat);
+        }
+    }
     delete f;
 }
 

@@ -101,6 +101,51 @@ using pivi::DevBuf;
 using pivi::HostBuf;
 using namespace piv;
 
+// Device memory of released frames and frame batches, kept for the next preparation: a
+// video loop prepares frames of one size over and over, and hipMalloc / hipFree of their
+// images and tables (a synchronising unmap per free) cost more than preparing them.  A
+// block is reused for a request of at most twice less than its size; at most kLimit bytes
+// are kept.  Shared with the frames and their batch storage, so it outlives the analyzer
+// while they do.
+struct BlockPool {
+    static constexpr size_t kLimit = (size_t)16 << 30;
+    std::mutex mu;
+    std::vector<std::pair<void *, size_t>> blocks;  // (pointer, capacity)
+    size_t held = 0;
+    // moves a fitting block into `b` (whose own memory is released first)
+    bool take(DevBuf &b, size_t bytes)
+    {
+        b.release();
+        std::lock_guard<std::mutex> lk(mu);
+        for (size_t i = 0; i < blocks.size(); ++i)
+            if (blocks[i].second >= bytes && blocks[i].second / 2 <= bytes) {
+                b.p = blocks[i].first;
+                b.cap = blocks[i].second;
+                held -= b.cap;
+                blocks[i] = blocks.back();
+                blocks.pop_back();
+                return true;
+            }
+        return false;
+    }
+    // takes `b`'s memory (b becomes empty), or leaves it to `b` when full; the caller has
+    // made sure no device work still uses it
+    void give(DevBuf &b)
+    {
+        if (!b.p) return;
+        std::lock_guard<std::mutex> lk(mu);
+        if (held + b.cap > kLimit) return;
+        blocks.emplace_back(b.p, b.cap);
+        held += b.cap;
+        b.p = nullptr;
+        b.cap = 0;
+    }
+    ~BlockPool()
+    {
+        for (auto &bl : blocks) (void)hipFree(bl.first);
+    }
+};
+
 struct piv_frame {
     int device = 0;
     int H = 0, W = 0;
@@ -116,6 +161,7 @@ struct piv_frame {
     float *base_ptr = nullptr;        // background-subtracted base: own `base` or batch storage
     const double *sat_ptr = nullptr;  // SAT (sum, then sum of squares): own `sat` or batch storage
     std::shared_ptr<DevBuf> batch;    // storage shared by the frames of one batch preparation
+    std::weak_ptr<BlockPool> pool;    // where piv_frame_release returns the owned buffers
     int in_fmt = FMT_F32;      // original sample format
     const void *raw_ptr = nullptr;  // device pointer of the original samples
     int raw_pitch = 0;
@@ -161,6 +207,7 @@ struct piv_analyzer {
     HostBuf h_summaries;
     DevBuf prep_tmp, prep_tmp2, prep_sat;  // conditioning scratch
     DevBuf sat_carry;                      // row carries of the SAT passes
+    std::shared_ptr<BlockPool> pool = std::make_shared<BlockPool>();
     DevBuf stage_in, stage_out;            // stage-level entry points
     DevBuf post_part, post_keys, post_out; // flow statistics (piv_flow_stats_device)
     // kernel timing (piv_profile_*)
