@@ -277,9 +277,101 @@ piv_status prepare_frame_pair(piv_analyzer *a, piv_frame *fa, piv_frame *fb, con
     return PIV_OK;
 }
 
-// Conditioning of `count` frames of one size and format (frames raw_frame_bytes apart): the
-// base image and SAT of every frame come from one launch per kernel into storage the frames
-// share.  Blur / local normalisation configurations condition frame by frame.
+// Device storage shared by the frames of one batch preparation, from the analyzer's pool
+// when a fitting block is free; it returns there when the last frame is released.
+static piv_status batch_storage(piv_analyzer *a, size_t bytes, std::shared_ptr<DevBuf> &store)
+{
+    std::weak_ptr<BlockPool> wp = a->pool;
+    const int dev = a->device;
+    store.reset(new DevBuf, [wp, dev](DevBuf *b) {
+        // the frames' last user is done before the block is reused (hipFree would
+        // synchronise the device as well)
+        if (b->p) {
+            int cur = dev;
+            (void)hipGetDevice(&cur);
+            if (cur != dev) (void)hipSetDevice(dev);
+            (void)hipDeviceSynchronize();
+            if (auto pool = wp.lock()) pool->give(*b);
+            if (cur != dev) (void)hipSetDevice(cur);
+        }
+        b->release();
+        delete b;
+    });
+    return frame_reserve(a, *store, bytes);
+}
+
+// PreparedFrame::new (preprocess.rs:137-153) for a batch of frames with conditioning
+// (background, blur, local normalisation): the same kernels as prepare_frame, each launched
+// once per chunk of frames (frames in blockIdx.y), so that the per-frame kernels are not
+// latency-bound launches of their own.  Base images, conditioned images and tables live in
+// storage the frames share; the blur and normalisation temporaries are sized for one chunk.
+static piv_status condition_frames_batch(piv_analyzer *a, piv_frame *const *fs, int count,
+                                         const float *bg_dev, long raw_frame_bytes, bool mask)
+{
+    constexpr int CHUNK = 32;
+    const piv_config &c = a->cfg;
+    piv_frame *f0 = fs[0];
+    const int H = f0->H, W = f0->W;
+    const size_t n = (size_t)H * W;
+    const int64_t cells = sat_table_elems(H, W);
+    const bool blur = c.smooth_sigma > 0.0f, norm = c.normalize_radius > 0;
+    const size_t img_bytes = align_up((size_t)count * n * sizeof(float), 256);
+    const size_t sat_bytes = mask ? (size_t)count * cells * 2 * sizeof(double) : 0;
+    std::shared_ptr<DevBuf> store;
+    PIV_TRY(batch_storage(a, 2 * img_bytes + sat_bytes, store));
+    float *base = store->as<float>();
+    float *image = reinterpret_cast<float *>(store->as<char>() + img_bytes);
+    double *sat = mask ? reinterpret_cast<double *>(store->as<char>() + 2 * img_bytes) + SAT_LEAD : nullptr;
+    const int k_max = std::min(CHUNK, count);
+    if (blur) PIV_TRY(a->prep_tmp.reserve((size_t)k_max * n * sizeof(float)));
+    if (blur && norm) PIV_TRY(a->prep_tmp2.reserve((size_t)k_max * n * sizeof(float)));
+    if (norm) PIV_TRY(a->prep_sat.reserve((size_t)k_max * cells * 2 * sizeof(double)));
+    if (mask || norm) PIV_TRY(a->sat_carry.reserve(sat_carry_bytes(H, W, k_max)));
+    hipStream_t st = a->stream;
+    double *carry = a->sat_carry.as<double>();
+    for (int i0 = 0; i0 < count; i0 += CHUNK) {
+        const int k = std::min(CHUNK, count - i0);
+        const void *raw = static_cast<const char *>(f0->raw_ptr) + (long)i0 * raw_frame_bytes;
+        float *b0 = base + (size_t)i0 * n, *img0 = image + (size_t)i0 * n;
+        HIP_TRY(launch_base(f0->in_fmt, raw, f0->raw_pitch, bg_dev, H, W, b0, st, k, raw_frame_bytes));
+        if (mask) {
+            double *s0 = sat + (size_t)i0 * 2 * cells;
+            HIP_TRY(launch_sat(b0, H, W, s0, s0 + cells, carry, st, k, 2 * cells));
+        }
+        const float *cur = b0;
+        if (blur) {
+            float *dst = norm ? a->prep_tmp2.as<float>() : img0;
+            HIP_TRY(launch_blur(cur, H, W, a->d_taps.as<float>(), a->taps_r, a->prep_tmp.as<float>(), dst,
+                                st, k));
+            cur = dst;
+        }
+        if (norm) {
+            double *ns = a->prep_sat.as<double>() + SAT_LEAD;
+            HIP_TRY(launch_sat(cur, H, W, ns, ns + cells, carry, st, k, 2 * cells));
+            HIP_TRY(launch_local_norm(cur, H, W, (int)c.normalize_radius, c.normalize_eps, ns, ns + cells,
+                                      img0, st, k, 2 * cells));
+        } else if (cur != img0) {
+            HIP_TRY(hipMemcpyAsync(img0, cur, (size_t)k * n * sizeof(float), hipMemcpyDeviceToDevice, st));
+        }
+    }
+    for (int i = 0; i < count; ++i) {
+        piv_frame *f = fs[i];
+        f->has_bg = bg_dev != nullptr;
+        f->cells = cells;
+        f->base_ptr = base + (size_t)i * n;
+        f->has_sat = mask;
+        f->sat_ptr = mask ? sat + (size_t)i * 2 * cells : nullptr;
+        f->fmt = FMT_F32;
+        f->img = image + (size_t)i * n;
+        f->pitch = W;
+        f->batch = store;
+    }
+    return PIV_OK;
+}
+
+// Preparation of `count` frames of one size and format (frames raw_frame_bytes apart): every
+// kernel is launched once for the whole set (per chunk when conditioning), into storage the
+// frames share.
 piv_status prepare_frames_batch(piv_analyzer *a, piv_frame *const *fs, int count,
                                 const float *bg_dev, long raw_frame_bytes)
 {
@@ -288,35 +380,16 @@ piv_status prepare_frames_batch(piv_analyzer *a, piv_frame *const *fs, int count
     piv_frame *f0 = fs[0];
     const bool cond = c.smooth_sigma > 0.0f || c.normalize_radius > 0 || bg_dev != nullptr;
     const bool mask = mask_needed(c, f0->in_fmt, bg_dev != nullptr);
-    if (cond) {
-        for (int i = 0; i < count; ++i) PIV_TRY(prepare_frame(a, fs[i], bg_dev));
-        return PIV_OK;
-    }
     const int H = f0->H, W = f0->W;
     const int64_t cells = sat_table_elems(H, W);
+    if (cond) return condition_frames_batch(a, fs, count, bg_dev, raw_frame_bytes, mask);
     std::shared_ptr<DevBuf> store;
     if (mask) {
         // one pass over the raw samples per frame; no base image is kept (piv_frame_download
         // recomputes it).  The tables go into a block from the analyzer's batch pool when one
         // of a fitting size is free.
         const size_t need = (size_t)count * cells * 2 * sizeof(double);
-        std::weak_ptr<BlockPool> wp = a->pool;
-        const int dev = a->device;
-        store.reset(new DevBuf, [wp, dev](DevBuf *b) {
-            // the frames' last user is done before the block is reused (hipFree would
-            // synchronise the device as well)
-            if (b->p) {
-                int cur = dev;
-                (void)hipGetDevice(&cur);
-                if (cur != dev) (void)hipSetDevice(dev);
-                (void)hipDeviceSynchronize();
-                if (auto pool = wp.lock()) pool->give(*b);
-                if (cur != dev) (void)hipSetDevice(cur);
-            }
-            b->release();
-            delete b;
-        });
-        PIV_TRY(frame_reserve(a, *store, need));
+        PIV_TRY(batch_storage(a, need, store));
         double *sat = store->as<double>() + SAT_LEAD;  // entry (0, 0) of frame 0's sums
         PIV_TRY(a->sat_carry.reserve(sat_carry_bytes(H, W, count)));
         HIP_TRY(launch_sat_raw(f0->in_fmt, f0->raw_ptr, f0->raw_pitch, raw_frame_bytes, H, W, sat,
@@ -899,8 +972,10 @@ piv_status piv_frame_download(const piv_frame *f, float *image, float *base)
         return PIV_OK;
     };
     if (image) {
-        if (f->fmt == FMT_F32 && f->img == f->image.p && f->image.p) {
-            HIP_TRY(hipMemcpy(image, f->image.p, n * sizeof(float), hipMemcpyDeviceToHost));
+        // a conditioned image (own or batch storage) is what the correlation reads; otherwise
+        // the image is the normalised samples
+        if (f->fmt == FMT_F32 && f->img && f->img != f->raw_ptr) {
+            HIP_TRY(hipMemcpy(image, f->img, n * sizeof(float), hipMemcpyDeviceToHost));
         } else {
             float *p = nullptr;
             PIV_TRY(normalized(&p));

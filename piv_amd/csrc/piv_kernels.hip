@@ -477,8 +477,11 @@ __global__ __launch_bounds__(64) void k_sat_strips(const void *__restrict__ img,
 __global__ void k_blur_rows(const float *__restrict__ in, int H, int W,
                             const float *__restrict__ taps, int r, float *__restrict__ out)
 {
+    // blockIdx.y: frame of a batch (images H*W apart)
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (long)H * W) return;
+    in += (long)blockIdx.y * H * W;
+    out += (long)blockIdx.y * H * W;
     const int y = (int)(i / W), x = (int)(i - (long)y * W);
     const float *row = in + (long)y * W;
     float acc = 0.0f;
@@ -495,6 +498,8 @@ __global__ void k_blur_cols(const float *__restrict__ in, int H, int W,
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (long)H * W) return;
+    in += (long)blockIdx.y * H * W;
+    out += (long)blockIdx.y * H * W;
     const int y = (int)(i / W), x = (int)(i - (long)y * W);
     float acc = 0.0f;
     for (int j = 0; j <= 2 * r; ++j) {
@@ -508,10 +513,15 @@ __global__ void k_blur_cols(const float *__restrict__ in, int H, int W,
 // local_normalize: (v - mean) / (std + eps) over the clipped (2r+1)^2 box
 __global__ void k_local_norm(const float *__restrict__ in, int H, int W, int radius, float eps,
                              const double *__restrict__ sum, const double *__restrict__ sq,
-                             float *__restrict__ out)
+                             float *__restrict__ out, long sat_stride)
 {
+    // blockIdx.y: frame of a batch (images H*W apart, tables sat_stride apart)
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (long)H * W) return;
+    in += (long)blockIdx.y * H * W;
+    out += (long)blockIdx.y * H * W;
+    sum += (long)blockIdx.y * sat_stride;
+    sq += (long)blockIdx.y * sat_stride;
     const int y = (int)(i / W), x = (int)(i - (long)y * W);
     const int y0 = y > radius ? y - radius : 0, y1 = min(y + radius + 1, H);
     const int x0 = x > radius ? x - radius : 0, x1 = min(x + radius + 1, W);
@@ -854,23 +864,36 @@ hipError_t launch_sat(const float *img, int H, int W, double *sum, double *sq, d
 }
 
 hipError_t launch_blur(const float *in, int H, int W, const float *taps, int r, float *tmp,
-                       float *out, hipStream_t st)
+                       float *out, hipStream_t st, int nframes)
 {
     const long n = (long)H * W;
-    hipLaunchKernelGGL(k_blur_rows, dim3(nblk(n, 256)), dim3(256), 0, st, in, H, W, taps, r, tmp);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_blur_cols, dim3(nblk(n, 256)), dim3(256), 0, st, tmp, H, W, taps, r, out);
-    return hipGetLastError();
+    for (int f0 = 0; f0 < nframes; f0 += MAX_GRID_Y) {
+        const dim3 grid(nblk(n, 256), (unsigned)min(MAX_GRID_Y, nframes - f0));
+        const long o = (long)f0 * n;
+        hipLaunchKernelGGL(k_blur_rows, grid, dim3(256), 0, st, in + o, H, W, taps, r, tmp + o);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_blur_cols, grid, dim3(256), 0, st, tmp + o, H, W, taps, r, out + o);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_local_norm(const float *in, int H, int W, int radius, float eps,
-                             const double *sum, const double *sq, float *out, hipStream_t st)
+                             const double *sum, const double *sq, float *out, hipStream_t st,
+                             int nframes, long sat_stride)
 {
     const long n = (long)H * W;
-    hipLaunchKernelGGL(k_local_norm, dim3(nblk(n, 256)), dim3(256), 0, st, in, H, W, radius, eps,
-                       sum, sq, out);
-    return hipGetLastError();
+    for (int f0 = 0; f0 < nframes; f0 += MAX_GRID_Y) {
+        const dim3 grid(nblk(n, 256), (unsigned)min(MAX_GRID_Y, nframes - f0));
+        const long o = (long)f0 * n, so = (long)f0 * sat_stride;
+        hipLaunchKernelGGL(k_local_norm, grid, dim3(256), 0, st, in + o, H, W, radius, eps,
+                           sum + so, sq + so, out + o, sat_stride);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_bg_add(int fmt, const void *raw, int pitch, int H, int W, int mode, int first,

@@ -39,10 +39,12 @@ hipError_t launch_sat_raw(int fmt, const void *raw, int pitch, long raw_frame_by
                           double *sum, double *sq, double *carry, hipStream_t st, int nframes = 1,
                           long sat_stride = 0);
 size_t sat_carry_bytes(int H, int W, int nframes);
+// nframes > 1: a batch of images H*W apart (tables sat_stride doubles apart)
 hipError_t launch_blur(const float *in, int H, int W, const float *taps, int r, float *tmp,
-                       float *out, hipStream_t st);
+                       float *out, hipStream_t st, int nframes = 1);
 hipError_t launch_local_norm(const float *in, int H, int W, int radius, float eps,
-                             const double *sum, const double *sq, float *out, hipStream_t st);
+                             const double *sum, const double *sq, float *out, hipStream_t st,
+                             int nframes = 1, long sat_stride = 0);
 hipError_t launch_normalize(int fmt, const void *raw, long count, float *out, hipStream_t st);
 // dst[i] = src[i * channels + plane] for `count` pixels of u8 / u16 / f32 samples
 hipError_t launch_extract_plane(int fmt, const void *src, long count, int channels, int plane,

@@ -294,11 +294,12 @@ def test_kernel_variants_bit_identical(piv, fmt):
 
 @pytest.mark.parametrize("dtype,pre,n", [(np.uint16, "raw", 5), (np.uint16, "raw", 12),
                                          (np.uint8, "raw", 5), (np.uint8, "default", 5),
+                                         (np.uint16, "default", 9), (np.uint8, "default", 40),
                                          (np.float32, "raw", 9)])
 def test_batch_prepared_frames_equal_single(piv, dtype, pre, n):
-    """Frames prepared as one batch (one launch per kernel, tables summed from the samples
-    into storage the frames share) give the single-frame preparation's analysis bitwise, masks
-    included."""
+    """Frames prepared as one batch (one launch per kernel, per chunk of 32 frames when
+    conditioning; images and tables in storage the frames share) give the single-frame
+    preparation's analysis and images bitwise, masks included."""
     h, w = 160, 224
     frames, _ = S.video_frames(h, w, n, 17, bits=12 if dtype is np.uint16 else 8,
                                container=np.uint8 if dtype is np.uint8 else np.uint16)
@@ -324,9 +325,11 @@ def test_batch_prepared_frames_equal_single(piv, dtype, pre, n):
         assert r.summary == ref.summary
         if dtype is not np.uint8:
             assert 0 < r.summary.no_signal < r.field.flags.size, r.summary
-    img, base = fs[2].download()
-    rimg, rbase = an.prepare(frames[2]).download()
-    assert np.array_equal(bits(img), bits(rimg))
+    for k in sorted({2, n - 2}):
+        img, base = fs[k].download()
+        rimg, rbase = an.prepare(frames[k]).download()
+        assert np.array_equal(bits(img), bits(rimg)), k
+        assert np.array_equal(bits(base), bits(rbase)), k
 
 
 def test_deterministic(piv):
