@@ -548,6 +548,8 @@ def main():
     ap.add_argument("--no-video", action="store_true", help="skip the batched-video measurement")
     ap.add_argument("--no-post", action="store_true",
                     help="skip the flow-statistics and smoothing measurement")
+    ap.add_argument("--no-cli", action="store_true",
+                    help="skip the piv-analyze host-loop measurement")
     ap.add_argument("--only-cli", type=int, default=0, metavar="FRAMES",
                     help="measure only piv-analyze end to end on a clip of this many frames")
     ap.add_argument("--only-video", type=int, default=0, metavar="PAIRS",
@@ -696,6 +698,10 @@ def main():
     if not args.no_post:
         post = post_bench(an, g, dx, dy, fl)
 
+    cli = None
+    if not args.no_cli:
+        cli = cli_bench(frames=1000)
+
     serving = None
     if args.serve_seconds > 0:
         serving = serve_bench(args.serve_rate, args.serve_seconds, args.serve_backlog)
@@ -740,6 +746,7 @@ def main():
         "reduced_precision": precision,
         "video": video,
         "post": post,
+        "cli": cli,
         "serving": serving,
         "cpu_baseline": cpu,
     }
