@@ -198,22 +198,29 @@ def serve_bench(rate, seconds, backlog, seed=77):
 
     def saturate(device_frames):
         kinds = rng.choice(len(streams), size=backlog, p=share)
-        total_bytes = 0
+        total_bytes = sum(streams[k][2] for k in kinds)
         t0 = time.perf_counter()
-        for i, k in enumerate(kinds):
-            sid, pairs, gb, _ = streams[k]
-            a, b = (dev_pairs[k] if device_frames else pairs)[i % 2]
-            while True:
-                try:
-                    if device_frames:
-                        srv.submit_device(sid, a, b)
-                    else:
-                        srv.submit(sid, a, b)
-                    break
-                except piv_amd.AdmissionError:
+        if device_frames:
+            # the whole backlog through the native batch submission, the rest retried
+            # whenever the HBM budget stops it
+            reqs = [(streams[k][0],) + dev_pairs[k][i % 2] for i, k in enumerate(kinds)]
+            while reqs:
+                n = len(srv.submit_device_many(reqs))
+                submitted[0] += n
+                reqs = reqs[n:]
+                if reqs:
                     time.sleep(0.0005)
-            submitted[0] += 1
-            total_bytes += gb
+        else:
+            for i, k in enumerate(kinds):
+                sid, pairs, gb, _ = streams[k]
+                a, b = pairs[i % 2]
+                while True:
+                    try:
+                        srv.submit(sid, a, b)
+                        break
+                    except piv_amd.AdmissionError:
+                        time.sleep(0.0005)
+                submitted[0] += 1
         while done_count[0] < submitted[0]:
             time.sleep(0.0005)
         el = time.perf_counter() - t0
@@ -230,11 +237,14 @@ def serve_bench(rate, seconds, backlog, seed=77):
     dev_pairs = [[(torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()) for a, b in s[1]]
                  for s in streams]
     torch.cuda.synchronize()
+    # one unmeasured backlog first: request slots for a full backlog are allocated and the
+    # batch shapes' graphs captured once, as in a server that has been running a while
+    saturate(True)
     b0 = srv.stats()
     sat_host, b1 = saturate(False)
     sat_host["note"] = "host frames: staging copy and PCIe upload included"
     sat_dev, b2 = saturate(True)
-    sat_dev["note"] = "frames resident in HBM (piv_submit_device)"
+    sat_dev["note"] = "frames resident in HBM, the backlog submitted with piv_submit_device_many"
     sat_host["mean_batch_pairs"] = round((b1["batched_pairs"] - b0["batched_pairs"]) /
                                          max(b1["batches"] - b0["batches"], 1), 2)
     sat_dev["mean_batch_pairs"] = round((b2["batched_pairs"] - b1["batched_pairs"]) /

@@ -288,6 +288,14 @@ piv_status piv_submit(piv_server *s, uint32_t stream_id, const void *pixels_a,
  * called): they are read in place, so keep them unchanged until the result is released. */
 piv_status piv_submit_device(piv_server *s, uint32_t stream_id, const void *pixels_a,
                              const void *pixels_b, size_t row_stride, uint64_t *request_id);
+/* `count` requests of frames in device memory in one call (each as piv_submit_device, in
+ * order), for a caller with many pairs ready.  Stops at the first request the server cannot
+ * admit (HBM budget or full queue): returns PIV_ERR_ADMISSION with *submitted the number
+ * admitted (their ids in request_ids), and the caller retries the rest after polling.  No
+ * counterpart in the reference, whose CLI analyses one pair per call. */
+piv_status piv_submit_device_many(piv_server *s, uint32_t count, const uint32_t *stream_ids,
+                                  const void *const *pixels_a, const void *const *pixels_b,
+                                  size_t row_stride, uint64_t *request_ids, uint32_t *submitted);
 /* Up to `capacity` finished results, oldest first; waits up to wait_ms (-1: forever) for one. */
 piv_status piv_poll(piv_server *s, piv_result *out, uint32_t capacity, uint32_t *count,
                     int wait_ms);

@@ -760,6 +760,31 @@ class PivServer:
                                            C.byref(rid)))
         return rid.value
 
+    def submit_device_many(self, requests):
+        """Several (stream_id, frame_a, frame_b) requests of device frames in one call
+        (piv_submit_device_many), in order.  Returns the ids of the requests admitted: fewer
+        than given when the server could not admit the next one (retry the rest after
+        polling)."""
+        n = len(requests)
+        if n == 0:
+            return []
+        sids = (C.c_uint32 * n)()
+        pa, pb = (C.c_void_p * n)(), (C.c_void_p * n)()
+        for i, (sid, fa, fb) in enumerate(requests):
+            h, w, dt, _ = self._streams[sid]
+            for f in (fa, fb):
+                if tuple(f.shape) != (h, w):
+                    raise ShapeMismatch(f"frame {tuple(f.shape)} vs stream {(h, w)}")
+            sids[i] = sid
+            pa[i] = L.ptr(fa).value
+            pb[i] = L.ptr(fb).value
+        ids = (C.c_uint64 * n)()
+        got = C.c_uint32()
+        st = self._lib.piv_submit_device_many(self._h, n, sids, pa, pb, 0, ids, C.byref(got))
+        if st != 0 and not (st == L.ERR_ADMISSION and got.value < n):
+            _check(st)
+        return list(ids[:got.value])
+
     def poll(self, max_results=64, wait_ms=0, copy=True):
         """Finished requests, oldest first, released after reading.  copy=False skips the
         field arrays (field is None): ids, status, timings and summaries only."""

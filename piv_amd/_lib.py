@@ -142,6 +142,9 @@ _SIGS = {
                     C.POINTER(C.c_uint64)], C.c_int),
     "piv_submit_device": ([C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_size_t,
                            C.POINTER(C.c_uint64)], C.c_int),
+    "piv_submit_device_many": ([C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_void_p),
+                                C.POINTER(C.c_void_p), C.c_size_t, C.POINTER(C.c_uint64),
+                                C.POINTER(C.c_uint32)], C.c_int),
     "piv_poll": ([C.c_void_p, C.POINTER(piv_result), C.c_uint32, C.POINTER(C.c_uint32), C.c_int],
                  C.c_int),
     "piv_result_release": ([C.c_void_p, C.c_uint64], C.c_int),

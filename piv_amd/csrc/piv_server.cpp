@@ -409,18 +409,21 @@ piv_status launch_batch(piv_server *s, Lane &L)
         steps = std::max(steps, r->st->plans.size());
     }
     const size_t nsum = steps * L.batch.size() * 5;
+    // lane buffers grow at least twofold: a server warming up to its batch sizes reallocates
+    // (and drops its graphs) a few times, not at every larger batch
+    auto grow = [](size_t need, size_t cap) { return need <= cap ? need : std::max(need, 2 * cap); };
     if (need_out > L.out.cap || nsum * 4 > L.sums.cap) L.drop_graphs();
-    PIV_TRY(L.out.reserve(need_out));
-    PIV_TRY(L.h_out.reserve(need_out));
-    PIV_TRY(L.sums.reserve(nsum * 4));
-    PIV_TRY(L.h_sums.reserve(nsum * 4));
+    PIV_TRY(L.out.reserve(grow(need_out, L.out.cap)));
+    PIV_TRY(L.h_out.reserve(grow(need_out, L.h_out.cap)));
+    PIV_TRY(L.sums.reserve(grow(nsum * 4, L.sums.cap)));
+    PIV_TRY(L.h_sums.reserve(grow(nsum * 4, L.h_sums.cap)));
     BatchPlan bp;
     build_batch(L, bp);
     if (bp.host.size() > L.desc.cap) {
         L.drop_graphs();
-        PIV_TRY(L.desc.reserve(bp.host.size()));
+        PIV_TRY(L.desc.reserve(grow(bp.host.size(), L.desc.cap)));
     }
-    PIV_TRY(L.h_desc.reserve(bp.host.size()));
+    PIV_TRY(L.h_desc.reserve(grow(bp.host.size(), L.h_desc.cap)));
     hipStream_t st = L.stream;
     for (Request *r : L.batch) HIP_TRY(hipStreamWaitEvent(st, r->slot->uploaded, 0));
     std::memcpy(L.h_desc.p, bp.host.data(), bp.host.size());
@@ -529,8 +532,8 @@ void completer_loop(piv_server *s)
         lk.unlock();
         const hipError_t e = hipEventSynchronize(L.done);
         const double t = s->now_ms();
-        lk.lock();
-        s->inflight.pop_front();
+        // the lane's requests belong to it until they are queued as done: their results are
+        // copied into the slots without the server lock, which submissions and polls take
         const int B = (int)L.batch.size();
         const char *ho = static_cast<const char *>(L.h_out.p);
         const uint32_t *hs = static_cast<const uint32_t *>(L.h_sums.p);
@@ -551,7 +554,11 @@ void completer_loop(piv_server *s)
                 to_summary(hs + L.sum_index[k] * 5, &r->summary);
                 r->status = PIV_OK;
             }
-            s->done.push_back(r);
+        }
+        lk.lock();
+        s->inflight.pop_front();
+        for (int k = 0; k < B; ++k) {
+            s->done.push_back(L.batch[k]);
             s->stats.completed += 1;
         }
         L.batch.clear();
@@ -800,6 +807,22 @@ piv_status piv_submit_device(piv_server *s, uint32_t stream_id, const void *pixe
                              const void *pixels_b, size_t row_stride, uint64_t *request_id)
 {
     return submit_impl(s, stream_id, pixels_a, pixels_b, row_stride, true, request_id);
+}
+
+piv_status piv_submit_device_many(piv_server *s, uint32_t count, const uint32_t *stream_ids,
+                                  const void *const *pixels_a, const void *const *pixels_b,
+                                  size_t row_stride, uint64_t *request_ids, uint32_t *submitted)
+{
+    if (!s || !stream_ids || !pixels_a || !pixels_b || !request_ids || !submitted)
+        return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+    *submitted = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        const piv_status r = submit_impl(s, stream_ids[i], pixels_a[i], pixels_b[i], row_stride, true,
+                                         request_ids + i);
+        if (r != PIV_OK) return r;
+        *submitted = i + 1;
+    }
+    return PIV_OK;
 }
 
 piv_status piv_poll(piv_server *s, piv_result *out, uint32_t capacity, uint32_t *count, int wait_ms)

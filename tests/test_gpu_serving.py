@@ -168,3 +168,45 @@ def test_device_submission_equals_host_submission(piv):
     _same(got[r_dev], _alone(piv, cfg, raw, a, b, None))
     _same(got[r_host], _alone(piv, cfg, raw, a, b, None))
     srv.close()
+
+
+def test_device_batch_submission(piv):
+    """Several device pairs submitted in one call: every result equals the pair analysed
+    alone; under an HBM budget for one request slot only the first is admitted, and the rest
+    go in after its result is released."""
+    cfg = piv.PivConfig(windows=[48, 24], final_iterations=1)
+    raw = piv.PreprocessConfig.raw()
+    pairs = [_pair(200, 320, np.uint16, 12, 5000 + i) for i in range(5)]
+    dev = [(piv.DeviceArray.from_numpy(a), piv.DeviceArray.from_numpy(b)) for a, b in pairs]
+    srv = piv.PivServer(max_batch_pairs=8)
+    sid = srv.add_stream(cfg, 200, 320, np.uint16, raw)
+    ids = srv.submit_device_many([(sid, da, db) for da, db in dev])
+    assert len(ids) == 5
+    got = {}
+    while len(got) < 5:
+        res = srv.poll(8, wait_ms=10000)
+        assert res, "no result within 10 s"
+        for r in res:
+            assert r.error is None, r.error
+            got[r.request_id] = r
+    for rid, (a, b) in zip(ids, pairs):
+        _same(got[rid], _alone(piv, cfg, raw, a, b, None))
+    srv.close()
+    # one request slot's worth of HBM: the first request is admitted, the next refused
+    probe = piv.PivServer()
+    psid = probe.add_stream(cfg, 200, 320, np.uint16, raw)
+    probe.submit_device(psid, *dev[0])
+    probe.poll(1, wait_ms=5000)
+    one = probe.stats()["hbm_peak"]
+    probe.close()
+    srv = piv.PivServer(hbm_budget=one)
+    sid = srv.add_stream(cfg, 200, 320, np.uint16, raw)
+    first = srv.submit_device_many([(sid, da, db) for da, db in dev[:3]])
+    assert len(first) == 1
+    res = srv.poll(1, wait_ms=10000)
+    assert len(res) == 1 and res[0].error is None and res[0].request_id == first[0]
+    _same(res[0], _alone(piv, cfg, raw, *pairs[0], None))
+    rest = srv.submit_device_many([(sid, da, db) for da, db in dev[1:2]])
+    assert len(rest) == 1
+    assert len(srv.poll(1, wait_ms=10000)) == 1
+    srv.close()
