@@ -127,6 +127,11 @@ bool mask_needed(const piv_config &c, int in_fmt, bool has_bg)
     return true;
 }
 
+bool prepare_launches(const piv_config &c, int in_fmt, bool has_bg)
+{
+    return c.smooth_sigma > 0.0f || c.normalize_radius > 0 || has_bg || mask_needed(c, in_fmt, has_bg);
+}
+
 piv_status set_device(int dev)
 {
     HIP_TRY(hipSetDevice(dev));

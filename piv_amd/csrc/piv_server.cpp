@@ -3,12 +3,13 @@
 // this loop keeps that per-pair result and changes only how pairs reach the device.
 //
 //   submit   -> admission against the HBM budget, the pixels go to a pinned staging slot and
-//               are uploaded (and conditioned) on the stream's own upload stream;
+//               are uploaded on the stream's own upload stream;
 //   scheduler-> takes every admitted request (up to max_batch_pairs) as soon as a compute lane
-//               is free; the pass chain of the whole batch runs as one launch per kernel and
-//               window size per pass step (requests of different streams, frame sizes and
-//               pass schedules share launches), replayed from a captured hipGraph when the
-//               batch has a launch structure seen before;
+//               is free, launches each request's conditioning and tables on its stream's upload
+//               stream (all launches come from this one thread); the pass chain of the whole
+//               batch runs as one launch per kernel and window size per pass step (requests of
+//               different streams, frame sizes and pass schedules share launches), replayed
+//               from a captured hipGraph when the batch has a launch structure seen before;
 //   completer-> waits for a lane's batch, copies each result to its slot's host buffers and
 //               hands it to piv_poll.
 // Up to max_inflight batches are on the device at once (one HIP stream per lane), so a batch
@@ -70,6 +71,7 @@ struct Request {
     piv_status status = PIV_OK;
     uint32_t batch_pairs = 0;
     piv_summary summary{};
+    bool prepared = false;  // conditioning and tables launched by the scheduler
 };
 
 struct Lane {
@@ -106,6 +108,7 @@ struct piv_server {
     std::condition_variable cv_sched, cv_done, cv_complete;
     std::vector<SrvStream *> streams;
     std::deque<Request *> queue;  // admitted, waiting for a lane
+    size_t unprepared = 0;        // queued requests the scheduler has not prepared yet
     std::deque<Request *> done;   // finished, not yet polled
     std::map<uint64_t, Request *> polled;
     std::vector<Lane> lanes;
@@ -464,36 +467,87 @@ piv_status launch_batch(piv_server *s, Lane &L)
 // for a slow upload of another stream.
 bool upload_done(Request *r) { return hipEventQuery(r->slot->uploaded) == hipSuccess; }
 
+// A request's conditioning and tables, launched on its stream's upload stream after its
+// upload; `uploaded` is recorded again after them, and the lane that runs the request waits
+// for it.
+piv_status prepare_request(Request *r)
+{
+    SrvStream *rs = r->st;
+    std::lock_guard<std::mutex> ul(rs->upload_mu);
+    const float *bg = rs->has_bg ? rs->bg.as<float>() : nullptr;
+    PIV_TRY(prepare_frame_pair(rs->an, r->slot->fa, r->slot->fb, bg));
+    // raw frames without tables need no device work: the upload's event stands
+    if (prepare_launches(rs->an->cfg, rs->fmt, bg != nullptr))
+        HIP_TRY(hipEventRecord(r->slot->uploaded, rs->an->stream));
+    return PIV_OK;
+}
+
 void scheduler_loop(piv_server *s)
 {
     (void)hipSetDevice(s->device);
     std::unique_lock<std::mutex> lk(s->mu);
     for (;;) {
         int free_lane = -1;
-        s->cv_sched.wait(lk, [&] {
-            if (s->stop) return true;
-            if (s->paused || s->queue.empty()) return false;
+        auto find_lane = [&] {
             for (size_t i = 0; i < s->lanes.size(); ++i)
                 if (!s->lanes[i].busy) {
                     free_lane = (int)i;
                     return true;
                 }
             return false;
+        };
+        // woken by a submission to prepare, or by a free lane with requests waiting
+        s->cv_sched.wait(lk, [&] {
+            if (s->stop) return true;
+            if (s->paused || s->queue.empty()) return false;
+            return s->unprepared > 0 || find_lane();
         });
         if (s->stop) return;
-        Lane &L = s->lanes[free_lane];
-        L.batch.clear();
-        for (auto it = s->queue.begin(); it != s->queue.end() && L.batch.size() < s->cfg.max_batch_pairs;) {
-            if (upload_done(*it)) {
-                L.batch.push_back(*it);
-                it = s->queue.erase(it);
-            } else {
-                ++it;
+        // Requests whose upload has landed are prepared at once, while the lanes may still be
+        // busy, so a batch launches as soon as a lane frees; all launches come from this
+        // thread (launches from several threads contend in the runtime).
+        std::vector<Request *> prep;
+        for (Request *r : s->queue)
+            if (!r->prepared && upload_done(r)) prep.push_back(r);
+        if (!prep.empty()) {
+            lk.unlock();
+            std::vector<piv_status> ps(prep.size());
+            for (size_t i = 0; i < prep.size(); ++i) ps[i] = prepare_request(prep[i]);
+            lk.lock();
+            for (size_t i = 0; i < prep.size(); ++i) {
+                prep[i]->prepared = true;
+                prep[i]->status = ps[i];
+                s->unprepared -= 1;
             }
         }
+        free_lane = -1;
+        if (s->paused || !find_lane()) {
+            // no lane: wait for one (or, with uploads in flight, look again shortly)
+            if (prep.empty()) s->cv_sched.wait_for(lk, std::chrono::microseconds(50));
+            continue;
+        }
+        Lane &L = s->lanes[free_lane];
+        L.batch.clear();
+        bool failed = false;
+        for (auto it = s->queue.begin(); it != s->queue.end() && L.batch.size() < s->cfg.max_batch_pairs;) {
+            Request *r = *it;
+            if (!r->prepared) {
+                ++it;
+            } else if (r->status != PIV_OK) {
+                // its preparation failed: the request completes with that error
+                r->t_done = s->now_ms();
+                s->done.push_back(r);
+                it = s->queue.erase(it);
+                failed = true;
+            } else {
+                L.batch.push_back(r);
+                it = s->queue.erase(it);
+            }
+        }
+        if (failed) s->cv_done.notify_all();
         if (L.batch.empty()) {
             // uploads still in flight: look again shortly (or when a submission arrives)
-            s->cv_sched.wait_for(lk, std::chrono::microseconds(50));
+            if (prep.empty()) s->cv_sched.wait_for(lk, std::chrono::microseconds(50));
             continue;
         }
         L.busy = true;
@@ -773,8 +827,8 @@ piv_status submit_impl(piv_server *s, uint32_t stream_id, const void *pa, const 
             sl->fb->raw_ptr = sl->raw_b;
             sl->fa->raw_pitch = sl->fb->raw_pitch = st->W;
         }
-        const float *bg = st->has_bg ? st->bg.as<float>() : nullptr;
-        if (r == PIV_OK) r = prepare_frame_pair(st->an, sl->fa, sl->fb, bg);
+        // conditioning and tables are launched by the scheduler when the request joins a batch
+        // (one thread issues all of a batch's launches)
         if (r == PIV_OK && hipEventRecord(sl->uploaded, us) != hipSuccess)
             r = fail(PIV_ERR_DEVICE, "event record failed");
     }
@@ -789,6 +843,7 @@ piv_status submit_impl(piv_server *s, uint32_t stream_id, const void *pa, const 
     req->slot = sl;
     req->t_submit = t_arrival;
     s->queue.push_back(req);
+    s->unprepared += 1;
     s->stats.submitted += 1;
     *request_id = req->id;
     s->cv_sched.notify_one();
