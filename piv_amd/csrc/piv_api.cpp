@@ -282,6 +282,47 @@ piv_status prepare_frame_pair(piv_analyzer *a, piv_frame *fa, piv_frame *fb, con
     return PIV_OK;
 }
 
+// Frames of one size and format at their own addresses (the requests of a serving batch),
+// prepared together: without conditioning, their tables come from one launch per kernel for
+// up to 64 frames (launch_sat_list); otherwise frame by frame.
+piv_status prepare_frames_list(piv_analyzer *a, piv_frame *const *fs, int count, const float *bg_dev)
+{
+    if (count <= 0) return PIV_OK;
+    const piv_config &c = a->cfg;
+    piv_frame *f0 = fs[0];
+    const bool cond = c.smooth_sigma > 0.0f || c.normalize_radius > 0 || bg_dev != nullptr;
+    bool same = true;
+    for (int i = 1; i < count; ++i)
+        same = same && fs[i]->H == f0->H && fs[i]->W == f0->W && fs[i]->in_fmt == f0->in_fmt &&
+               fs[i]->raw_pitch == f0->raw_pitch;
+    if (cond || !same || !mask_needed(c, f0->in_fmt, false)) {
+        for (int i = 0; i < count; ++i) PIV_TRY(prepare_frame(a, fs[i], bg_dev));
+        return PIV_OK;
+    }
+    const int H = f0->H, W = f0->W;
+    const int64_t cells = sat_table_elems(H, W);
+    std::vector<const void *> imgs(count);
+    std::vector<double *> sums(count);
+    for (int i = 0; i < count; ++i) {
+        piv_frame *f = fs[i];
+        PIV_TRY(frame_reserve(a, f->sat, (size_t)cells * 2 * sizeof(double)));
+        sums[i] = f->sat.as<double>() + SAT_LEAD;
+        imgs[i] = f->raw_ptr;
+        f->sat_ptr = sums[i];
+        f->has_sat = true;
+        f->has_bg = false;
+        f->cells = cells;
+        f->base_ptr = nullptr;
+        f->fmt = f->in_fmt;
+        f->img = f->raw_ptr;
+        f->pitch = f->raw_pitch;
+    }
+    PIV_TRY(a->sat_carry.reserve(sat_carry_bytes(H, W, std::min(count, 64))));
+    HIP_TRY(launch_sat_list(f0->in_fmt, imgs.data(), sums.data(), count, f0->raw_pitch, H, W,
+                            a->sat_carry.as<double>(), a->stream));
+    return PIV_OK;
+}
+
 // Device storage shared by the frames of one batch preparation, from the analyzer's pool
 // when a fitting block is free; it returns there when the last frame is released.
 static piv_status batch_storage(piv_analyzer *a, size_t bytes, std::shared_ptr<DevBuf> &store)

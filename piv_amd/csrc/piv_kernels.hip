@@ -342,16 +342,22 @@ __global__ void k_base(const void *__restrict__ raw, int pitch, long raw_frame_b
 // columns), so every lane carries one chain.
 constexpr int SAT_SW = 32, SAT_BAND = 32, SATK_TW = 64;
 
+// Frames of a launch at their own addresses (a serving batch's frames in their request
+// slots): frame blockIdx.y reads img[y] and writes the tables at sum[y] (entry (0, 0) of the
+// sums; the squares sat_table_elems(H, W) entries later).  Passed by value.
+constexpr int SAT_LIST_MAX = 64;
+struct SatList {
+    const void *img[SAT_LIST_MAX];
+    double *sum[SAT_LIST_MAX];
+};
+
+// one frame's carries; carry: [nb boundaries][2: sum, square][H rows], boundary b holding
+// the running sums through column SAT_SW * (b + 1) - 1
 template <int FMT>
-__global__ __launch_bounds__(64) void k_sat_carry(const void *__restrict__ img, int pitch,
-                                                  long frame_bytes, int H, int W,
-                                                  double *__restrict__ carry, int nb)
+__device__ __forceinline__ void sat_carry_frame(const void *__restrict__ img, int pitch, int H,
+                                                int W, double *__restrict__ carry, int nb)
 {
-    // carry: per frame [nb boundaries][2: sum, square][H rows]; boundary b holds the running
-    // sums through column SAT_SW * (b + 1) - 1
     __shared__ float tin[SAT_BAND][SATK_TW + 1];
-    img = static_cast<const char *>(img) + (long)blockIdx.y * frame_bytes;
-    carry += (long)blockIdx.y * nb * 2 * H;
     const int lane = threadIdx.x, r = lane & 31, sq = lane >> 5, y0 = blockIdx.x * SAT_BAND;
     const int rows = min(SAT_BAND, H - y0);
     const int ntile = (nb * SAT_SW + SATK_TW - 1) / SATK_TW;
@@ -393,19 +399,33 @@ __global__ __launch_bounds__(64) void k_sat_carry(const void *__restrict__ img, 
 }
 
 template <int FMT>
-__global__ __launch_bounds__(64) void k_sat_strips(const void *__restrict__ img, int pitch,
-                                                   long frame_bytes, int H, int W,
-                                                   const double *__restrict__ carry, int nb,
-                                                   double *__restrict__ sum,
-                                                   double *__restrict__ sumsq, long sat_stride)
+__global__ __launch_bounds__(64) void k_sat_carry(const void *__restrict__ img, int pitch,
+                                                  long frame_bytes, int H, int W,
+                                                  double *__restrict__ carry, int nb)
+{
+    sat_carry_frame<FMT>(static_cast<const char *>(img) + (long)blockIdx.y * frame_bytes, pitch, H,
+                         W, carry + (long)blockIdx.y * nb * 2 * H, nb);
+}
+
+template <int FMT>
+__global__ __launch_bounds__(64) void k_sat_carry_list(SatList list, int pitch, int H, int W,
+                                                       double *__restrict__ carry, int nb)
+{
+    sat_carry_frame<FMT>(list.img[blockIdx.y], pitch, H, W, carry + (long)blockIdx.y * nb * 2 * H, nb);
+}
+
+// one frame's tables, strip blockIdx.x
+template <int FMT>
+__device__ __forceinline__ void sat_strips_frame(const void *__restrict__ img, int pitch, int H,
+                                                 int W, const double *__restrict__ carry, int nb,
+                                                 double *__restrict__ sum,
+                                                 double *__restrict__ sumsq)
 {
     __shared__ float tin[SAT_BAND][SAT_SW + 1];
     __shared__ double tv[2][SAT_BAND][SAT_SW + 1];
-    img = static_cast<const char *>(img) + (long)blockIdx.y * frame_bytes;
-    carry += (long)blockIdx.y * nb * 2 * H;
     const int lane = threadIdx.x, c = lane & 31, sq = lane >> 5, strip = blockIdx.x;
     // this lane's table: sums (lanes 0-31) or squares (32-63)
-    double *tab = (sq ? sumsq : sum) + (long)blockIdx.y * sat_stride;
+    double *tab = sq ? sumsq : sum;
     const long s = (long)sat_pitch(W);
     const int x = strip * SAT_SW + c;  // sample column of table column x + 1
     if (x < W) tab[x + 1] = 0.0;       // row 0
@@ -471,6 +491,28 @@ __global__ __launch_bounds__(64) void k_sat_strips(const void *__restrict__ img,
             }
         }
     }
+}
+
+template <int FMT>
+__global__ __launch_bounds__(64) void k_sat_strips(const void *__restrict__ img, int pitch,
+                                                   long frame_bytes, int H, int W,
+                                                   const double *__restrict__ carry, int nb,
+                                                   double *__restrict__ sum,
+                                                   double *__restrict__ sumsq, long sat_stride)
+{
+    const long y = blockIdx.y;
+    sat_strips_frame<FMT>(static_cast<const char *>(img) + y * frame_bytes, pitch, H, W,
+                          carry + y * nb * 2 * H, nb, sum + y * sat_stride, sumsq + y * sat_stride);
+}
+
+template <int FMT>
+__global__ __launch_bounds__(64) void k_sat_strips_list(SatList list, int pitch, int H, int W,
+                                                        const double *__restrict__ carry, int nb)
+{
+    const long y = blockIdx.y;
+    double *sum = list.sum[y];
+    sat_strips_frame<FMT>(list.img[y], pitch, H, W, carry + y * nb * 2 * H, nb, sum,
+                          sum + sat_table_elems(H, W));
 }
 
 // gaussian_blur rows: sum over taps in kernel order, clamped edges
@@ -849,6 +891,39 @@ hipError_t launch_sat_raw(int fmt, const void *raw, int pitch, long raw_frame_by
         case FMT_U8: hipLaunchKernelGGL(k_sat_strips<FMT_U8>, grid, dim3(64), 0, st, r, pitch, raw_frame_bytes, H, W, carry, nb, su, sqq, sat_stride); break;
         case FMT_U16: hipLaunchKernelGGL(k_sat_strips<FMT_U16>, grid, dim3(64), 0, st, r, pitch, raw_frame_bytes, H, W, carry, nb, su, sqq, sat_stride); break;
         default: hipLaunchKernelGGL(k_sat_strips<FMT_F32>, grid, dim3(64), 0, st, r, pitch, raw_frame_bytes, H, W, carry, nb, su, sqq, sat_stride); break;
+        }
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_sat_list(int fmt, const void *const *imgs, double *const *sums, int nframes,
+                           int pitch, int H, int W, double *carry, hipStream_t st)
+{
+    const int nb = sat_boundaries(W);
+    for (int f0 = 0; f0 < nframes; f0 += SAT_LIST_MAX) {
+        const int nf = min(SAT_LIST_MAX, nframes - f0);
+        SatList list{};
+        for (int i = 0; i < nf; ++i) {
+            list.img[i] = imgs[f0 + i];
+            list.sum[i] = sums[f0 + i];
+        }
+        if (nb > 0) {
+            const dim3 grid(nblk(H, SAT_BAND), (unsigned)nf);
+            switch (fmt) {
+            case FMT_U8: hipLaunchKernelGGL(k_sat_carry_list<FMT_U8>, grid, dim3(64), 0, st, list, pitch, H, W, carry, nb); break;
+            case FMT_U16: hipLaunchKernelGGL(k_sat_carry_list<FMT_U16>, grid, dim3(64), 0, st, list, pitch, H, W, carry, nb); break;
+            default: hipLaunchKernelGGL(k_sat_carry_list<FMT_F32>, grid, dim3(64), 0, st, list, pitch, H, W, carry, nb); break;
+            }
+            const hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+        }
+        const dim3 grid(nblk(W, SAT_SW), (unsigned)nf);
+        switch (fmt) {
+        case FMT_U8: hipLaunchKernelGGL(k_sat_strips_list<FMT_U8>, grid, dim3(64), 0, st, list, pitch, H, W, carry, nb); break;
+        case FMT_U16: hipLaunchKernelGGL(k_sat_strips_list<FMT_U16>, grid, dim3(64), 0, st, list, pitch, H, W, carry, nb); break;
+        default: hipLaunchKernelGGL(k_sat_strips_list<FMT_F32>, grid, dim3(64), 0, st, list, pitch, H, W, carry, nb); break;
         }
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;

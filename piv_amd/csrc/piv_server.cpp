@@ -467,18 +467,24 @@ piv_status launch_batch(piv_server *s, Lane &L)
 // for a slow upload of another stream.
 bool upload_done(Request *r) { return hipEventQuery(r->slot->uploaded) == hipSuccess; }
 
-// A request's conditioning and tables, launched on its stream's upload stream after its
-// upload; `uploaded` is recorded again after them, and the lane that runs the request waits
-// for it.
-piv_status prepare_request(Request *r)
+// Conditioning and tables of requests of one stream, launched on its upload stream after
+// their uploads (the tables of all their frames in one launch per kernel where the stream's
+// configuration allows); `uploaded` is recorded again after them, and the lane that runs a
+// request waits for it.
+piv_status prepare_requests(SrvStream *rs, const std::vector<Request *> &reqs)
 {
-    SrvStream *rs = r->st;
     std::lock_guard<std::mutex> ul(rs->upload_mu);
     const float *bg = rs->has_bg ? rs->bg.as<float>() : nullptr;
-    PIV_TRY(prepare_frame_pair(rs->an, r->slot->fa, r->slot->fb, bg));
-    // raw frames without tables need no device work: the upload's event stands
+    std::vector<piv_frame *> fs;
+    fs.reserve(2 * reqs.size());
+    for (Request *r : reqs) {
+        fs.push_back(r->slot->fa);
+        fs.push_back(r->slot->fb);
+    }
+    PIV_TRY(prepare_frames_list(rs->an, fs.data(), (int)fs.size(), bg));
+    // raw frames without tables need no device work: the uploads' events stand
     if (prepare_launches(rs->an->cfg, rs->fmt, bg != nullptr))
-        HIP_TRY(hipEventRecord(r->slot->uploaded, rs->an->stream));
+        for (Request *r : reqs) HIP_TRY(hipEventRecord(r->slot->uploaded, rs->an->stream));
     return PIV_OK;
 }
 
@@ -511,12 +517,21 @@ void scheduler_loop(piv_server *s)
             if (!r->prepared && upload_done(r)) prep.push_back(r);
         if (!prep.empty()) {
             lk.unlock();
-            std::vector<piv_status> ps(prep.size());
-            for (size_t i = 0; i < prep.size(); ++i) ps[i] = prepare_request(prep[i]);
+            // stream by stream, oldest stream first
+            std::vector<SrvStream *> order;
+            for (Request *r : prep)
+                if (std::find(order.begin(), order.end(), r->st) == order.end()) order.push_back(r->st);
+            std::vector<Request *> group;
+            for (SrvStream *rs : order) {
+                group.clear();
+                for (Request *r : prep)
+                    if (r->st == rs) group.push_back(r);
+                const piv_status ps = prepare_requests(rs, group);
+                for (Request *r : group) r->status = ps;
+            }
             lk.lock();
-            for (size_t i = 0; i < prep.size(); ++i) {
-                prep[i]->prepared = true;
-                prep[i]->status = ps[i];
+            for (Request *r : prep) {
+                r->prepared = true;
                 s->unprepared -= 1;
             }
         }
@@ -718,7 +733,7 @@ piv_status piv_server_add_stream(piv_server *s, const piv_config *cfg, uint32_t 
     if (r == PIV_OK) {
         const size_t n = (size_t)height * width, cells = (size_t)sat_table_elems(st->H, st->W);
         const piv_config &c = st->an->cfg;
-        r = st->an->sat_carry.reserve(sat_carry_bytes(st->H, st->W, 2));
+        r = st->an->sat_carry.reserve(sat_carry_bytes(st->H, st->W, 64));
         if (r == PIV_OK && c.smooth_sigma > 0.0f) r = st->an->prep_tmp.reserve(n * sizeof(float));
         if (r == PIV_OK && c.smooth_sigma > 0.0f && c.normalize_radius > 0)
             r = st->an->prep_tmp2.reserve(n * sizeof(float));
