@@ -244,44 +244,6 @@ piv_status prepare_frame(piv_analyzer *a, piv_frame *f, const float *bg_dev)
     return PIV_OK;
 }
 
-// The two frames of one request (serving).  Without conditioning both tables come from one
-// launch per kernel: the launch's frame strides are the distances between the two frames'
-// samples and between their tables, wherever each lies.
-piv_status prepare_frame_pair(piv_analyzer *a, piv_frame *fa, piv_frame *fb, const float *bg_dev)
-{
-    const piv_config &c = a->cfg;
-    const bool cond = c.smooth_sigma > 0.0f || c.normalize_radius > 0 || bg_dev != nullptr;
-    const bool mask = mask_needed(c, fa->in_fmt, bg_dev != nullptr);
-    if (cond || !mask || fa->in_fmt != fb->in_fmt || fa->H != fb->H || fa->W != fb->W ||
-        fa->raw_pitch != fb->raw_pitch) {
-        PIV_TRY(prepare_frame(a, fa, bg_dev));
-        return prepare_frame(a, fb, bg_dev);
-    }
-    const int H = fa->H, W = fa->W;
-    const int64_t cells = sat_table_elems(H, W);
-    double *sum[2];
-    piv_frame *const fs[2] = {fa, fb};
-    for (int i = 0; i < 2; ++i) {
-        piv_frame *f = fs[i];
-        PIV_TRY(frame_reserve(a, f->sat, (size_t)cells * 2 * sizeof(double)));
-        sum[i] = f->sat.as<double>() + SAT_LEAD;
-        f->sat_ptr = sum[i];
-        f->has_sat = true;
-        f->has_bg = false;
-        f->cells = cells;
-        f->base_ptr = nullptr;
-        f->fmt = f->in_fmt;
-        f->img = f->raw_ptr;
-        f->pitch = f->raw_pitch;
-    }
-    const long frame_bytes = (long)((intptr_t)fb->raw_ptr - (intptr_t)fa->raw_ptr);
-    const long sat_stride = (long)(((intptr_t)sum[1] - (intptr_t)sum[0]) / (intptr_t)sizeof(double));
-    PIV_TRY(a->sat_carry.reserve(sat_carry_bytes(H, W, 2)));
-    HIP_TRY(launch_sat_raw(fa->in_fmt, fa->raw_ptr, fa->raw_pitch, frame_bytes, H, W, sum[0],
-                           sum[0] + cells, a->sat_carry.as<double>(), a->stream, 2, sat_stride));
-    return PIV_OK;
-}
-
 // Frames of one size and format at their own addresses (the requests of a serving batch),
 // prepared together: without conditioning, their tables come from one launch per kernel for
 // up to 64 frames (launch_sat_list); otherwise frame by frame.

@@ -272,7 +272,6 @@ int variant_of(const piv_analyzer *a, int n, int fmt);
 int mf_precision(const piv_analyzer *a, int n);
 piv_status check_config(const piv_config &c);
 piv_status prepare_frame(piv_analyzer *a, piv_frame *f, const float *bg_dev);
-piv_status prepare_frame_pair(piv_analyzer *a, piv_frame *fa, piv_frame *fb, const float *bg_dev);
 piv_status prepare_frames_list(piv_analyzer *a, piv_frame *const *fs, int count, const float *bg_dev);
 // whether preparing a frame of this format launches device work (conditioning or tables)
 bool prepare_launches(const piv_config &c, int in_fmt, bool has_bg);
