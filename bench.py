@@ -108,13 +108,17 @@ def serve_bench(rate, seconds, backlog, seed=77):
     """Continuous-batching server on the mixed workload: p50/p99 latency (arrival to result
     on the host) under Poisson arrivals at `rate` pairs/s, then the saturated rate with a
     backlog of `backlog` requests (host staging and PCIe upload included)."""
+    import gc
     import threading
 
     import piv_amd
     from piv_amd import synthetic as S
 
-    # the load generator and the result collector are Python threads: short GIL slices
+    # the load generator and the result collector are Python threads: short GIL slices, and
+    # no garbage-collection pauses inside the measured phases (as timeit does)
     sys.setswitchinterval(2e-4)
+    gc.collect()
+    gc.disable()
     srv = piv_amd.PivServer(hbm_budget=16 << 30, max_batch_pairs=64, max_inflight=2)
     streams = []
     for k, sp in enumerate(SERVE_STREAMS):
@@ -243,14 +247,24 @@ def serve_bench(rate, seconds, backlog, seed=77):
     b0 = srv.stats()
     sat_host, b1 = saturate(False)
     sat_host["note"] = "host frames: staging copy and PCIe upload included"
-    sat_dev, b2 = saturate(True)
-    sat_dev["note"] = "frames resident in HBM, the backlog submitted with piv_submit_device_many"
     sat_host["mean_batch_pairs"] = round((b1["batched_pairs"] - b0["batched_pairs"]) /
                                          max(b1["batches"] - b0["batches"], 1), 2)
-    sat_dev["mean_batch_pairs"] = round((b2["batched_pairs"] - b1["batched_pairs"]) /
-                                        max(b2["batches"] - b1["batches"], 1), 2)
+    # three backlogs with frames in HBM: the median is reported, all three beside it
+    runs = []
+    for _ in range(3):
+        r, b2 = saturate(True)
+        r["mean_batch_pairs"] = round((b2["batched_pairs"] - b1["batched_pairs"]) /
+                                      max(b2["batches"] - b1["batches"], 1), 2)
+        runs.append(r)
+        b1 = b2
+    runs.sort(key=lambda r: r["pairs_per_s"])
+    sat_dev = dict(runs[1])
+    sat_dev["runs_pairs_per_s"] = [r["pairs_per_s"] for r in runs]
+    sat_dev["note"] = ("frames resident in HBM, the backlog submitted with piv_submit_device_many; "
+                       "median of three backlogs")
     stop.set()
     th.join()
+    gc.enable()
     st = srv.stats()
     srv.close()
     return {"streams": [sp["name"] for sp in SERVE_STREAMS],
@@ -568,7 +582,8 @@ def main():
                     help="analyse one batch, then measure only the post-analysis stages")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the reduced-precision (DFT-GEMM) measurement")
-    ap.add_argument("--serve-backlog", type=int, default=400)
+    ap.add_argument("--serve-backlog", type=int, default=400,
+                    help="saturation backlog (400 fit the 16 GiB serving budget together)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default correlation-kernel variants")
     ap.add_argument("--only-batched", action="store_true",
