@@ -5,7 +5,8 @@ Mask thresholds are set to window means and standard deviations the oracle compu
 window exactly at a threshold passes only if the device computed the identical f32 value: a
 one-ulp difference in any table entry a window reads flips a decision.  Frame shapes cover
 partial 32-column strips and 32-row bands, frames no wider than one strip and frames an exact
-multiple of both; the single-frame, batched and served (paired) preparations all run."""
+multiple of both; the single-frame, batched and served (grouped) preparations all run, and
+frames prepared in memory that released frames left in the analyzer's pool."""
 import numpy as np
 import pytest
 
@@ -105,9 +106,9 @@ def test_batch_tables_decide_the_mask_like_the_oracle(piv, oracle, shape, dtype)
 
 
 def test_served_pair_tables_match_single_frames(piv, oracle):
-    """A served request's two frames get their tables from one launch: with a mask at the
-    median window mean, every served field equals the pair analysed alone, and the mask
-    splits the windows."""
+    """The requests a server prepares together get their tables from one launch per kernel:
+    with a mask at the median window mean, every served field equals the pair analysed alone,
+    and the mask splits the windows."""
     h, w = 97, 130
     frames = _frames(h, w, np.uint16, 99, count=6)
     norm = np.stack(frames).astype(np.float32) / np.float32(65535)
@@ -132,3 +133,37 @@ def test_served_pair_tables_match_single_frames(piv, oracle):
                               np.ascontiguousarray(ref.field.dx).view(np.uint32))
         masked = (ref.field.flags & NO_SIGNAL) != 0
         assert 0 < masked.sum() < masked.size
+
+
+@pytest.mark.parametrize("pre", ["raw", "default"])
+def test_reused_frame_memory_gives_fresh_results(piv, pre):
+    """Frames released to the analyzer's pool leave nothing behind: frames prepared afterwards
+    in the reused memory (single and batched, tables and conditioned images) analyse bitwise
+    as with a fresh analyzer."""
+    h, w = 97, 130
+    first = _frames(h, w, np.uint16, 1, count=6)
+    second = _frames(h, w, np.uint16, 2, count=6)
+    norm = np.stack(second).astype(np.float32) / np.float32(65535)
+    cfg = piv.PivConfig(windows=[24], final_iterations=1,
+                        mask=piv.MaskConfig(float(np.float32(np.median(norm))), 0.0))
+    prep = piv.PreprocessConfig.raw() if pre == "raw" else piv.PreprocessConfig()
+    an = piv.PivAnalyzer(cfg, prep)
+    # fill the pool: single frames and a batch, analysed, then released
+    old = [an.prepare(f) for f in first]
+    an.analyze(old[0], old[1])
+    del old
+    stack1 = piv.DeviceArray.from_numpy(np.stack(first))
+    old = piv.PreparedFrame.from_device_batch(an, stack1)
+    an.analyze(old[2], old[3])
+    del old
+    fresh = piv.PivAnalyzer(cfg, prep)
+    stack2 = piv.DeviceArray.from_numpy(np.stack(second))
+    got_single = [an.prepare(f) for f in second[:2]]
+    got_batch = piv.PreparedFrame.from_device_batch(an, stack2)
+    ref = fresh.analyze(fresh.prepare(second[0]), fresh.prepare(second[1]))
+    for r in (an.analyze(got_single[0], got_single[1]), an.analyze(got_batch[0], got_batch[1])):
+        assert np.array_equal(r.field.flags, ref.field.flags)
+        assert np.array_equal(np.ascontiguousarray(r.field.dx).view(np.uint32),
+                              np.ascontiguousarray(ref.field.dx).view(np.uint32))
+    masked = (ref.field.flags & NO_SIGNAL) != 0
+    assert 0 < masked.sum() < masked.size
