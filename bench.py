@@ -32,6 +32,7 @@ sys.path.insert(0, ROOT)
 H = W = 1024
 WINDOWS = [64, 32, 16]
 HBM_PEAK_GBS = 8000.0
+HOST_SUBMITTERS = 4  # client threads submitting host frames in the saturated serving run
 
 
 def make_pairs(count, seed0=2024):
@@ -215,16 +216,29 @@ def serve_bench(rate, seconds, backlog, seed=77):
                 if reqs:
                     time.sleep(0.0005)
         else:
-            for i, k in enumerate(kinds):
-                sid, pairs, gb, _ = streams[k]
-                a, b = pairs[i % 2]
-                while True:
-                    try:
-                        srv.submit(sid, a, b)
-                        break
-                    except piv_amd.AdmissionError:
-                        time.sleep(0.0005)
-                submitted[0] += 1
+            # several submitting threads, as several clients would: each request's staging
+            # copy runs outside the server's locks (and ctypes releases the GIL), so the
+            # copies of different requests overlap
+            def submitter(part):
+                for i, k in part:
+                    sid, pairs, gb, _ = streams[k]
+                    a, b = pairs[i % 2]
+                    while True:
+                        try:
+                            srv.submit(sid, a, b)
+                            break
+                        except piv_amd.AdmissionError:
+                            time.sleep(0.0005)
+                    with lock:
+                        submitted[0] += 1
+
+            items = list(enumerate(kinds))
+            ths = [threading.Thread(target=submitter, args=(items[j::HOST_SUBMITTERS],))
+                   for j in range(HOST_SUBMITTERS)]
+            for t in ths:
+                t.start()
+            for t in ths:
+                t.join()
         while done_count[0] < submitted[0]:
             time.sleep(0.0005)
         el = time.perf_counter() - t0
@@ -246,7 +260,8 @@ def serve_bench(rate, seconds, backlog, seed=77):
     saturate(True)
     b0 = srv.stats()
     sat_host, b1 = saturate(False)
-    sat_host["note"] = "host frames: staging copy and PCIe upload included"
+    sat_host["note"] = ("host frames: staging copy and PCIe upload included, submitted from "
+                        f"{HOST_SUBMITTERS} threads")
     sat_host["mean_batch_pairs"] = round((b1["batched_pairs"] - b0["batched_pairs"]) /
                                          max(b1["batches"] - b0["batches"], 1), 2)
     # three backlogs with frames in HBM: the median is reported, all three beside it
@@ -578,6 +593,8 @@ def main():
                     help="measure only piv-analyze end to end on a clip of this many frames")
     ap.add_argument("--only-video", type=int, default=0, metavar="PAIRS",
                     help="measure only the batched-video workload over this many pairs")
+    ap.add_argument("--only-serve", action="store_true",
+                    help="measure only the serving loop (Poisson arrivals and saturation)")
     ap.add_argument("--only-post", action="store_true",
                     help="analyse one batch, then measure only the post-analysis stages")
     ap.add_argument("--no-precision", action="store_true",
@@ -606,6 +623,11 @@ def main():
 
     if args.only_cli:
         print(json.dumps(cli_bench(frames=args.only_cli)), flush=True)
+        return
+
+    if args.only_serve:
+        print(json.dumps(serve_bench(args.serve_rate, args.serve_seconds, args.serve_backlog)),
+              flush=True)
         return
 
     if args.only_video:

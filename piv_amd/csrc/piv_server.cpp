@@ -812,17 +812,19 @@ piv_status submit_impl(piv_server *s, uint32_t stream_id, const void *pa, const 
     lk.unlock();
     piv_status r = PIV_OK;
     {
-        std::lock_guard<std::mutex> ul(st->upload_mu);
         hipStream_t us = st->an->stream;
         if (on_device) {
             // the caller's samples are read in place until the result is released
+            std::lock_guard<std::mutex> ul(st->upload_mu);
             sl->fa->raw_ptr = pa;
             sl->fb->raw_ptr = pb;
             sl->fa->raw_pitch = sl->fb->raw_pitch = (int)(row_stride / bpp);
+            if (hipEventRecord(sl->uploaded, us) != hipSuccess) r = fail(PIV_ERR_DEVICE, "event record failed");
         } else {
-            // staged: the caller's buffers are free when this returns.  Copied into pinned
-            // staging in chunks of about 1 MiB, each chunk's upload queued as soon as it is
-            // staged, so the DMA of one chunk overlaps the host copy of the next.
+            // staged: the caller's buffers are free when this returns.  Copied into the slot's
+            // own pinned staging in chunks of about 1 MiB without the stream's lock (requests of
+            // one stream stage in parallel from several threads); each chunk's upload is queued
+            // as soon as it is staged, so the DMA of one chunk overlaps the copy of the next.
             const int chunk_rows = (int)std::max<size_t>(1, ((size_t)1 << 20) / rowb);
             const void *src[2] = {pa, pb};
             char *stage[2] = {sl->h_raw_a, sl->h_raw_b};
@@ -833,19 +835,21 @@ piv_status submit_impl(piv_server *s, uint32_t stream_id, const void *pa, const 
                     for (int h = h0; h < h1; ++h)
                         std::memcpy(stage[i] + h * rowb,
                                     static_cast<const char *>(src[i]) + h * row_stride, rowb);
+                    std::lock_guard<std::mutex> ul(st->upload_mu);
                     if (hipMemcpyAsync(dst[i] + h0 * rowb, stage[i] + h0 * rowb,
                                        (size_t)(h1 - h0) * rowb, hipMemcpyHostToDevice,
                                        us) != hipSuccess)
                         r = fail(PIV_ERR_DEVICE, "upload failed");
                 }
+            std::lock_guard<std::mutex> ul(st->upload_mu);
             sl->fa->raw_ptr = sl->raw_a;
             sl->fb->raw_ptr = sl->raw_b;
             sl->fa->raw_pitch = sl->fb->raw_pitch = st->W;
+            // conditioning and tables are launched by the scheduler when the request is taken
+            // up (one thread issues those launches)
+            if (r == PIV_OK && hipEventRecord(sl->uploaded, us) != hipSuccess)
+                r = fail(PIV_ERR_DEVICE, "event record failed");
         }
-        // conditioning and tables are launched by the scheduler when the request joins a batch
-        // (one thread issues all of a batch's launches)
-        if (r == PIV_OK && hipEventRecord(sl->uploaded, us) != hipSuccess)
-            r = fail(PIV_ERR_DEVICE, "event record failed");
     }
     lk.lock();
     if (r != PIV_OK) {

@@ -210,3 +210,47 @@ def test_device_batch_submission(piv):
     assert len(rest) == 1
     assert len(srv.poll(1, wait_ms=10000)) == 1
     srv.close()
+
+
+def test_concurrent_host_submission(piv):
+    """Four client threads submit host frames to the same streams at once: the staging copies
+    of different requests run concurrently, each into its own request slot, and every result
+    equals the pair analysed alone."""
+    import threading
+
+    srv = piv.PivServer(max_batch_pairs=8, max_inflight=2)
+    specs = _streams(piv)
+    sids = [srv.add_stream(cfg, h, w, dt, pre, bg) for cfg, pre, h, w, dt, bg, _ in specs]
+    jobs = []
+    for rnd in range(6):
+        for si, (cfg, pre, h, w, dt, bg, b) in enumerate(specs):
+            jobs.append((si,) + _pair(h, w, dt, b, 3000 + 10 * rnd + si))
+    pairs, lock, errors = {}, threading.Lock(), []
+
+    def client(part):
+        try:
+            for si, a_, b_ in part:
+                rid = srv.submit(sids[si], a_, b_)
+                with lock:
+                    pairs[rid] = (si, a_, b_)
+        except Exception as e:  # surfaced below
+            errors.append(e)
+
+    ths = [threading.Thread(target=client, args=(jobs[j::4],)) for j in range(4)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    assert not errors, errors
+    got = {}
+    while len(got) < len(jobs):
+        res = srv.poll(64, wait_ms=10000)
+        assert res, "no result within 10 s"
+        for r in res:
+            assert r.error is None, r.error
+            got[r.request_id] = r
+    assert sorted(got) == sorted(pairs)
+    for rid, (si, a_, b_) in pairs.items():
+        cfg, pre, h, w, dt, bg, _ = specs[si]
+        _same(got[rid], _alone(piv, cfg, pre, a_, b_, bg))
+    srv.close()
