@@ -255,7 +255,7 @@ def serve_bench(rate, seconds, backlog, seed=77):
     dev_pairs = [[(torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()) for a, b in s[1]]
                  for s in streams]
     torch.cuda.synchronize()
-    # one unmeasured backlog first: request slots for a full backlog are allocated and the
+    # one unmeasured backlog first: request slots up to the HBM budget are allocated and the
     # batch shapes' graphs captured once, as in a server that has been running a while
     saturate(True)
     b0 = srv.stats()
@@ -599,8 +599,9 @@ def main():
                     help="analyse one batch, then measure only the post-analysis stages")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the reduced-precision (DFT-GEMM) measurement")
-    ap.add_argument("--serve-backlog", type=int, default=400,
-                    help="saturation backlog (400 fit the 16 GiB serving budget together)")
+    ap.add_argument("--serve-backlog", type=int, default=2000,
+                    help="saturation backlog (about five times what the 16 GiB serving budget "
+                         "admits at once: submissions past it are refused and retried)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default correlation-kernel variants")
     ap.add_argument("--only-batched", action="store_true",

@@ -250,7 +250,10 @@ piv_status piv_profile_read(piv_analyzer *a, piv_kernel_stat *out, uint32_t capa
 typedef struct piv_server piv_server;
 
 typedef struct {
-    size_t hbm_budget;         /* device bytes requests may hold at once (0: 4 GiB) */
+    size_t hbm_budget;         /* device bytes request slots may hold at once (0: 4 GiB).  A
+                                * released slot stays cached for its stream; another stream
+                                * takes it over the budget only while the owner caches more
+                                * slots than it has requests in flight. */
     uint32_t max_batch_pairs;  /* pairs per batch (0: 64) */
     uint32_t max_inflight;     /* batches on the device at once, one HIP stream each (0: 2) */
 } piv_server_config;

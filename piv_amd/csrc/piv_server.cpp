@@ -60,6 +60,7 @@ struct SrvStream {
     std::vector<PassPlan> plans;
     size_t slot_dev = 0, slot_host = 0;
     std::vector<Slot *> free_slots;
+    int in_use = 0;        // slots held by admitted requests not yet released
     std::mutex upload_mu;  // orders uploads + conditioning on the analyzer's stream
 };
 
@@ -140,12 +141,20 @@ void free_slot(piv_server *s, Slot *sl)
 }
 
 // Drop cached free slots (largest streams first) until `need` more bytes fit the budget.
+// A stream gives up free slots only while it holds more of them than it has requests in
+// flight: a stream being served reuses a free slot for its next request as soon as one of
+// its results is released, and taking those slots would only have it reallocate them
+// (hipFree / hipMalloc, each synchronising) a moment later.  Past that, the submission is
+// refused and retried by the client once results are released; slots left over from an
+// earlier burst still go to other streams.
 bool make_room(piv_server *s, size_t need)
 {
     while (s->allocated + need > s->cfg.hbm_budget) {
         SrvStream *best = nullptr;
         for (SrvStream *st : s->streams)
-            if (!st->free_slots.empty() && (!best || st->slot_dev > best->slot_dev)) best = st;
+            if (st->free_slots.size() > (size_t)st->in_use &&
+                (!best || st->slot_dev > best->slot_dev))
+                best = st;
         if (!best) return false;
         Slot *sl = best->free_slots.back();
         best->free_slots.pop_back();
@@ -809,6 +818,7 @@ piv_status submit_impl(piv_server *s, uint32_t stream_id, const void *pa, const 
         }
         PIV_TRY(new_slot(s, st, &sl));
     }
+    st->in_use += 1;
     lk.unlock();
     piv_status r = PIV_OK;
     {
@@ -854,6 +864,7 @@ piv_status submit_impl(piv_server *s, uint32_t stream_id, const void *pa, const 
     lk.lock();
     if (r != PIV_OK) {
         st->free_slots.push_back(sl);
+        st->in_use -= 1;
         return r;
     }
     auto *req = new Request();
@@ -943,7 +954,10 @@ piv_status piv_result_release(piv_server *s, uint64_t request_id)
                                            (unsigned long long)request_id);
     Request *r = it->second;
     s->polled.erase(it);
-    if (r->slot) r->st->free_slots.push_back(r->slot);
+    if (r->slot) {
+        r->st->free_slots.push_back(r->slot);
+        r->st->in_use -= 1;
+    }
     delete r;
     s->cv_sched.notify_one();
     return PIV_OK;

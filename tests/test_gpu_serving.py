@@ -254,3 +254,53 @@ def test_concurrent_host_submission(piv):
         cfg, pre, h, w, dt, bg, _ = specs[si]
         _same(got[rid], _alone(piv, cfg, pre, a_, b_, bg))
     srv.close()
+
+
+def test_free_slots_move_between_streams_only_when_idle(piv):
+    """Over the HBM budget, a stream's cached free slots go to another stream only while the
+    stream holds more free slots than requests in flight; otherwise the submission is refused
+    (and succeeds once results are released), so a saturated mix does not reallocate slots
+    back and forth."""
+    cfg = piv.PivConfig(windows=[32], final_iterations=1)
+    raw = piv.PreprocessConfig.raw()
+    a, b = S.particle_frames(256, 256, 5, field=S.uniform_field(1.5, 0.5))
+    probe = piv.PivServer()
+    psid = probe.add_stream(cfg, 256, 256, np.uint8, raw)
+    probe.submit(psid, a, b)
+    probe.poll(1, wait_ms=5000)
+    one = probe.stats()["hbm_peak"]
+    probe.close()
+
+    def drain(srv, n):
+        got = []
+        while len(got) < n:
+            res = srv.poll(8, wait_ms=5000)
+            assert res, "no result within 5 s"
+            assert all(r.error is None for r in res)
+            got += res
+        return got
+
+    srv = piv.PivServer(hbm_budget=3 * one)
+    sa = srv.add_stream(cfg, 256, 256, np.uint8, raw)
+    sb = srv.add_stream(cfg, 256, 256, np.uint8, raw)
+    srv.pause(True)
+    for _ in range(3):
+        srv.submit(sa, a, b)
+    with pytest.raises(piv.AdmissionError):  # stream a holds the budget, all in flight
+        srv.submit(sb, a, b)
+    srv.pause(False)
+    drain(srv, 3)  # stream a: three free slots, none in flight
+    srv.submit(sb, a, b)  # takes one of them
+    srv.pause(True)
+    srv.submit(sa, a, b)  # stream a: one free slot, one request in flight
+    with pytest.raises(piv.AdmissionError):
+        srv.submit(sb, a, b)
+    srv.pause(False)
+    drain(srv, 2)
+    srv.submit(sb, a, b)
+    srv.submit(sb, a, b)  # stream a is idle again: its free slot moves to stream b
+    drain(srv, 2)
+    st = srv.stats()
+    assert st["rejected"] == 2
+    assert st["hbm_peak"] <= 3 * one
+    srv.close()
