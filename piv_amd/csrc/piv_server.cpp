@@ -15,6 +15,7 @@
 // Up to max_inflight batches are on the device at once (one HIP stream per lane), so a batch
 // admitted while another runs overlaps with it.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -97,6 +98,83 @@ struct Lane {
 
 constexpr size_t kMaxGraphsPerLane = 64;
 constexpr size_t kMaxQueued = 1 << 16;
+// Host pairs of at least this many bytes are staged by the submitting thread together with
+// the server's staging helpers (a single thread copies pageable memory at about 16 GB/s, so
+// one 2048x2048 u16 pair alone would take a millisecond).
+constexpr size_t kSplitStageBytes = (size_t)3 << 20;
+constexpr int kStageHelpers = 3;
+
+// One request's staging copy split into row chunks of about 1 MiB, claimed by whichever
+// thread (submitter or helper) asks next.  `done[k]` is set once chunk k is in the pinned
+// buffer; the submitter queues the uploads in chunk order as chunks complete.
+struct StageTask {
+    const char *src[2] = {nullptr, nullptr};
+    char *stage[2] = {nullptr, nullptr};
+    size_t row_stride = 0, rowb = 0;
+    int H = 0, chunk_rows = 1, per_frame = 0, total = 0;
+    std::atomic<int> next{0};
+    std::unique_ptr<std::atomic<int>[]> done;
+    // copies the next unclaimed chunk; false when every chunk has been claimed
+    bool run_one()
+    {
+        const int k = next.fetch_add(1, std::memory_order_relaxed);
+        if (k >= total) return false;
+        const int i = k / per_frame;
+        const int h0 = (k % per_frame) * chunk_rows, h1 = std::min(H, h0 + chunk_rows);
+        for (int h = h0; h < h1; ++h)
+            std::memcpy(stage[i] + (size_t)h * rowb, src[i] + (size_t)h * row_stride, rowb);
+        done[k].store(1, std::memory_order_release);
+        return true;
+    }
+};
+
+// Helper threads that join the staging of large host pairs.  A task stays at the front of
+// the queue until a helper finds every chunk claimed; the submitter holds its own reference
+// and returns only when all chunks are done, so helpers never touch a caller's buffer after
+// the call that passed it has returned.
+struct StagePool {
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<std::shared_ptr<StageTask>> tasks;
+    std::vector<std::thread> threads;
+    bool stop = false;
+    void start(int n)
+    {
+        for (int i = 0; i < n; ++i) threads.emplace_back([this] { loop(); });
+    }
+    void loop()
+    {
+        std::unique_lock<std::mutex> lk(mu);
+        for (;;) {
+            cv.wait(lk, [&] { return stop || !tasks.empty(); });
+            if (stop) return;
+            std::shared_ptr<StageTask> t = tasks.front();
+            lk.unlock();
+            while (t->run_one()) {
+            }
+            lk.lock();
+            if (!tasks.empty() && tasks.front() == t) tasks.pop_front();
+        }
+    }
+    void post(const std::shared_ptr<StageTask> &t)
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            tasks.push_back(t);
+        }
+        cv.notify_all();
+    }
+    void shutdown()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            stop = true;
+        }
+        cv.notify_all();
+        for (std::thread &th : threads) th.join();
+        threads.clear();
+    }
+};
 
 size_t fields_bytes(size_t c) { return align_up(c * 13, 256); }
 
@@ -120,6 +198,7 @@ struct piv_server {
     bool stop = false;
     bool paused = false;  // piv_server_pause: admitted requests wait, nothing is launched
     std::thread scheduler, completer;
+    StagePool stagers;    // helpers for the staging copies of large host pairs
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     double now_ms() const
     {
@@ -673,6 +752,7 @@ piv_status piv_server_create(int device, const piv_server_config *cfg, piv_serve
     }
     s->scheduler = std::thread(scheduler_loop, s);
     s->completer = std::thread(completer_loop, s);
+    s->stagers.start(kStageHelpers);
     *out = s;
     return PIV_OK;
 }
@@ -686,6 +766,7 @@ void piv_server_destroy(piv_server *s)
     }
     s->cv_sched.notify_all();
     s->cv_complete.notify_all();
+    s->stagers.shutdown();
     if (s->scheduler.joinable()) s->scheduler.join();
     // the completer drains every launched batch before it exits
     if (s->completer.joinable()) s->completer.join();
@@ -835,22 +916,47 @@ piv_status submit_impl(piv_server *s, uint32_t stream_id, const void *pa, const 
             // own pinned staging in chunks of about 1 MiB without the stream's lock (requests of
             // one stream stage in parallel from several threads); each chunk's upload is queued
             // as soon as it is staged, so the DMA of one chunk overlaps the copy of the next.
+            // Large pairs are copied by the server's staging helpers and this thread together.
             const int chunk_rows = (int)std::max<size_t>(1, ((size_t)1 << 20) / rowb);
-            const void *src[2] = {pa, pb};
-            char *stage[2] = {sl->h_raw_a, sl->h_raw_b};
+            const size_t pair_bytes = 2 * (size_t)st->H * rowb;
+            auto t = std::make_shared<StageTask>();
+            t->src[0] = static_cast<const char *>(pa);
+            t->src[1] = static_cast<const char *>(pb);
+            t->stage[0] = sl->h_raw_a;
+            t->stage[1] = sl->h_raw_b;
+            t->row_stride = row_stride;
+            t->rowb = rowb;
+            t->H = st->H;
+            t->chunk_rows = chunk_rows;
+            t->per_frame = (st->H + chunk_rows - 1) / chunk_rows;
+            t->total = 2 * t->per_frame;
+            t->done.reset(new std::atomic<int>[t->total]());
             char *dst[2] = {static_cast<char *>(sl->raw_a), static_cast<char *>(sl->raw_b)};
-            for (int i = 0; i < 2 && r == PIV_OK; ++i)
-                for (int h0 = 0; h0 < st->H && r == PIV_OK; h0 += chunk_rows) {
-                    const int h1 = std::min(st->H, h0 + chunk_rows);
-                    for (int h = h0; h < h1; ++h)
-                        std::memcpy(stage[i] + h * rowb,
-                                    static_cast<const char *>(src[i]) + h * row_stride, rowb);
-                    std::lock_guard<std::mutex> ul(st->upload_mu);
-                    if (hipMemcpyAsync(dst[i] + h0 * rowb, stage[i] + h0 * rowb,
-                                       (size_t)(h1 - h0) * rowb, hipMemcpyHostToDevice,
-                                       us) != hipSuccess)
-                        r = fail(PIV_ERR_DEVICE, "upload failed");
+            const bool split = pair_bytes >= kSplitStageBytes && !s->stagers.threads.empty();
+            if (split) s->stagers.post(t);
+            int queued = 0;
+            // queues the uploads of the staged chunks that follow the last one queued (after
+            // an upload error the remaining chunks are still waited for, not uploaded)
+            auto queue_staged = [&]() {
+                while (queued < t->total && t->done[queued].load(std::memory_order_acquire)) {
+                    if (r == PIV_OK) {
+                        const int i = queued / t->per_frame;
+                        const int h0 = (queued % t->per_frame) * chunk_rows;
+                        const int h1 = std::min(st->H, h0 + chunk_rows);
+                        std::lock_guard<std::mutex> ul(st->upload_mu);
+                        if (hipMemcpyAsync(dst[i] + (size_t)h0 * rowb, t->stage[i] + (size_t)h0 * rowb,
+                                           (size_t)(h1 - h0) * rowb, hipMemcpyHostToDevice,
+                                           us) != hipSuccess)
+                            r = fail(PIV_ERR_DEVICE, "upload failed");
+                    }
+                    ++queued;
                 }
+            };
+            while (t->run_one()) queue_staged();
+            while (queued < t->total) {
+                queue_staged();
+                if (queued < t->total) std::this_thread::yield();
+            }
             std::lock_guard<std::mutex> ul(st->upload_mu);
             sl->fa->raw_ptr = sl->raw_a;
             sl->fb->raw_ptr = sl->raw_b;

@@ -304,3 +304,40 @@ def test_free_slots_move_between_streams_only_when_idle(piv):
     assert st["rejected"] == 2
     assert st["hbm_peak"] <= 3 * one
     srv.close()
+
+
+def test_large_host_pairs_staged_with_helpers(piv):
+    """Host pairs of 3 MiB and more are staged by the submitting thread and the server's
+    staging helpers together, in row chunks of about 1 MiB (the last one partial here), from
+    two client threads at once: every result equals the pair analysed alone."""
+    import threading
+
+    h, w = 1100, 1536  # 3.2 MiB per u16 frame
+    cfg = piv.PivConfig(windows=[64, 32], final_iterations=1)
+    raw = piv.PreprocessConfig.raw()
+    srv = piv.PivServer(max_batch_pairs=4)
+    sid = srv.add_stream(cfg, h, w, np.uint16, raw)
+    pairs = [_pair(h, w, np.uint16, 12, 500 + k) for k in range(4)]
+    ids, lock = {}, threading.Lock()
+
+    def client(ks):
+        for k in ks:
+            rid = srv.submit(sid, *pairs[k])
+            with lock:
+                ids[rid] = k
+
+    ths = [threading.Thread(target=client, args=(ks,)) for ks in ([0, 2], [1, 3])]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    got = {}
+    while len(got) < len(pairs):
+        res = srv.poll(8, wait_ms=10000)
+        assert res, "no result within 10 s"
+        for r in res:
+            assert r.error is None, r.error
+            got[r.request_id] = r
+    srv.close()
+    for rid, k in ids.items():
+        _same(got[rid], _alone(piv, cfg, raw, *pairs[k], None))
