@@ -166,6 +166,9 @@ __device__ __forceinline__ void count_add(uint32_t *ctr, bool pred)
 }
 
 constexpr int VBLOCK = 256;
+// Vectors whose usability and components a k_val_median block stages in LDS: its own 256 and
+// the grid rows above and below them (256 + 2 nx + 2), for grids up to 895 vectors wide.
+constexpr int VSTAGE = 2048;
 
 __global__ __launch_bounds__(VBLOCK) void k_val_median(const ValidatePair *__restrict__ jobs,
                                                        uint32_t bpp)
@@ -175,6 +178,24 @@ __global__ __launch_bounds__(VBLOCK) void k_val_median(const ValidatePair *__res
     const int nx = J.nx, ny = J.ny;
     const int c = (int)(blk * VBLOCK + threadIdx.x);
     const bool in = c < nx * ny;
+    // Each vector's per-vector tests run once per block instead of once per neighbourhood
+    // that holds it (nine times): the block first stages the usability and components of
+    // every vector its threads' 3x3 neighbourhoods read.  Same arithmetic, same results.
+    __shared__ uint8_t s_ok[VSTAGE];
+    __shared__ float s_u[VSTAGE], s_v[VSTAGE];
+    const int lo = (int)(blk * VBLOCK) - nx - 1;
+    const int span = VBLOCK + 2 * nx + 2;
+    const bool staged = span <= VSTAGE;  // uniform over the block
+    if (staged) {
+        for (int t = threadIdx.x; t < span; t += VBLOCK) {
+            const int j = lo + t;
+            const bool inside = j >= 0 && j < nx * ny;
+            s_ok[t] = inside && (vector_flags(J, j) & INVALID) == 0;
+            s_u[t] = inside ? J.raw_dx[j] : 0.0f;
+            s_v[t] = inside ? J.raw_dy[j] : 0.0f;
+        }
+        __syncthreads();
+    }
     uint8_t f = 0;
     bool med_out = false;
     if (in) {
@@ -192,10 +213,16 @@ __global__ __launch_bounds__(VBLOCK) void k_val_median(const ValidatePair *__res
                     bool use = false;
                     float u = 0.0f, v = 0.0f;
                     if (x >= 0 && y >= 0 && x < nx && y < ny) {
-                        const long j = (long)y * nx + x;
-                        use = (vector_flags(J, j) & INVALID) == 0;
-                        u = J.raw_dx[j];
-                        v = J.raw_dy[j];
+                        const int j = y * nx + x;
+                        if (staged) {
+                            use = s_ok[j - lo] != 0;
+                            u = s_u[j - lo];
+                            v = s_v[j - lo];
+                        } else {
+                            use = (vector_flags(J, j) & INVALID) == 0;
+                            u = J.raw_dx[j];
+                            v = J.raw_dy[j];
+                        }
                     }
                     // compact: slot cnt takes this neighbour
 #pragma unroll
