@@ -328,16 +328,17 @@ piv_status piv_analyzer_set_precision(piv_analyzer *a, piv_precision p);
 
 /* The correlation kernel of each window size has launch variants (occupancy targets of the
  * register allocator).  Variants differ only in scheduling: every variant gives bit-identical
- * results.  piv_autotune runs the analyzer's pass plan on the caller's batch once untimed and
- * `repeats` times timed per variant, keeps the fastest variant per window size for the frames'
- * sample format on this analyzer, and reports one entry per distinct window size. */
+ * results.  piv_autotune sweeps the variants twice; per sweep it runs the analyzer's pass plan
+ * on the caller's batch once untimed and `repeats` times timed per variant.  It keeps the
+ * fastest variant per window size (each timed by its faster sweep) for the frames' sample
+ * format on this analyzer, and reports one entry per distinct window size. */
 typedef struct {
     uint32_t window;
     int32_t fmt;             /* piv_pixel_format of the frames */
     uint32_t chosen;         /* variant index now used by the analyzer */
     uint32_t num_variants;
     uint32_t occupancy[4];   /* waves per SIMD targeted by each variant */
-    float ms[4];             /* mean kernel time per launch of each variant */
+    float ms[4];             /* mean kernel time per launch of each variant (faster sweep) */
 } piv_tune_entry;
 
 /* Number of variants of the correlation kernel for a window size (0 if unsupported). */
