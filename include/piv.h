@@ -284,7 +284,9 @@ piv_status piv_server_add_stream(piv_server *s, const piv_config *cfg, uint32_t 
                                  uint32_t width, piv_pixel_format fmt, const float *background,
                                  uint32_t *stream_id);
 /* Copies the pair (row_stride bytes per row, 0 = packed) and queues it.  PIV_ERR_ADMISSION:
- * the HBM budget or the queue is full and nothing was queued (retry after releasing results). */
+ * the HBM budget or the queue is full and nothing was queued (retry after releasing results).
+ * Concurrent calls, for one stream or several, stage their copies in parallel; pairs of 3 MiB
+ * or more are staged by the calling thread and the server's staging threads together. */
 piv_status piv_submit(piv_server *s, uint32_t stream_id, const void *pixels_a,
                       const void *pixels_b, size_t row_stride, uint64_t *request_id);
 /* As piv_submit for frames already in device memory on the server's GPU (complete when
