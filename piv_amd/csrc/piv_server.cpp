@@ -19,6 +19,8 @@
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <exception>
+#include <new>
 #include <thread>
 
 #include "piv_internal.hpp"
@@ -138,8 +140,11 @@ struct StagePool {
     std::deque<std::shared_ptr<StageTask>> tasks;
     std::vector<std::thread> threads;
     bool stop = false;
+    // may throw std::system_error (no thread) or std::bad_alloc; the threads started so far
+    // stay and are joined by shutdown()
     void start(int n)
     {
+        threads.reserve(n);
         for (int i = 0; i < n; ++i) threads.emplace_back([this] { loop(); });
     }
     void loop()
@@ -156,13 +161,16 @@ struct StagePool {
             if (!tasks.empty() && tasks.front() == t) tasks.pop_front();
         }
     }
-    void post(const std::shared_ptr<StageTask> &t)
+    bool post(const std::shared_ptr<StageTask> &t)
     {
-        {
+        try {
             std::lock_guard<std::mutex> lk(mu);
             tasks.push_back(t);
+        } catch (const std::bad_alloc &) {
+            return false;
         }
         cv.notify_all();
+        return true;
     }
     void shutdown()
     {
@@ -750,9 +758,18 @@ piv_status piv_server_create(int device, const piv_server_config *cfg, piv_serve
             return fail(PIV_ERR_DEVICE, "stream creation failed");
         }
     }
-    s->scheduler = std::thread(scheduler_loop, s);
-    s->completer = std::thread(completer_loop, s);
-    s->stagers.start(kStageHelpers);
+    try {
+        s->scheduler = std::thread(scheduler_loop, s);
+        s->completer = std::thread(completer_loop, s);
+    } catch (const std::exception &) {
+        piv_server_destroy(s);
+        return fail(PIV_ERR_DEVICE, "server threads could not be started");
+    }
+    try {
+        s->stagers.start(kStageHelpers);
+    } catch (const std::exception &) {
+        // staging helpers are optional: submitters copy their pairs alone without them
+    }
     *out = s;
     return PIV_OK;
 }
@@ -919,43 +936,53 @@ piv_status submit_impl(piv_server *s, uint32_t stream_id, const void *pa, const 
             // Large pairs are copied by the server's staging helpers and this thread together.
             const int chunk_rows = (int)std::max<size_t>(1, ((size_t)1 << 20) / rowb);
             const size_t pair_bytes = 2 * (size_t)st->H * rowb;
-            auto t = std::make_shared<StageTask>();
-            t->src[0] = static_cast<const char *>(pa);
-            t->src[1] = static_cast<const char *>(pb);
-            t->stage[0] = sl->h_raw_a;
-            t->stage[1] = sl->h_raw_b;
-            t->row_stride = row_stride;
-            t->rowb = rowb;
-            t->H = st->H;
-            t->chunk_rows = chunk_rows;
-            t->per_frame = (st->H + chunk_rows - 1) / chunk_rows;
-            t->total = 2 * t->per_frame;
-            t->done.reset(new std::atomic<int>[t->total]());
-            char *dst[2] = {static_cast<char *>(sl->raw_a), static_cast<char *>(sl->raw_b)};
-            const bool split = pair_bytes >= kSplitStageBytes && !s->stagers.threads.empty();
-            if (split) s->stagers.post(t);
-            int queued = 0;
-            // queues the uploads of the staged chunks that follow the last one queued (after
-            // an upload error the remaining chunks are still waited for, not uploaded)
-            auto queue_staged = [&]() {
-                while (queued < t->total && t->done[queued].load(std::memory_order_acquire)) {
-                    if (r == PIV_OK) {
-                        const int i = queued / t->per_frame;
-                        const int h0 = (queued % t->per_frame) * chunk_rows;
-                        const int h1 = std::min(st->H, h0 + chunk_rows);
-                        std::lock_guard<std::mutex> ul(st->upload_mu);
-                        if (hipMemcpyAsync(dst[i] + (size_t)h0 * rowb, t->stage[i] + (size_t)h0 * rowb,
-                                           (size_t)(h1 - h0) * rowb, hipMemcpyHostToDevice,
-                                           us) != hipSuccess)
-                            r = fail(PIV_ERR_DEVICE, "upload failed");
+            // no C++ exception leaves the C ABI: a failed host allocation is PIV_ERR_OOM
+            std::shared_ptr<StageTask> t;
+            try {
+                t = std::make_shared<StageTask>();
+                t->per_frame = (st->H + chunk_rows - 1) / chunk_rows;
+                t->total = 2 * t->per_frame;
+                t->done.reset(new std::atomic<int>[t->total]());
+            } catch (const std::bad_alloc &) {
+                t.reset();
+                r = fail(PIV_ERR_OOM, "host allocation failed");
+            }
+            if (t) {
+                t->src[0] = static_cast<const char *>(pa);
+                t->src[1] = static_cast<const char *>(pb);
+                t->stage[0] = sl->h_raw_a;
+                t->stage[1] = sl->h_raw_b;
+                t->row_stride = row_stride;
+                t->rowb = rowb;
+                t->H = st->H;
+                t->chunk_rows = chunk_rows;
+                char *dst[2] = {static_cast<char *>(sl->raw_a), static_cast<char *>(sl->raw_b)};
+                // when the task cannot be queued, this thread copies every chunk itself
+                if (pair_bytes >= kSplitStageBytes && !s->stagers.threads.empty())
+                    (void)s->stagers.post(t);
+                int queued = 0;
+                // queues the uploads of the staged chunks that follow the last one queued (after
+                // an upload error the remaining chunks are still waited for, not uploaded)
+                auto queue_staged = [&]() {
+                    while (queued < t->total && t->done[queued].load(std::memory_order_acquire)) {
+                        if (r == PIV_OK) {
+                            const int i = queued / t->per_frame;
+                            const int h0 = (queued % t->per_frame) * chunk_rows;
+                            const int h1 = std::min(st->H, h0 + chunk_rows);
+                            std::lock_guard<std::mutex> ul(st->upload_mu);
+                            if (hipMemcpyAsync(dst[i] + (size_t)h0 * rowb, t->stage[i] + (size_t)h0 * rowb,
+                                               (size_t)(h1 - h0) * rowb, hipMemcpyHostToDevice,
+                                               us) != hipSuccess)
+                                r = fail(PIV_ERR_DEVICE, "upload failed");
+                        }
+                        ++queued;
                     }
-                    ++queued;
+                };
+                while (t->run_one()) queue_staged();
+                while (queued < t->total) {
+                    queue_staged();
+                    if (queued < t->total) std::this_thread::yield();
                 }
-            };
-            while (t->run_one()) queue_staged();
-            while (queued < t->total) {
-                queue_staged();
-                if (queued < t->total) std::this_thread::yield();
             }
             std::lock_guard<std::mutex> ul(st->upload_mu);
             sl->fa->raw_ptr = sl->raw_a;
