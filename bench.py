@@ -421,6 +421,7 @@ def cli_bench(frames=400, batch=256, seed=41):
         m = re.search(r"analysed (\d+) pairs in ([0-9.]+) s \(([0-9.]+) pairs/s on the GPU loop, "
                       r"([0-9.]+) s total\)", p.stderr)
         lp = re.search(r"loop: (.*)", p.stderr)
+        su = re.search(r"set-up: ([0-9.]+) s before the loop", p.stderr)
         files = os.listdir(out)
         out_bytes = sum(os.path.getsize(os.path.join(out, f)) for f in files)
         pairs = int(m.group(1))
@@ -432,6 +433,7 @@ def cli_bench(frames=400, batch=256, seed=41):
                 "loop_seconds": float(m.group(2)), "pairs_per_s_loop": float(m.group(3)),
                 "cli_total_seconds": float(m.group(4)),
                 "loop_phases": lp.group(1) if lp else None,
+                "setup_seconds": float(su.group(1)) if su else None,
                 "output_files": len(files), "output_MB": round(out_bytes / 1e6, 1)}
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
