@@ -322,39 +322,70 @@ int main(int argc, char **argv)
     piv_grid grid;
     check(piv_output_grid(an, H, W, &grid), "grid");
 
-    // the clip in HBM (one plane).  Uploaded in 32 MiB pieces from several threads at once: the
-    // mapped file's pages are faulted in and staged by all of them, not by one.
-    auto upload = [&](void *dst, const unsigned char *src, size_t bytes) {
-        constexpr size_t piece = (size_t)32 << 20;
-        const size_t pieces = (bytes + piece - 1) / piece;
-        const unsigned nt = (unsigned)std::min<size_t>(pieces, 8);
+    // the clip in HBM (one plane).  Uploaded in 32 MiB pieces by several threads at once (the
+    // mapped file's pages are faulted in and staged by all of them), in the background: the
+    // loop waits only for the frames each batch needs, so the upload of the rest of the clip
+    // overlaps the analysis of its start.
+    struct ClipUpload {
+        const size_t kPiece = (size_t)32 << 20;
+        int device = 0;
+        char *dst = nullptr;
+        const unsigned char *src = nullptr;
+        size_t bytes = 0, pieces = 0, prefix = 0;  // pieces [0, prefix) are in HBM
         std::atomic<size_t> next{0};
-        std::mutex emu;
+        std::vector<char> done;
+        std::mutex mu;
+        std::condition_variable cv;
         std::string err;
-        auto work = [&]() {
-            for (size_t k; (k = next.fetch_add(1)) < pieces;) {
-                const size_t off = k * piece, len = std::min(piece, bytes - off);
-                if (piv_copy_to_device(A.device, static_cast<char *>(dst) + off, src + off, len) != PIV_OK) {
-                    std::lock_guard<std::mutex> lk(emu);
-                    if (err.empty()) err = piv_last_error();
-                }
-            }
-        };
         std::vector<std::thread> ts;
-        for (unsigned i = 1; i < nt; ++i) ts.emplace_back(work);
-        work();
-        for (auto &t : ts) t.join();
-        if (!err.empty()) die("upload: " + err);
+        void start(int dev, void *d, const unsigned char *s, size_t b)
+        {
+            device = dev;
+            dst = static_cast<char *>(d);
+            src = s;
+            bytes = b;
+            pieces = (b + kPiece - 1) / kPiece;
+            done.assign(pieces, 0);
+            const unsigned nt = (unsigned)std::min<size_t>(std::max<size_t>(pieces, 1), 8);
+            for (unsigned i = 0; i < nt; ++i) ts.emplace_back([this] { work(); });
+        }
+        void work()
+        {
+            for (size_t k; (k = next.fetch_add(1)) < pieces;) {
+                const size_t off = k * kPiece, len = std::min(kPiece, bytes - off);
+                const bool ok = piv_copy_to_device(device, dst + off, src + off, len) == PIV_OK;
+                std::lock_guard<std::mutex> lk(mu);
+                if (!ok && err.empty()) err = piv_last_error();
+                done[k] = 1;
+                while (prefix < pieces && done[prefix]) ++prefix;
+                cv.notify_all();
+            }
+        }
+        // blocks until the first `b` bytes are in HBM
+        void wait_bytes(size_t b)
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv.wait(lk, [&] { return !err.empty() || prefix == pieces || prefix * kPiece >= b; });
+            if (!err.empty()) die("upload: " + err);
+        }
+        void finish()
+        {
+            wait_bytes(bytes);
+            for (auto &t : ts) t.join();
+            ts.clear();
+        }
     };
+    ClipUpload up;
     void *clip = nullptr;
     check(piv_device_alloc(A.device, in.frames * frame_bytes, &clip), "device memory for the clip");
     if (in.channels == 1) {
-        upload(clip, in.data, in.frames * frame_bytes);
+        up.start(A.device, clip, in.data, in.frames * frame_bytes);
     } else {
         void *rgb = nullptr;
         const size_t bytes = in.frames * frame_bytes * in.channels;
         check(piv_device_alloc(A.device, bytes, &rgb), "device memory for the colour clip");
-        upload(rgb, in.data, bytes);
+        up.start(A.device, rgb, in.data, bytes);
+        up.finish();
         check(piv_extract_plane_device(A.device, rgb, in.fmt, in.frames * in.height * in.width,
                                        (uint32_t)in.channels, 1, clip),
               "green plane");
@@ -374,6 +405,7 @@ int main(int argc, char **argv)
         long n = c == std::string::npos ? 40 : std::stol(A.background.substr(c + 1));
         if ((mode != "min" && mode != "mean") || n < 1) die("--background: none, min:N or mean:N");
         n = std::min(n, total);
+        up.finish();  // frames sampled across the whole clip
         check(piv_background_create(an, mode == "min" ? PIV_BACKGROUND_MIN : PIV_BACKGROUND_MEAN, H, W, &bg),
               "background");
         for (long k = 0; k < n; ++k) {
@@ -481,9 +513,9 @@ int main(int argc, char **argv)
     };
 
     uint64_t valid = 0, vectors = 0;
-    // seconds of the loop per phase: preparation, analysis, flow statistics, download, and
-    // waiting for the writers
-    double ph_prep = 0, ph_an = 0, ph_stats = 0, ph_down = 0, ph_wait = 0;
+    // seconds of the loop per phase: waiting for the clip's upload, preparation, analysis, flow
+    // statistics, download, and waiting for the writers
+    double ph_upload = 0, ph_prep = 0, ph_an = 0, ph_stats = 0, ph_down = 0, ph_wait = 0;
     auto lap = [](std::chrono::steady_clock::time_point &t) {
         const auto now = std::chrono::steady_clock::now();
         const double s = std::chrono::duration<double>(now - t).count();
@@ -501,6 +533,8 @@ int main(int argc, char **argv)
             need.push_back(pair_starts[p0 + k] + A.stride);
         }
         auto tp = std::chrono::steady_clock::now();
+        up.wait_bytes((size_t)(*std::max_element(need.begin(), need.end()) + 1) * frame_bytes);
+        ph_upload += lap(tp);
         prepare_frames(need);
         ph_prep += lap(tp);
         std::vector<const piv_frame *> fa(n), fb(n);
@@ -564,11 +598,13 @@ int main(int argc, char **argv)
         ph_wait += lap(tw);
     }
     if (!write_error.empty()) die("output: " + write_error);
-    std::fprintf(stderr, "loop: preparation %.3f s, analysis %.3f s, flow statistics %.3f s, "
-                         "download %.3f s, waiting for %u writers %.3f s\n",
-                 ph_prep, ph_an, ph_stats, ph_down, nwriters, ph_wait);
+    up.finish();
+    std::fprintf(stderr, "loop: waiting for the upload %.3f s, preparation %.3f s, analysis %.3f s, "
+                         "flow statistics %.3f s, download %.3f s, waiting for %u writers %.3f s\n",
+                 ph_upload, ph_prep, ph_an, ph_stats, ph_down, nwriters, ph_wait);
     // before the loop: opening the clip, the output directory, the analyzer (the device's
-    // first use), the background estimate and the clip's upload
+    // first use), the start of the clip's upload and the background estimate (which waits
+    // for the whole upload)
     std::fprintf(stderr, "set-up: %.3f s before the loop\n",
                  std::chrono::duration<double>(t_loop - t0).count());
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_loop).count();
