@@ -145,3 +145,33 @@ def test_cli_errors(piv, tmp_path):
     assert rc != 0 and "--force" in log
     rc, log = _run([str(src), "-o", str(tmp_path / "o2"), "--format", "hdf5"])
     assert rc != 0 and "HDF5" in log
+
+
+def test_cli_streamed_upload_multi_piece_clip(piv, tmp_path):
+    """A one-plane clip of several 32 MiB upload pieces and no background: the loop starts
+    while the rest of the clip is still uploading and each batch waits only for its own
+    frames.  Pairs spread over the whole clip (the last ones in the last piece) equal the
+    library analysing the same pair alone, bitwise."""
+    h, w, n = 1024, 1536, 36  # 3 MiB per u16 frame: 108 MiB, four pieces
+    base, _ = S.video_frames(256, 384, 6, 29, bits=12, container=np.uint16)
+    clip = np.stack([np.tile(base[k % len(base)], (4, 4)) for k in range(n)])
+    assert clip.shape == (n, h, w) and clip.nbytes > 3 * (32 << 20)
+    src = tmp_path / "clip.npy"
+    np.save(src, clip)
+    out = tmp_path / "out"
+    rc, log = _run([str(src), "-o", str(out), "--stride", "1", "--pair-step", "5",
+                    "--capture-fps", "100", "--background", "none", "--windows", "64",
+                    "--final-iterations", "1", "--min-mean", "0", "--min-std", "0",
+                    "--smooth-sigma", "0", "--normalize-radius", "0", "--batch", "2"])
+    assert rc == 0, log
+    assert "waiting for the upload" in log and "set-up:" in log
+    cfg = piv.PivConfig(windows=[64], final_iterations=1, mask=piv.MaskConfig(0.0, 0.0))
+    an = piv.PivAnalyzer(cfg, piv.PreprocessConfig.raw())
+    starts = list(range(0, n - 1, 5))
+    assert starts[-1] == 30
+    for a in (0, 15, 30):
+        d = np.load(out / f"pair_{a:06d}_{a + 1:06d}.npz")
+        r = an.analyze(an.prepare(clip[a]), an.prepare(clip[a + 1]))
+        assert np.array_equal(bits(d["dx"]), bits(r.field.dx)), a
+        assert np.array_equal(bits(d["dy"]), bits(r.field.dy)), a
+        assert np.array_equal(d["flags"], r.field.flags), a
