@@ -752,159 +752,177 @@ int piv_abi_version(void) { return PIV_ABI_VERSION; }
 
 piv_status piv_device_count(int *count)
 {
-    if (!count) return fail(PIV_ERR_INVALID_ARGUMENT, "count is null");
-    int n = 0;
-    hipError_t e = hipGetDeviceCount(&n);
-    if (e != hipSuccess) n = 0;
-    *count = n;
-    return PIV_OK;
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!count) return fail(PIV_ERR_INVALID_ARGUMENT, "count is null");
+        int n = 0;
+        hipError_t e = hipGetDeviceCount(&n);
+        if (e != hipSuccess) n = 0;
+        *count = n;
+        return PIV_OK;
+    });
 }
 
 void piv_config_default(piv_config *c)
 {
-    if (!c) return;
-    std::memset(c, 0, sizeof *c);
-    c->windows[0] = 96;
-    c->windows[1] = 64;
-    c->num_windows = 2;
-    c->overlap = 0.5f;
-    c->final_iterations = 2;
-    c->min_mean = 0.0f;
-    c->min_std = 0.0f;
-    c->ppr_threshold = 1.3f;
-    c->median_threshold = 2.0f;
-    c->median_eps = 0.1f;
-    c->replace = 1;
-    c->smooth_sigma = 1.0f;
-    c->normalize_radius = 4;
-    c->normalize_eps = 0.02f;
+    try {
+        if (!c) return;
+        std::memset(c, 0, sizeof *c);
+        c->windows[0] = 96;
+        c->windows[1] = 64;
+        c->num_windows = 2;
+        c->overlap = 0.5f;
+        c->final_iterations = 2;
+        c->min_mean = 0.0f;
+        c->min_std = 0.0f;
+        c->ppr_threshold = 1.3f;
+        c->median_threshold = 2.0f;
+        c->median_eps = 0.1f;
+        c->replace = 1;
+        c->smooth_sigma = 1.0f;
+        c->normalize_radius = 4;
+        c->normalize_eps = 0.02f;
+    } catch (...) {
+    }
 }
 
 piv_status piv_grid_new(uint32_t height, uint32_t width, uint32_t window, float overlap,
                         piv_grid *out)
 {
-    if (!out) return fail(PIV_ERR_INVALID_ARGUMENT, "out is null");
-    GridDev g{};
-    PIV_TRY(grid_new(height, width, window, overlap, g));
-    *out = to_pub(g);
-    return PIV_OK;
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!out) return fail(PIV_ERR_INVALID_ARGUMENT, "out is null");
+        GridDev g{};
+        PIV_TRY(grid_new(height, width, window, overlap, g));
+        *out = to_pub(g);
+        return PIV_OK;
+    });
 }
 
 piv_status piv_analyzer_create(const piv_config *cfg, int device, piv_analyzer **out)
 {
-    if (!cfg || !out) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
-    *out = nullptr;
-    PIV_TRY(check_config(*cfg));
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
-        return fail(PIV_ERR_DEVICE, "no HIP device available");
-    if (device < 0 || device >= ndev)
-        return fail(PIV_ERR_INVALID_ARGUMENT, "device %d out of range (%d devices)", device, ndev);
-    PIV_TRY(set_device(device));
-    auto *a = new (std::nothrow) piv_analyzer();
-    if (!a) return fail(PIV_ERR_OOM, "host allocation failed");
-    a->cfg = *cfg;
-    a->device = device;
-    for (uint32_t i = 0; i < cfg->num_windows; ++i) a->passes.push_back((int)cfg->windows[i]);
-    for (uint32_t i = 1; i < cfg->final_iterations; ++i)
-        a->passes.push_back((int)cfg->windows[cfg->num_windows - 1]);
-    hipError_t e = hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking);
-    if (e != hipSuccess) {
-        delete a;
-        return fail(PIV_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
-    }
-    if (cfg->smooth_sigma > 0.0f) {
-        a->taps_r = gauss_taps(cfg->smooth_sigma, a->taps);
-        piv_status s = a->d_taps.reserve(a->taps.size() * sizeof(float));
-        if (s == PIV_OK &&
-            hipMemcpy(a->d_taps.p, a->taps.data(), a->taps.size() * sizeof(float),
-                      hipMemcpyHostToDevice) != hipSuccess)
-            s = fail(PIV_ERR_DEVICE, "copying blur taps failed");
-        if (s != PIV_OK) {
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!cfg || !out) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+        *out = nullptr;
+        PIV_TRY(check_config(*cfg));
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+            return fail(PIV_ERR_DEVICE, "no HIP device available");
+        if (device < 0 || device >= ndev)
+            return fail(PIV_ERR_INVALID_ARGUMENT, "device %d out of range (%d devices)", device, ndev);
+        PIV_TRY(set_device(device));
+        auto *a = new (std::nothrow) piv_analyzer();
+        if (!a) return fail(PIV_ERR_OOM, "host allocation failed");
+        a->cfg = *cfg;
+        a->device = device;
+        for (uint32_t i = 0; i < cfg->num_windows; ++i) a->passes.push_back((int)cfg->windows[i]);
+        for (uint32_t i = 1; i < cfg->final_iterations; ++i)
+            a->passes.push_back((int)cfg->windows[cfg->num_windows - 1]);
+        hipError_t e = hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
             delete a;
-            return s;
+            return fail(PIV_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
         }
-    }
-    *out = a;
-    return PIV_OK;
+        if (cfg->smooth_sigma > 0.0f) {
+            a->taps_r = gauss_taps(cfg->smooth_sigma, a->taps);
+            piv_status s = a->d_taps.reserve(a->taps.size() * sizeof(float));
+            if (s == PIV_OK &&
+                hipMemcpy(a->d_taps.p, a->taps.data(), a->taps.size() * sizeof(float),
+                          hipMemcpyHostToDevice) != hipSuccess)
+                s = fail(PIV_ERR_DEVICE, "copying blur taps failed");
+            if (s != PIV_OK) {
+                delete a;
+                return s;
+            }
+        }
+        *out = a;
+        return PIV_OK;
+    });
 }
 
 void piv_analyzer_destroy(piv_analyzer *a)
 {
-    if (!a) return;
-    (void)hipSetDevice(a->device);
-    if (a->stream) (void)hipStreamSynchronize(a->stream);
-    delete a;
+    try {
+        if (!a) return;
+        (void)hipSetDevice(a->device);
+        if (a->stream) (void)hipStreamSynchronize(a->stream);
+        delete a;
+    } catch (...) {
+    }
 }
 
 uint32_t piv_analyzer_num_passes(const piv_analyzer *a) { return a ? (uint32_t)a->passes.size() : 0; }
 
 piv_status piv_output_grid(const piv_analyzer *a, uint32_t height, uint32_t width, piv_grid *out)
 {
-    if (!a || !out) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
-    return piv_grid_new(height, width, a->cfg.windows[a->cfg.num_windows - 1], a->cfg.overlap,
-                        out);
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!a || !out) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+        return piv_grid_new(height, width, a->cfg.windows[a->cfg.num_windows - 1], a->cfg.overlap,
+                            out);
+    });
 }
 
 piv_status piv_frame_prepare(piv_analyzer *a, const void *pixels, piv_pixel_format fmt,
                              uint32_t height, uint32_t width, size_t row_stride,
                              const float *background, piv_frame **out)
 {
-    if (!a || !out) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
-    *out = nullptr;
-    PIV_TRY(check_pixels_args(pixels, fmt, height, width, row_stride));
-    std::lock_guard<std::mutex> lk(a->mu);
-    PIV_TRY(set_device(a->device));
-    auto *f = new (std::nothrow) piv_frame();
-    if (!f) return fail(PIV_ERR_OOM, "host allocation failed");
-    f->device = a->device;
-    f->pool = a->pool;
-    f->H = (int)height;
-    f->W = (int)width;
-    f->in_fmt = fmt;
-    const size_t bpp = (size_t)bytes_per_px(fmt);
-    piv_status s = frame_reserve(a, f->raw, (size_t)height * width * bpp);
-    DevBuf bg;
-    if (s == PIV_OK) {
-        hipError_t e = hipMemcpy2DAsync(f->raw.p, width * bpp, pixels, row_stride, width * bpp,
-                                        height, hipMemcpyHostToDevice, a->stream);
-        if (e != hipSuccess) s = fail(PIV_ERR_DEVICE, "upload: %s", hipGetErrorString(e));
-    }
-    if (s == PIV_OK && background) {
-        s = bg.reserve((size_t)height * width * sizeof(float));
-        if (s == PIV_OK &&
-            hipMemcpyAsync(bg.p, background, (size_t)height * width * sizeof(float),
-                           hipMemcpyHostToDevice, a->stream) != hipSuccess)
-            s = fail(PIV_ERR_DEVICE, "background upload failed");
-    }
-    if (s == PIV_OK) {
-        f->raw_ptr = f->raw.p;
-        f->raw_pitch = (int)width;
-        s = prepare_frame(a, f, background ? bg.as<float>() : nullptr);
-    }
-    if (s == PIV_OK && hipStreamSynchronize(a->stream) != hipSuccess)
-        s = fail(PIV_ERR_DEVICE, "frame preparation failed");
-    bg.release();
-    if (s != PIV_OK) {
-        delete f;
-        return s;
-    }
-    *out = f;
-    return PIV_OK;
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!a || !out) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+        *out = nullptr;
+        PIV_TRY(check_pixels_args(pixels, fmt, height, width, row_stride));
+        std::lock_guard<std::mutex> lk(a->mu);
+        PIV_TRY(set_device(a->device));
+        auto *f = new (std::nothrow) piv_frame();
+        if (!f) return fail(PIV_ERR_OOM, "host allocation failed");
+        f->device = a->device;
+        f->pool = a->pool;
+        f->H = (int)height;
+        f->W = (int)width;
+        f->in_fmt = fmt;
+        const size_t bpp = (size_t)bytes_per_px(fmt);
+        piv_status s = frame_reserve(a, f->raw, (size_t)height * width * bpp);
+        DevBuf bg;
+        if (s == PIV_OK) {
+            hipError_t e = hipMemcpy2DAsync(f->raw.p, width * bpp, pixels, row_stride, width * bpp,
+                                            height, hipMemcpyHostToDevice, a->stream);
+            if (e != hipSuccess) s = fail(PIV_ERR_DEVICE, "upload: %s", hipGetErrorString(e));
+        }
+        if (s == PIV_OK && background) {
+            s = bg.reserve((size_t)height * width * sizeof(float));
+            if (s == PIV_OK &&
+                hipMemcpyAsync(bg.p, background, (size_t)height * width * sizeof(float),
+                               hipMemcpyHostToDevice, a->stream) != hipSuccess)
+                s = fail(PIV_ERR_DEVICE, "background upload failed");
+        }
+        if (s == PIV_OK) {
+            f->raw_ptr = f->raw.p;
+            f->raw_pitch = (int)width;
+            s = prepare_frame(a, f, background ? bg.as<float>() : nullptr);
+        }
+        if (s == PIV_OK && hipStreamSynchronize(a->stream) != hipSuccess)
+            s = fail(PIV_ERR_DEVICE, "frame preparation failed");
+        bg.release();
+        if (s != PIV_OK) {
+            delete f;
+            return s;
+        }
+        *out = f;
+        return PIV_OK;
+    });
 }
 
 piv_status piv_frame_prepare_device(piv_analyzer *a, const void *pixels, piv_pixel_format fmt,
                                     uint32_t height, uint32_t width, size_t row_stride,
                                     const float *background, piv_frame **out)
 {
-    if (!a || !out) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
-    *out = nullptr;
-    PIV_TRY(check_pixels_args(pixels, fmt, height, width, row_stride));
-    std::lock_guard<std::mutex> lk(a->mu);
-    PIV_TRY(set_device(a->device));
-    PIV_TRY(frame_from_device(a, pixels, fmt, height, width, row_stride, false, background, out));
-    HIP_TRY(hipStreamSynchronize(a->stream));
-    return PIV_OK;
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!a || !out) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+        *out = nullptr;
+        PIV_TRY(check_pixels_args(pixels, fmt, height, width, row_stride));
+        std::lock_guard<std::mutex> lk(a->mu);
+        PIV_TRY(set_device(a->device));
+        PIV_TRY(frame_from_device(a, pixels, fmt, height, width, row_stride, false, background, out));
+        HIP_TRY(hipStreamSynchronize(a->stream));
+        return PIV_OK;
+    });
 }
 
 piv_status piv_frame_prepare_device_batch(piv_analyzer *a, const void *pixels,
@@ -912,95 +930,102 @@ piv_status piv_frame_prepare_device_batch(piv_analyzer *a, const void *pixels,
                                           uint32_t width, size_t row_stride, size_t frame_stride,
                                           const float *background, piv_frame **out)
 {
-    if (!a || !out) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
-    if (count == 0) return PIV_OK;
-    PIV_TRY(check_pixels_args(pixels, fmt, height, width, row_stride));
-    if (frame_stride < row_stride * height)
-        return fail(PIV_ERR_INVALID_ARGUMENT, "frame_stride %zu < %zu", frame_stride, row_stride * height);
-    std::lock_guard<std::mutex> lk(a->mu);
-    PIV_TRY(set_device(a->device));
-    std::vector<piv_frame *> fs(count, nullptr);
-    piv_status s = PIV_OK;
-    for (uint32_t i = 0; i < count && s == PIV_OK; ++i) {
-        fs[i] = new (std::nothrow) piv_frame();
-        if (!fs[i]) {
-            s = fail(PIV_ERR_OOM, "host allocation failed");
-            break;
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!a || !out) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+        if (count == 0) return PIV_OK;
+        PIV_TRY(check_pixels_args(pixels, fmt, height, width, row_stride));
+        if (frame_stride < row_stride * height)
+            return fail(PIV_ERR_INVALID_ARGUMENT, "frame_stride %zu < %zu", frame_stride, row_stride * height);
+        std::lock_guard<std::mutex> lk(a->mu);
+        PIV_TRY(set_device(a->device));
+        std::vector<piv_frame *> fs(count, nullptr);
+        piv_status s = PIV_OK;
+        for (uint32_t i = 0; i < count && s == PIV_OK; ++i) {
+            fs[i] = new (std::nothrow) piv_frame();
+            if (!fs[i]) {
+                s = fail(PIV_ERR_OOM, "host allocation failed");
+                break;
+            }
+            piv_frame *f = fs[i];
+            f->device = a->device;
+            f->pool = a->pool;
+            f->H = (int)height;
+            f->W = (int)width;
+            f->in_fmt = (int)fmt;
+            f->raw_ptr = static_cast<const char *>(pixels) + (size_t)i * frame_stride;
+            f->raw_pitch = (int)(row_stride / bytes_per_px((int)fmt));
         }
-        piv_frame *f = fs[i];
-        f->device = a->device;
-        f->pool = a->pool;
-        f->H = (int)height;
-        f->W = (int)width;
-        f->in_fmt = (int)fmt;
-        f->raw_ptr = static_cast<const char *>(pixels) + (size_t)i * frame_stride;
-        f->raw_pitch = (int)(row_stride / bytes_per_px((int)fmt));
-    }
-    if (s == PIV_OK) s = prepare_frames_batch(a, fs.data(), (int)count, background, (long)frame_stride);
-    if (s == PIV_OK && hipStreamSynchronize(a->stream) != hipSuccess)
-        s = fail(PIV_ERR_DEVICE, "frame preparation failed");
-    if (s != PIV_OK) {
-        for (piv_frame *f : fs) delete f;
-        return s;
-    }
-    for (uint32_t i = 0; i < count; ++i) out[i] = fs[i];
-    return PIV_OK;
+        if (s == PIV_OK) s = prepare_frames_batch(a, fs.data(), (int)count, background, (long)frame_stride);
+        if (s == PIV_OK && hipStreamSynchronize(a->stream) != hipSuccess)
+            s = fail(PIV_ERR_DEVICE, "frame preparation failed");
+        if (s != PIV_OK) {
+            for (piv_frame *f : fs) delete f;
+            return s;
+        }
+        for (uint32_t i = 0; i < count; ++i) out[i] = fs[i];
+        return PIV_OK;
+    });
 }
 
 void piv_frame_release(piv_frame *f)
 {
-    if (!f) return;
-    (void)hipSetDevice(f->device);
-    if (f->raw.p || f->image.p || f->base.p || f->sat.p) {
-        if (auto pool = f->pool.lock()) {
-            // the frame's last user is done before its memory is reused (hipFree would
-            // synchronise the device as well)
-            (void)hipDeviceSynchronize();
-            pool->give(f->raw);
-            pool->give(f->image);
-            pool->give(f->base);
-            pool->give(f->sat);
+    try {
+        if (!f) return;
+        (void)hipSetDevice(f->device);
+        if (f->raw.p || f->image.p || f->base.p || f->sat.p) {
+            if (auto pool = f->pool.lock()) {
+                // the frame's last user is done before its memory is reused (hipFree would
+                // synchronise the device as well)
+                (void)hipDeviceSynchronize();
+                pool->give(f->raw);
+                pool->give(f->image);
+                pool->give(f->base);
+                pool->give(f->sat);
+            }
         }
+        delete f;
+    } catch (...) {
     }
-    delete f;
 }
 
 piv_status piv_frame_download(const piv_frame *f, float *image, float *base)
 {
-    if (!f) return fail(PIV_ERR_INVALID_ARGUMENT, "null frame");
-    PIV_TRY(set_device(f->device));
-    const size_t n = (size_t)f->H * f->W;
-    DevBuf tmp;
-    auto normalized = [&](float **p) -> piv_status {
-        PIV_TRY(tmp.reserve(n * sizeof(float)));
-        // the original samples, normalised (pitch-aware via base kernel without background)
-        HIP_TRY(launch_base(f->in_fmt, f->raw_ptr, f->raw_pitch, nullptr, f->H, f->W,
-                            tmp.as<float>(), nullptr));
-        *p = tmp.as<float>();
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!f) return fail(PIV_ERR_INVALID_ARGUMENT, "null frame");
+        PIV_TRY(set_device(f->device));
+        const size_t n = (size_t)f->H * f->W;
+        DevBuf tmp;
+        auto normalized = [&](float **p) -> piv_status {
+            PIV_TRY(tmp.reserve(n * sizeof(float)));
+            // the original samples, normalised (pitch-aware via base kernel without background)
+            HIP_TRY(launch_base(f->in_fmt, f->raw_ptr, f->raw_pitch, nullptr, f->H, f->W,
+                                tmp.as<float>(), nullptr));
+            *p = tmp.as<float>();
+            return PIV_OK;
+        };
+        if (image) {
+            // a conditioned image (own or batch storage) is what the correlation reads; otherwise
+            // the image is the normalised samples
+            if (f->fmt == FMT_F32 && f->img && f->img != f->raw_ptr) {
+                HIP_TRY(hipMemcpy(image, f->img, n * sizeof(float), hipMemcpyDeviceToHost));
+            } else {
+                float *p = nullptr;
+                PIV_TRY(normalized(&p));
+                HIP_TRY(hipMemcpy(image, p, n * sizeof(float), hipMemcpyDeviceToHost));
+            }
+        }
+        if (base) {
+            if (f->base_ptr) {
+                HIP_TRY(hipMemcpy(base, f->base_ptr, n * sizeof(float), hipMemcpyDeviceToHost));
+            } else {
+                float *p = nullptr;
+                PIV_TRY(normalized(&p));
+                HIP_TRY(hipMemcpy(base, p, n * sizeof(float), hipMemcpyDeviceToHost));
+            }
+        }
+        tmp.release();
         return PIV_OK;
-    };
-    if (image) {
-        // a conditioned image (own or batch storage) is what the correlation reads; otherwise
-        // the image is the normalised samples
-        if (f->fmt == FMT_F32 && f->img && f->img != f->raw_ptr) {
-            HIP_TRY(hipMemcpy(image, f->img, n * sizeof(float), hipMemcpyDeviceToHost));
-        } else {
-            float *p = nullptr;
-            PIV_TRY(normalized(&p));
-            HIP_TRY(hipMemcpy(image, p, n * sizeof(float), hipMemcpyDeviceToHost));
-        }
-    }
-    if (base) {
-        if (f->base_ptr) {
-            HIP_TRY(hipMemcpy(base, f->base_ptr, n * sizeof(float), hipMemcpyDeviceToHost));
-        } else {
-            float *p = nullptr;
-            PIV_TRY(normalized(&p));
-            HIP_TRY(hipMemcpy(base, p, n * sizeof(float), hipMemcpyDeviceToHost));
-        }
-    }
-    tmp.release();
-    return PIV_OK;
+    });
 }
 
 static piv_status analyze_impl(piv_analyzer *a, const piv_frame *fa, const piv_frame *fb,
@@ -1075,14 +1100,18 @@ static piv_status analyze_impl(piv_analyzer *a, const piv_frame *fa, const piv_f
 piv_status piv_analyze_pair(piv_analyzer *a, const piv_frame *fa, const piv_frame *fb,
                             piv_field_out *out)
 {
-    return analyze_impl(a, fa, fb, out, nullptr, 0);
+    return pivi::guard(__func__, [&]() -> piv_status {
+        return analyze_impl(a, fa, fb, out, nullptr, 0);
+    });
 }
 
 piv_status piv_analyze_traced(piv_analyzer *a, const piv_frame *fa, const piv_frame *fb,
                               piv_field_out *out, piv_pass_trace *traces, uint32_t num_traces)
 {
-    if (!traces) return fail(PIV_ERR_INVALID_ARGUMENT, "traces is null");
-    return analyze_impl(a, fa, fb, out, traces, num_traces);
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!traces) return fail(PIV_ERR_INVALID_ARGUMENT, "traces is null");
+        return analyze_impl(a, fa, fb, out, traces, num_traces);
+    });
 }
 
 piv_status piv_analyze_batch_device(piv_analyzer *a, const piv_frame *const *fa,
@@ -1090,73 +1119,77 @@ piv_status piv_analyze_batch_device(piv_analyzer *a, const piv_frame *const *fa,
                                     float *dy, float *ppr, uint8_t *flags,
                                     piv_summary *summaries, void *stream, int async)
 {
-    if (!a || !fa || !fb || !dx || !dy || !ppr || !flags)
-        return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
-    if (npairs == 0) return PIV_OK;
-    std::lock_guard<std::mutex> lk(a->mu);
-    PIV_TRY(set_device(a->device));
-    hipStream_t saved = a->stream;
-    if (stream) a->stream = static_cast<hipStream_t>(stream);
-    std::vector<PassPlan> plans;
-    std::vector<FieldPtrs> raw, val;
-    std::vector<float *> org;
-    std::vector<int32_t *> pk;
-    uint32_t *sums = nullptr;
-    BatchOut bo;
-    bo.dx = dx;
-    bo.dy = dy;
-    bo.ppr = ppr;
-    bo.flags = flags;
-    piv_status s = run_batch(a, fa, fb, npairs, bo, false, plans, raw, val, org, pk, &sums);
-    if (s == PIV_OK && summaries) {
-        const size_t np = plans.size();
-        s = a->h_summaries.reserve(np * npairs * 5 * sizeof(uint32_t));
-        if (s == PIV_OK &&
-            (hipMemcpyAsync(a->h_summaries.p, sums + (np - 1) * npairs * 5,
-                            npairs * 5 * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                            a->stream) != hipSuccess ||
-             hipStreamSynchronize(a->stream) != hipSuccess))
-            s = fail(PIV_ERR_DEVICE, "summary download failed");
-        if (s == PIV_OK) {
-            const uint32_t *hs = static_cast<const uint32_t *>(a->h_summaries.p);
-            for (uint32_t k = 0; k < npairs; ++k) to_summary(hs + k * 5, &summaries[k]);
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!a || !fa || !fb || !dx || !dy || !ppr || !flags)
+            return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+        if (npairs == 0) return PIV_OK;
+        std::lock_guard<std::mutex> lk(a->mu);
+        PIV_TRY(set_device(a->device));
+        hipStream_t saved = a->stream;
+        if (stream) a->stream = static_cast<hipStream_t>(stream);
+        std::vector<PassPlan> plans;
+        std::vector<FieldPtrs> raw, val;
+        std::vector<float *> org;
+        std::vector<int32_t *> pk;
+        uint32_t *sums = nullptr;
+        BatchOut bo;
+        bo.dx = dx;
+        bo.dy = dy;
+        bo.ppr = ppr;
+        bo.flags = flags;
+        piv_status s = run_batch(a, fa, fb, npairs, bo, false, plans, raw, val, org, pk, &sums);
+        if (s == PIV_OK && summaries) {
+            const size_t np = plans.size();
+            s = a->h_summaries.reserve(np * npairs * 5 * sizeof(uint32_t));
+            if (s == PIV_OK &&
+                (hipMemcpyAsync(a->h_summaries.p, sums + (np - 1) * npairs * 5,
+                                npairs * 5 * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                a->stream) != hipSuccess ||
+                 hipStreamSynchronize(a->stream) != hipSuccess))
+                s = fail(PIV_ERR_DEVICE, "summary download failed");
+            if (s == PIV_OK) {
+                const uint32_t *hs = static_cast<const uint32_t *>(a->h_summaries.p);
+                for (uint32_t k = 0; k < npairs; ++k) to_summary(hs + k * 5, &summaries[k]);
+            }
+        } else if (s == PIV_OK && !async) {
+            if (hipStreamSynchronize(a->stream) != hipSuccess) s = fail(PIV_ERR_DEVICE, "batch failed");
         }
-    } else if (s == PIV_OK && !async) {
-        if (hipStreamSynchronize(a->stream) != hipSuccess) s = fail(PIV_ERR_DEVICE, "batch failed");
-    }
-    a->stream = saved;
-    return s;
+        a->stream = saved;
+        return s;
+    });
 }
 
 piv_status piv_place_windows(piv_analyzer *a, const piv_grid *grid, uint32_t height,
                              uint32_t width, const piv_grid *prev, const float *prev_dx,
                              const float *prev_dy, float *origins)
 {
-    if (!a || !grid || !origins || (prev && (!prev_dx || !prev_dy)))
-        return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
-    std::lock_guard<std::mutex> lk(a->mu);
-    PIV_TRY(set_device(a->device));
-    GridDev g{(int)grid->window, (int)grid->step, (int)grid->nx, (int)grid->ny, (int)grid->x0, (int)grid->y0};
-    GridDev pg{};
-    const size_t n = (size_t)g.nx * g.ny;
-    size_t pn = 0;
-    if (prev) {
-        pg = GridDev{(int)prev->window, (int)prev->step, (int)prev->nx, (int)prev->ny, (int)prev->x0, (int)prev->y0};
-        pn = (size_t)pg.nx * pg.ny;
-    }
-    PIV_TRY(a->stage_in.reserve(2 * pn * sizeof(float) + 16));
-    PIV_TRY(a->stage_out.reserve(n * 4 * sizeof(float)));
-    float *pdx = a->stage_in.as<float>(), *pdy = pdx + pn;
-    hipStream_t st = a->stream;
-    if (prev) {
-        HIP_TRY(hipMemcpyAsync(pdx, prev_dx, pn * 4, hipMemcpyHostToDevice, st));
-        HIP_TRY(hipMemcpyAsync(pdy, prev_dy, pn * 4, hipMemcpyHostToDevice, st));
-    }
-    HIP_TRY(launch_place(g, pg, prev ? 1 : 0, pdx, pdy, (int)height, (int)width,
-                         a->stage_out.as<float>(), st));
-    HIP_TRY(hipMemcpyAsync(origins, a->stage_out.p, n * 16, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    return PIV_OK;
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!a || !grid || !origins || (prev && (!prev_dx || !prev_dy)))
+            return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+        std::lock_guard<std::mutex> lk(a->mu);
+        PIV_TRY(set_device(a->device));
+        GridDev g{(int)grid->window, (int)grid->step, (int)grid->nx, (int)grid->ny, (int)grid->x0, (int)grid->y0};
+        GridDev pg{};
+        const size_t n = (size_t)g.nx * g.ny;
+        size_t pn = 0;
+        if (prev) {
+            pg = GridDev{(int)prev->window, (int)prev->step, (int)prev->nx, (int)prev->ny, (int)prev->x0, (int)prev->y0};
+            pn = (size_t)pg.nx * pg.ny;
+        }
+        PIV_TRY(a->stage_in.reserve(2 * pn * sizeof(float) + 16));
+        PIV_TRY(a->stage_out.reserve(n * 4 * sizeof(float)));
+        float *pdx = a->stage_in.as<float>(), *pdy = pdx + pn;
+        hipStream_t st = a->stream;
+        if (prev) {
+            HIP_TRY(hipMemcpyAsync(pdx, prev_dx, pn * 4, hipMemcpyHostToDevice, st));
+            HIP_TRY(hipMemcpyAsync(pdy, prev_dy, pn * 4, hipMemcpyHostToDevice, st));
+        }
+        HIP_TRY(launch_place(g, pg, prev ? 1 : 0, pdx, pdy, (int)height, (int)width,
+                             a->stage_out.as<float>(), st));
+        HIP_TRY(hipMemcpyAsync(origins, a->stage_out.p, n * 16, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        return PIV_OK;
+    });
 }
 
 piv_status piv_correlate_windows(piv_analyzer *a, const piv_frame *fa, const piv_frame *fb,
@@ -1164,293 +1197,324 @@ piv_status piv_correlate_windows(piv_analyzer *a, const piv_frame *fa, const piv
                                  float *dx, float *dy, float *ppr, uint8_t *flags,
                                  int32_t *peak_index)
 {
-    if (!a || !fa || !fb || !origins || !dx || !dy || !ppr || !flags)
-        return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
-    if (fa->H != fb->H || fa->W != fb->W || fa->fmt != fb->fmt || fa->pitch != fb->pitch)
-        return fail(PIV_ERR_SHAPE_MISMATCH, "frames differ in shape or format");
-    if (!correlate_supported((int)window))
-        return fail(PIV_ERR_UNSUPPORTED, "window size %u has no gfx950 kernel", window);
-    if ((int)window > fa->W || (int)window > fa->H)
-        return fail(PIV_ERR_IMAGE_TOO_SMALL, "window %u larger than the frame", window);
-    if (count == 0) return PIV_OK;
-    for (uint32_t k = 0; k < count; ++k) {
-        // origins must keep the (n+1)^2 bilinear footprint inside the frame
-        const float ax = origins[4 * k], ay = origins[4 * k + 1], bx = origins[4 * k + 2], by = origins[4 * k + 3];
-        if (!(ax >= 0 && ay >= 0 && bx >= 0 && by >= 0 && ax + window <= fa->W && bx + window <= fa->W &&
-              ay + window <= fa->H && by + window <= fa->H))
-            return fail(PIV_ERR_INVALID_ARGUMENT, "origin %u outside the frame", k);
-    }
-    std::lock_guard<std::mutex> lk(a->mu);
-    PIV_TRY(set_device(a->device));
-    const size_t o_pp = align_up(count * 16, 256);
-    const size_t o_sig = o_pp + align_up(sizeof(PassPair), 256);
-    const size_t o_rec = o_sig + align_up(count, 256);
-    PIV_TRY(a->stage_in.reserve(o_rec + count * sizeof(PeakRec)));
-    PIV_TRY(a->stage_out.reserve((size_t)count * 17 + 1024));
-    char *din = a->stage_in.as<char>();
-    float *d_org = reinterpret_cast<float *>(din);
-    auto *d_pp = reinterpret_cast<PassPair *>(din + o_pp);
-    float *o_dx = a->stage_out.as<float>(), *o_dy = o_dx + count, *o_ppr = o_dy + count;
-    int32_t *o_pk = reinterpret_cast<int32_t *>(o_ppr + count);
-    uint8_t *o_fl = reinterpret_cast<uint8_t *>(o_pk + count);
-    PassPair d{};
-    d.img_a = fa->img;
-    d.img_b = fb->img;
-    d.use_mask = (fa->has_sat && fb->has_sat) ? 1 : 0;
-    d.sat_a = fa->has_sat ? fa->sat_ptr : nullptr;
-    d.sat_b = fb->has_sat ? fb->sat_ptr : nullptr;
-    d.sat_cells = fa->cells;
-    d.origins = d_org;
-    d.origins_given = 1;
-    d.sig = reinterpret_cast<uint8_t *>(din + o_sig);
-    d.peaks = reinterpret_cast<PeakRec *>(din + o_rec);
-    d.out_dx = o_dx;
-    d.out_dy = o_dy;
-    d.out_ppr = o_ppr;
-    d.out_flags = o_fl;
-    d.out_peak = o_pk;
-    d.H = fa->H;
-    d.W = fa->W;
-    d.pitch = fa->pitch;
-    d.fmt = fa->fmt;
-    d.grid = GridDev{(int)window, 1, (int)count, 1, 0, 0};
-    d.min_mean = a->cfg.min_mean;
-    d.min_std = a->cfg.min_std;
-    hipStream_t st = a->stream;
-    HIP_TRY(hipMemcpyAsync(d_org, origins, count * 16, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(d_pp, &d, sizeof d, hipMemcpyHostToDevice, st));
-    if (const int mp = mf_precision(a, (int)window)) {
-        HIP_TRY(launch_place_pass(d_pp, 1, count, st));
-        HIP_TRY(launch_correlate_mf((int)window, fa->fmt, mp, d_pp, 1, count, st));
-        HIP_TRY(launch_peak_finish(d_pp, 1, count, st, true));
-    } else {
-        HIP_TRY(launch_pass((int)window, fa->fmt, variant_of(a, (int)window, fa->fmt), d_pp, 1, count, st));
-    }
-    HIP_TRY(hipMemcpyAsync(dx, o_dx, count * 4, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(dy, o_dy, count * 4, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(ppr, o_ppr, count * 4, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(flags, o_fl, count, hipMemcpyDeviceToHost, st));
-    if (peak_index) HIP_TRY(hipMemcpyAsync(peak_index, o_pk, count * 4, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    return PIV_OK;
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!a || !fa || !fb || !origins || !dx || !dy || !ppr || !flags)
+            return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+        if (fa->H != fb->H || fa->W != fb->W || fa->fmt != fb->fmt || fa->pitch != fb->pitch)
+            return fail(PIV_ERR_SHAPE_MISMATCH, "frames differ in shape or format");
+        if (!correlate_supported((int)window))
+            return fail(PIV_ERR_UNSUPPORTED, "window size %u has no gfx950 kernel", window);
+        if ((int)window > fa->W || (int)window > fa->H)
+            return fail(PIV_ERR_IMAGE_TOO_SMALL, "window %u larger than the frame", window);
+        if (count == 0) return PIV_OK;
+        for (uint32_t k = 0; k < count; ++k) {
+            // origins must keep the (n+1)^2 bilinear footprint inside the frame
+            const float ax = origins[4 * k], ay = origins[4 * k + 1], bx = origins[4 * k + 2], by = origins[4 * k + 3];
+            if (!(ax >= 0 && ay >= 0 && bx >= 0 && by >= 0 && ax + window <= fa->W && bx + window <= fa->W &&
+                  ay + window <= fa->H && by + window <= fa->H))
+                return fail(PIV_ERR_INVALID_ARGUMENT, "origin %u outside the frame", k);
+        }
+        std::lock_guard<std::mutex> lk(a->mu);
+        PIV_TRY(set_device(a->device));
+        const size_t o_pp = align_up(count * 16, 256);
+        const size_t o_sig = o_pp + align_up(sizeof(PassPair), 256);
+        const size_t o_rec = o_sig + align_up(count, 256);
+        PIV_TRY(a->stage_in.reserve(o_rec + count * sizeof(PeakRec)));
+        PIV_TRY(a->stage_out.reserve((size_t)count * 17 + 1024));
+        char *din = a->stage_in.as<char>();
+        float *d_org = reinterpret_cast<float *>(din);
+        auto *d_pp = reinterpret_cast<PassPair *>(din + o_pp);
+        float *o_dx = a->stage_out.as<float>(), *o_dy = o_dx + count, *o_ppr = o_dy + count;
+        int32_t *o_pk = reinterpret_cast<int32_t *>(o_ppr + count);
+        uint8_t *o_fl = reinterpret_cast<uint8_t *>(o_pk + count);
+        PassPair d{};
+        d.img_a = fa->img;
+        d.img_b = fb->img;
+        d.use_mask = (fa->has_sat && fb->has_sat) ? 1 : 0;
+        d.sat_a = fa->has_sat ? fa->sat_ptr : nullptr;
+        d.sat_b = fb->has_sat ? fb->sat_ptr : nullptr;
+        d.sat_cells = fa->cells;
+        d.origins = d_org;
+        d.origins_given = 1;
+        d.sig = reinterpret_cast<uint8_t *>(din + o_sig);
+        d.peaks = reinterpret_cast<PeakRec *>(din + o_rec);
+        d.out_dx = o_dx;
+        d.out_dy = o_dy;
+        d.out_ppr = o_ppr;
+        d.out_flags = o_fl;
+        d.out_peak = o_pk;
+        d.H = fa->H;
+        d.W = fa->W;
+        d.pitch = fa->pitch;
+        d.fmt = fa->fmt;
+        d.grid = GridDev{(int)window, 1, (int)count, 1, 0, 0};
+        d.min_mean = a->cfg.min_mean;
+        d.min_std = a->cfg.min_std;
+        hipStream_t st = a->stream;
+        HIP_TRY(hipMemcpyAsync(d_org, origins, count * 16, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(d_pp, &d, sizeof d, hipMemcpyHostToDevice, st));
+        if (const int mp = mf_precision(a, (int)window)) {
+            HIP_TRY(launch_place_pass(d_pp, 1, count, st));
+            HIP_TRY(launch_correlate_mf((int)window, fa->fmt, mp, d_pp, 1, count, st));
+            HIP_TRY(launch_peak_finish(d_pp, 1, count, st, true));
+        } else {
+            HIP_TRY(launch_pass((int)window, fa->fmt, variant_of(a, (int)window, fa->fmt), d_pp, 1, count, st));
+        }
+        HIP_TRY(hipMemcpyAsync(dx, o_dx, count * 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(dy, o_dy, count * 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(ppr, o_ppr, count * 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(flags, o_fl, count, hipMemcpyDeviceToHost, st));
+        if (peak_index) HIP_TRY(hipMemcpyAsync(peak_index, o_pk, count * 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        return PIV_OK;
+    });
 }
 
 piv_status piv_validate_field(piv_analyzer *a, uint32_t ny, uint32_t nx, float *dx, float *dy,
                               const float *ppr, uint8_t *flags, float max_displacement,
                               piv_summary *summary)
 {
-    if (!a || !dx || !dy || !ppr || !flags) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
-    if (nx == 0 || ny == 0) return fail(PIV_ERR_INVALID_ARGUMENT, "empty field");
-    std::lock_guard<std::mutex> lk(a->mu);
-    PIV_TRY(set_device(a->device));
-    const size_t n = (size_t)nx * ny;
-    const size_t fb = align_up(n * 13, 256);
-    PIV_TRY(a->stage_in.reserve(2 * fb + align_up(n, 256) + 1024));
-    char *b = a->stage_in.as<char>();
-    float *r_dx = reinterpret_cast<float *>(b), *r_dy = r_dx + n, *r_ppr = r_dy + n;
-    uint8_t *r_fl = reinterpret_cast<uint8_t *>(r_ppr + n);
-    float *v_dx = reinterpret_cast<float *>(b + fb), *v_dy = v_dx + n, *v_ppr = v_dy + n;
-    uint8_t *v_fl = reinterpret_cast<uint8_t *>(v_ppr + n);
-    uint8_t *snapd = reinterpret_cast<uint8_t *>(b + 2 * fb);
-    char *tail = b + 2 * fb + align_up(n, 256);
-    auto *d_job = reinterpret_cast<ValidatePair *>(tail);
-    auto *d_sum = reinterpret_cast<uint32_t *>(tail + 512);
-    ValidatePair v{};
-    v.raw_dx = r_dx;
-    v.raw_dy = r_dy;
-    v.raw_ppr = r_ppr;
-    v.raw_flags = r_fl;
-    v.dx = v_dx;
-    v.dy = v_dy;
-    v.ppr = v_ppr;
-    v.flags = v_fl;
-    v.snapshot = snapd;
-    v.summary = d_sum;
-    v.nx = (int)nx;
-    v.ny = (int)ny;
-    v.ppr_threshold = a->cfg.ppr_threshold;
-    v.median_threshold = a->cfg.median_threshold;
-    v.median_eps = a->cfg.median_eps;
-    v.max_displacement = max_displacement;
-    v.replace = a->cfg.replace ? 1 : 0;
-    uint32_t hs[5];
-    hipStream_t st = a->stream;
-    HIP_TRY(hipMemcpyAsync(r_dx, dx, n * 4, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(r_dy, dy, n * 4, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(r_ppr, ppr, n * 4, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(r_fl, flags, n, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(d_job, &v, sizeof v, hipMemcpyHostToDevice, st));
-    HIP_TRY(launch_zero_u32(d_sum, 5, st));
-    HIP_TRY(launch_validate(d_job, 1, (uint32_t)n, st));
-    HIP_TRY(hipMemcpyAsync(dx, v_dx, n * 4, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(dy, v_dy, n * 4, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(flags, v_fl, n, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(hs, d_sum, sizeof hs, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    if (summary) to_summary(hs, summary);
-    return PIV_OK;
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!a || !dx || !dy || !ppr || !flags) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+        if (nx == 0 || ny == 0) return fail(PIV_ERR_INVALID_ARGUMENT, "empty field");
+        std::lock_guard<std::mutex> lk(a->mu);
+        PIV_TRY(set_device(a->device));
+        const size_t n = (size_t)nx * ny;
+        const size_t fb = align_up(n * 13, 256);
+        PIV_TRY(a->stage_in.reserve(2 * fb + align_up(n, 256) + 1024));
+        char *b = a->stage_in.as<char>();
+        float *r_dx = reinterpret_cast<float *>(b), *r_dy = r_dx + n, *r_ppr = r_dy + n;
+        uint8_t *r_fl = reinterpret_cast<uint8_t *>(r_ppr + n);
+        float *v_dx = reinterpret_cast<float *>(b + fb), *v_dy = v_dx + n, *v_ppr = v_dy + n;
+        uint8_t *v_fl = reinterpret_cast<uint8_t *>(v_ppr + n);
+        uint8_t *snapd = reinterpret_cast<uint8_t *>(b + 2 * fb);
+        char *tail = b + 2 * fb + align_up(n, 256);
+        auto *d_job = reinterpret_cast<ValidatePair *>(tail);
+        auto *d_sum = reinterpret_cast<uint32_t *>(tail + 512);
+        ValidatePair v{};
+        v.raw_dx = r_dx;
+        v.raw_dy = r_dy;
+        v.raw_ppr = r_ppr;
+        v.raw_flags = r_fl;
+        v.dx = v_dx;
+        v.dy = v_dy;
+        v.ppr = v_ppr;
+        v.flags = v_fl;
+        v.snapshot = snapd;
+        v.summary = d_sum;
+        v.nx = (int)nx;
+        v.ny = (int)ny;
+        v.ppr_threshold = a->cfg.ppr_threshold;
+        v.median_threshold = a->cfg.median_threshold;
+        v.median_eps = a->cfg.median_eps;
+        v.max_displacement = max_displacement;
+        v.replace = a->cfg.replace ? 1 : 0;
+        uint32_t hs[5];
+        hipStream_t st = a->stream;
+        HIP_TRY(hipMemcpyAsync(r_dx, dx, n * 4, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(r_dy, dy, n * 4, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(r_ppr, ppr, n * 4, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(r_fl, flags, n, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(d_job, &v, sizeof v, hipMemcpyHostToDevice, st));
+        HIP_TRY(launch_zero_u32(d_sum, 5, st));
+        HIP_TRY(launch_validate(d_job, 1, (uint32_t)n, st));
+        HIP_TRY(hipMemcpyAsync(dx, v_dx, n * 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(dy, v_dy, n * 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(flags, v_fl, n, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(hs, d_sum, sizeof hs, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        if (summary) to_summary(hs, summary);
+        return PIV_OK;
+    });
 }
 
 piv_status piv_background_create(piv_analyzer *a, piv_background_mode mode, uint32_t height,
                                  uint32_t width, piv_background **out)
 {
-    if (!a || !out) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
-    *out = nullptr;
-    if ((int)mode != 0 && (int)mode != 1) return fail(PIV_ERR_INVALID_ARGUMENT, "unknown background mode");
-    if (height == 0 || width == 0 || height > (1u << 15) || width > (1u << 15))
-        return fail(PIV_ERR_INVALID_ARGUMENT, "bad frame size %ux%u", height, width);
-    std::lock_guard<std::mutex> lk(a->mu);
-    PIV_TRY(set_device(a->device));
-    auto *bg = new (std::nothrow) piv_background();
-    if (!bg) return fail(PIV_ERR_OOM, "host allocation failed");
-    bg->an = a;
-    bg->mode = (int)mode;
-    bg->H = (int)height;
-    bg->W = (int)width;
-    piv_status s = bg->acc.reserve((size_t)height * width * sizeof(float));
-    if (s != PIV_OK) {
-        delete bg;
-        return s;
-    }
-    *out = bg;
-    return PIV_OK;
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!a || !out) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+        *out = nullptr;
+        if ((int)mode != 0 && (int)mode != 1) return fail(PIV_ERR_INVALID_ARGUMENT, "unknown background mode");
+        if (height == 0 || width == 0 || height > (1u << 15) || width > (1u << 15))
+            return fail(PIV_ERR_INVALID_ARGUMENT, "bad frame size %ux%u", height, width);
+        std::lock_guard<std::mutex> lk(a->mu);
+        PIV_TRY(set_device(a->device));
+        auto *bg = new (std::nothrow) piv_background();
+        if (!bg) return fail(PIV_ERR_OOM, "host allocation failed");
+        bg->an = a;
+        bg->mode = (int)mode;
+        bg->H = (int)height;
+        bg->W = (int)width;
+        piv_status s = bg->acc.reserve((size_t)height * width * sizeof(float));
+        if (s != PIV_OK) {
+            delete bg;
+            return s;
+        }
+        *out = bg;
+        return PIV_OK;
+    });
 }
 
 piv_status piv_background_add(piv_background *bg, const void *pixels, piv_pixel_format fmt,
                               size_t row_stride, int on_device)
 {
-    if (!bg) return fail(PIV_ERR_INVALID_ARGUMENT, "null background");
-    PIV_TRY(check_pixels_args(pixels, fmt, (uint32_t)bg->H, (uint32_t)bg->W, row_stride));
-    if (bg->finished) return fail(PIV_ERR_INVALID_ARGUMENT, "background already finished");
-    piv_analyzer *a = bg->an;
-    std::lock_guard<std::mutex> lk(a->mu);
-    PIV_TRY(set_device(a->device));
-    const size_t bpp = (size_t)bytes_per_px(fmt);
-    const void *src = pixels;
-    int pitch = (int)(row_stride / bpp);
-    if (!on_device) {
-        PIV_TRY(bg->stage.reserve((size_t)bg->H * bg->W * bpp));
-        HIP_TRY(hipMemcpy2DAsync(bg->stage.p, bg->W * bpp, pixels, row_stride, bg->W * bpp, bg->H,
-                                 hipMemcpyHostToDevice, a->stream));
-        src = bg->stage.p;
-        pitch = bg->W;
-    }
-    HIP_TRY(launch_bg_add(fmt, src, pitch, bg->H, bg->W, bg->mode, bg->count == 0 ? 1 : 0,
-                          bg->acc.as<float>(), a->stream));
-    HIP_TRY(hipStreamSynchronize(a->stream));
-    bg->count += 1;
-    return PIV_OK;
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!bg) return fail(PIV_ERR_INVALID_ARGUMENT, "null background");
+        PIV_TRY(check_pixels_args(pixels, fmt, (uint32_t)bg->H, (uint32_t)bg->W, row_stride));
+        if (bg->finished) return fail(PIV_ERR_INVALID_ARGUMENT, "background already finished");
+        piv_analyzer *a = bg->an;
+        std::lock_guard<std::mutex> lk(a->mu);
+        PIV_TRY(set_device(a->device));
+        const size_t bpp = (size_t)bytes_per_px(fmt);
+        const void *src = pixels;
+        int pitch = (int)(row_stride / bpp);
+        if (!on_device) {
+            PIV_TRY(bg->stage.reserve((size_t)bg->H * bg->W * bpp));
+            HIP_TRY(hipMemcpy2DAsync(bg->stage.p, bg->W * bpp, pixels, row_stride, bg->W * bpp, bg->H,
+                                     hipMemcpyHostToDevice, a->stream));
+            src = bg->stage.p;
+            pitch = bg->W;
+        }
+        HIP_TRY(launch_bg_add(fmt, src, pitch, bg->H, bg->W, bg->mode, bg->count == 0 ? 1 : 0,
+                              bg->acc.as<float>(), a->stream));
+        HIP_TRY(hipStreamSynchronize(a->stream));
+        bg->count += 1;
+        return PIV_OK;
+    });
 }
 
 piv_status piv_background_finish(piv_background *bg, float *out)
 {
-    if (!bg) return fail(PIV_ERR_INVALID_ARGUMENT, "null background");
-    if (bg->count == 0) return fail(PIV_ERR_INVALID_ARGUMENT, "no frames were added");
-    piv_analyzer *a = bg->an;
-    std::lock_guard<std::mutex> lk(a->mu);
-    PIV_TRY(set_device(a->device));
-    const long n = (long)bg->H * bg->W;
-    if (!bg->finished && bg->mode == 1)
-        HIP_TRY(launch_bg_finish(bg->acc.as<float>(), n, (float)bg->count, a->stream));
-    bg->finished = true;
-    if (out) HIP_TRY(hipMemcpyAsync(out, bg->acc.p, n * sizeof(float), hipMemcpyDeviceToHost, a->stream));
-    HIP_TRY(hipStreamSynchronize(a->stream));
-    return PIV_OK;
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!bg) return fail(PIV_ERR_INVALID_ARGUMENT, "null background");
+        if (bg->count == 0) return fail(PIV_ERR_INVALID_ARGUMENT, "no frames were added");
+        piv_analyzer *a = bg->an;
+        std::lock_guard<std::mutex> lk(a->mu);
+        PIV_TRY(set_device(a->device));
+        const long n = (long)bg->H * bg->W;
+        if (!bg->finished && bg->mode == 1)
+            HIP_TRY(launch_bg_finish(bg->acc.as<float>(), n, (float)bg->count, a->stream));
+        bg->finished = true;
+        if (out) HIP_TRY(hipMemcpyAsync(out, bg->acc.p, n * sizeof(float), hipMemcpyDeviceToHost, a->stream));
+        HIP_TRY(hipStreamSynchronize(a->stream));
+        return PIV_OK;
+    });
 }
 
 piv_status piv_background_device(const piv_background *bg, const float **device_ptr)
 {
-    if (!bg || !device_ptr) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
-    if (!bg->finished) return fail(PIV_ERR_INVALID_ARGUMENT, "background not finished");
-    *device_ptr = bg->acc.as<float>();
-    return PIV_OK;
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!bg || !device_ptr) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+        if (!bg->finished) return fail(PIV_ERR_INVALID_ARGUMENT, "background not finished");
+        *device_ptr = bg->acc.as<float>();
+        return PIV_OK;
+    });
 }
 
 void piv_background_destroy(piv_background *bg)
 {
-    if (!bg) return;
-    (void)hipSetDevice(bg->an->device);
-    delete bg;
+    try {
+        if (!bg) return;
+        (void)hipSetDevice(bg->an->device);
+        delete bg;
+    } catch (...) {
+    }
 }
 
 piv_status piv_device_alloc(int device, size_t bytes, void **out)
 {
-    if (!out) return fail(PIV_ERR_INVALID_ARGUMENT, "out is null");
-    *out = nullptr;
-    PIV_TRY(set_device(device));
-    HIP_TRY(hipMalloc(out, bytes ? bytes : 1));
-    return PIV_OK;
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!out) return fail(PIV_ERR_INVALID_ARGUMENT, "out is null");
+        *out = nullptr;
+        PIV_TRY(set_device(device));
+        HIP_TRY(hipMalloc(out, bytes ? bytes : 1));
+        return PIV_OK;
+    });
 }
 
 piv_status piv_device_free(int device, void *ptr)
 {
-    if (!ptr) return PIV_OK;
-    PIV_TRY(set_device(device));
-    HIP_TRY(hipFree(ptr));
-    return PIV_OK;
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!ptr) return PIV_OK;
+        PIV_TRY(set_device(device));
+        HIP_TRY(hipFree(ptr));
+        return PIV_OK;
+    });
 }
 
 piv_status piv_copy_to_device(int device, void *dst, const void *src, size_t bytes)
 {
-    if (!dst || !src) return fail(PIV_ERR_INVALID_ARGUMENT, "null pointer");
-    PIV_TRY(set_device(device));
-    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
-    return PIV_OK;
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!dst || !src) return fail(PIV_ERR_INVALID_ARGUMENT, "null pointer");
+        PIV_TRY(set_device(device));
+        HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+        return PIV_OK;
+    });
 }
 
 piv_status piv_copy_to_host(int device, void *dst, const void *src, size_t bytes)
 {
-    if (!dst || !src) return fail(PIV_ERR_INVALID_ARGUMENT, "null pointer");
-    PIV_TRY(set_device(device));
-    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
-    return PIV_OK;
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!dst || !src) return fail(PIV_ERR_INVALID_ARGUMENT, "null pointer");
+        PIV_TRY(set_device(device));
+        HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+        return PIV_OK;
+    });
 }
 
 piv_status piv_analyzer_synchronize(piv_analyzer *a)
 {
-    if (!a) return fail(PIV_ERR_INVALID_ARGUMENT, "null analyzer");
-    std::lock_guard<std::mutex> lk(a->mu);
-    PIV_TRY(set_device(a->device));
-    HIP_TRY(hipStreamSynchronize(a->stream));
-    return PIV_OK;
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!a) return fail(PIV_ERR_INVALID_ARGUMENT, "null analyzer");
+        std::lock_guard<std::mutex> lk(a->mu);
+        PIV_TRY(set_device(a->device));
+        HIP_TRY(hipStreamSynchronize(a->stream));
+        return PIV_OK;
+    });
 }
 
 piv_status piv_profile_enable(piv_analyzer *a, int enable)
 {
-    if (!a) return fail(PIV_ERR_INVALID_ARGUMENT, "null analyzer");
-    std::lock_guard<std::mutex> lk(a->mu);
-    PIV_TRY(set_device(a->device));
-    HIP_TRY(hipStreamSynchronize(a->stream));
-    prof_harvest(a);
-    a->prof_stats.clear();
-    a->prof_on = enable != 0;
-    return PIV_OK;
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!a) return fail(PIV_ERR_INVALID_ARGUMENT, "null analyzer");
+        std::lock_guard<std::mutex> lk(a->mu);
+        PIV_TRY(set_device(a->device));
+        HIP_TRY(hipStreamSynchronize(a->stream));
+        prof_harvest(a);
+        a->prof_stats.clear();
+        a->prof_on = enable != 0;
+        return PIV_OK;
+    });
 }
 
 piv_status piv_profile_read(piv_analyzer *a, piv_kernel_stat *out, uint32_t capacity,
                             uint32_t *count)
 {
-    if (!a || !count) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
-    std::lock_guard<std::mutex> lk(a->mu);
-    PIV_TRY(set_device(a->device));
-    HIP_TRY(hipStreamSynchronize(a->stream));
-    prof_harvest(a);
-    uint32_t i = 0;
-    for (auto &kv : a->prof_stats) {
-        if (out && i < capacity) out[i] = kv.second;
-        ++i;
-    }
-    *count = i;
-    return PIV_OK;
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!a || !count) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+        std::lock_guard<std::mutex> lk(a->mu);
+        PIV_TRY(set_device(a->device));
+        HIP_TRY(hipStreamSynchronize(a->stream));
+        prof_harvest(a);
+        uint32_t i = 0;
+        for (auto &kv : a->prof_stats) {
+            if (out && i < capacity) out[i] = kv.second;
+            ++i;
+        }
+        *count = i;
+        return PIV_OK;
+    });
 }
 
 piv_status piv_analyzer_set_precision(piv_analyzer *a, piv_precision p)
 {
-    if (!a) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
-    if ((int)p < 0 || (int)p > 2) return fail(PIV_ERR_INVALID_ARGUMENT, "unknown precision %d", (int)p);
-    std::lock_guard<std::mutex> lk(a->mu);
-    a->precision = (int)p;
-    return PIV_OK;
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!a) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+        if ((int)p < 0 || (int)p > 2) return fail(PIV_ERR_INVALID_ARGUMENT, "unknown precision %d", (int)p);
+        std::lock_guard<std::mutex> lk(a->mu);
+        a->precision = (int)p;
+        return PIV_OK;
+    });
 }
 
 uint32_t piv_kernel_variant_count(uint32_t window)
@@ -1461,147 +1525,155 @@ uint32_t piv_kernel_variant_count(uint32_t window)
 piv_status piv_kernel_variant_set(piv_analyzer *a, uint32_t window, piv_pixel_format fmt,
                                   uint32_t variant)
 {
-    if (!a) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
-    if (!correlate_supported((int)window))
-        return fail(PIV_ERR_UNSUPPORTED, "window size %u has no gfx950 kernel", window);
-    if ((int)fmt != FMT_U8 && (int)fmt != FMT_U16 && (int)fmt != FMT_F32)
-        return fail(PIV_ERR_INVALID_ARGUMENT, "unknown pixel format %d", (int)fmt);
-    if (variant >= (uint32_t)corr_variants((int)window).count)
-        return fail(PIV_ERR_INVALID_ARGUMENT, "window %u has %d variants", window,
-                    corr_variants((int)window).count);
-    std::lock_guard<std::mutex> lk(a->mu);
-    a->variants[{(int)window, (int)fmt}] = (int)variant;
-    return PIV_OK;
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!a) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+        if (!correlate_supported((int)window))
+            return fail(PIV_ERR_UNSUPPORTED, "window size %u has no gfx950 kernel", window);
+        if ((int)fmt != FMT_U8 && (int)fmt != FMT_U16 && (int)fmt != FMT_F32)
+            return fail(PIV_ERR_INVALID_ARGUMENT, "unknown pixel format %d", (int)fmt);
+        if (variant >= (uint32_t)corr_variants((int)window).count)
+            return fail(PIV_ERR_INVALID_ARGUMENT, "window %u has %d variants", window,
+                        corr_variants((int)window).count);
+        std::lock_guard<std::mutex> lk(a->mu);
+        a->variants[{(int)window, (int)fmt}] = (int)variant;
+        return PIV_OK;
+    });
 }
 
 piv_status piv_autotune(piv_analyzer *a, const piv_frame *const *fa, const piv_frame *const *fb,
                         uint32_t npairs, uint32_t repeats, piv_tune_entry *out, uint32_t capacity,
                         uint32_t *count)
 {
-    if (!a || !fa || !fb || !count || npairs == 0) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
-    if (repeats == 0) repeats = 1;
-    std::lock_guard<std::mutex> lk(a->mu);
-    PIV_TRY(set_device(a->device));
-    const int fmt = fa[0]->fmt;
-    std::vector<PassPlan> plans;
-    PIV_TRY(plan_passes(a, fa[0]->H, fa[0]->W, plans));
-    std::vector<int> sizes;
-    for (const PassPlan &pl : plans)
-        if (std::find(sizes.begin(), sizes.end(), pl.n) == sizes.end()) sizes.push_back(pl.n);
-    int maxv = 1;
-    for (int n : sizes) maxv = std::max(maxv, corr_variants(n).count);
-    HIP_TRY(hipStreamSynchronize(a->stream));
-    prof_harvest(a);
-    const bool saved_on = a->prof_on;
-    const auto saved_stats = a->prof_stats;
-    std::map<int, std::vector<double>> ms;
-    for (int n : sizes) ms[n].assign(corr_variants(n).count, 0.0);
-    piv_status s = PIV_OK;
-    // Two sweeps over the variants, each variant keeping its faster sweep: kernel durations
-    // fall over the first launches of a process as the GPU clocks up, which in a single
-    // sweep would favour the variants timed last.
-    for (int sweep = 0; sweep < 2 && s == PIV_OK; ++sweep)
-    for (int v = 0; v < maxv && s == PIV_OK; ++v) {
-        for (int n : sizes) a->variants[{n, fmt}] = std::min(v, corr_variants(n).count - 1);
-        std::vector<PassPlan> pl;
-        std::vector<FieldPtrs> raw, val;
-        std::vector<float *> org;
-        std::vector<int32_t *> pk;
-        uint32_t *sums = nullptr;
-        a->prof_on = false;  // one untimed run per variant (first launch of each kernel)
-        s = run_batch(a, fa, fb, npairs, BatchOut{}, false, pl, raw, val, org, pk, &sums);
-        if (s != PIV_OK) break;
-        if (hipStreamSynchronize(a->stream) != hipSuccess) {
-            s = fail(PIV_ERR_DEVICE, "autotune run failed");
-            break;
-        }
-        a->prof_stats.clear();
-        a->prof_on = true;
-        for (uint32_t r = 0; r < repeats && s == PIV_OK; ++r)
-            s = run_batch(a, fa, fb, npairs, BatchOut{}, false, pl, raw, val, org, pk, &sums);
-        if (s != PIV_OK) break;
-        if (hipStreamSynchronize(a->stream) != hipSuccess) {
-            s = fail(PIV_ERR_DEVICE, "autotune run failed");
-            break;
-        }
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!a || !fa || !fb || !count || npairs == 0) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+        if (repeats == 0) repeats = 1;
+        std::lock_guard<std::mutex> lk(a->mu);
+        PIV_TRY(set_device(a->device));
+        const int fmt = fa[0]->fmt;
+        std::vector<PassPlan> plans;
+        PIV_TRY(plan_passes(a, fa[0]->H, fa[0]->W, plans));
+        std::vector<int> sizes;
+        for (const PassPlan &pl : plans)
+            if (std::find(sizes.begin(), sizes.end(), pl.n) == sizes.end()) sizes.push_back(pl.n);
+        int maxv = 1;
+        for (int n : sizes) maxv = std::max(maxv, corr_variants(n).count);
+        HIP_TRY(hipStreamSynchronize(a->stream));
         prof_harvest(a);
+        const bool saved_on = a->prof_on;
+        const auto saved_stats = a->prof_stats;
+        std::map<int, std::vector<double>> ms;
+        for (int n : sizes) ms[n].assign(corr_variants(n).count, 0.0);
+        piv_status s = PIV_OK;
+        // Two sweeps over the variants, each variant keeping its faster sweep: kernel durations
+        // fall over the first launches of a process as the GPU clocks up, which in a single
+        // sweep would favour the variants timed last.
+        for (int sweep = 0; sweep < 2 && s == PIV_OK; ++sweep)
+        for (int v = 0; v < maxv && s == PIV_OK; ++v) {
+            for (int n : sizes) a->variants[{n, fmt}] = std::min(v, corr_variants(n).count - 1);
+            std::vector<PassPlan> pl;
+            std::vector<FieldPtrs> raw, val;
+            std::vector<float *> org;
+            std::vector<int32_t *> pk;
+            uint32_t *sums = nullptr;
+            a->prof_on = false;  // one untimed run per variant (first launch of each kernel)
+            s = run_batch(a, fa, fb, npairs, BatchOut{}, false, pl, raw, val, org, pk, &sums);
+            if (s != PIV_OK) break;
+            if (hipStreamSynchronize(a->stream) != hipSuccess) {
+                s = fail(PIV_ERR_DEVICE, "autotune run failed");
+                break;
+            }
+            a->prof_stats.clear();
+            a->prof_on = true;
+            for (uint32_t r = 0; r < repeats && s == PIV_OK; ++r)
+                s = run_batch(a, fa, fb, npairs, BatchOut{}, false, pl, raw, val, org, pk, &sums);
+            if (s != PIV_OK) break;
+            if (hipStreamSynchronize(a->stream) != hipSuccess) {
+                s = fail(PIV_ERR_DEVICE, "autotune run failed");
+                break;
+            }
+            prof_harvest(a);
+            for (int n : sizes) {
+                if (v >= corr_variants(n).count) continue;
+                auto it = a->prof_stats.find({"correlate", n});
+                if (it != a->prof_stats.end() && it->second.launches) {
+                    const double t = it->second.total_ms / (double)it->second.launches;
+                    ms[n][v] = sweep == 0 ? t : std::min(ms[n][v], t);
+                }
+            }
+        }
+        a->prof_on = saved_on;
+        a->prof_stats = saved_stats;
+        if (s != PIV_OK) {
+            for (int n : sizes) a->variants.erase({n, fmt});
+            return s;
+        }
+        uint32_t i = 0;
         for (int n : sizes) {
-            if (v >= corr_variants(n).count) continue;
-            auto it = a->prof_stats.find({"correlate", n});
-            if (it != a->prof_stats.end() && it->second.launches) {
-                const double t = it->second.total_ms / (double)it->second.launches;
-                ms[n][v] = sweep == 0 ? t : std::min(ms[n][v], t);
+            const VariantSet vs = corr_variants(n);
+            int best = 0;
+            for (int v = 1; v < vs.count; ++v)
+                if (ms[n][v] < ms[n][best]) best = v;
+            a->variants[{n, fmt}] = best;
+            if (out && i < capacity) {
+                piv_tune_entry &e = out[i];
+                e = piv_tune_entry{};
+                e.window = (uint32_t)n;
+                e.fmt = fmt;
+                e.chosen = (uint32_t)best;
+                e.num_variants = (uint32_t)vs.count;
+                for (int v = 0; v < vs.count && v < 4; ++v) {
+                    e.occupancy[v] = (uint32_t)vs.occ[v];
+                    e.ms[v] = (float)ms[n][v];
+                }
             }
+            ++i;
         }
-    }
-    a->prof_on = saved_on;
-    a->prof_stats = saved_stats;
-    if (s != PIV_OK) {
-        for (int n : sizes) a->variants.erase({n, fmt});
-        return s;
-    }
-    uint32_t i = 0;
-    for (int n : sizes) {
-        const VariantSet vs = corr_variants(n);
-        int best = 0;
-        for (int v = 1; v < vs.count; ++v)
-            if (ms[n][v] < ms[n][best]) best = v;
-        a->variants[{n, fmt}] = best;
-        if (out && i < capacity) {
-            piv_tune_entry &e = out[i];
-            e = piv_tune_entry{};
-            e.window = (uint32_t)n;
-            e.fmt = fmt;
-            e.chosen = (uint32_t)best;
-            e.num_variants = (uint32_t)vs.count;
-            for (int v = 0; v < vs.count && v < 4; ++v) {
-                e.occupancy[v] = (uint32_t)vs.occ[v];
-                e.ms[v] = (float)ms[n][v];
-            }
-        }
-        ++i;
-    }
-    *count = i;
-    return PIV_OK;
+        *count = i;
+        return PIV_OK;
+    });
 }
 
 piv_status piv_extract_plane_device(int device, const void *pixels, piv_pixel_format fmt,
                                     size_t count, uint32_t channels, uint32_t plane, void *out)
 {
-    if (!pixels || !out) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
-    if ((int)fmt != FMT_U8 && (int)fmt != FMT_U16 && (int)fmt != FMT_F32)
-        return fail(PIV_ERR_INVALID_ARGUMENT, "unknown pixel format %d", (int)fmt);
-    if (channels < 1 || channels > 4 || plane >= channels)
-        return fail(PIV_ERR_INVALID_ARGUMENT, "plane %u of %u channels", plane, channels);
-    if (count == 0) return PIV_OK;
-    PIV_TRY(set_device(device));
-    hipStream_t st = nullptr;
-    HIP_TRY(hipStreamCreate(&st));
-    hipError_t e = launch_extract_plane((int)fmt, pixels, (long)count, (int)channels, (int)plane,
-                                        out, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    (void)hipStreamDestroy(st);
-    HIP_TRY(e);
-    return PIV_OK;
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!pixels || !out) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+        if ((int)fmt != FMT_U8 && (int)fmt != FMT_U16 && (int)fmt != FMT_F32)
+            return fail(PIV_ERR_INVALID_ARGUMENT, "unknown pixel format %d", (int)fmt);
+        if (channels < 1 || channels > 4 || plane >= channels)
+            return fail(PIV_ERR_INVALID_ARGUMENT, "plane %u of %u channels", plane, channels);
+        if (count == 0) return PIV_OK;
+        PIV_TRY(set_device(device));
+        hipStream_t st = nullptr;
+        HIP_TRY(hipStreamCreate(&st));
+        hipError_t e = launch_extract_plane((int)fmt, pixels, (long)count, (int)channels, (int)plane,
+                                            out, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        (void)hipStreamDestroy(st);
+        HIP_TRY(e);
+        return PIV_OK;
+    });
 }
 
 piv_status piv_normalize_pixels(int device, const void *pixels, piv_pixel_format fmt,
                                 size_t count, float *out)
 {
-    if (!pixels || !out) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
-    if ((int)fmt != FMT_U8 && (int)fmt != FMT_U16 && (int)fmt != FMT_F32)
-        return fail(PIV_ERR_INVALID_ARGUMENT, "unknown pixel format %d", (int)fmt);
-    PIV_TRY(set_device(device));
-    DevBuf in, o;
-    const size_t bpp = (size_t)bytes_per_px(fmt);
-    PIV_TRY(in.reserve(count * bpp + 4));
-    PIV_TRY(o.reserve(count * 4 + 4));
-    HIP_TRY(hipMemcpy(in.p, pixels, count * bpp, hipMemcpyHostToDevice));
-    HIP_TRY(launch_normalize(fmt, in.p, (long)count, o.as<float>(), nullptr));
-    HIP_TRY(hipMemcpy(out, o.p, count * 4, hipMemcpyDeviceToHost));
-    in.release();
-    o.release();
-    return PIV_OK;
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!pixels || !out) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+        if ((int)fmt != FMT_U8 && (int)fmt != FMT_U16 && (int)fmt != FMT_F32)
+            return fail(PIV_ERR_INVALID_ARGUMENT, "unknown pixel format %d", (int)fmt);
+        PIV_TRY(set_device(device));
+        DevBuf in, o;
+        const size_t bpp = (size_t)bytes_per_px(fmt);
+        PIV_TRY(in.reserve(count * bpp + 4));
+        PIV_TRY(o.reserve(count * 4 + 4));
+        HIP_TRY(hipMemcpy(in.p, pixels, count * bpp, hipMemcpyHostToDevice));
+        HIP_TRY(launch_normalize(fmt, in.p, (long)count, o.as<float>(), nullptr));
+        HIP_TRY(hipMemcpy(out, o.p, count * 4, hipMemcpyDeviceToHost));
+        in.release();
+        o.release();
+        return PIV_OK;
+    });
 }
 
 }  // extern "C"

@@ -26,6 +26,22 @@ using namespace piv;
 // Sets the calling thread's piv_last_error() message and returns `st`.
 piv_status fail(piv_status st, const char *fmt, ...);
 
+// Runs the body of a C entry point: a C++ exception never crosses the ABI, it becomes
+// PIV_ERR_OOM (std::bad_alloc) or PIV_ERR_DEVICE (anything else) with the message set.
+template <class F>
+piv_status guard(const char *entry, F &&body) noexcept
+{
+    try {
+        return body();
+    } catch (const std::bad_alloc &) {
+        return fail(PIV_ERR_OOM, "%s: host allocation failed", entry);
+    } catch (const std::exception &e) {
+        return fail(PIV_ERR_DEVICE, "%s: %s", entry, e.what());
+    } catch (...) {
+        return fail(PIV_ERR_DEVICE, "%s: unexpected exception", entry);
+    }
+}
+
 #define HIP_TRY(expr)                                                                         \
     do {                                                                                      \
         hipError_t e_ = (expr);                                                               \

@@ -158,77 +158,81 @@ piv_status piv_write_pair_npz(const char *dir, uint64_t frame_a, uint64_t frame_
                               const piv_grid *grid, const float *dx, const float *dy,
                               const float *peak_ratio, const uint8_t *flags)
 {
-    if (!dir || !grid || !dx || !dy || !peak_ratio || !flags)
-        return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
-    if (!(dt_s > 0.0)) return fail(PIV_ERR_INVALID_ARGUMENT, "dt must be positive");
-    const size_t nx = grid->nx, ny = grid->ny, n = nx * ny;
-    // grid.rs:60-74 centres, plus the ROI offset (write.rs grid_coords)
-    std::vector<float> xs(nx), ys(ny);
-    const float half = ((float)grid->window - 1.0f) / 2.0f;
-    for (size_t i = 0; i < nx; ++i)
-        xs[i] = ((float)(grid->x0 + i * grid->step) + half) + (float)roi_x;
-    for (size_t i = 0; i < ny; ++i)
-        ys[i] = ((float)(grid->y0 + i * grid->step) + half) + (float)roi_y;
-    // write.rs derive(): u = dx * (1 / dt) as f32, physical = u * (m_per_px as f32)
-    const float inv_dt = (float)(1.0 / dt_s);
-    std::vector<float> u(n), v(n), um, vm;
-    for (size_t i = 0; i < n; ++i) {
-        u[i] = dx[i] * inv_dt;
-        v[i] = dy[i] * inv_dt;
-    }
-    const bool physical = m_per_px > 0.0;
-    if (physical) {
-        const float k = (float)m_per_px;
-        um.resize(n);
-        vm.resize(n);
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!dir || !grid || !dx || !dy || !peak_ratio || !flags)
+            return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+        if (!(dt_s > 0.0)) return fail(PIV_ERR_INVALID_ARGUMENT, "dt must be positive");
+        const size_t nx = grid->nx, ny = grid->ny, n = nx * ny;
+        // grid.rs:60-74 centres, plus the ROI offset (write.rs grid_coords)
+        std::vector<float> xs(nx), ys(ny);
+        const float half = ((float)grid->window - 1.0f) / 2.0f;
+        for (size_t i = 0; i < nx; ++i)
+            xs[i] = ((float)(grid->x0 + i * grid->step) + half) + (float)roi_x;
+        for (size_t i = 0; i < ny; ++i)
+            ys[i] = ((float)(grid->y0 + i * grid->step) + half) + (float)roi_y;
+        // write.rs derive(): u = dx * (1 / dt) as f32, physical = u * (m_per_px as f32)
+        const float inv_dt = (float)(1.0 / dt_s);
+        std::vector<float> u(n), v(n), um, vm;
         for (size_t i = 0; i < n; ++i) {
-            um[i] = u[i] * k;
-            vm[i] = v[i] * k;
+            u[i] = dx[i] * inv_dt;
+            v[i] = dy[i] * inv_dt;
         }
-    }
-    const uint64_t frames[2] = {frame_a, frame_b};
-    std::vector<NpyArray> arrays;
-    arrays.push_back({"x", "<f4", {nx}, xs.data(), nx * 4});
-    arrays.push_back({"y", "<f4", {ny}, ys.data(), ny * 4});
-    arrays.push_back({"dx", "<f4", {ny, nx}, dx, n * 4});
-    arrays.push_back({"dy", "<f4", {ny, nx}, dy, n * 4});
-    arrays.push_back({"u", "<f4", {ny, nx}, u.data(), n * 4});
-    arrays.push_back({"v", "<f4", {ny, nx}, v.data(), n * 4});
-    if (physical) {
-        arrays.push_back({"u_mps", "<f4", {ny, nx}, um.data(), n * 4});
-        arrays.push_back({"v_mps", "<f4", {ny, nx}, vm.data(), n * 4});
-    }
-    arrays.push_back({"peak_ratio", "<f4", {ny, nx}, peak_ratio, n * 4});
-    arrays.push_back({"flags", "|u1", {ny, nx}, flags, n});
-    arrays.push_back({"frames", "<u8", {2}, frames, 16});
-    arrays.push_back({"t", "<f8", {}, &time_s, 8});
-    char name[64];
-    std::snprintf(name, sizeof name, "/pair_%06llu_%06llu.npz", (unsigned long long)frame_a,
-                  (unsigned long long)frame_b);
-    return write_npz(std::string(dir) + name, arrays);
+        const bool physical = m_per_px > 0.0;
+        if (physical) {
+            const float k = (float)m_per_px;
+            um.resize(n);
+            vm.resize(n);
+            for (size_t i = 0; i < n; ++i) {
+                um[i] = u[i] * k;
+                vm[i] = v[i] * k;
+            }
+        }
+        const uint64_t frames[2] = {frame_a, frame_b};
+        std::vector<NpyArray> arrays;
+        arrays.push_back({"x", "<f4", {nx}, xs.data(), nx * 4});
+        arrays.push_back({"y", "<f4", {ny}, ys.data(), ny * 4});
+        arrays.push_back({"dx", "<f4", {ny, nx}, dx, n * 4});
+        arrays.push_back({"dy", "<f4", {ny, nx}, dy, n * 4});
+        arrays.push_back({"u", "<f4", {ny, nx}, u.data(), n * 4});
+        arrays.push_back({"v", "<f4", {ny, nx}, v.data(), n * 4});
+        if (physical) {
+            arrays.push_back({"u_mps", "<f4", {ny, nx}, um.data(), n * 4});
+            arrays.push_back({"v_mps", "<f4", {ny, nx}, vm.data(), n * 4});
+        }
+        arrays.push_back({"peak_ratio", "<f4", {ny, nx}, peak_ratio, n * 4});
+        arrays.push_back({"flags", "|u1", {ny, nx}, flags, n});
+        arrays.push_back({"frames", "<u8", {2}, frames, 16});
+        arrays.push_back({"t", "<f8", {}, &time_s, 8});
+        char name[64];
+        std::snprintf(name, sizeof name, "/pair_%06llu_%06llu.npz", (unsigned long long)frame_a,
+                      (unsigned long long)frame_b);
+        return write_npz(std::string(dir) + name, arrays);
+    });
 }
 
 piv_status piv_write_stats_npz(const char *dir, uint32_t npairs, const double *t,
                                const piv_flow_stats *stats)
 {
-    if (!dir || (npairs && (!t || !stats))) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
-    // flow.rs:133-145 write_stats_npz / :147-156 columns: t (f64), then the f32 columns
-    const size_t n = npairs;
-    std::vector<float> col[5];
-    for (auto &c : col) c.resize(n);
-    for (size_t i = 0; i < n; ++i) {
-        col[0][i] = stats[i].vorticity_mean;
-        col[1][i] = stats[i].vorticity_abs_mean;
-        col[2][i] = stats[i].ke_mean;
-        col[3][i] = stats[i].ke_sum;
-        col[4][i] = stats[i].coverage;
-    }
-    static const char *names[5] = {"vorticity_mean", "vorticity_abs_mean", "ke_mean", "ke_sum",
-                                   "coverage"};
-    std::vector<NpyArray> arrays;
-    arrays.push_back({"t", "<f8", {n}, t, n * 8});
-    for (int c = 0; c < 5; ++c) arrays.push_back({names[c], "<f4", {n}, col[c].data(), n * 4});
-    return write_npz(std::string(dir) + "/stats.npz", arrays);
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!dir || (npairs && (!t || !stats))) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+        // flow.rs:133-145 write_stats_npz / :147-156 columns: t (f64), then the f32 columns
+        const size_t n = npairs;
+        std::vector<float> col[5];
+        for (auto &c : col) c.resize(n);
+        for (size_t i = 0; i < n; ++i) {
+            col[0][i] = stats[i].vorticity_mean;
+            col[1][i] = stats[i].vorticity_abs_mean;
+            col[2][i] = stats[i].ke_mean;
+            col[3][i] = stats[i].ke_sum;
+            col[4][i] = stats[i].coverage;
+        }
+        static const char *names[5] = {"vorticity_mean", "vorticity_abs_mean", "ke_mean", "ke_sum",
+                                       "coverage"};
+        std::vector<NpyArray> arrays;
+        arrays.push_back({"t", "<f8", {n}, t, n * 8});
+        for (int c = 0; c < 5; ++c) arrays.push_back({names[c], "<f4", {n}, col[c].data(), n * 4});
+        return write_npz(std::string(dir) + "/stats.npz", arrays);
+    });
 }
 
 }  // extern "C"

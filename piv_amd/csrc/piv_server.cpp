@@ -739,147 +739,154 @@ extern "C" {
 
 piv_status piv_server_create(int device, const piv_server_config *cfg, piv_server **out)
 {
-    if (!out) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
-    *out = nullptr;
-    PIV_TRY(set_device(device));
-    auto *s = new (std::nothrow) piv_server();
-    if (!s) return fail(PIV_ERR_OOM, "host allocation failed");
-    s->device = device;
-    if (cfg) s->cfg = *cfg;
-    if (s->cfg.hbm_budget == 0) s->cfg.hbm_budget = (size_t)4 << 30;
-    if (s->cfg.max_batch_pairs == 0) s->cfg.max_batch_pairs = 64;
-    if (s->cfg.max_inflight == 0) s->cfg.max_inflight = 2;
-    if (s->cfg.max_inflight > 8) s->cfg.max_inflight = 8;
-    s->lanes.resize(s->cfg.max_inflight);
-    for (Lane &L : s->lanes) {
-        if (hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&L.done, hipEventDisableTiming | hipEventBlockingSync) != hipSuccess) {
-            piv_server_destroy(s);
-            return fail(PIV_ERR_DEVICE, "stream creation failed");
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!out) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+        *out = nullptr;
+        PIV_TRY(set_device(device));
+        auto *s = new (std::nothrow) piv_server();
+        if (!s) return fail(PIV_ERR_OOM, "host allocation failed");
+        s->device = device;
+        if (cfg) s->cfg = *cfg;
+        if (s->cfg.hbm_budget == 0) s->cfg.hbm_budget = (size_t)4 << 30;
+        if (s->cfg.max_batch_pairs == 0) s->cfg.max_batch_pairs = 64;
+        if (s->cfg.max_inflight == 0) s->cfg.max_inflight = 2;
+        if (s->cfg.max_inflight > 8) s->cfg.max_inflight = 8;
+        s->lanes.resize(s->cfg.max_inflight);
+        for (Lane &L : s->lanes) {
+            if (hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&L.done, hipEventDisableTiming | hipEventBlockingSync) != hipSuccess) {
+                piv_server_destroy(s);
+                return fail(PIV_ERR_DEVICE, "stream creation failed");
+            }
         }
-    }
-    try {
-        s->scheduler = std::thread(scheduler_loop, s);
-        s->completer = std::thread(completer_loop, s);
-    } catch (const std::exception &) {
-        piv_server_destroy(s);
-        return fail(PIV_ERR_DEVICE, "server threads could not be started");
-    }
-    try {
-        s->stagers.start(kStageHelpers);
-    } catch (const std::exception &) {
-        // staging helpers are optional: submitters copy their pairs alone without them
-    }
-    *out = s;
-    return PIV_OK;
+        try {
+            s->scheduler = std::thread(scheduler_loop, s);
+            s->completer = std::thread(completer_loop, s);
+        } catch (const std::exception &) {
+            piv_server_destroy(s);
+            return fail(PIV_ERR_DEVICE, "server threads could not be started");
+        }
+        try {
+            s->stagers.start(kStageHelpers);
+        } catch (const std::exception &) {
+            // staging helpers are optional: submitters copy their pairs alone without them
+        }
+        *out = s;
+        return PIV_OK;
+    });
 }
 
 void piv_server_destroy(piv_server *s)
 {
-    if (!s) return;
-    {
-        std::lock_guard<std::mutex> lk(s->mu);
-        s->stop = true;
+    try {
+        if (!s) return;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->stop = true;
+        }
+        s->cv_sched.notify_all();
+        s->cv_complete.notify_all();
+        s->stagers.shutdown();
+        if (s->scheduler.joinable()) s->scheduler.join();
+        // the completer drains every launched batch before it exits
+        if (s->completer.joinable()) s->completer.join();
+        (void)hipSetDevice(s->device);
+        for (Lane &L : s->lanes) {
+            if (L.stream) (void)hipStreamSynchronize(L.stream);
+            L.drop_graphs();
+            L.desc.release();
+            L.sums.release();
+            L.out.release();
+            L.h_desc.release();
+            L.h_sums.release();
+            L.h_out.release();
+            if (L.done) (void)hipEventDestroy(L.done);
+            if (L.stream) (void)hipStreamDestroy(L.stream);
+        }
+        std::vector<Request *> all(s->queue.begin(), s->queue.end());
+        all.insert(all.end(), s->done.begin(), s->done.end());
+        for (auto &kv : s->polled) all.push_back(kv.second);
+        for (Request *r : all) {
+            if (r->slot) free_slot(s, r->slot);
+            delete r;
+        }
+        for (SrvStream *st : s->streams) {
+            if (st->an) (void)hipStreamSynchronize(st->an->stream);
+            for (Slot *sl : st->free_slots) free_slot(s, sl);
+            st->bg.release();
+            piv_analyzer_destroy(st->an);
+            delete st;
+        }
+        delete s;
+    } catch (...) {
     }
-    s->cv_sched.notify_all();
-    s->cv_complete.notify_all();
-    s->stagers.shutdown();
-    if (s->scheduler.joinable()) s->scheduler.join();
-    // the completer drains every launched batch before it exits
-    if (s->completer.joinable()) s->completer.join();
-    (void)hipSetDevice(s->device);
-    for (Lane &L : s->lanes) {
-        if (L.stream) (void)hipStreamSynchronize(L.stream);
-        L.drop_graphs();
-        L.desc.release();
-        L.sums.release();
-        L.out.release();
-        L.h_desc.release();
-        L.h_sums.release();
-        L.h_out.release();
-        if (L.done) (void)hipEventDestroy(L.done);
-        if (L.stream) (void)hipStreamDestroy(L.stream);
-    }
-    std::vector<Request *> all(s->queue.begin(), s->queue.end());
-    all.insert(all.end(), s->done.begin(), s->done.end());
-    for (auto &kv : s->polled) all.push_back(kv.second);
-    for (Request *r : all) {
-        if (r->slot) free_slot(s, r->slot);
-        delete r;
-    }
-    for (SrvStream *st : s->streams) {
-        if (st->an) (void)hipStreamSynchronize(st->an->stream);
-        for (Slot *sl : st->free_slots) free_slot(s, sl);
-        st->bg.release();
-        piv_analyzer_destroy(st->an);
-        delete st;
-    }
-    delete s;
 }
 
 piv_status piv_server_add_stream(piv_server *s, const piv_config *cfg, uint32_t height,
                                  uint32_t width, piv_pixel_format fmt, const float *background,
                                  uint32_t *stream_id)
 {
-    if (!s || !cfg || !stream_id) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
-    if ((int)fmt != FMT_U8 && (int)fmt != FMT_U16 && (int)fmt != FMT_F32)
-        return fail(PIV_ERR_INVALID_ARGUMENT, "unknown pixel format %d", (int)fmt);
-    auto *st = new (std::nothrow) SrvStream();
-    if (!st) return fail(PIV_ERR_OOM, "host allocation failed");
-    piv_status r = piv_analyzer_create(cfg, s->device, &st->an);
-    if (r != PIV_OK) {
-        delete st;
-        return r;
-    }
-    st->H = (int)height;
-    st->W = (int)width;
-    st->fmt = (int)fmt;
-    r = plan_passes(st->an, st->H, st->W, st->plans);
-    // the conditioning scratch at its final size, so that no submission allocates (its users,
-    // serialised by upload_mu, are ordered on the analyzer's stream)
-    if (r == PIV_OK) {
-        const size_t n = (size_t)height * width, cells = (size_t)sat_table_elems(st->H, st->W);
-        const piv_config &c = st->an->cfg;
-        r = st->an->sat_carry.reserve(sat_carry_bytes(st->H, st->W, 64));
-        if (r == PIV_OK && c.smooth_sigma > 0.0f) r = st->an->prep_tmp.reserve(n * sizeof(float));
-        if (r == PIV_OK && c.smooth_sigma > 0.0f && c.normalize_radius > 0)
-            r = st->an->prep_tmp2.reserve(n * sizeof(float));
-        if (r == PIV_OK && c.normalize_radius > 0) r = st->an->prep_sat.reserve(cells * 2 * sizeof(double));
-    }
-    if (r == PIV_OK && background) {
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!s || !cfg || !stream_id) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+        if ((int)fmt != FMT_U8 && (int)fmt != FMT_U16 && (int)fmt != FMT_F32)
+            return fail(PIV_ERR_INVALID_ARGUMENT, "unknown pixel format %d", (int)fmt);
+        auto *st = new (std::nothrow) SrvStream();
+        if (!st) return fail(PIV_ERR_OOM, "host allocation failed");
+        piv_status r = piv_analyzer_create(cfg, s->device, &st->an);
+        if (r != PIV_OK) {
+            delete st;
+            return r;
+        }
+        st->H = (int)height;
+        st->W = (int)width;
+        st->fmt = (int)fmt;
+        r = plan_passes(st->an, st->H, st->W, st->plans);
+        // the conditioning scratch at its final size, so that no submission allocates (its users,
+        // serialised by upload_mu, are ordered on the analyzer's stream)
+        if (r == PIV_OK) {
+            const size_t n = (size_t)height * width, cells = (size_t)sat_table_elems(st->H, st->W);
+            const piv_config &c = st->an->cfg;
+            r = st->an->sat_carry.reserve(sat_carry_bytes(st->H, st->W, 64));
+            if (r == PIV_OK && c.smooth_sigma > 0.0f) r = st->an->prep_tmp.reserve(n * sizeof(float));
+            if (r == PIV_OK && c.smooth_sigma > 0.0f && c.normalize_radius > 0)
+                r = st->an->prep_tmp2.reserve(n * sizeof(float));
+            if (r == PIV_OK && c.normalize_radius > 0) r = st->an->prep_sat.reserve(cells * 2 * sizeof(double));
+        }
+        if (r == PIV_OK && background) {
+            const size_t n = (size_t)height * width;
+            r = st->bg.reserve(n * 4);
+            if (r == PIV_OK && hipMemcpy(st->bg.p, background, n * 4, hipMemcpyHostToDevice) != hipSuccess)
+                r = fail(PIV_ERR_DEVICE, "background upload failed");
+            st->has_bg = r == PIV_OK;
+        }
+        if (r != PIV_OK) {
+            st->bg.release();
+            piv_analyzer_destroy(st->an);
+            delete st;
+            return r;
+        }
+        // slot sizes (the layout new_slot carves)
+        const size_t fb = (size_t)height * width * bytes_per_px((int)fmt);
+        size_t dev = 2 * align_up(fb, 256);
+        for (const PassPlan &pl : st->plans) {
+            const size_t c = pl.cells;
+            dev += 2 * fields_bytes(c) + 2 * align_up(c, 256) + align_up(c * 16, 256) +
+                   align_up(c * sizeof(PeakRec), 256);
+        }
+        // conditioned frames hold their own image, base and SAT buffers
         const size_t n = (size_t)height * width;
-        r = st->bg.reserve(n * 4);
-        if (r == PIV_OK && hipMemcpy(st->bg.p, background, n * 4, hipMemcpyHostToDevice) != hipSuccess)
-            r = fail(PIV_ERR_DEVICE, "background upload failed");
-        st->has_bg = r == PIV_OK;
-    }
-    if (r != PIV_OK) {
-        st->bg.release();
-        piv_analyzer_destroy(st->an);
-        delete st;
-        return r;
-    }
-    // slot sizes (the layout new_slot carves)
-    const size_t fb = (size_t)height * width * bytes_per_px((int)fmt);
-    size_t dev = 2 * align_up(fb, 256);
-    for (const PassPlan &pl : st->plans) {
-        const size_t c = pl.cells;
-        dev += 2 * fields_bytes(c) + 2 * align_up(c, 256) + align_up(c * 16, 256) +
-               align_up(c * sizeof(PeakRec), 256);
-    }
-    // conditioned frames hold their own image, base and SAT buffers
-    const size_t n = (size_t)height * width;
-    const size_t cells = (size_t)sat_table_elems(st->H, st->W);
-    const piv_config &c = st->an->cfg;
-    if (c.smooth_sigma > 0.0f || c.normalize_radius > 0 || background) dev += 2 * 2 * n * 4;
-    dev += 2 * cells * 16;  // SAT (upper bound; only when the mask is needed)
-    st->slot_dev = dev;
-    st->slot_host = 2 * align_up(fb, 256) + fields_bytes(st->plans.back().cells);
-    std::lock_guard<std::mutex> lk(s->mu);
-    st->id = (uint32_t)s->streams.size();
-    s->streams.push_back(st);
-    *stream_id = st->id;
-    return PIV_OK;
+        const size_t cells = (size_t)sat_table_elems(st->H, st->W);
+        const piv_config &c = st->an->cfg;
+        if (c.smooth_sigma > 0.0f || c.normalize_radius > 0 || background) dev += 2 * 2 * n * 4;
+        dev += 2 * cells * 16;  // SAT (upper bound; only when the mask is needed)
+        st->slot_dev = dev;
+        st->slot_host = 2 * align_up(fb, 256) + fields_bytes(st->plans.back().cells);
+        std::lock_guard<std::mutex> lk(s->mu);
+        st->id = (uint32_t)s->streams.size();
+        s->streams.push_back(st);
+        *stream_id = st->id;
+        return PIV_OK;
+    });
 }
 
 namespace {
@@ -1018,104 +1025,118 @@ piv_status submit_impl(piv_server *s, uint32_t stream_id, const void *pa, const 
 piv_status piv_submit(piv_server *s, uint32_t stream_id, const void *pixels_a,
                       const void *pixels_b, size_t row_stride, uint64_t *request_id)
 {
-    return submit_impl(s, stream_id, pixels_a, pixels_b, row_stride, false, request_id);
+    return pivi::guard(__func__, [&]() -> piv_status {
+        return submit_impl(s, stream_id, pixels_a, pixels_b, row_stride, false, request_id);
+    });
 }
 
 piv_status piv_submit_device(piv_server *s, uint32_t stream_id, const void *pixels_a,
                              const void *pixels_b, size_t row_stride, uint64_t *request_id)
 {
-    return submit_impl(s, stream_id, pixels_a, pixels_b, row_stride, true, request_id);
+    return pivi::guard(__func__, [&]() -> piv_status {
+        return submit_impl(s, stream_id, pixels_a, pixels_b, row_stride, true, request_id);
+    });
 }
 
 piv_status piv_submit_device_many(piv_server *s, uint32_t count, const uint32_t *stream_ids,
                                   const void *const *pixels_a, const void *const *pixels_b,
                                   size_t row_stride, uint64_t *request_ids, uint32_t *submitted)
 {
-    if (!s || !stream_ids || !pixels_a || !pixels_b || !request_ids || !submitted)
-        return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
-    *submitted = 0;
-    for (uint32_t i = 0; i < count; ++i) {
-        const piv_status r = submit_impl(s, stream_ids[i], pixels_a[i], pixels_b[i], row_stride, true,
-                                         request_ids + i);
-        if (r != PIV_OK) return r;
-        *submitted = i + 1;
-    }
-    return PIV_OK;
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!s || !stream_ids || !pixels_a || !pixels_b || !request_ids || !submitted)
+            return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+        *submitted = 0;
+        for (uint32_t i = 0; i < count; ++i) {
+            const piv_status r = submit_impl(s, stream_ids[i], pixels_a[i], pixels_b[i], row_stride, true,
+                                             request_ids + i);
+            if (r != PIV_OK) return r;
+            *submitted = i + 1;
+        }
+        return PIV_OK;
+    });
 }
 
 piv_status piv_poll(piv_server *s, piv_result *out, uint32_t capacity, uint32_t *count, int wait_ms)
 {
-    if (!s || !count || (!out && capacity)) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
-    std::unique_lock<std::mutex> lk(s->mu);
-    if (s->done.empty() && wait_ms != 0) {
-        auto pred = [&] { return !s->done.empty(); };
-        if (wait_ms < 0) s->cv_done.wait(lk, pred);
-        else s->cv_done.wait_for(lk, std::chrono::milliseconds(wait_ms), pred);
-    }
-    uint32_t n = 0;
-    while (n < capacity && !s->done.empty()) {
-        Request *r = s->done.front();
-        s->done.pop_front();
-        piv_result &o = out[n++];
-        o = piv_result{};
-        o.request_id = r->id;
-        o.stream_id = r->st->id;
-        o.status = r->status;
-        o.grid = to_pub(r->st->plans.back().grid);
-        if (r->status == PIV_OK) {
-            o.dx = r->slot->h_dx;
-            o.dy = r->slot->h_dy;
-            o.peak_ratio = r->slot->h_ppr;
-            o.flags = r->slot->h_flags;
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!s || !count || (!out && capacity)) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+        std::unique_lock<std::mutex> lk(s->mu);
+        if (s->done.empty() && wait_ms != 0) {
+            auto pred = [&] { return !s->done.empty(); };
+            if (wait_ms < 0) s->cv_done.wait(lk, pred);
+            else s->cv_done.wait_for(lk, std::chrono::milliseconds(wait_ms), pred);
         }
-        o.summary = r->summary;
-        o.submit_ms = r->t_submit;
-        o.done_ms = r->t_done;
-        o.batch_pairs = r->batch_pairs;
-        s->polled[r->id] = r;
-    }
-    *count = n;
-    return PIV_OK;
+        uint32_t n = 0;
+        while (n < capacity && !s->done.empty()) {
+            Request *r = s->done.front();
+            s->done.pop_front();
+            piv_result &o = out[n++];
+            o = piv_result{};
+            o.request_id = r->id;
+            o.stream_id = r->st->id;
+            o.status = r->status;
+            o.grid = to_pub(r->st->plans.back().grid);
+            if (r->status == PIV_OK) {
+                o.dx = r->slot->h_dx;
+                o.dy = r->slot->h_dy;
+                o.peak_ratio = r->slot->h_ppr;
+                o.flags = r->slot->h_flags;
+            }
+            o.summary = r->summary;
+            o.submit_ms = r->t_submit;
+            o.done_ms = r->t_done;
+            o.batch_pairs = r->batch_pairs;
+            s->polled[r->id] = r;
+        }
+        *count = n;
+        return PIV_OK;
+    });
 }
 
 piv_status piv_result_release(piv_server *s, uint64_t request_id)
 {
-    if (!s) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
-    std::lock_guard<std::mutex> lk(s->mu);
-    auto it = s->polled.find(request_id);
-    if (it == s->polled.end()) return fail(PIV_ERR_INVALID_ARGUMENT, "unknown request %llu",
-                                           (unsigned long long)request_id);
-    Request *r = it->second;
-    s->polled.erase(it);
-    if (r->slot) {
-        r->st->free_slots.push_back(r->slot);
-        r->st->in_use -= 1;
-    }
-    delete r;
-    s->cv_sched.notify_one();
-    return PIV_OK;
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!s) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->polled.find(request_id);
+        if (it == s->polled.end()) return fail(PIV_ERR_INVALID_ARGUMENT, "unknown request %llu",
+                                               (unsigned long long)request_id);
+        Request *r = it->second;
+        s->polled.erase(it);
+        if (r->slot) {
+            r->st->free_slots.push_back(r->slot);
+            r->st->in_use -= 1;
+        }
+        delete r;
+        s->cv_sched.notify_one();
+        return PIV_OK;
+    });
 }
 
 piv_status piv_server_pause(piv_server *s, int paused)
 {
-    if (!s) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
-    {
-        std::lock_guard<std::mutex> lk(s->mu);
-        s->paused = paused != 0;
-    }
-    s->cv_sched.notify_one();
-    return PIV_OK;
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!s) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->paused = paused != 0;
+        }
+        s->cv_sched.notify_one();
+        return PIV_OK;
+    });
 }
 
 piv_status piv_server_stats_read(piv_server *s, piv_server_stats *out)
 {
-    if (!s || !out) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
-    std::lock_guard<std::mutex> lk(s->mu);
-    *out = s->stats;
-    out->hbm_allocated = s->allocated;
-    out->hbm_peak = s->peak;
-    out->queued = s->queue.size();
-    return PIV_OK;
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!s || !out) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+        std::lock_guard<std::mutex> lk(s->mu);
+        *out = s->stats;
+        out->hbm_allocated = s->allocated;
+        out->hbm_peak = s->peak;
+        out->queued = s->queue.size();
+        return PIV_OK;
+    });
 }
 
 }  // extern "C"
