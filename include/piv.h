@@ -178,6 +178,27 @@ piv_status piv_analyze_batch_device(piv_analyzer *a, const piv_frame *const *fa,
                                     float *dy, float *ppr, uint8_t *flags,
                                     piv_summary *summaries, void *stream, int async);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the analysis
+ * of piv_analyze_batch_device, with the same kernels and bitwise the same results, on `npairs`
+ * pairs of raw DEVICE samples -- pair i is pixels_a + i * frame_stride and pixels_b +
+ * i * frame_stride (a clip of consecutive pairs: pixels_b = pixels_a + frame_stride) -- into
+ * DEVICE outputs dx/dy/ppr/flags [npairs][ny][nx] and, when not NULL, device summaries
+ * uint32 [npairs][5] (no_signal, low_ppr, median_outliers, out_of_range, replaced).
+ * All workspace (per-pass fields, descriptors, mask tables) lives in one 64 MiB device array
+ * per GPU that the library holds for the life of the process: the call allocates, frees,
+ * copies and synchronises nothing; it only enqueues kernels on `stream` (NULL: the analyzer's
+ * stream) and returns.  A call whose workspace exceeds 64 MiB fails with
+ * PIV_ERR_INVALID_ARGUMENT (analyse fewer pairs per call); a configuration with conditioning
+ * (smooth_sigma > 0 or normalize_radius > 0) fails with PIV_ERR_UNSUPPORTED.  Every call on a
+ * GPU shares that GPU's workspace: calls that overlap on one GPU race, so the caller
+ * serialises them (one stream, or waits between streams). */
+piv_status piv_analyze_batch_noalloc(piv_analyzer *a, const void *pixels_a, const void *pixels_b,
+                                     piv_pixel_format fmt, uint32_t npairs, uint32_t height,
+                                     uint32_t width, size_t row_stride, size_t frame_stride,
+                                     float *dx, float *dy, float *ppr, uint8_t *flags,
+                                     uint32_t *summaries, void *stream);
+
 /* ---- stage-level entry points (parity tests replay each stage from the Rust fixtures) --- */
 
 /* multipass.rs:207-231 window placement for `grid`; prev == NULL on the first pass.

@@ -115,6 +115,10 @@ _SIGS = {
     "piv_analyze_batch_device": ([C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                   C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_void_p, C.c_int], C.c_int),
+    "piv_analyze_batch_noalloc": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_uint32,
+                                   C.c_uint32, C.c_uint32, C.c_size_t, C.c_size_t, C.c_void_p,
+                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
+                                  C.c_int),
     "piv_place_windows": ([C.c_void_p, C.POINTER(piv_grid), C.c_uint32, C.c_uint32,
                            C.POINTER(piv_grid), C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "piv_correlate_windows": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,

@@ -809,6 +809,9 @@ piv_status piv_analyzer_create(const piv_config *cfg, int device, piv_analyzer *
         if (device < 0 || device >= ndev)
             return fail(PIV_ERR_INVALID_ARGUMENT, "device %d out of range (%d devices)", device, ndev);
         PIV_TRY(set_device(device));
+        // set-up of the allocation-free entry point's workspace address on this device; a
+        // failure is reported by that entry point, not here
+        (void)noalloc_init(device);
         auto *a = new (std::nothrow) piv_analyzer();
         if (!a) return fail(PIV_ERR_OOM, "host allocation failed");
         a->cfg = *cfg;

@@ -287,6 +287,11 @@ int variant_of(const piv_analyzer *a, int n, int fmt);
 // DFT-GEMM precision for a pass of window n (0: the FP32 FFT kernel)
 int mf_precision(const piv_analyzer *a, int n);
 piv_status check_config(const piv_config &c);
+// whether the signal mask can change a result (else its tables are not built)
+bool mask_needed(const piv_config &c, int in_fmt, bool has_bg);
+// device address of this GPU's workspace for piv_analyze_batch_noalloc, taken once per device
+// (device `dev` current); later calls return the stored address or failure
+piv_status noalloc_init(int dev);
 piv_status prepare_frame(piv_analyzer *a, piv_frame *f, const float *bg_dev);
 piv_status prepare_frames_list(piv_analyzer *a, piv_frame *const *fs, int count, const float *bg_dev);
 // whether preparing a frame of this format launches device work (conditioning or tables)
