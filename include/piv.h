@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define PIV_ABI_VERSION 1
+#define PIV_ABI_VERSION 2
 #define PIV_MAX_PASSES 16
 
 /* PivError (error.rs:3-20) plus device-side conditions. */
@@ -48,8 +48,22 @@ typedef enum { PIV_PIXEL_U8 = 0, PIV_PIXEL_U16 = 1, PIV_PIXEL_F32 = 2 } piv_pixe
 #define PIV_FLAG_OUT_OF_RANGE 16u
 #define PIV_FLAG_INVALID 23u
 
+/* Window deformation between passes (no counterpart in the reference, which only offsets the
+ * windows rigidly by the predictor at the window centre, multipass.rs:207-231).  With
+ * BILINEAR or BICUBIC, every pass after the first keeps that rigid placement and in addition
+ * moves each pixel of window A by -delta and of window B by +delta, delta = (D(p) - D(c)) / 2,
+ * where D is the bilinear interpolant of the previous pass's validated field, p the pixel's
+ * position and c the window centre; the windows are resampled bilinearly or with 4x4 cubic
+ * (Lagrange) weights.  A uniform predictor gives delta = 0 and, with BILINEAR, exactly the
+ * rigid result.  NONE (the default) is the reference's method. */
+typedef enum {
+    PIV_DEFORM_NONE = 0,
+    PIV_DEFORM_BILINEAR = 1,
+    PIV_DEFORM_BICUBIC = 2
+} piv_deformation;
+
 /* PivConfig + MaskConfig (multipass.rs:22-63), ValidationConfig (validate.rs:7-37) and
- * PreprocessConfig (preprocess.rs:8-29) in one struct. */
+ * PreprocessConfig (preprocess.rs:8-29) in one struct, plus the deformation mode. */
 typedef struct {
     uint32_t windows[PIV_MAX_PASSES]; /* window size per pass, largest first */
     uint32_t num_windows;
@@ -61,6 +75,7 @@ typedef struct {
     float smooth_sigma;               /* PreprocessConfig */
     uint32_t normalize_radius;
     float normalize_eps;
+    uint32_t deformation;             /* piv_deformation (no counterpart in the reference) */
 } piv_config;
 
 /* Grid (grid.rs:9-17) */
@@ -104,7 +119,8 @@ int piv_abi_version(void);
 piv_status piv_device_count(int *count);
 
 /* PivConfig::default() etc.: windows [96, 64], overlap 0.5, final_iterations 2, mask 0/0,
- * validation 1.3 / 2 / 0.1 / replace, preprocessing sigma 1, radius 4, eps 0.02. */
+ * validation 1.3 / 2 / 0.1 / replace, preprocessing sigma 1, radius 4, eps 0.02; no
+ * deformation. */
 void piv_config_default(piv_config *cfg);
 
 /* Grid::new (grid.rs:20-44) */

@@ -115,7 +115,20 @@ def lib():
         L.or_correlate_pass.argtypes = [f32, f32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_int32, C.c_int32, C.POINTER(Grid), f32, C.c_float,
                                         C.c_float, C.c_int32, C.c_int32, f32, f32, f32, u8,
-                                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+                                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.or_deform_sample.argtypes = [C.POINTER(Grid), f32, f32, C.c_float, C.c_float,
+                                       C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.or_deform_shifts.argtypes = [C.POINTER(Grid), f32, f32, C.c_int32, C.c_float,
+                                       C.c_float, C.c_float, C.c_float, f32, f32]
+        L.or_load_window_deformed.argtypes = [f32, f32, C.c_int32, C.c_int32, C.c_float,
+                                              C.c_float, C.c_int32, C.c_int32, f32, f32,
+                                              C.c_int32]
+        L.or_load_window_deformed.restype = C.c_double
+        L.or_load_window_deformed64.argtypes = [f64, f32, C.c_int32, C.c_int32, C.c_float,
+                                                C.c_float, C.c_int32, C.c_int32, f32, f32,
+                                                C.c_int32]
+        L.or_load_window_deformed64.restype = C.c_double
         L.or_validate.argtypes = [C.c_int32, C.c_int32, f32, f32, f32, u8, C.POINTER(VCfg),
                                   C.c_float, C.POINTER(Summary)]
         L.or_median.argtypes = [f32, C.c_int32]
@@ -319,8 +332,14 @@ class Field:
                      self.flags.copy(), self.peak_iy, self.peak_ix, self.dx64, self.dy64)
 
 
+DEFORM = {"none": 0, "bilinear": 1, "bicubic": 2}
+
+
 def correlate_pass(pa: Prepared, pb: Prepared, g: Grid, origins, min_mean=0.0, min_std=0.0,
-                   precision=32, threads=0, use_mask=True) -> Field:
+                   precision=32, threads=0, use_mask=True, deform=None) -> Field:
+    """PivAnalyzer::pass given origins (multipass.rs:233-259).  deform: None (rigid windows,
+    the reference) or (mode, prev_grid, prev_dx, prev_dy), mode 1 bilinear / 2 bicubic: the
+    library's window deformation from the previous pass's validated field."""
     h, w = pa.image.shape
     cnt = len(g)
     dx = np.zeros(cnt, np.float32)
@@ -336,12 +355,48 @@ def correlate_pass(pa: Prepared, pb: Prepared, g: Grid, origins, min_mean=0.0, m
     lib().or_correlate_pass(pa.image.ravel(), pb.image.ravel(), _ptr(sa[0]), _ptr(sa[1]),
                             _ptr(sb[0]), _ptr(sb[1]), h, w, C.byref(g),
                             _f32(origins).ravel(), min_mean, min_std, precision, threads, dx, dy,
-                            ppr, fl, _ptr(piy), _ptr(pix), _ptr(d64), _ptr(e64))
+                            ppr, fl, _ptr(piy), _ptr(pix), _ptr(d64), _ptr(e64),
+                            *_deform_args(deform))
     shp = (g.ny, g.nx)
     return Field(g.as_tuple(), dx.reshape(shp), dy.reshape(shp), ppr.reshape(shp),
                  fl.reshape(shp), piy.reshape(shp), pix.reshape(shp),
                  None if d64 is None else d64.reshape(shp),
                  None if e64 is None else e64.reshape(shp))
+
+
+def _deform_args(deform):
+    if deform is None or deform[0] == 0 or deform[1] is None:
+        return 0, None, None, None
+    mode, pg, pdx, pdy = deform
+    pdx, pdy = _f32(pdx).ravel(), _f32(pdy).ravel()
+    # the arrays must outlive the call: keep them on the function object for its duration
+    _deform_args.keep = (pg, pdx, pdy)
+    return (mode, C.cast(C.pointer(pg), C.c_void_p), pdx.ctypes.data_as(C.c_void_p),
+            pdy.ctypes.data_as(C.c_void_p))
+
+
+def deform_sample(g: Grid, dx, dy, x, y):
+    u, v = C.c_float(), C.c_float()
+    lib().or_deform_sample(C.byref(g), _f32(dx).ravel(), _f32(dy).ravel(), x, y, C.byref(u),
+                           C.byref(v))
+    return u.value, v.value
+
+
+def deform_shifts(prev: Grid, pdx, pdy, n, ax, ay, bx, by):
+    hx = np.zeros(n * n, np.float32)
+    hy = np.zeros_like(hx)
+    lib().or_deform_shifts(C.byref(prev), _f32(pdx).ravel(), _f32(pdy).ravel(), n, ax, ay, bx, by,
+                           hx, hy)
+    return hx.reshape(n, n), hy.reshape(n, n)
+
+
+def load_window_deformed(img, x, y, n, mode, hx, hy, sgn, precision=32):
+    img = _f32(img)
+    h, w = img.shape
+    dst = np.zeros(n * n, np.float64 if precision == 64 else np.float32)
+    fn = lib().or_load_window_deformed64 if precision == 64 else lib().or_load_window_deformed
+    e = fn(dst, img.ravel(), h, w, x, y, n, mode, _f32(hx).ravel(), _f32(hy).ravel(), sgn)
+    return dst.reshape(n, n), e
 
 
 def validate(f: Field, ppr_threshold=1.3, median_threshold=2.0, median_eps=0.1, replace=True,
@@ -400,8 +455,11 @@ def check_config(cfg: PivConfig):
 
 
 def analyze(pa: Prepared, pb: Prepared, cfg: PivConfig, precision=32, threads=0,
-            traced=False):
-    """PivAnalyzer::run (multipass.rs:152-192)."""
+            traced=False, deformation="none"):
+    """PivAnalyzer::run (multipass.rs:152-192); deformation "bilinear" / "bicubic" deforms
+    the windows of every pass after the first by the previous validated field (the library's
+    definition, no counterpart in the reference)."""
+    mode = DEFORM[deformation]
     check_config(cfg)
     if pa.image.shape != pb.image.shape:
         raise PivError(f"frames have different shapes: {pa.image.shape} vs {pb.image.shape}")
@@ -413,7 +471,9 @@ def analyze(pa: Prepared, pb: Prepared, cfg: PivConfig, precision=32, threads=0,
         g = grid_new(h, w, win, cfg.overlap)
         origins = place_windows(g, h, w, prev_grid, None if prev is None else prev.dx,
                                 None if prev is None else prev.dy)
-        raw = correlate_pass(pa, pb, g, origins, cfg.min_mean, cfg.min_std, precision, threads)
+        dfm = None if prev is None else (mode, prev_grid, prev.dx, prev.dy)
+        raw = correlate_pass(pa, pb, g, origins, cfg.min_mean, cfg.min_std, precision, threads,
+                             deform=dfm)
         val, summary = validate(raw, cfg.ppr_threshold, cfg.median_threshold, cfg.median_eps,
                                 cfg.replace, np.float32(cfg.windows[0]) / np.float32(2.0))
         if traced:

@@ -325,6 +325,182 @@ static double load_window64(double *dst, const float *img, int32_t w, float x, f
     return energy;
 }
 
+/* ---- window deformation ------------------------------------------------------------
+ * No counterpart in the reference, which moves windows rigidly by the predictor at the
+ * window centre (multipass.rs:207-231).  The definition restated here is the library's
+ * (include/piv.h, piv_deformation): the rigid placement stays; pixel (c, r) of window A moves
+ * by -h and of window B by +h, h = (D(p) - D(centre)) / 2, where D is the lerp-form bilinear
+ * interpolant of the previous validated field (VectorField::sample's NaN-skipping weighted
+ * form, field.rs:56-81, wherever a neighbour is not finite) and p = (mx + c, my + r) the
+ * pixel's position midway between its A and B samples (mx = (ax + bx) / 2).  Each pixel is
+ * resampled from its own fractional position: bilinear with the weights of load_window
+ * (correlate.rs:189-200), or 4x4 Lagrange (cubic) weights, indices clamped into the frame. */
+void or_deform_sample(const or_grid *g, const float *dx, const float *dy, float x, float y,
+                      float *u, float *v)
+{
+    float fx = clamp_rs((x - center(g->x0, 0, g->step, g->window)) / (float)g->step, 0.0f,
+                        (float)(g->nx - 1));
+    float fy = clamp_rs((y - center(g->y0, 0, g->step, g->window)) / (float)g->step, 0.0f,
+                        (float)(g->ny - 1));
+    int32_t ix = (int32_t)floorf(fx), iy = (int32_t)floorf(fy);
+    float tx = fx - (float)ix, ty = fy - (float)iy;
+    int32_t ix1 = ix + 1 < g->nx - 1 ? ix + 1 : g->nx - 1;
+    int32_t iy1 = iy + 1 < g->ny - 1 ? iy + 1 : g->ny - 1;
+    const float u00 = dx[iy * g->nx + ix], u01 = dx[iy * g->nx + ix1];
+    const float u10 = dx[iy1 * g->nx + ix], u11 = dx[iy1 * g->nx + ix1];
+    const float v00 = dy[iy * g->nx + ix], v01 = dy[iy * g->nx + ix1];
+    const float v10 = dy[iy1 * g->nx + ix], v11 = dy[iy1 * g->nx + ix1];
+    if (isfinite(u00) && isfinite(u01) && isfinite(u10) && isfinite(u11) && isfinite(v00) &&
+        isfinite(v01) && isfinite(v10) && isfinite(v11)) {
+        const float ut = u00 + tx * (u01 - u00), ub = u10 + tx * (u11 - u10);
+        const float vt = v00 + tx * (v01 - v00), vb = v10 + tx * (v11 - v10);
+        *u = ut + ty * (ub - ut);
+        *v = vt + ty * (vb - vt);
+    } else {
+        or_field_sample(g, dx, dy, x, y, u, v);
+    }
+}
+
+/* Lagrange weights at nodes -1, 0, 1, 2 (reproduce cubics exactly) */
+static void lagrange4(float t, float l[4])
+{
+    const float tp1 = t + 1.0f, tm1 = t - 1.0f, tm2 = t - 2.0f;
+    l[0] = -(t * tm1 * tm2) / 6.0f;
+    l[1] = (tp1 * tm1 * tm2) / 2.0f;
+    l[2] = -(tp1 * t * tm2) / 2.0f;
+    l[3] = (tp1 * t * tm1) / 6.0f;
+}
+
+static void lagrange4_64(double t, double l[4])
+{
+    const double tp1 = t + 1.0, tm1 = t - 1.0, tm2 = t - 2.0;
+    l[0] = -(t * tm1 * tm2) / 6.0;
+    l[1] = (tp1 * tm1 * tm2) / 2.0;
+    l[2] = -(tp1 * t * tm2) / 2.0;
+    l[3] = (tp1 * t * tm1) / 6.0;
+}
+
+static float px_clamped(const float *img, int32_t h, int32_t w, int32_t x, int32_t y)
+{
+    x = x < 0 ? 0 : (x > w - 1 ? w - 1 : x);
+    y = y < 0 ? 0 : (y > h - 1 ? h - 1 : y);
+    return img[(int64_t)y * w + x];
+}
+
+/* The per-pixel shift h of the window pair with rigid origins (ax, ay), (bx, by). */
+void or_deform_shifts(const or_grid *prev, const float *pdx, const float *pdy, int32_t n,
+                      float ax, float ay, float bx, float by, float *hx, float *hy)
+{
+    const float mx = (ax + bx) * 0.5f, my = (ay + by) * 0.5f;
+    const float half = ((float)n - 1.0f) / 2.0f;
+    float uc, vc;
+    or_deform_sample(prev, pdx, pdy, mx + half, my + half, &uc, &vc);
+    for (int32_t r = 0; r < n; r++)
+        for (int32_t c = 0; c < n; c++) {
+            float u, v;
+            or_deform_sample(prev, pdx, pdy, mx + (float)c, my + (float)r, &u, &v);
+            hx[r * n + c] = (u - uc) * 0.5f;
+            hy[r * n + c] = (v - vc) * 0.5f;
+        }
+}
+
+/* load_window (correlate.rs:182-210) of a deformed window: sgn -1 for A (moves by -h), +1 for
+ * B; mode 1 bilinear, 2 Lagrange.  With h = 0 and mode 1 every sample is load_window's. */
+double or_load_window_deformed(float *dst, const float *img, int32_t h, int32_t w, float x,
+                               float y, int32_t n, int32_t mode, const float *hx,
+                               const float *hy, int32_t sgn)
+{
+    const int32_t x0 = (int32_t)floorf(x), y0 = (int32_t)floorf(y);
+    const float tx = x - (float)x0, ty = y - (float)y0;
+    for (int32_t r = 0; r < n; r++)
+        for (int32_t c = 0; c < n; c++) {
+            const int32_t k = r * n + c;
+            const float fx = sgn < 0 ? tx - hx[k] : tx + hx[k];
+            const float fy = sgn < 0 ? ty - hy[k] : ty + hy[k];
+            const float ixf = floorf(fx), iyf = floorf(fy);
+            const float wx = fx - ixf, wy = fy - iyf;
+            const int32_t xi = x0 + c + (int32_t)ixf, yi = y0 + r + (int32_t)iyf;
+            float v;
+            if (wx < 1e-4f && wy < 1e-4f) {
+                v = px_clamped(img, h, w, xi, yi);
+            } else if (mode == 1) {
+                const float w00 = (1.0f - wx) * (1.0f - wy), w01 = wx * (1.0f - wy);
+                const float w10 = (1.0f - wx) * wy, w11 = wx * wy;
+                v = w00 * px_clamped(img, h, w, xi, yi) + w01 * px_clamped(img, h, w, xi + 1, yi) +
+                    w10 * px_clamped(img, h, w, xi, yi + 1) + w11 * px_clamped(img, h, w, xi + 1, yi + 1);
+            } else {
+                float lx[4], ly[4];
+                lagrange4(wx, lx);
+                lagrange4(wy, ly);
+                v = 0.0f;
+                for (int32_t j = 0; j < 4; j++) {
+                    float hsum = 0.0f;
+                    for (int32_t i = 0; i < 4; i++)
+                        hsum = hsum + lx[i] * px_clamped(img, h, w, xi - 1 + i, yi - 1 + j);
+                    v = v + ly[j] * hsum;
+                }
+            }
+            dst[k] = v;
+        }
+    double s = 0.0;
+    for (int32_t i = 0; i < n * n; i++) s += (double)dst[i];
+    const double mean = s / (double)(n * n);
+    double energy = 0.0;
+    for (int32_t i = 0; i < n * n; i++) {
+        dst[i] -= (float)mean;
+        energy += (double)dst[i] * (double)dst[i];
+    }
+    return energy;
+}
+
+/* FP64 form: positions as in the FP32 form (placement is f32), weights and sums in double */
+double or_load_window_deformed64(double *dst, const float *img, int32_t h, int32_t w, float x,
+                                 float y, int32_t n, int32_t mode, const float *hx,
+                                 const float *hy, int32_t sgn)
+{
+    const int32_t x0 = (int32_t)floorf(x), y0 = (int32_t)floorf(y);
+    const float tx = x - (float)x0, ty = y - (float)y0;
+    for (int32_t r = 0; r < n; r++)
+        for (int32_t c = 0; c < n; c++) {
+            const int32_t k = r * n + c;
+            const float fx = sgn < 0 ? tx - hx[k] : tx + hx[k];
+            const float fy = sgn < 0 ? ty - hy[k] : ty + hy[k];
+            const float ixf = floorf(fx), iyf = floorf(fy);
+            const int32_t xi = x0 + c + (int32_t)ixf, yi = y0 + r + (int32_t)iyf;
+            const double wx = (double)fx - (double)ixf, wy = (double)fy - (double)iyf;
+            double v;
+            if ((float)wx < 1e-4f && (float)wy < 1e-4f) {
+                v = px_clamped(img, h, w, xi, yi);
+            } else if (mode == 1) {
+                const double w00 = (1.0 - wx) * (1.0 - wy), w01 = wx * (1.0 - wy);
+                const double w10 = (1.0 - wx) * wy, w11 = wx * wy;
+                v = w00 * px_clamped(img, h, w, xi, yi) + w01 * px_clamped(img, h, w, xi + 1, yi) +
+                    w10 * px_clamped(img, h, w, xi, yi + 1) + w11 * px_clamped(img, h, w, xi + 1, yi + 1);
+            } else {
+                double lx[4], ly[4];
+                lagrange4_64(wx, lx);
+                lagrange4_64(wy, ly);
+                v = 0.0;
+                for (int32_t j = 0; j < 4; j++) {
+                    double hsum = 0.0;
+                    for (int32_t i = 0; i < 4; i++)
+                        hsum += lx[i] * px_clamped(img, h, w, xi - 1 + i, yi - 1 + j);
+                    v += ly[j] * hsum;
+                }
+            }
+            dst[k] = v;
+        }
+    double s = 0.0;
+    for (int32_t i = 0; i < n * n; i++) s += dst[i];
+    const double mean = s / (double)(n * n);
+    double energy = 0.0;
+    for (int32_t i = 0; i < n * n; i++) {
+        dst[i] -= mean;
+        energy += dst[i] * dst[i];
+    }
+    return energy;
+}
+
 /* ---- correlate.rs:215-303 find_peak ----------------------------------------------- */
 #define FIND_PEAK_BODY(TT, LOG, PEAK_T)                                                      \
     const int32_t half = n / 2;                                                              \
@@ -463,8 +639,34 @@ void or_find_peak64(const double *corr, int32_t n, or_peak64 *out)
 CORR_FWD(float, or_fft_f32_impl)
 CORR_FWD(double, or_fft_f64_impl)
 
+/* deform: NULL (rigid windows, the reference) or {mode, hx, hy} of or_deform_shifts */
+typedef struct {
+    int32_t mode;
+    const float *hx, *hy;
+} deform_t;
+
+static int correlate_f32(const float *a, const float *b, int32_t h, int32_t w, int32_t n,
+                         float ax, float ay, float bx, float by, or_peak *out, float *corr_out,
+                         const deform_t *def);
+static int correlate_f64(const float *a, const float *b, int32_t h, int32_t w, int32_t n,
+                         float ax, float ay, float bx, float by, or_peak64 *out,
+                         double *corr_out, const deform_t *def);
+
 int or_correlate_f32(const float *a, const float *b, int32_t h, int32_t w, int32_t n, float ax,
                      float ay, float bx, float by, or_peak *out, float *corr_out)
+{
+    return correlate_f32(a, b, h, w, n, ax, ay, bx, by, out, corr_out, NULL);
+}
+
+int or_correlate_f64(const float *a, const float *b, int32_t h, int32_t w, int32_t n, float ax,
+                     float ay, float bx, float by, or_peak64 *out, double *corr_out)
+{
+    return correlate_f64(a, b, h, w, n, ax, ay, bx, by, out, corr_out, NULL);
+}
+
+static int correlate_f32(const float *a, const float *b, int32_t h, int32_t w, int32_t n,
+                         float ax, float ay, float bx, float by, or_peak *out, float *corr_out,
+                         const deform_t *def)
 {
     const int32_t m = n / 2 + 1;
     float *wa = (float *)malloc(sizeof(float) * (size_t)n * n * 3);
@@ -474,8 +676,10 @@ int or_correlate_f32(const float *a, const float *b, int32_t h, int32_t w, int32
     float *tb = ta + 2 * m * n;
     float *row = (float *)malloc(sizeof(float) * 2 * (size_t)n);
     int ok = 0;
-    const double ea = or_load_window(wa, a, h, w, ax, ay, n);
-    const double eb = or_load_window(wb, b, h, w, bx, by, n);
+    const double ea = def ? or_load_window_deformed(wa, a, h, w, ax, ay, n, def->mode, def->hx, def->hy, -1)
+                          : or_load_window(wa, a, h, w, ax, ay, n);
+    const double eb = def ? or_load_window_deformed(wb, b, h, w, bx, by, n, def->mode, def->hx, def->hy, 1)
+                          : or_load_window(wb, b, h, w, bx, by, n);
     if (!(ea <= 1e-12 || eb <= 1e-12)) {
         fwd_float(wa, ta, row, n);
         fwd_float(wb, tb, row, n);
@@ -501,10 +705,10 @@ int or_correlate_f32(const float *a, const float *b, int32_t h, int32_t w, int32
     return ok;
 }
 
-int or_correlate_f64(const float *a, const float *b, int32_t h, int32_t w, int32_t n, float ax,
-                     float ay, float bx, float by, or_peak64 *out, double *corr_out)
+static int correlate_f64(const float *a, const float *b, int32_t h, int32_t w, int32_t n,
+                         float ax, float ay, float bx, float by, or_peak64 *out,
+                         double *corr_out, const deform_t *def)
 {
-    (void)h;
     const int32_t m = n / 2 + 1;
     double *wa = (double *)malloc(sizeof(double) * (size_t)n * n * 3);
     double *wb = wa + n * n;
@@ -513,8 +717,10 @@ int or_correlate_f64(const float *a, const float *b, int32_t h, int32_t w, int32
     double *tb = ta + 2 * m * n;
     double *row = (double *)malloc(sizeof(double) * 2 * (size_t)n);
     int ok = 0;
-    const double ea = load_window64(wa, a, w, ax, ay, n);
-    const double eb = load_window64(wb, b, w, bx, by, n);
+    const double ea = def ? or_load_window_deformed64(wa, a, h, w, ax, ay, n, def->mode, def->hx, def->hy, -1)
+                          : load_window64(wa, a, w, ax, ay, n);
+    const double eb = def ? or_load_window_deformed64(wb, b, h, w, bx, by, n, def->mode, def->hx, def->hy, 1)
+                          : load_window64(wb, b, w, bx, by, n);
     if (!(ea <= 1e-12 || eb <= 1e-12)) {
         fwd_double(wa, ta, row, n);
         fwd_double(wb, tb, row, n);
@@ -543,10 +749,12 @@ void or_correlate_pass(const float *img_a, const float *img_b, const double *sat
                        int32_t h, int32_t w, const or_grid *g, const float *origins,
                        float min_mean, float min_std, int32_t precision, int32_t threads,
                        float *dx, float *dy, float *ppr, uint8_t *flags, int32_t *peak_iy,
-                       int32_t *peak_ix, double *dx64, double *dy64)
+                       int32_t *peak_ix, double *dx64, double *dy64, int32_t deform,
+                       const or_grid *prev, const float *prev_dx, const float *prev_dy)
 {
     const int32_t n = g->window;
     const int32_t count = g->nx * g->ny;
+    const int deformed = deform != 0 && prev != NULL;
 #ifdef _OPENMP
     if (threads <= 0) threads = omp_get_max_threads();
 #pragma omp parallel for schedule(dynamic, 4) num_threads(threads)
@@ -571,10 +779,18 @@ void or_correlate_pass(const float *img_a, const float *img_b, const double *sat
         int ok = 0;
         float rdx = NAN, rdy = NAN, rppr = 0.0f;
         int32_t piy = -1, pix = -1;
+        float *shifts = NULL;
+        deform_t def = {deform, NULL, NULL};
+        if (sig && deformed) {
+            shifts = (float *)malloc(sizeof(float) * 2 * (size_t)n * n);
+            or_deform_shifts(prev, prev_dx, prev_dy, n, ax, ay, bx, by, shifts, shifts + n * n);
+            def.hx = shifts;
+            def.hy = shifts + n * n;
+        }
         if (sig) {
             if (precision == 64) {
                 or_peak64 p;
-                ok = or_correlate_f64(img_a, img_b, h, w, n, ax, ay, bx, by, &p, NULL);
+                ok = correlate_f64(img_a, img_b, h, w, n, ax, ay, bx, by, &p, NULL, shifts ? &def : NULL);
                 if (ok) {
                     double ddx = ((double)bx - (double)ax) + p.dx;
                     double ddy = ((double)by - (double)ay) + p.dy;
@@ -588,7 +804,7 @@ void or_correlate_pass(const float *img_a, const float *img_b, const double *sat
                 }
             } else {
                 or_peak p;
-                ok = or_correlate_f32(img_a, img_b, h, w, n, ax, ay, bx, by, &p, NULL);
+                ok = correlate_f32(img_a, img_b, h, w, n, ax, ay, bx, by, &p, NULL, shifts ? &def : NULL);
                 if (ok) {
                     rdx = bx - ax + p.dx;
                     rdy = by - ay + p.dy;
@@ -598,6 +814,7 @@ void or_correlate_pass(const float *img_a, const float *img_b, const double *sat
                 }
             }
         }
+        free(shifts);
         dx[k] = ok ? rdx : NAN;
         dy[k] = ok ? rdy : NAN;
         ppr[k] = ok ? rppr : 0.0f;

@@ -114,7 +114,24 @@ void or_correlate_pass(const float *img_a, const float *img_b, const double *sat
                        int32_t h, int32_t w, const or_grid *g, const float *origins,
                        float min_mean, float min_std, int32_t precision, int32_t threads,
                        float *dx, float *dy, float *ppr, uint8_t *flags, int32_t *peak_iy,
-                       int32_t *peak_ix, double *dx64, double *dy64);
+                       int32_t *peak_ix, double *dx64, double *dy64, int32_t deform,
+                       const or_grid *prev, const float *prev_dx, const float *prev_dy);
+
+/* Window deformation (the library's definition; no counterpart in the reference): deform 0
+ * rigid, 1 bilinear, 2 Lagrange-cubic resampling, with the previous pass's validated field
+ * prev_dx / prev_dy on grid prev (NULL on a first pass: rigid).  The pieces, exposed for
+ * tests: the predictor interpolant D, the per-pixel shifts h of one window pair, and the
+ * deformed load_window (sgn -1: window A, +1: window B). */
+void or_deform_sample(const or_grid *g, const float *dx, const float *dy, float x, float y,
+                      float *u, float *v);
+void or_deform_shifts(const or_grid *prev, const float *pdx, const float *pdy, int32_t n,
+                      float ax, float ay, float bx, float by, float *hx, float *hy);
+double or_load_window_deformed(float *dst, const float *img, int32_t h, int32_t w, float x,
+                               float y, int32_t n, int32_t mode, const float *hx,
+                               const float *hy, int32_t sgn);
+double or_load_window_deformed64(double *dst, const float *img, int32_t h, int32_t w, float x,
+                                 float y, int32_t n, int32_t mode, const float *hx,
+                                 const float *hy, int32_t sgn);
 
 /* validate.rs:51-130 */
 void or_validate(int32_t ny, int32_t nx, float *dx, float *dy, const float *ppr, uint8_t *flags,

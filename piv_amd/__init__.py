@@ -135,6 +135,9 @@ class PivConfig:
     final_iterations: int = 2
     mask: MaskConfig = field(default_factory=MaskConfig)
     validation: ValidationConfig = field(default_factory=ValidationConfig)
+    # window deformation between passes: "none" (the reference's rigid offset), "bilinear"
+    # or "bicubic" (include/piv.h, piv_deformation)
+    deformation: str = "none"
 
     def to_c(self, pre: PreprocessConfig) -> L.piv_config:
         c = L.piv_config()
@@ -156,6 +159,9 @@ class PivConfig:
         c.smooth_sigma = pre.smooth_sigma
         c.normalize_radius = int(pre.normalize_radius)
         c.normalize_eps = pre.normalize_eps
+        if self.deformation not in L.DEFORM:
+            raise ConfigError(f"invalid configuration: deformation {self.deformation!r}")
+        c.deformation = L.DEFORM[self.deformation]
         return c
 
 

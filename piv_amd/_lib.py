@@ -28,7 +28,9 @@ class piv_config(C.Structure):
                 ("ppr_threshold", C.c_float), ("median_threshold", C.c_float),
                 ("median_eps", C.c_float), ("replace", C.c_int32),
                 ("smooth_sigma", C.c_float), ("normalize_radius", C.c_uint32),
-                ("normalize_eps", C.c_float)]
+                ("normalize_eps", C.c_float), ("deformation", C.c_uint32)]
+
+DEFORM = {"none": 0, "bilinear": 1, "bicubic": 2}
 
 
 class piv_grid(C.Structure):

@@ -67,6 +67,7 @@ struct Args {
     double px_per_mm = 0;
     uint32_t batch = 64;
     std::string precision = "fp32";
+    std::string deformation = "none";
     std::string channel = "luma";
     int device = 0;
 };
@@ -81,7 +82,8 @@ void usage()
                  "                 --background none|min:N|mean:N (min:40),\n"
                  "                 --smooth-sigma 1, --normalize-radius 4\n"
                  "  interrogation: --windows 96,64, --overlap 0.5, --final-iterations 2,\n"
-                 "                 --min-mean 0.04, --min-std 0.015\n"
+                 "                 --min-mean 0.04, --min-std 0.015,\n"
+                 "                 --deformation none|bilinear|bicubic (between passes; none)\n"
                  "  validation:    --ppr-threshold, --median-threshold, --median-eps, --no-replace\n"
                  "  output:        --format npz (hdf5 needs libhdf5), --px-per-mm X\n"
                  "  device:        --batch 64, --precision fp32|fp16|bf16, --device 0\n");
@@ -143,6 +145,7 @@ Args parse(int argc, char **argv)
         else if (s == "--px-per-mm") a.px_per_mm = std::stod(need(i));
         else if (s == "--batch") a.batch = (uint32_t)std::stoul(need(i));
         else if (s == "--precision") a.precision = need(i);
+        else if (s == "--deformation") a.deformation = need(i);
         else if (s == "--device") a.device = std::stoi(need(i));
         else if (!s.empty() && s[0] == '-') die("unknown option " + s);
         else if (a.input.empty()) a.input = s;
@@ -311,6 +314,10 @@ int main(int argc, char **argv)
     cfg.replace = A.no_replace ? 0 : 1;
     cfg.smooth_sigma = A.smooth_sigma;
     cfg.normalize_radius = A.normalize_radius;
+    cfg.deformation = A.deformation == "none"       ? PIV_DEFORM_NONE
+                      : A.deformation == "bilinear" ? PIV_DEFORM_BILINEAR
+                      : A.deformation == "bicubic"  ? PIV_DEFORM_BICUBIC
+                                                    : (die("unknown deformation " + A.deformation), PIV_DEFORM_NONE);
     piv_analyzer *an = nullptr;
     check(piv_analyzer_create(&cfg, A.device, &an), "analyzer");
     if (A.precision != "fp32") {

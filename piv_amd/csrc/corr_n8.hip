@@ -2,9 +2,9 @@
 #include "correlate.hpp"
 
 namespace piv {
-hipError_t launch_correlate_8(int fmt, int variant, const PassPair *pairs, int npairs, uint32_t max_cells,
-                              hipStream_t st)
+hipError_t launch_correlate_8(int fmt, int variant, int deform, const PassPair *pairs, int npairs,
+                              uint32_t max_cells, hipStream_t st)
 {
-    return launch_corr_n<8>(fmt, variant, pairs, npairs, max_cells, st);
+    return launch_corr_n<8>(fmt, variant, deform, pairs, npairs, max_cells, st);
 }
 }  // namespace piv

@@ -1,0 +1,7 @@
+// Window sizes with an instantiated correlation kernel: the register-resident FFT kernel
+// k_correlate<N> (one corr_n<N>.hip per size) and the DFT-GEMM kernel k_correlate_mf<N>
+// (corrmf_n<N>.hip).  The dispatch in piv_kernels.hip is generated from these lists.
+#pragma once
+
+#define PIV_FFT_SIZES(X) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96)
+#define PIV_MF_SIZES(X) X(16) X(32) X(64)

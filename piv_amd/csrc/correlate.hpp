@@ -175,6 +175,99 @@ __device__ __forceinline__ void load_column_int(const void *__restrict__ img, in
     }
 }
 
+// The sample at (x, y) with both indices clamped into the frame, in raw units.
+template <int FMT>
+__device__ __forceinline__ float clamped_sample(const void *__restrict__ img, int pitch, int H,
+                                                int W, int x, int y)
+{
+    x = min(max(x, 0), W - 1);
+    y = min(max(y, 0), H - 1);
+    return raw_sample<FMT>(img, (uint32_t)y * (uint32_t)pitch + (uint32_t)x);
+}
+
+// One sample of a deformed window in raw units: the pixel whose rigid position has integer
+// part (xi, yi) and fractional part (fx, fy), where the fractions already carry the pixel's
+// deformation shift and may leave [0, 1).  MODE 1: bilinear as the rigid loaders compute it
+// (a row lerp along x, then along y), so a zero shift gives their exact value; MODE 2: 4x4
+// Lagrange.  Both fractions below 1e-4: the sample itself, as correlate.rs:186-188 does for
+// a window.  Sample indices clamp into the frame.
+template <int FMT, int MODE>
+__device__ __forceinline__ float deformed_sample(const void *__restrict__ img, int pitch, int H,
+                                                 int W, int xi, int yi, float fx, float fy)
+{
+    const float ixf = floorf(fx), iyf = floorf(fy);
+    const float wx = fx - ixf, wy = fy - iyf;
+    const int x = xi + (int)ixf, y = yi + (int)iyf;
+    const bool ints = wx < 1e-4f && wy < 1e-4f;
+#define at(xx, yy) clamped_sample<FMT>(img, pitch, H, W, (xx), (yy))
+    if constexpr (MODE == 1) {
+        const float p00 = at(x, y), p01 = at(x + 1, y), p10 = at(x, y + 1), p11 = at(x + 1, y + 1);
+        const float ha = __builtin_fmaf(wx, p01 - p00, p00);
+        const float hb = __builtin_fmaf(wx, p11 - p10, p10);
+        const float v = __builtin_fmaf(wy, hb - ha, ha);
+        return ints ? p00 : v;
+    } else {
+        float lx[4], ly[4];
+        lagrange4(wx, lx);
+        lagrange4(wy, ly);
+        float v = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            float h = 0.0f;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) h = __builtin_fmaf(lx[i], at(x - 1 + i, y - 1 + j), h);
+            v = __builtin_fmaf(ly[j], h, v);
+        }
+        return ints ? at(x, y) : v;
+    }
+#undef at
+}
+
+// Column c of both windows of a deformed pair.  The windows keep the rigid placement of the
+// predictor at the window centre (origins (ax, ay), (bx, by)); pixel (c, r) then moves by
+// -h in A and +h in B, h = (D(P) - D(C)) / 2, with D the predictor's deformation interpolant
+// (deform_sample), P = (mx + c, my + r) the pixel's position midway between its A and B
+// samples and C the window centre.  A uniform predictor gives h = 0 exactly and so the rigid
+// samples.  Each window's column is sampled by a plain loop into the window's LDS plane (row
+// stride N + 1) and read back with constant indices: the per-pixel work stays one loop body
+// rather than N unrolled copies, so the column registers never spill.  Every lane of the
+// workgroup calls this (it holds barriers); windows without signal get zeros.
+template <int N, int FMT, int MODE>
+__device__ __forceinline__ void load_columns_deformed(const PassPair &pp, bool sig, float ax,
+                                                      float ay, float bx, float by, int c,
+                                                      float *pl, float (&za)[N], float (&zb)[N])
+{
+    constexpr int P = N + 1;
+    const float mx = (ax + bx) * 0.5f, my = (ay + by) * 0.5f;
+    const float half = ((float)N - 1.0f) / 2.0f;
+    float uc = 0.0f, vc = 0.0f;
+    if (sig) deform_sample(pp.prev, pp.prev_dx, pp.prev_dy, mx + half, my + half, uc, vc);
+    const float X = mx + (float)c;
+#pragma unroll
+    for (int w = 0; w < 2; ++w) {
+        // window A (w = 0) moves by -h, window B by +h
+        const float o_x = w ? bx : ax, o_y = w ? by : ay;
+        const int xi = (int)floorf(o_x), yi = (int)floorf(o_y);
+        const float tx = o_x - (float)xi, ty = o_y - (float)yi;
+        const void *img = w ? pp.img_b : pp.img_a;
+        if (sig) {
+#pragma unroll 2
+            for (int r = 0; r < N; ++r) {
+                float u, v;
+                deform_sample(pp.prev, pp.prev_dx, pp.prev_dy, X, my + (float)r, u, v);
+                const float hx = (u - uc) * 0.5f, hy = (v - vc) * 0.5f;
+                const float fx = w ? tx + hx : tx - hx, fy = w ? ty + hy : ty - hy;
+                pl[r * P + c] = deformed_sample<FMT, MODE>(img, pp.pitch, pp.H, pp.W, xi + c, yi + r, fx, fy);
+            }
+        }
+        __syncthreads();
+        float(&z)[N] = w ? zb : za;
+#pragma unroll
+        for (int r = 0; r < N; ++r) z[r] = sig ? pl[r * P + c] : 0.0f;
+        __syncthreads();
+    }
+}
+
 // energy threshold of correlate.rs:110 in raw sample units
 template <int FMT>
 __device__ __forceinline__ double energy_floor()
@@ -187,16 +280,20 @@ __device__ __forceinline__ double energy_floor()
 // Window pair of one lane's column: raw samples bilinearly sampled at the window origins
 // (zeros for a masked window), with the approximate window means removed (the DC bin is
 // cleared exactly later, which is what the reference's mean removal amounts to).
-template <int N, int FMT>
+template <int N, int FMT, int DEF>
 __device__ __forceinline__ void ingest(const PassPair &pp, bool sig, float ax, float ay, float bx,
-                                       float by, int lane, float *red, float (&zr)[N],
+                                       float by, int lane, float *red, float *pl, float (&zr)[N],
                                        float (&zi)[N])
 {
     // every window of the wave keeps its (N+1)^2 footprint inside the frame: no clamping
     const bool fast = __all(!sig || (interior<N>(ax, ay, pp.W, pp.H) && interior<N>(bx, by, pp.W, pp.H)));
     // every window of the wave at integer origins (a first pass without predictor)
     const bool ints = __all(!sig || (integer_origin(ax, ay) && integer_origin(bx, by)));
-    if (sig) {
+    // DEF: the launch's deformation mode (a pass with a predictor); it is a template
+    // parameter so that the rigid kernels carry none of the deformed loader's registers
+    if (DEF != 0 && pp.has_prev) {
+        load_columns_deformed<N, FMT, DEF>(pp, sig, ax, ay, bx, by, lane, pl, zr, zi);
+    } else if (sig) {
         if (fast && ints) {
             load_column_int<N, FMT>(pp.img_a, pp.pitch, ax, ay, lane, zr);
             load_column_int<N, FMT>(pp.img_b, pp.pitch, bx, by, lane, zi);
@@ -462,7 +559,7 @@ __device__ __forceinline__ void find_peak_plane(const float *pr, int lane, int t
     }
 }
 
-template <int N, int FMT, int OCC>
+template <int N, int FMT, int OCC, int DEF>
 __global__ __launch_bounds__(KCfg<N>::BLOCK) __attribute__((amdgpu_waves_per_eu(OCC)))
 void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblocks)
 {
@@ -520,7 +617,7 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
 
     // ---- fused ingest + mean removal --------------------------------------------------
     float zr[N], zi[N];
-    ingest<N, FMT>(pp, sig, ax, ay, bx, by, lane, red, zr, zi);
+    ingest<N, FMT, DEF>(pp, sig, ax, ay, bx, by, lane, red, pr, zr, zi);
 
     // ---- forward transform of z = a + i b ----------------------------------------------
     // Transposes go through LDS.  A window inside one wave uses a single float plane and
@@ -735,23 +832,32 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
     find_peak_plane<N, FMT>(pr, lane, tid, pkv, pki, active, has_peak, pp.peaks + k);
 }
 
-template <int N, int FMT, int OCC>
+template <int N, int FMT, int OCC, int DEF = 0>
 hipError_t launch_corr_t(const PassPair *pairs, int npairs, uint32_t max_cells, hipStream_t st)
 {
     using C = KCfg<N>;
     const uint32_t bpp = (max_cells + C::WPB - 1) / C::WPB;
     const uint32_t blocks = bpp * (uint32_t)npairs;
     if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_correlate<N, FMT, OCC>), dim3(blocks), dim3(C::BLOCK), 0, st, pairs, bpp,
+    hipLaunchKernelGGL((k_correlate<N, FMT, OCC, DEF>), dim3(blocks), dim3(C::BLOCK), 0, st, pairs, bpp,
                        blocks);
     return hipGetLastError();
 }
 
+// deform != 0 (a deformed pass) runs the default variant's occupancy with the deformed loader;
+// only one-wave sizes (N <= 64) have deformed instances here (larger windows deform in the
+// generic-size kernel)
 template <int N, int FMT>
-hipError_t launch_corr_v(int variant, const PassPair *pairs, int npairs, uint32_t max_cells,
-                         hipStream_t st)
+hipError_t launch_corr_v(int variant, int deform, const PassPair *pairs, int npairs,
+                         uint32_t max_cells, hipStream_t st)
 {
     constexpr VariantSet vs = corr_variants(N);
+    if constexpr (KCfg<N>::ONE_WAVE) {
+        if (deform == 1) return launch_corr_t<N, FMT, vs.occ[0], 1>(pairs, npairs, max_cells, st);
+        if (deform == 2) return launch_corr_t<N, FMT, vs.occ[0], 2>(pairs, npairs, max_cells, st);
+    } else {
+        if (deform != 0) return hipErrorInvalidValue;
+    }
     if (variant == 0) return launch_corr_t<N, FMT, vs.occ[0]>(pairs, npairs, max_cells, st);
     if constexpr (vs.count > 1)
         if (variant == 1) return launch_corr_t<N, FMT, vs.occ[1]>(pairs, npairs, max_cells, st);
@@ -761,13 +867,13 @@ hipError_t launch_corr_v(int variant, const PassPair *pairs, int npairs, uint32_
 }
 
 template <int N>
-hipError_t launch_corr_n(int fmt, int variant, const PassPair *pairs, int npairs,
+hipError_t launch_corr_n(int fmt, int variant, int deform, const PassPair *pairs, int npairs,
                          uint32_t max_cells, hipStream_t st)
 {
     switch (fmt) {
-    case FMT_U8: return launch_corr_v<N, FMT_U8>(variant, pairs, npairs, max_cells, st);
-    case FMT_U16: return launch_corr_v<N, FMT_U16>(variant, pairs, npairs, max_cells, st);
-    default: return launch_corr_v<N, FMT_F32>(variant, pairs, npairs, max_cells, st);
+    case FMT_U8: return launch_corr_v<N, FMT_U8>(variant, deform, pairs, npairs, max_cells, st);
+    case FMT_U16: return launch_corr_v<N, FMT_U16>(variant, deform, pairs, npairs, max_cells, st);
+    default: return launch_corr_v<N, FMT_F32>(variant, deform, pairs, npairs, max_cells, st);
     }
 }
 

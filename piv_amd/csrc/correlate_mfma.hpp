@@ -146,7 +146,7 @@ __device__ __forceinline__ void mf_col(const typename Mfma<E>::v8 (&A)[T][T], co
     }
 }
 
-template <int N, int FMT, typename E>
+template <int N, int FMT, typename E, int DEF>
 __global__ __launch_bounds__(MfCfg<N>::BLOCK) void k_correlate_mf(const PassPair *__restrict__ pairs,
                                                                   uint32_t bpp, uint32_t nblocks)
 {
@@ -200,7 +200,7 @@ __global__ __launch_bounds__(MfCfg<N>::BLOCK) void k_correlate_mf(const PassPair
         sig = pp.sig[k] != 0;
     }
     float zr[N], zi[N];
-    ingest<N, FMT>(pp, sig, ax, ay, bx, by, lane, red, zr, zi);
+    ingest<N, FMT, DEF>(pp, sig, ax, ay, bx, by, lane, red, pr, zr, zi);
     float ea = 0.f, eb = 0.f;
 #pragma unroll
     for (int r = 0; r < N; ++r) {
@@ -315,35 +315,42 @@ __global__ __launch_bounds__(MfCfg<N>::BLOCK) void k_correlate_mf(const PassPair
                             pp.peaks + k);
 }
 
-template <int N, int FMT, typename E>
+template <int N, int FMT, typename E, int DEF>
 hipError_t launch_corr_mf_t(const PassPair *pairs, int npairs, uint32_t max_cells, hipStream_t st)
 {
     using M = MfCfg<N>;
     const uint32_t bpp = (max_cells + M::WPB - 1) / M::WPB;
     const uint32_t blocks = bpp * (uint32_t)npairs;
     if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_correlate_mf<N, FMT, E>), dim3(blocks), dim3(M::BLOCK), 0, st, pairs, bpp,
+    hipLaunchKernelGGL((k_correlate_mf<N, FMT, E, DEF>), dim3(blocks), dim3(M::BLOCK), 0, st, pairs, bpp,
                        blocks);
     return hipGetLastError();
 }
 
-// precision: 1 = fp16, 2 = bf16
+template <int N, typename E, int DEF>
+hipError_t launch_corr_mf_f(int fmt, const PassPair *pairs, int npairs, uint32_t max_cells,
+                            hipStream_t st)
+{
+    switch (fmt) {
+    case FMT_U8: return launch_corr_mf_t<N, FMT_U8, E, DEF>(pairs, npairs, max_cells, st);
+    case FMT_U16: return launch_corr_mf_t<N, FMT_U16, E, DEF>(pairs, npairs, max_cells, st);
+    default: return launch_corr_mf_t<N, FMT_F32, E, DEF>(pairs, npairs, max_cells, st);
+    }
+}
+
+// precision: 1 = fp16, 2 = bf16; deform: the pass's deformation mode (0 rigid)
 template <int N>
-hipError_t launch_corr_mf_n(int fmt, int precision, const PassPair *pairs, int npairs,
+hipError_t launch_corr_mf_n(int fmt, int precision, int deform, const PassPair *pairs, int npairs,
                             uint32_t max_cells, hipStream_t st)
 {
     if (precision == 1) {
-        switch (fmt) {
-        case FMT_U8: return launch_corr_mf_t<N, FMT_U8, _Float16>(pairs, npairs, max_cells, st);
-        case FMT_U16: return launch_corr_mf_t<N, FMT_U16, _Float16>(pairs, npairs, max_cells, st);
-        default: return launch_corr_mf_t<N, FMT_F32, _Float16>(pairs, npairs, max_cells, st);
-        }
+        if (deform == 1) return launch_corr_mf_f<N, _Float16, 1>(fmt, pairs, npairs, max_cells, st);
+        if (deform == 2) return launch_corr_mf_f<N, _Float16, 2>(fmt, pairs, npairs, max_cells, st);
+        return launch_corr_mf_f<N, _Float16, 0>(fmt, pairs, npairs, max_cells, st);
     }
-    switch (fmt) {
-    case FMT_U8: return launch_corr_mf_t<N, FMT_U8, __bf16>(pairs, npairs, max_cells, st);
-    case FMT_U16: return launch_corr_mf_t<N, FMT_U16, __bf16>(pairs, npairs, max_cells, st);
-    default: return launch_corr_mf_t<N, FMT_F32, __bf16>(pairs, npairs, max_cells, st);
-    }
+    if (deform == 1) return launch_corr_mf_f<N, __bf16, 1>(fmt, pairs, npairs, max_cells, st);
+    if (deform == 2) return launch_corr_mf_f<N, __bf16, 2>(fmt, pairs, npairs, max_cells, st);
+    return launch_corr_mf_f<N, __bf16, 0>(fmt, pairs, npairs, max_cells, st);
 }
 
 }  // namespace piv

@@ -219,6 +219,7 @@ piv_status piv_analyze_batch_noalloc(piv_analyzer *a, const void *pixels_a, cons
             d.grid = pl.grid;
             d.min_mean = c.min_mean;
             d.min_std = c.min_std;
+            d.deform = (int32_t)c.deformation;
             ValidatePair v{};
             const bool last = i + 1 == np;
             v.dx = last ? dx : val[i];
@@ -242,10 +243,11 @@ piv_status piv_analyze_batch_noalloc(piv_analyzer *a, const void *pixels_a, cons
             const uint32_t cells = (uint32_t)pl.cells;
             HIP_TRY(launch_place_pass(pp[i], (int)npairs, cells, st));
             const int mp = mf_precision(a, pl.n);
+            const int def = i > 0 ? (int)c.deformation : 0;
             if (mp)
-                HIP_TRY(launch_correlate_mf(pl.n, fmt, mp, pp[i], (int)npairs, cells, st));
+                HIP_TRY(launch_correlate_mf(pl.n, fmt, mp, pp[i], (int)npairs, cells, st, def));
             else
-                HIP_TRY(launch_correlate(pl.n, fmt, variant_of(a, pl.n, fmt), pp[i], (int)npairs, cells, st));
+                HIP_TRY(launch_correlate(pl.n, fmt, variant_of(a, pl.n, fmt), pp[i], (int)npairs, cells, st, def));
             HIP_TRY(launch_peak_finish(pp[i], (int)npairs, cells, st, mp != 0));
             HIP_TRY(launch_validate(vp[i], (int)npairs, cells, st));
         }

@@ -110,6 +110,8 @@ piv_status check_config(const piv_config &c)
                         "window size %u is valid in the reference but has no gfx950 kernel yet "
                         "(supported: 8, 12, 16, 24, 32, 48, 64, 96)",
                         c.windows[i]);
+    if (c.deformation > PIV_DEFORM_BICUBIC)
+        return fail(PIV_ERR_CONFIG, "invalid configuration: deformation mode %u", c.deformation);
     if (c.smooth_sigma > 0.0f && std::ceil(3.0f * c.smooth_sigma) > 255.0f)
         return fail(PIV_ERR_CONFIG, "smooth_sigma %g too large", c.smooth_sigma);
     return PIV_OK;
@@ -579,6 +581,7 @@ piv_status run_batch(piv_analyzer *a, const piv_frame *const *fa, const piv_fram
             d.grid = pl.grid;
             d.min_mean = a->cfg.min_mean;
             d.min_std = a->cfg.min_std;
+            d.deform = (int32_t)a->cfg.deformation;
             if (B->pitch != A->pitch)
                 return fail(PIV_ERR_INVALID_ARGUMENT, "frames of a pair must share a row pitch");
             pp[k] = d;
@@ -635,11 +638,12 @@ piv_status run_batch(piv_analyzer *a, const piv_frame *const *fa, const piv_fram
                 return launch_place_pass(ppd, (int)npairs, (uint32_t)pl.cells, st);
             }));
             const int mp = mf_precision(a, pl.n);
+            const int def = i > 0 ? (int)a->cfg.deformation : 0;
             HIP_TRY(timed(a, mp ? "correlate_mf" : "correlate", pl.n, cbytes, total, [&] {
                 return mp ? launch_correlate_mf(pl.n, fa[0]->fmt, mp, ppd, (int)npairs,
-                                                (uint32_t)pl.cells, st)
+                                                (uint32_t)pl.cells, st, def)
                           : launch_correlate(pl.n, fa[0]->fmt, variant_of(a, pl.n, fa[0]->fmt),
-                                             ppd, (int)npairs, (uint32_t)pl.cells, st);
+                                             ppd, (int)npairs, (uint32_t)pl.cells, st, def);
             }));
             // the finishing kernel reads peak records + origins (48 B) and writes 13 B/window
             HIP_TRY(timed(a, "finish", 0, (double)total * 61.0, total, [&] {

@@ -65,6 +65,62 @@ __device__ __forceinline__ void sample_field(const GridDev &g, const float *__re
     }
 }
 
+// Window deformation (no counterpart in the reference, which offsets windows rigidly):
+// the predictor's displacement at pixel (x, y) as the bilinear interpolant of the grid field
+// in lerp form, a + t (b - a) along x then y, so that equal neighbours give exactly their
+// value (a uniform field deforms nothing).  Positions clamp to the grid as
+// VectorField::sample does; where a neighbour is not finite the weighted NaN-skipping form
+// of VectorField::sample (field.rs:56-81) is used instead.
+__device__ __forceinline__ void deform_sample(const GridDev &g, const float *__restrict__ dx,
+                                              const float *__restrict__ dy, float x, float y,
+                                              float &u, float &v)
+{
+    const float fx = clamp_rs((x - center(g.x0, 0, g.step, g.window)) / (float)g.step, 0.0f,
+                              (float)(g.nx - 1));
+    const float fy = clamp_rs((y - center(g.y0, 0, g.step, g.window)) / (float)g.step, 0.0f,
+                              (float)(g.ny - 1));
+    const int ix = (int)floorf(fx), iy = (int)floorf(fy);
+    const float tx = fx - (float)ix, ty = fy - (float)iy;
+    const int ix1 = min(ix + 1, g.nx - 1), iy1 = min(iy + 1, g.ny - 1);
+    const float u00 = dx[iy * g.nx + ix], u01 = dx[iy * g.nx + ix1];
+    const float u10 = dx[iy1 * g.nx + ix], u11 = dx[iy1 * g.nx + ix1];
+    const float v00 = dy[iy * g.nx + ix], v01 = dy[iy * g.nx + ix1];
+    const float v10 = dy[iy1 * g.nx + ix], v11 = dy[iy1 * g.nx + ix1];
+    // both forms are evaluated and one selected (straight-line code: the windows' loaders
+    // call this once per pixel row, unrolled)
+    const float ut = u00 + tx * (u01 - u00), ub = u10 + tx * (u11 - u10);
+    const float vt = v00 + tx * (v01 - v00), vb = v10 + tx * (v11 - v10);
+    const float ul = ut + ty * (ub - ut), vl = vt + ty * (vb - vt);
+    const float w[4] = {(1.0f - tx) * (1.0f - ty), tx * (1.0f - ty), (1.0f - tx) * ty, tx * ty};
+    const float su[4] = {u00, u01, u10, u11}, sv[4] = {v00, v01, v10, v11};
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+    bool all = true;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const bool fin = isfinite(su[k]) && isfinite(sv[k]);
+        all = all && fin;
+        const bool use = w[k] > 0.0f && fin;
+        a0 = use ? a0 + w[k] * su[k] : a0;
+        a1 = use ? a1 + w[k] * sv[k] : a1;
+        a2 = use ? a2 + w[k] : a2;
+    }
+    const bool any = a2 > 0.0f;
+    const float uw = any ? a0 / a2 : 0.0f, vw = any ? a1 / a2 : 0.0f;
+    u = all ? ul : uw;
+    v = all ? vl : vw;
+}
+
+// 4-point Lagrange weights at nodes -1, 0, 1, 2 for a fraction t in [0, 1): the cubic
+// resampling kernel of the deformation (reproduces polynomials of degree <= 3 exactly).
+__device__ __forceinline__ void lagrange4(float t, float (&l)[4])
+{
+    const float tp1 = t + 1.0f, tm1 = t - 1.0f, tm2 = t - 2.0f;
+    l[0] = -(t * tm1 * tm2) / 6.0f;
+    l[1] = (tp1 * tm1 * tm2) / 2.0f;
+    l[2] = -(tp1 * t * tm2) / 2.0f;
+    l[3] = (tp1 * t * tm1) / 6.0f;
+}
+
 // multipass.rs:224-229 place closure
 __device__ __forceinline__ void place(int origin, float shift, float maxv, float &a, float &b)
 {

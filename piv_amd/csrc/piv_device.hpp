@@ -65,7 +65,8 @@ struct PassPair {
     int32_t use_mask;
     int32_t origins_given;
     float min_mean, min_std;
-    int32_t pad_;
+    int32_t deform;           // window deformation from the predictor (passes with has_prev):
+                              //   0 none (rigid offset), 1 bilinear, 2 bicubic resampling
 };
 
 // One image pair's validation job (piv-core/src/validate.rs:51-130).

@@ -22,31 +22,18 @@
 
 #include "piv_common.hpp"
 #include "piv_kernels.hpp"
+#include "corr_sizes.hpp"
 
 namespace piv {
 
-hipError_t launch_correlate_8(int fmt, int variant, const PassPair *pairs, int npairs, uint32_t max_cells,
-                              hipStream_t st);
-hipError_t launch_correlate_12(int fmt, int variant, const PassPair *pairs, int npairs, uint32_t max_cells,
-                              hipStream_t st);
-hipError_t launch_correlate_16(int fmt, int variant, const PassPair *pairs, int npairs, uint32_t max_cells,
-                              hipStream_t st);
-hipError_t launch_correlate_24(int fmt, int variant, const PassPair *pairs, int npairs, uint32_t max_cells,
-                              hipStream_t st);
-hipError_t launch_correlate_32(int fmt, int variant, const PassPair *pairs, int npairs, uint32_t max_cells,
-                              hipStream_t st);
-hipError_t launch_correlate_48(int fmt, int variant, const PassPair *pairs, int npairs, uint32_t max_cells,
-                              hipStream_t st);
-hipError_t launch_correlate_64(int fmt, int variant, const PassPair *pairs, int npairs, uint32_t max_cells,
-                              hipStream_t st);
-hipError_t launch_correlate_mf_16(int fmt, int precision, const PassPair *pairs, int npairs,
-                                  uint32_t max_cells, hipStream_t st);
-hipError_t launch_correlate_mf_32(int fmt, int precision, const PassPair *pairs, int npairs,
-                                  uint32_t max_cells, hipStream_t st);
-hipError_t launch_correlate_mf_64(int fmt, int precision, const PassPair *pairs, int npairs,
-                                  uint32_t max_cells, hipStream_t st);
-hipError_t launch_correlate_96(int fmt, int variant, const PassPair *pairs, int npairs, uint32_t max_cells,
-                              hipStream_t st);
+#define PIV_DECL_FFT(N)                                                                       \
+    hipError_t launch_correlate_##N(int fmt, int variant, int deform, const PassPair *pairs,       \
+                                    int npairs, uint32_t max_cells, hipStream_t st);
+#define PIV_DECL_MF(N)                                                                        \
+    hipError_t launch_correlate_mf_##N(int fmt, int precision, int deform, const PassPair *pairs, \
+                                       int npairs, uint32_t max_cells, hipStream_t st);
+PIV_FFT_SIZES(PIV_DECL_FFT)
+PIV_MF_SIZES(PIV_DECL_MF)
 
 // ---------------------------------------------------------------------------------------
 // validation (validate.rs:51-171): two launches per pass, one thread per vector
@@ -781,26 +768,24 @@ __global__ __launch_bounds__(WBLOCK) void k_peak_finish(const PassPair *__restri
 
 bool correlate_supported(int n)
 {
+#define PIV_CASE_SUP(N) case N:
     switch (n) {
-    case 8: case 12: case 16: case 24: case 32: case 48: case 64: case 96: return true;
+    PIV_FFT_SIZES(PIV_CASE_SUP) return true;
     default: return false;
     }
+#undef PIV_CASE_SUP
 }
 
 hipError_t launch_correlate(int n, int fmt, int variant, const PassPair *pairs, int npairs,
-                            uint32_t max_cells, hipStream_t st)
+                            uint32_t max_cells, hipStream_t st, int deform)
 {
+#define PIV_CASE_FFT(N)                                                                       \
+    case N: return launch_correlate_##N(fmt, variant, deform, pairs, npairs, max_cells, st);
     switch (n) {
-    case 8: return launch_correlate_8(fmt, variant, pairs, npairs, max_cells, st);
-    case 12: return launch_correlate_12(fmt, variant, pairs, npairs, max_cells, st);
-    case 16: return launch_correlate_16(fmt, variant, pairs, npairs, max_cells, st);
-    case 24: return launch_correlate_24(fmt, variant, pairs, npairs, max_cells, st);
-    case 32: return launch_correlate_32(fmt, variant, pairs, npairs, max_cells, st);
-    case 48: return launch_correlate_48(fmt, variant, pairs, npairs, max_cells, st);
-    case 64: return launch_correlate_64(fmt, variant, pairs, npairs, max_cells, st);
-    case 96: return launch_correlate_96(fmt, variant, pairs, npairs, max_cells, st);
+    PIV_FFT_SIZES(PIV_CASE_FFT)
     default: return hipErrorInvalidValue;
     }
+#undef PIV_CASE_FFT
 }
 
 hipError_t launch_validate(const ValidatePair *jobs, int njobs, uint32_t max_cells,
@@ -839,14 +824,15 @@ hipError_t launch_peak_finish(const PassPair *pairs, int npairs, uint32_t max_ce
 bool correlate_mf_supported(int n) { return n == 16 || n == 32 || n == 64; }
 
 hipError_t launch_correlate_mf(int n, int fmt, int precision, const PassPair *pairs, int npairs,
-                               uint32_t max_cells, hipStream_t st)
+                               uint32_t max_cells, hipStream_t st, int deform)
 {
+#define PIV_CASE_MF(N)                                                                        \
+    case N: return launch_correlate_mf_##N(fmt, precision, deform, pairs, npairs, max_cells, st);
     switch (n) {
-    case 16: return launch_correlate_mf_16(fmt, precision, pairs, npairs, max_cells, st);
-    case 32: return launch_correlate_mf_32(fmt, precision, pairs, npairs, max_cells, st);
-    case 64: return launch_correlate_mf_64(fmt, precision, pairs, npairs, max_cells, st);
+    PIV_MF_SIZES(PIV_CASE_MF)
     default: return hipErrorInvalidValue;
     }
+#undef PIV_CASE_MF
 }
 
 hipError_t launch_pass(int n, int fmt, int variant, const PassPair *pairs, int npairs,

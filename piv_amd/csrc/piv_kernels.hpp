@@ -9,8 +9,10 @@ namespace piv {
 
 bool correlate_supported(int n);
 // variant: index into corr_variants(n) (variants.hpp); 0 is the default
+// deform: the pass's deformation mode (0 rigid, 1 bilinear, 2 bicubic; only pairs with a
+// predictor deform)
 hipError_t launch_correlate(int n, int fmt, int variant, const PassPair *pairs, int npairs,
-                            uint32_t max_cells, hipStream_t st);
+                            uint32_t max_cells, hipStream_t st, int deform = 0);
 // placement + mask (one thread per window), and sub-pixel fit + assembly of the raw field
 hipError_t launch_place_pass(const PassPair *pairs, int npairs, uint32_t max_cells, hipStream_t st);
 hipError_t launch_peak_finish(const PassPair *pairs, int npairs, uint32_t max_cells, hipStream_t st,
@@ -18,7 +20,7 @@ hipError_t launch_peak_finish(const PassPair *pairs, int npairs, uint32_t max_ce
 // DFT-GEMM correlation on the matrix cores (precision 1 = fp16, 2 = bf16), n in {16, 32, 64}
 bool correlate_mf_supported(int n);
 hipError_t launch_correlate_mf(int n, int fmt, int precision, const PassPair *pairs, int npairs,
-                               uint32_t max_cells, hipStream_t st);
+                               uint32_t max_cells, hipStream_t st, int deform = 0);
 // placement + mask, correlation, sub-pixel fit and assembly of one pass (three launches)
 hipError_t launch_pass(int n, int fmt, int variant, const PassPair *pairs, int npairs,
                        uint32_t max_cells, hipStream_t st);

@@ -15,6 +15,7 @@
 // Up to max_inflight batches are on the device at once (one HIP stream per lane), so a batch
 // admitted while another runs overlaps with it.
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -321,7 +322,7 @@ piv_status new_slot(piv_server *s, SrvStream *st, Slot **out)
 // correlation launch, then the ValidatePairs of the same requests.
 struct BatchPlan {
     struct Group {
-        int n, fmt, count, variant;
+        int n, fmt, deform, count, variant;
         uint32_t max_cells;
         size_t pp_off;
     };
@@ -359,20 +360,25 @@ void build_batch(Lane &L, BatchPlan &bp)
     for (size_t si = 0; si < maxsteps; ++si) {
         BatchPlan::Step &S = bp.steps[si];
         S.pp_off = doff;
-        std::vector<std::pair<int, int>> keys;
+        // groups: (window, format, deformation mode of this step), one launch each
+        auto group_key = [&](Request *r) {
+            const int def = si > 0 ? (int)r->st->an->cfg.deformation : 0;
+            return std::array<int, 3>{r->st->plans[si].n, r->slot->fa->fmt, def};
+        };
+        std::vector<std::array<int, 3>> keys;
         for (Request *r : reqs) {
             if (r->st->plans.size() <= si) continue;
-            const std::pair<int, int> kk{r->st->plans[si].n, r->slot->fa->fmt};
+            const std::array<int, 3> kk = group_key(r);
             if (std::find(keys.begin(), keys.end(), kk) == keys.end()) keys.push_back(kk);
         }
         std::sort(keys.begin(), keys.end());
         for (auto &kk : keys) {
-            BatchPlan::Group g{kk.first, kk.second, 0, -1, 0, doff};
+            BatchPlan::Group g{kk[0], kk[1], kk[2], 0, -1, 0, doff};
             for (Request *r : reqs) {
                 if (r->st->plans.size() <= si) continue;
-                if (r->st->plans[si].n != kk.first || r->slot->fa->fmt != kk.second) continue;
+                if (group_key(r) != kk) continue;
                 // the kernel variant of the group's first stream (variants are bit-identical)
-                if (g.variant < 0) g.variant = variant_of(r->st->an, kk.first, kk.second);
+                if (g.variant < 0) g.variant = variant_of(r->st->an, kk[0], kk[1]);
                 ++g.count;
                 g.max_cells = std::max(g.max_cells, (uint32_t)r->st->plans[si].cells);
             }
@@ -398,7 +404,9 @@ void build_batch(Lane &L, BatchPlan &bp)
                 Request *r = reqs[k];
                 SrvStream *st = r->st;
                 if (st->plans.size() <= si) continue;
-                if (st->plans[si].n != g.n || r->slot->fa->fmt != g.fmt) continue;
+                if (st->plans[si].n != g.n || r->slot->fa->fmt != g.fmt ||
+                    (si > 0 ? (int)st->an->cfg.deformation : 0) != g.deform)
+                    continue;
                 const PassPlan &pl = st->plans[si];
                 const size_t c = pl.cells;
                 Slot *sl = r->slot;
@@ -432,6 +440,7 @@ void build_batch(Lane &L, BatchPlan &bp)
                 d.grid = pl.grid;
                 d.min_mean = st->an->cfg.min_mean;
                 d.min_std = st->an->cfg.min_std;
+                d.deform = (int32_t)st->an->cfg.deformation;
                 reinterpret_cast<PassPair *>(bp.host.data() + g.pp_off)[gi++] = d;
                 ValidatePair v{};
                 v.raw_dx = d.out_dx;
@@ -471,6 +480,7 @@ void build_batch(Lane &L, BatchPlan &bp)
             bp.key.push_back(g.count);
             bp.key.push_back(g.max_cells);
             bp.key.push_back(g.variant);
+            bp.key.push_back(g.deform);
         }
     }
 }
@@ -487,7 +497,7 @@ hipError_t enqueue_batch(Lane &L, const BatchPlan &bp)
         for (const BatchPlan::Group &g : S.groups) {
             if (e != hipSuccess) break;
             e = launch_correlate(g.n, g.fmt, g.variant, reinterpret_cast<const PassPair *>(dd + g.pp_off), g.count,
-                                 g.max_cells, st);
+                                 g.max_cells, st, g.deform);
         }
         if (e == hipSuccess) e = launch_peak_finish(pp, S.count, S.max_cells, st);
         if (e == hipSuccess)

@@ -27,7 +27,7 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(lib, n), n
     assert set(names) == set(L._SIGS), set(names) ^ set(L._SIGS)
-    assert lib.piv_abi_version() == 1
+    assert lib.piv_abi_version() == 2
 
 
 def test_config_defaults_match_reference():
