@@ -6,9 +6,9 @@
 // exchanged between lanes only at the 2-D transpose points, through LDS.
 //
 // The transform is a Stockham autosort FFT (natural order in and out) with radix-4 stages
-// first, then radix 2 and 3, so every even N = 2^a 3^b is covered.  All loops are fully
-// unrolled, indices and twiddles are compile-time constants: twiddles come from a constexpr
-// table evaluated in double and rounded to float, and multiplications by 1, -1, +-i are
+// first, then radix 2, 3, 5 and 7, so every even N = 2^a 3^b 5^c 7^d is covered.  All loops
+// are fully unrolled, indices and twiddles are compile-time constants: twiddles come from a
+// constexpr table evaluated in double and rounded to float, and multiplications by 1, -1, +-i are
 // elided.  Complex products use explicit fma so the rest of the library can be built with
 // -ffp-contract=off (bit-exact scalar paths) without slowing the butterflies down.
 #pragma once
@@ -97,13 +97,21 @@ struct Tw {
     static constexpr Twiddles<N> t{};
 };
 
-constexpr int pick_radix(int rem) { return rem % 4 == 0 ? 4 : (rem % 2 == 0 ? 2 : (rem % 3 == 0 ? 3 : rem)); }
+constexpr int pick_radix(int rem)
+{
+    return rem % 4 == 0 ? 4
+         : rem % 2 == 0 ? 2
+         : rem % 3 == 0 ? 3
+         : rem % 5 == 0 ? 5
+         : rem % 7 == 0 ? 7
+                        : rem;
+}
 
 constexpr bool supported(int n)
 {
     if (n < 2) return false;
-    while (n % 2 == 0) n /= 2;
-    while (n % 3 == 0) n /= 3;
+    for (int p : {2, 3, 5, 7})
+        while (n % p == 0) n /= p;
     return n == 1;
 }
 
@@ -170,8 +178,47 @@ __device__ __forceinline__ void dft(float (&vr)[R], float (&vi)[R])
         vi[1] = mi + qi;
         vr[2] = mr - qr;
         vi[2] = mi - qi;
+    } else if constexpr (R == 5 || R == 7) {
+        // odd prime R from the symmetric pairs j, R - j: with s_j = v_j + v_{R-j} and
+        // d_j = v_j - v_{R-j},  Y_m = v_0 + sum_j cos(2 pi j m / R) s_j + S i sum_j
+        // sin(2 pi j m / R) d_j,  and Y_{R-m} the same with -S
+        constexpr int H = R / 2;
+        float sr[H + 1], si[H + 1], dr[H + 1], di[H + 1];
+#pragma unroll
+        for (int j = 1; j <= H; ++j) {
+            sr[j] = vr[j] + vr[R - j];
+            si[j] = vi[j] + vi[R - j];
+            dr[j] = vr[j] - vr[R - j];
+            di[j] = vi[j] - vi[R - j];
+        }
+        float y0r = vr[0], y0i = vi[0];
+#pragma unroll
+        for (int j = 1; j <= H; ++j) {
+            y0r += sr[j];
+            y0i += si[j];
+        }
+#pragma unroll
+        for (int m = 1; m <= H; ++m) {
+            float ar = vr[0], ai = vi[0], br = 0.0f, bi = 0.0f;
+#pragma unroll
+            for (int j = 1; j <= H; ++j) {
+                const float c = Tw<R>::t.c[(j * m) % R], sn = Tw<R>::t.s[(j * m) % R];
+                ar = __builtin_fmaf(c, sr[j], ar);
+                ai = __builtin_fmaf(c, si[j], ai);
+                br = __builtin_fmaf(sn, dr[j], br);
+                bi = __builtin_fmaf(sn, di[j], bi);
+            }
+            // S i B = (-S bi, S br)
+            const float qr = S < 0 ? bi : -bi, qi = S < 0 ? -br : br;
+            vr[m] = ar + qr;
+            vi[m] = ai + qi;
+            vr[R - m] = ar - qr;
+            vi[R - m] = ai - qi;
+        }
+        vr[0] = y0r;
+        vi[0] = y0i;
     } else {
-        static_assert(R == 2 || R == 3 || R == 4, "radix");
+        static_assert(R == 2 || R == 3 || R == 4 || R == 5 || R == 7, "radix");
     }
 }
 
@@ -218,7 +265,7 @@ __device__ __forceinline__ void stockham(float (&xr)[N], float (&xi)[N])
 template <int N, int S>
 __device__ __forceinline__ void fft(float (&xr)[N], float (&xi)[N])
 {
-    static_assert(supported(N), "N must be 2^a 3^b");
+    static_assert(supported(N), "N must be 2^a 3^b 5^c 7^d");
     stockham<N, S, 1, N>(xr, xi);
 }
 

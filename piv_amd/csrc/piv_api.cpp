@@ -107,9 +107,9 @@ piv_status check_config(const piv_config &c)
     for (uint32_t i = 0; i < c.num_windows; ++i)
         if (!correlate_supported((int)c.windows[i]))
             return fail(PIV_ERR_UNSUPPORTED,
-                        "window size %u is valid in the reference but has no gfx950 kernel yet "
-                        "(supported: 8, 12, 16, 24, 32, 48, 64, 96)",
-                        c.windows[i]);
+                        "window size %u is valid in the reference but above this library's largest "
+                        "(%d px; every even size from 8 up to it is supported)",
+                        c.windows[i], kMaxWindow);
     if (c.deformation > PIV_DEFORM_BICUBIC)
         return fail(PIV_ERR_CONFIG, "invalid configuration: deformation mode %u", c.deformation);
     if (c.smooth_sigma > 0.0f && std::ceil(3.0f * c.smooth_sigma) > 255.0f)

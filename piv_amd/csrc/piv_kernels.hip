@@ -34,6 +34,12 @@ namespace piv {
                                        int npairs, uint32_t max_cells, hipStream_t st);
 PIV_FFT_SIZES(PIV_DECL_FFT)
 PIV_MF_SIZES(PIV_DECL_MF)
+hipError_t launch_correlate_any_32(int fmt, int deform, const PassPair *pairs, int npairs,
+                                   uint32_t max_cells, hipStream_t st);
+hipError_t launch_correlate_any_64(int fmt, int deform, const PassPair *pairs, int npairs,
+                                   uint32_t max_cells, hipStream_t st);
+hipError_t launch_correlate_any_128(int fmt, int deform, const PassPair *pairs, int npairs,
+                                    uint32_t max_cells, hipStream_t st);
 
 // ---------------------------------------------------------------------------------------
 // validation (validate.rs:51-171): two launches per pass, one thread per vector
@@ -766,7 +772,8 @@ __global__ __launch_bounds__(WBLOCK) void k_peak_finish(const PassPair *__restri
 // host-side launchers
 // ---------------------------------------------------------------------------------------
 
-bool correlate_supported(int n)
+// sizes with an instantiated register-resident FFT kernel (corr_sizes.hpp)
+static bool fft_kernel(int n)
 {
 #define PIV_CASE_SUP(N) case N:
     switch (n) {
@@ -776,9 +783,19 @@ bool correlate_supported(int n)
 #undef PIV_CASE_SUP
 }
 
+bool correlate_supported(int n) { return n >= 8 && n <= kMaxWindow && n % 2 == 0; }
+
+bool correlate_generic(int n, int deform) { return !fft_kernel(n) || (deform != 0 && n > 64); }
+
 hipError_t launch_correlate(int n, int fmt, int variant, const PassPair *pairs, int npairs,
                             uint32_t max_cells, hipStream_t st, int deform)
 {
+    if (!correlate_supported(n)) return hipErrorInvalidValue;
+    if (correlate_generic(n, deform)) {
+        if (n <= 32) return launch_correlate_any_32(fmt, deform, pairs, npairs, max_cells, st);
+        if (n <= 64) return launch_correlate_any_64(fmt, deform, pairs, npairs, max_cells, st);
+        return launch_correlate_any_128(fmt, deform, pairs, npairs, max_cells, st);
+    }
 #define PIV_CASE_FFT(N)                                                                       \
     case N: return launch_correlate_##N(fmt, variant, deform, pairs, npairs, max_cells, st);
     switch (n) {

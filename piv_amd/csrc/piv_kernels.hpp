@@ -7,7 +7,12 @@
 
 namespace piv {
 
+// Largest window size with a correlation kernel; every even size from 8 up to it has one.
+constexpr int kMaxWindow = 128;
 bool correlate_supported(int n);
+// whether a pass of window n (deformation mode `deform`) runs the generic-size kernel
+// (k_correlate_any) rather than the register-resident FFT kernel of its size
+bool correlate_generic(int n, int deform);
 // variant: index into corr_variants(n) (variants.hpp); 0 is the default
 // deform: the pass's deformation mode (0 rigid, 1 bilinear, 2 bicubic; only pairs with a
 // predictor deform)

@@ -19,13 +19,14 @@ struct VariantSet {
 constexpr VariantSet corr_variants(int n)
 {
     switch (n) {
-    case 8: case 16: return {3, {4, 6, 8}};
+    case 8: case 10: case 14: case 16: return {3, {4, 6, 8}};
     case 12: return {3, {6, 4, 8}};
-    case 24: return {3, {4, 3, 5}};
-    case 32: return {2, {3, 4}};
+    case 20: case 24: case 28: return {3, {4, 3, 5}};
+    case 32: case 40: return {2, {3, 4}};
     case 48: return {3, {2, 3, 4}};
+    case 56: return {2, {2, 3}};
     case 64: return {2, {2, 1}};
-    default: return {1, {1, 0, 0}};
+    default: return {1, {1, 0, 0}};  // 96 and the generic-size kernel
     }
 }
 

@@ -90,9 +90,12 @@ def test_no_device_is_a_device_error():
 
 
 def test_kernel_variant_counts():
-    """Every supported window size has at least one correlation-kernel variant; others none."""
+    """Every even window size from 8 to 128 has at least one correlation-kernel variant (the
+    generic-size kernel has one); other sizes none."""
     import piv_amd
-    for n in (8, 12, 16, 24, 32, 48, 64, 96):
-        assert 1 <= piv_amd.kernel_variant_count(n) <= 4
-    for n in (0, 10, 40, 128):
-        assert piv_amd.kernel_variant_count(n) == 0
+    for n in range(8, 129, 2):
+        assert 1 <= piv_amd.kernel_variant_count(n) <= 4, n
+    for n in (10, 14, 20, 28, 40, 56):  # sizes with radix-5 / radix-7 stages
+        assert piv_amd.kernel_variant_count(n) >= 2, n
+    for n in (0, 6, 9, 130, 256):
+        assert piv_amd.kernel_variant_count(n) == 0, n

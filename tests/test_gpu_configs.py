@@ -383,7 +383,7 @@ def test_errors(piv):
     with pytest.raises(piv.ImageTooSmall):
         an.analyze(small, small)
     with pytest.raises(piv.Unsupported):
-        piv.PivAnalyzer(piv.PivConfig(windows=[40], final_iterations=1))
+        piv.PivAnalyzer(piv.PivConfig(windows=[130], final_iterations=1))
 
 
 def test_background_estimator(piv):

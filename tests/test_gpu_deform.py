@@ -61,25 +61,24 @@ def test_uniform_predictor_is_rigid_bitwise(piv):
         assert np.array_equal(bits(out[1].peak_ratio), bits(out[0].peak_ratio)), seed
 
 
-@pytest.mark.parametrize("mode", ["bilinear", "bicubic"])
-def test_vortex_1024_stage_parity(piv, oracle, vortex, mode):
-    bench, pairs, field = vortex
-    a, b = pairs[0]
-    cfg = piv.PivConfig(windows=[64, 32, 16], overlap=0.5, final_iterations=1, deformation=mode)
-    an = piv.PivAnalyzer(cfg, piv.PreprocessConfig.raw())
+def deformed_stage_parity(piv, oracle, an, cfg, a, b, mode, tol=1e-4):
+    """Replay every pass of a traced deformed GPU run through the oracle (from the GPU's own
+    origins and previous validated field), then compare the final field with an independent
+    oracle run.  Returns the GPU result."""
     res, traces = an.analyze_traced(an.prepare(a), an.prepare(b))
     pa, pb = oracle.prepare(a), oracle.prepare(b)
     h, w = a.shape
-    maxd = np.float32(32.0)
+    maxd = np.float32(cfg.windows[0]) / np.float32(2)
     v = cfg.validation
     prev_g = None
     for i, t in enumerate(traces):
         g = t.raw.grid
-        og = oracle.grid_new(h, w, g.window, 0.5)
+        og = oracle.grid_new(h, w, g.window, cfg.overlap)
         deform = None
         if i > 0:
             deform = (MODES[mode], prev_g, traces[i - 1].validated.dx, traces[i - 1].validated.dy)
-        ref = oracle.correlate_pass(pa, pb, og, t.origins, deform=deform)
+        ref = oracle.correlate_pass(pa, pb, og, t.origins, cfg.mask.min_mean, cfg.mask.min_std,
+                                    deform=deform)
         assert np.array_equal(t.raw.flags, ref.flags), f"pass {i} NO_SIGNAL"
         ref_pk = np.where(ref.peak_iy >= 0, ref.peak_iy * g.window + ref.peak_ix, -1)
         mism = np.count_nonzero(t.peak_index != ref_pk)
@@ -87,21 +86,33 @@ def test_vortex_1024_stage_parity(piv, oracle, vortex, mode):
         fin = np.isfinite(ref.dx)
         e = max(np.abs(t.raw.dx - ref.dx)[fin].max(initial=0),
                 np.abs(t.raw.dy - ref.dy)[fin].max(initial=0))
-        assert e < 1e-4, (i, e)
+        assert e < tol, (i, e)
         f = oracle.Field(None, t.raw.dx, t.raw.dy, t.raw.peak_ratio, t.raw.flags)
         val, s = oracle.validate(f, v.ppr_threshold, v.median_threshold, v.median_eps, v.replace,
                                  maxd)
         assert np.array_equal(val.flags, t.validated.flags)
         assert np.array_equal(bits(val.dx), bits(t.validated.dx))
-        print(f"{mode} pass {i}: max |raw d - oracle| = {e:.2e} px")
+        print(f"{mode} pass {i} (n = {g.window}): max |raw d - oracle| = {e:.2e} px")
         prev_g = og
-    ocfg = oracle.PivConfig(windows=[64, 32, 16], overlap=0.5, final_iterations=1)
+    ocfg = oracle.PivConfig(windows=list(cfg.windows), overlap=cfg.overlap,
+                            final_iterations=cfg.final_iterations, min_mean=cfg.mask.min_mean,
+                            min_std=cfg.mask.min_std)
     ref, s = oracle.analyze(pa, pb, ocfg, deformation=mode)
     assert np.array_equal(res.field.flags, ref.flags)
     fin = np.isfinite(ref.dx)
     assert np.abs(res.field.dx - ref.dx)[fin].max(initial=0) < 1e-3
     assert np.abs(res.field.dy - ref.dy)[fin].max(initial=0) < 1e-3
     assert s == res.summary.__dict__
+    return res
+
+
+@pytest.mark.parametrize("mode", ["bilinear", "bicubic"])
+def test_vortex_1024_stage_parity(piv, oracle, vortex, mode):
+    bench, pairs, field = vortex
+    a, b = pairs[0]
+    cfg = piv.PivConfig(windows=[64, 32, 16], overlap=0.5, final_iterations=1, deformation=mode)
+    an = piv.PivAnalyzer(cfg, piv.PreprocessConfig.raw())
+    res = deformed_stage_parity(piv, oracle, an, cfg, a, b, mode)
     rigid = piv.PivAnalyzer(piv.PivConfig(windows=[64, 32, 16], final_iterations=1),
                             piv.PreprocessConfig.raw())
     r0 = rigid.analyze(rigid.prepare(a), rigid.prepare(b))

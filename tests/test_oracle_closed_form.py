@@ -13,7 +13,10 @@ import pytest
 from piv_amd import synthetic as S
 
 
-@pytest.mark.parametrize("n", [8, 12, 16, 18, 24, 32, 36, 40, 48, 50, 64, 96, 128, 14])
+# every radix of the oracle's FFT, and lengths 2p with a large prime p (the generic-size
+# correlation kernel's sizes)
+@pytest.mark.parametrize("n", [8, 12, 16, 18, 24, 32, 36, 40, 48, 50, 64, 96, 128, 14, 22, 26, 34,
+                               58, 62, 122, 126])
 @pytest.mark.parametrize("prec", [32, 64])
 def test_fft_matches_direct_dft(oracle, n, prec):
     rng = np.random.default_rng(n)
