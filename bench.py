@@ -32,6 +32,7 @@ sys.path.insert(0, ROOT)
 H = W = 1024
 WINDOWS = [64, 32, 16]
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
 HOST_SUBMITTERS = 4  # client threads submitting host frames in the saturated serving run
 
 
@@ -57,9 +58,24 @@ def truth_rms(res_dx, res_dy, fl, grid, field, margin=32.0):
     return float(np.sqrt(e2[keep].mean())), int(keep.sum())
 
 
-def cpu_baseline(pair, seconds):
+def cpu_model():
+    """The host CPU's model name as lscpu reports it (None when unavailable)."""
+    import subprocess
+
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+    except (OSError, subprocess.SubprocessError):
+        return None
+    for line in out.splitlines():
+        if line.lower().startswith("model name"):
+            return line.split(":", 1)[1].strip()
+    return None
+
+
+def cpu_baseline(pair, seconds, single_core_seconds):
     """The oracle (C restatement of the Rust CPU path, FP32, OpenMP over windows as rayon's
-    map_init loop) on this host's cores: fields/s over a bounded sample of whole pairs."""
+    map_init loop) on this host: fields/s over a bounded sample of whole pairs, on all the
+    cores this process may use and again on one core."""
     from oracle import oracle as O
 
     try:
@@ -69,16 +85,25 @@ def cpu_baseline(pair, seconds):
     cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
     pa, pb = O.prepare(pair[0]), O.prepare(pair[1])
     cfg = O.PivConfig(windows=list(WINDOWS), overlap=0.5, final_iterations=1)
-    n, t0 = 0, time.perf_counter()
-    while True:
-        O.analyze(pa, pb, cfg, precision=32, threads=cores)
-        n += 1
-        el = time.perf_counter() - t0
-        if el >= seconds or n >= 1000:
-            break
+
+    def rate(threads, budget):
+        n, t0 = 0, time.perf_counter()
+        while True:
+            O.analyze(pa, pb, cfg, precision=32, threads=threads)
+            n += 1
+            el = time.perf_counter() - t0
+            if el >= budget or n >= 1000:
+                return n, el
+
+    n, el = rate(cores, seconds)
+    n1, el1 = rate(1, single_core_seconds)
+    model = cpu_model()
     return {"value": n / el, "unit": "fields/s", "cores": cores, "kind": "port",
-            "sample": f"{n} whole 1024x1024 3-pass pairs ({el:.1f} s), C oracle FP32, "
-                      f"{cores} OpenMP threads"}
+            "cpu_model": model,
+            "sample": f"{n} whole 1024x1024 3-pass pairs ({el:.1f} s), C oracle FP32 "
+                      f"(restatement of the Rust CPU path), {cores} OpenMP threads",
+            "single_core": {"value": n1 / el1, "unit": "fields/s", "cores": 1,
+                            "sample": f"{n1} whole pairs ({el1:.1f} s) on one thread"}}
 
 
 # Mixed serving workload: four streams (frame size, sample format, passes, particle model,
@@ -502,6 +527,69 @@ def precision_bench(an, fa, fb, dx, dy, pr, fl, g, field, uniq, steps):
     return out
 
 
+def deform_bench(A, B, field, uniq, steps):
+    """The 1024x1024 three-pass workload with window deformation between passes (bilinear and
+    bicubic resampling), in FP32 and, with bicubic deformation, with DFT-GEMM correlation in
+    fp16 / bf16: fields/s over the same batch as the headline, kernel durations, and the
+    displacement error against the generating field and against the oracle run with the same
+    deformation (the FP32 oracle for FP32, the FP64 oracle for fp16 / bf16)."""
+    import torch
+
+    from oracle import oracle as O
+    from piv_amd import PivAnalyzer, PivConfig, PreparedFrame, PreprocessConfig
+
+    P = A.shape[0]
+    dev = A.device
+    pa, pb = O.prepare(uniq[0][0]), O.prepare(uniq[0][1])
+    ocfg = O.PivConfig(windows=list(WINDOWS), overlap=0.5, final_iterations=1)
+    out = {}
+    for mode in ("bilinear", "bicubic"):
+        cfg = PivConfig(windows=list(WINDOWS), overlap=0.5, final_iterations=1, deformation=mode)
+        an = PivAnalyzer(cfg, PreprocessConfig.raw(), device=0)
+        fa = [PreparedFrame.from_device(an, A[i]) for i in range(P)]
+        fb = [PreparedFrame.from_device(an, B[i]) for i in range(P)]
+        g = an.output_grid(H, W)
+        dx = torch.empty((P, g.ny, g.nx), dtype=torch.float32, device=dev)
+        dy, pr = torch.empty_like(dx), torch.empty_like(dx)
+        fl = torch.empty((P, g.ny, g.nx), dtype=torch.uint8, device=dev)
+        refs = {"fp32": O.analyze(pa, pb, ocfg, deformation=mode)[0]}
+        precs = ("fp32", "fp16", "bf16") if mode == "bicubic" else ("fp32",)
+        if len(precs) > 1:
+            refs["fp64"] = O.analyze(pa, pb, ocfg, precision=64, deformation=mode)[0]
+        for prec in precs:
+            an.set_precision(prec)
+            for _ in range(2):
+                an.analyze_batch_device(fa, fb, dx, dy, pr, fl, summaries=False, sync=True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                an.analyze_batch_device(fa, fb, dx, dy, pr, fl, summaries=False, sync=False)
+            an.synchronize()
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+            an.profile(True)
+            an.analyze_batch_device(fa, fb, dx, dy, pr, fl, summaries=False, sync=True)
+            stats = an.profile_read()
+            an.profile(False)
+            hdx, hdy, hfl = dx[0].cpu().numpy(), dy[0].cpu().numpy(), fl[0].cpu().numpy()
+            rms_t, _ = truth_rms(hdx, hdy, hfl, g, field)
+            ref = refs["fp32" if prec == "fp32" else "fp64"]
+            both = ((hfl & 23) == 0) & ((ref.flags & 23) == 0)
+            d2 = (hdx - ref.dx)[both] ** 2 + (hdy - ref.dy)[both] ** 2
+            out[f"{mode}_{prec}"] = {
+                "fields_per_s": round(P * steps / el, 1),
+                "kernel_avg_ms": {f'{s["name"]}{s["window"] or ""}': round(s["total_ms"] / s["launches"], 4)
+                                  for s in stats},
+                "rms_px_vs_truth": round(rms_t, 5),
+                "rms_px_vs_oracle": round(float(np.sqrt(d2.mean())), 7),
+                "max_px_vs_oracle": round(float(np.sqrt(d2.max())), 6),
+                "oracle": "FP32 C oracle" if prec == "fp32" else "FP64 C oracle",
+                "flags_equal_oracle": float(np.mean(hfl == ref.flags))}
+        an.set_precision("fp32")
+        del fa, fb
+    return out
+
+
 def post_bench(an, g, dx, dy, fl, repeats=20, seed=3):
     """Post-analysis stages on the device: the flow statistics of the step's validated fields
     (one launch pair for the whole batch), and the temporal / spatial filters on a video-length
@@ -580,13 +668,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--pairs", type=int, default=64, help="pairs per step")
     ap.add_argument("--unique", type=int, default=4, help="distinct synthetic pairs")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="CPU baseline sample on all host cores")
+    ap.add_argument("--cpu-single-seconds", type=float, default=8.0,
+                    help="CPU baseline sample on one core")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--latency-requests", type=int, default=50)
     ap.add_argument("--serve-seconds", type=float, default=6.0,
                     help="Poisson phase of the serving measurement (0 skips serving)")
     ap.add_argument("--serve-rate", type=float, default=500.0)
     ap.add_argument("--no-video", action="store_true", help="skip the batched-video measurement")
+    ap.add_argument("--no-deform", action="store_true",
+                    help="skip the window-deformation measurement")
     ap.add_argument("--no-post", action="store_true",
                     help="skip the flow-statistics and smoothing measurement")
     ap.add_argument("--no-cli", action="store_true",
@@ -706,6 +799,10 @@ def main():
     avg_ms = dom["total_ms"] / dom["launches"]
     bytes_per_launch = dom["bytes"] / dom["launches"]
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    # the correlation is FP32 VALU work: three real 2-D FFTs of 5 n^2 log2 n flop per window
+    n_dom = dom["window"]
+    flop_per_launch = dom["units"] / dom["launches"] * 15.0 * n_dom * n_dom * math.log2(n_dom)
+    tflops = flop_per_launch / (avg_ms * 1e-3) / 1e12
     kernels = {f'{s["name"]}{s["window"] or ""}': round(s["total_ms"] / s["launches"], 4)
                for s in stats}
     traffic = None
@@ -740,6 +837,10 @@ def main():
     if not args.no_precision:
         precision = precision_bench(an, fa, fb, dx, dy, pr, fl, g, field, uniq, args.steps)
 
+    deformation = None
+    if not args.no_deform:
+        deformation = deform_bench(A, B, field, uniq, args.steps)
+
     video = None
     if not args.no_video:
         video = video_bench()
@@ -758,7 +859,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(uniq[0], args.cpu_seconds)
+        cpu = cpu_baseline(uniq[0], args.cpu_seconds, args.cpu_single_seconds)
 
     windows_per_field = 0
     for n in WINDOWS:
@@ -789,11 +890,29 @@ def main():
         "rms_vectors": nvec,
         "kernel_avg_ms": kernels,
         "autotune": tune,
-        "roofline": {"bound": "hbm", "kernel": f"k_correlate<{dom['window']}>",
-                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "bytes_per_launch": bytes_per_launch, "avg_launch_ms": round(avg_ms, 5)},
+        "roofline": {"bound": "valu", "kernel": f"k_correlate<{dom['window']}>",
+                     "achieved": round(tflops, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(tflops / FP32_PEAK_TFLOPS, 4),
+                     "frac_of_nonpacked_fp32": round(tflops / (FP32_PEAK_TFLOPS / 2), 4),
+                     "flop_per_launch": flop_per_launch,
+                     "flop_model": "15 n^2 log2 n per window (three real 2-D FFTs of 5 n^2 log2 n)",
+                     "avg_launch_ms": round(avg_ms, 5),
+                     "traffic": traffic,
+                     "traffic_source": "profiles/traffic.json: rocprofv3 FETCH_SIZE + WRITE_SIZE per "
+                                       "launch of this kernel variant, scaled by a 256 MiB copy "
+                                       "(measured in a profiler run, not in this run)",
+                     "note": "the measured HBM traffic is a small fraction of the window-gather "
+                             "bytes (overlapping windows are served from L2), so the kernel is "
+                             "VALU-issue bound; this pool forbids packed FP32, so half the "
+                             "157.3 TFLOP/s peak is the attainable FP32 rate",
+                     "hbm_gather_model": {"achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                                          "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                                          "bytes_per_launch": bytes_per_launch,
+                                          "per_step_frac": round(P * gather_bytes(H, W, 1, WINDOWS) /
+                                                                 (ms_per_step * 1e-3) / 1e9 /
+                                                                 HBM_PEAK_GBS, 4)}},
         "reduced_precision": precision,
+        "deformation": deformation,
         "video": video,
         "post": post,
         "cli": cli,

@@ -228,42 +228,66 @@ __device__ __forceinline__ float deformed_sample(const void *__restrict__ img, i
 // -h in A and +h in B, h = (D(P) - D(C)) / 2, with D the predictor's deformation interpolant
 // (deform_sample), P = (mx + c, my + r) the pixel's position midway between its A and B
 // samples and C the window centre.  A uniform predictor gives h = 0 exactly and so the rigid
-// samples.  Each window's column is sampled by a plain loop into the window's LDS plane (row
-// stride N + 1) and read back with constant indices: the per-pixel work stays one loop body
-// rather than N unrolled copies, so the column registers never spill.  Every lane of the
-// workgroup calls this (it holds barriers); windows without signal get zeros.
+// samples.  D's x part is the lane's (one column), its y part is the row's, and the taps of
+// a grid row pair are loaded once while the rows stay in it.  Both windows' columns are
+// sampled by a plain loop into two LDS planes (row stride N + 1) and read back with constant
+// indices, so the per-pixel work stays one loop body and the column registers never spill.
+// Every lane of the workgroup calls this (it holds barriers); windows without signal get
+// zeros.  With pb == nullptr (a kernel without room for a second plane) window A and then
+// window B go through pa, and the interpolant is evaluated for each.
 template <int N, int FMT, int MODE>
 __device__ __forceinline__ void load_columns_deformed(const PassPair &pp, bool sig, float ax,
                                                       float ay, float bx, float by, int c,
-                                                      float *pl, float (&za)[N], float (&zb)[N])
+                                                      float *pa, float *pb, float (&za)[N],
+                                                      float (&zb)[N])
 {
     constexpr int P = N + 1;
+    const GridDev &g = pp.prev;
     const float mx = (ax + bx) * 0.5f, my = (ay + by) * 0.5f;
-    const float half = ((float)N - 1.0f) / 2.0f;
+    const int xa = (int)floorf(ax), ya = (int)floorf(ay), xb = (int)floorf(bx), yb = (int)floorf(by);
+    const float txa = ax - (float)xa, tya = ay - (float)ya, txb = bx - (float)xb, tyb = by - (float)yb;
     float uc = 0.0f, vc = 0.0f;
-    if (sig) deform_sample(pp.prev, pp.prev_dx, pp.prev_dy, mx + half, my + half, uc, vc);
-    const float X = mx + (float)c;
+    if (sig) deform_sample(g, pp.prev_dx, pp.prev_dy, mx + ((float)N - 1.0f) / 2.0f,
+                           my + ((float)N - 1.0f) / 2.0f, uc, vc);
+    const DeformAxis dxa = deform_axis(g.x0, g.step, g.window, g.nx, mx + (float)c);
+    // which: 0 both windows (pb), 1 window A only, 2 window B only (into pa)
+    auto sample_rows = [&](int which) {
+        DeformRow R;
+        for (int r = 0; r < N; ++r) {
+            const DeformAxis dya = deform_axis(g.y0, g.step, g.window, g.ny, my + (float)r);
+            if (dya.i0 != R.iy) deform_row(g, pp.prev_dx, pp.prev_dy, dxa, dya, R);
+            float u, v;
+            deform_eval(dxa, dya, R, u, v);
+            const float hx = (u - uc) * 0.5f, hy = (v - vc) * 0.5f;
+            if (which != 2)
+                pa[r * P + c] = deformed_sample<FMT, MODE>(pp.img_a, pp.pitch, pp.H, pp.W, xa + c, ya + r,
+                                                           txa - hx, tya - hy);
+            const float vb = which != 1 ? deformed_sample<FMT, MODE>(pp.img_b, pp.pitch, pp.H, pp.W, xb + c,
+                                                                     yb + r, txb + hx, tyb + hy)
+                                        : 0.0f;
+            if (which == 0) pb[r * P + c] = vb;
+            if (which == 2) pa[r * P + c] = vb;
+        }
+    };
+    if (pb) {
+        if (sig) sample_rows(0);
+        __syncthreads();
 #pragma unroll
-    for (int w = 0; w < 2; ++w) {
-        // window A (w = 0) moves by -h, window B by +h
-        const float o_x = w ? bx : ax, o_y = w ? by : ay;
-        const int xi = (int)floorf(o_x), yi = (int)floorf(o_y);
-        const float tx = o_x - (float)xi, ty = o_y - (float)yi;
-        const void *img = w ? pp.img_b : pp.img_a;
-        if (sig) {
-#pragma unroll 2
-            for (int r = 0; r < N; ++r) {
-                float u, v;
-                deform_sample(pp.prev, pp.prev_dx, pp.prev_dy, X, my + (float)r, u, v);
-                const float hx = (u - uc) * 0.5f, hy = (v - vc) * 0.5f;
-                const float fx = w ? tx + hx : tx - hx, fy = w ? ty + hy : ty - hy;
-                pl[r * P + c] = deformed_sample<FMT, MODE>(img, pp.pitch, pp.H, pp.W, xi + c, yi + r, fx, fy);
-            }
+        for (int r = 0; r < N; ++r) {
+            za[r] = sig ? pa[r * P + c] : 0.0f;
+            zb[r] = sig ? pb[r * P + c] : 0.0f;
         }
         __syncthreads();
-        float(&z)[N] = w ? zb : za;
+    } else {
+        if (sig) sample_rows(1);
+        __syncthreads();
 #pragma unroll
-        for (int r = 0; r < N; ++r) z[r] = sig ? pl[r * P + c] : 0.0f;
+        for (int r = 0; r < N; ++r) za[r] = sig ? pa[r * P + c] : 0.0f;
+        __syncthreads();
+        if (sig) sample_rows(2);
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < N; ++r) zb[r] = sig ? pa[r * P + c] : 0.0f;
         __syncthreads();
     }
 }
@@ -282,8 +306,8 @@ __device__ __forceinline__ double energy_floor()
 // cleared exactly later, which is what the reference's mean removal amounts to).
 template <int N, int FMT, int DEF>
 __device__ __forceinline__ void ingest(const PassPair &pp, bool sig, float ax, float ay, float bx,
-                                       float by, int lane, float *red, float *pl, float (&zr)[N],
-                                       float (&zi)[N])
+                                       float by, int lane, float *red, float *pl, float *pl2,
+                                       float (&zr)[N], float (&zi)[N])
 {
     // every window of the wave keeps its (N+1)^2 footprint inside the frame: no clamping
     const bool fast = __all(!sig || (interior<N>(ax, ay, pp.W, pp.H) && interior<N>(bx, by, pp.W, pp.H)));
@@ -292,7 +316,7 @@ __device__ __forceinline__ void ingest(const PassPair &pp, bool sig, float ax, f
     // DEF: the launch's deformation mode (a pass with a predictor); it is a template
     // parameter so that the rigid kernels carry none of the deformed loader's registers
     if (DEF != 0 && pp.has_prev) {
-        load_columns_deformed<N, FMT, DEF>(pp, sig, ax, ay, bx, by, lane, pl, zr, zi);
+        load_columns_deformed<N, FMT, DEF>(pp, sig, ax, ay, bx, by, lane, pl, pl2, zr, zi);
     } else if (sig) {
         if (fast && ints) {
             load_column_int<N, FMT>(pp.img_a, pp.pitch, ax, ay, lane, zr);
@@ -566,7 +590,8 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
     using C = KCfg<N>;
     constexpr int WPB = C::WPB, P = C::P, PL = C::PL, H2 = N / 2;
     __shared__ float s_re[WPB * PL];
-    __shared__ float s_im[C::ONE_WAVE ? 1 : WPB * PL];
+    // a second plane: windows of more than one wave, and the deformed loader (window B)
+    __shared__ float s_im[(C::ONE_WAVE && DEF == 0) ? 1 : WPB * PL];
     __shared__ float s_red[WPB * N];
     __shared__ float s_pkv[WPB * N];
     __shared__ int s_pki[WPB * N];
@@ -598,7 +623,7 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
     const uint32_t k = blk * WPB + wl;
     const bool active = k < cells;
     float *pr = s_re + wl * PL;
-    float *pim = C::ONE_WAVE ? s_im : s_im + wl * PL;
+    float *pim = (C::ONE_WAVE && DEF == 0) ? s_im : s_im + wl * PL;
     float *red = s_red + wl * N;
     float *pkv = s_pkv + wl * N;
     int *pki = s_pki + wl * N;
@@ -617,7 +642,7 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
 
     // ---- fused ingest + mean removal --------------------------------------------------
     float zr[N], zi[N];
-    ingest<N, FMT, DEF>(pp, sig, ax, ay, bx, by, lane, red, pr, zr, zi);
+    ingest<N, FMT, DEF>(pp, sig, ax, ay, bx, by, lane, red, pr, pim, zr, zi);
 
     // ---- forward transform of z = a + i b ----------------------------------------------
     // Transposes go through LDS.  A window inside one wave uses a single float plane and
@@ -853,8 +878,12 @@ hipError_t launch_corr_v(int variant, int deform, const PassPair *pairs, int npa
 {
     constexpr VariantSet vs = corr_variants(N);
     if constexpr (KCfg<N>::ONE_WAVE) {
-        if (deform == 1) return launch_corr_t<N, FMT, vs.occ[0], 1>(pairs, npairs, max_cells, st);
-        if (deform == 2) return launch_corr_t<N, FMT, vs.occ[0], 2>(pairs, npairs, max_cells, st);
+        // the deformed loader's second LDS plane bounds the workgroups per CU (160 KiB)
+        constexpr int lds = 2 * KCfg<N>::WPB * KCfg<N>::PL * 4;
+        constexpr int occ_lds = (163840 / lds + 3) / 4;
+        constexpr int occ = vs.occ[0] < occ_lds ? vs.occ[0] : (occ_lds < 1 ? 1 : occ_lds);
+        if (deform == 1) return launch_corr_t<N, FMT, occ, 1>(pairs, npairs, max_cells, st);
+        if (deform == 2) return launch_corr_t<N, FMT, occ, 2>(pairs, npairs, max_cells, st);
     } else {
         if (deform != 0) return hipErrorInvalidValue;
     }

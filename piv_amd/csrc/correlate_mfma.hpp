@@ -155,6 +155,9 @@ __global__ __launch_bounds__(MfCfg<N>::BLOCK) void k_correlate_mf(const PassPair
     using V4 = typename Mfma<E>::v4;
     constexpr int T = M::T, WPB = M::WPB, P = M::P, PL = M::PL;
     __shared__ float s_pl[WPB * PL];      // per window: 16-bit input planes, then the FP32 plane
+    // window B's samples in the deformed loader (N = 64 has no room left: one plane, twice)
+    constexpr bool PL2 = DEF != 0 && N < 64;
+    __shared__ float s_pl2[PL2 ? WPB * PL : 1];
     __shared__ float s_red[WPB * N];
     __shared__ float s_pkv[WPB * N];
     __shared__ int s_pki[WPB * N];
@@ -200,7 +203,7 @@ __global__ __launch_bounds__(MfCfg<N>::BLOCK) void k_correlate_mf(const PassPair
         sig = pp.sig[k] != 0;
     }
     float zr[N], zi[N];
-    ingest<N, FMT, DEF>(pp, sig, ax, ay, bx, by, lane, red, pr, zr, zi);
+    ingest<N, FMT, DEF>(pp, sig, ax, ay, bx, by, lane, red, pr, PL2 ? s_pl2 + wl * PL : nullptr, zr, zi);
     float ea = 0.f, eb = 0.f;
 #pragma unroll
     for (int r = 0; r < N; ++r) {

@@ -110,6 +110,81 @@ __device__ __forceinline__ void deform_sample(const GridDev &g, const float *__r
     v = all ? vl : vw;
 }
 
+// deform_sample split into its separable parts, for loaders that evaluate it on a window's
+// pixel grid: the x part of a column and the y part of a row are computed once, and the
+// predictor taps of one grid row pair are loaded once while successive pixel rows fall in it.
+// Every value equals deform_sample's.
+struct DeformAxis {
+    int i0, i1;  // grid nodes below and above
+    float t;     // fraction between them
+};
+
+__device__ __forceinline__ DeformAxis deform_axis(int o0, int step, int window, int count, float x)
+{
+    const float f = clamp_rs((x - center(o0, 0, step, window)) / (float)step, 0.0f,
+                             (float)(count - 1));
+    DeformAxis a;
+    a.i0 = (int)floorf(f);
+    a.t = f - (float)a.i0;
+    a.i1 = min(a.i0 + 1, count - 1);
+    return a;
+}
+
+// The predictor's taps of one grid row pair (iy, iy1) at a column's nodes (ix, ix1), and
+// their lerps along x; `fin` when all eight are finite (the lerp form applies).
+struct DeformRow {
+    int iy = -1;
+    float u00, u01, u10, u11, v00, v01, v10, v11;
+    float ut, ub, vt, vb;
+    bool fin;
+};
+
+__device__ __forceinline__ void deform_row(const GridDev &g, const float *__restrict__ dx,
+                                           const float *__restrict__ dy, const DeformAxis &ax,
+                                           const DeformAxis &ay, DeformRow &R)
+{
+    R.iy = ay.i0;
+    const int r0 = ay.i0 * g.nx, r1 = ay.i1 * g.nx;
+    R.u00 = dx[r0 + ax.i0];
+    R.u01 = dx[r0 + ax.i1];
+    R.u10 = dx[r1 + ax.i0];
+    R.u11 = dx[r1 + ax.i1];
+    R.v00 = dy[r0 + ax.i0];
+    R.v01 = dy[r0 + ax.i1];
+    R.v10 = dy[r1 + ax.i0];
+    R.v11 = dy[r1 + ax.i1];
+    R.fin = isfinite(R.u00) && isfinite(R.u01) && isfinite(R.u10) && isfinite(R.u11) &&
+            isfinite(R.v00) && isfinite(R.v01) && isfinite(R.v10) && isfinite(R.v11);
+    R.ut = R.u00 + ax.t * (R.u01 - R.u00);
+    R.ub = R.u10 + ax.t * (R.u11 - R.u10);
+    R.vt = R.v00 + ax.t * (R.v01 - R.v00);
+    R.vb = R.v10 + ax.t * (R.v11 - R.v10);
+}
+
+// deform_sample at the point whose axis parts are ax, ay, from the row pair's taps R
+__device__ __forceinline__ void deform_eval(const DeformAxis &ax, const DeformAxis &ay,
+                                            const DeformRow &R, float &u, float &v)
+{
+    if (R.fin) {
+        u = R.ut + ay.t * (R.ub - R.ut);
+        v = R.vt + ay.t * (R.vb - R.vt);
+        return;
+    }
+    const float tx = ax.t, ty = ay.t;
+    const float w[4] = {(1.0f - tx) * (1.0f - ty), tx * (1.0f - ty), (1.0f - tx) * ty, tx * ty};
+    const float su[4] = {R.u00, R.u01, R.u10, R.u11}, sv[4] = {R.v00, R.v01, R.v10, R.v11};
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (w[k] > 0.0f && isfinite(su[k]) && isfinite(sv[k])) {
+            a0 = a0 + w[k] * su[k];
+            a1 = a1 + w[k] * sv[k];
+            a2 = a2 + w[k];
+        }
+    u = a2 > 0.0f ? a0 / a2 : 0.0f;
+    v = a2 > 0.0f ? a1 / a2 : 0.0f;
+}
+
 // 4-point Lagrange weights at nodes -1, 0, 1, 2 for a fraction t in [0, 1): the cubic
 // resampling kernel of the deformation (reproduces polynomials of degree <= 3 exactly).
 __device__ __forceinline__ void lagrange4(float t, float (&l)[4])
