@@ -130,7 +130,7 @@ def gather_bytes(h, w, s_in, windows):
     return b
 
 
-def serve_bench(rate, seconds, backlog, seed=77):
+def serve_bench(rate, seconds, backlog, seed=77, inflight=2, batch_pairs=64):
     """Continuous-batching server on the mixed workload: p50/p99 latency (arrival to result
     on the host) under Poisson arrivals at `rate` pairs/s, then the saturated rate with a
     backlog of `backlog` requests (host staging and PCIe upload included)."""
@@ -145,7 +145,7 @@ def serve_bench(rate, seconds, backlog, seed=77):
     sys.setswitchinterval(2e-4)
     gc.collect()
     gc.disable()
-    srv = piv_amd.PivServer(hbm_budget=16 << 30, max_batch_pairs=64, max_inflight=2)
+    srv = piv_amd.PivServer(hbm_budget=16 << 30, max_batch_pairs=batch_pairs, max_inflight=inflight)
     streams = []
     for k, sp in enumerate(SERVE_STREAMS):
         dt = np.uint8 if sp["dtype"] == "u8" else np.uint16
@@ -677,6 +677,9 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=6.0,
                     help="Poisson phase of the serving measurement (0 skips serving)")
     ap.add_argument("--serve-rate", type=float, default=500.0)
+    ap.add_argument("--serve-inflight", type=int, default=2,
+                    help="serving: batches on the device at once (compute lanes)")
+    ap.add_argument("--serve-batch", type=int, default=64, help="serving: pairs per batch")
     ap.add_argument("--no-video", action="store_true", help="skip the batched-video measurement")
     ap.add_argument("--no-deform", action="store_true",
                     help="skip the window-deformation measurement")
@@ -701,6 +704,9 @@ def main():
                     help="keep the default correlation-kernel variants")
     ap.add_argument("--only-batched", action="store_true",
                     help="time the batched steps only (for profiler runs)")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16"],
+                    help="correlation precision of the batched steps (profiler runs with "
+                         "--only-batched; the default run measures every precision)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (a kernel of "
                          "known traffic for scaling profiler byte counters)")
@@ -722,7 +728,8 @@ def main():
         return
 
     if args.only_serve:
-        print(json.dumps(serve_bench(args.serve_rate, args.serve_seconds, args.serve_backlog)),
+        print(json.dumps(serve_bench(args.serve_rate, args.serve_seconds, args.serve_backlog,
+                                     inflight=args.serve_inflight, batch_pairs=args.serve_batch)),
               flush=True)
         return
 
@@ -772,6 +779,9 @@ def main():
                 for t in an.autotune(fa, fb, repeats=3)}
         an.analyze_batch_device(fa, fb, dx, dy, pr, fl, summaries=False, sync=True)
 
+    if args.only_batched and args.precision != "fp32":
+        an.set_precision(args.precision)
+
     def timed_steps():
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -794,7 +804,7 @@ def main():
     stats = an.profile_read()
     an.profile(False)
 
-    corr = [s for s in stats if s["name"] == "correlate"]
+    corr = [s for s in stats if s["name"].startswith("correlate")]
     dom = max(corr, key=lambda s: s["total_ms"])
     avg_ms = dom["total_ms"] / dom["launches"]
     bytes_per_launch = dom["bytes"] / dom["launches"]
@@ -855,7 +865,8 @@ def main():
 
     serving = None
     if args.serve_seconds > 0:
-        serving = serve_bench(args.serve_rate, args.serve_seconds, args.serve_backlog)
+        serving = serve_bench(args.serve_rate, args.serve_seconds, args.serve_backlog,
+                              inflight=args.serve_inflight, batch_pairs=args.serve_batch)
 
     cpu = None
     if not args.no_cpu:

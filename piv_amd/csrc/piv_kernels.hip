@@ -163,10 +163,26 @@ constexpr int VBLOCK = 256;
 // the grid rows above and below them (256 + 2 nx + 2), for grids up to 895 vectors wide.
 constexpr int VSTAGE = 2048;
 
-__global__ __launch_bounds__(VBLOCK) void k_val_median(const ValidatePair *__restrict__ jobs,
-                                                       uint32_t bpp)
+// Pair and block-in-pair of block b: bpp blocks per pair, or, with a block map (serving
+// batches of frames of different sizes), pair p owns blocks begins[p] .. begins[p + 1] - 1.
+__device__ __forceinline__ void block_pair(uint32_t b, uint32_t bpp, const uint32_t *begins,
+                                           int npairs, uint32_t &p, uint32_t &blk)
 {
-    const uint32_t p = blockIdx.x / bpp, blk = blockIdx.x - p * bpp;
+    if (begins) {
+        p = find_pair_u(begins, npairs, b);
+        blk = b - begins[p];
+    } else {
+        p = b / bpp;
+        blk = b - p * bpp;
+    }
+}
+
+__global__ __launch_bounds__(VBLOCK) void k_val_median(const ValidatePair *__restrict__ jobs,
+                                                       uint32_t bpp, const uint32_t *begins,
+                                                       int njobs)
+{
+    uint32_t p, blk;
+    block_pair(blockIdx.x, bpp, begins, njobs, p, blk);
     const ValidatePair &J = jobs[p];
     const int nx = J.nx, ny = J.ny;
     const int c = (int)(blk * VBLOCK + threadIdx.x);
@@ -245,9 +261,11 @@ __global__ __launch_bounds__(VBLOCK) void k_val_median(const ValidatePair *__res
 }
 
 __global__ __launch_bounds__(VBLOCK) void k_val_replace(const ValidatePair *__restrict__ jobs,
-                                                        uint32_t bpp)
+                                                        uint32_t bpp, const uint32_t *begins,
+                                                        int njobs)
 {
-    const uint32_t p = blockIdx.x / bpp, blk = blockIdx.x - p * bpp;
+    uint32_t p, blk;
+    block_pair(blockIdx.x, bpp, begins, njobs, p, blk);
     const ValidatePair &J = jobs[p];
     const int nx = J.nx, ny = J.ny;
     const int c = (int)(blk * VBLOCK + threadIdx.x);
@@ -656,12 +674,14 @@ __global__ void k_place(GridDev g, GridDev prev, int has_prev, const float *__re
 constexpr int WBLOCK = 256;
 
 __global__ __launch_bounds__(WBLOCK) void k_place_pass(const PassPair *__restrict__ pairs,
-                                                       uint32_t bpp)
+                                                       uint32_t bpp, const uint32_t *begins,
+                                                       int npairs)
 {
     // window rows k and k + 2 read the same table rows (at 50 % overlap the bottom corners of
     // one row of windows are the top corners of the row two below): keep them in one L2
     const uint32_t b = xcd_block();
-    const uint32_t p = b / bpp, blk = b - p * bpp;
+    uint32_t p, blk;
+    block_pair(b, bpp, begins, npairs, p, blk);
     const PassPair &pp = pairs[p];
     const GridDev g = pp.grid;
     const uint32_t k = blk * WBLOCK + threadIdx.x;
@@ -727,9 +747,11 @@ __device__ __forceinline__ double subpixel64(float m32, float c32, float q32)
 // FP64 (only the final value is rounded to f32).
 template <bool F64>
 __global__ __launch_bounds__(WBLOCK) void k_peak_finish(const PassPair *__restrict__ pairs,
-                                                        uint32_t bpp)
+                                                        uint32_t bpp, const uint32_t *begins,
+                                                        int npairs)
 {
-    const uint32_t p = blockIdx.x / bpp, blk = blockIdx.x - p * bpp;
+    uint32_t p, blk;
+    block_pair(blockIdx.x, bpp, begins, npairs, p, blk);
     const PassPair &pp = pairs[p];
     const GridDev g = pp.grid;
     const uint32_t k = blk * WBLOCK + threadIdx.x;
@@ -806,35 +828,38 @@ hipError_t launch_correlate(int n, int fmt, int variant, const PassPair *pairs, 
 }
 
 hipError_t launch_validate(const ValidatePair *jobs, int njobs, uint32_t max_cells,
-                           hipStream_t st)
+                           hipStream_t st, const uint32_t *begins, uint32_t total_blocks)
 {
     const uint32_t bpp = (max_cells + VBLOCK - 1) / VBLOCK;
-    const uint32_t blocks = bpp * (uint32_t)njobs;
+    const uint32_t blocks = begins ? total_blocks : bpp * (uint32_t)njobs;
     if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_val_median, dim3(blocks), dim3(VBLOCK), 0, st, jobs, bpp);
+    hipLaunchKernelGGL(k_val_median, dim3(blocks), dim3(VBLOCK), 0, st, jobs, bpp, begins, njobs);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_val_replace, dim3(blocks), dim3(VBLOCK), 0, st, jobs, bpp);
+    hipLaunchKernelGGL(k_val_replace, dim3(blocks), dim3(VBLOCK), 0, st, jobs, bpp, begins, njobs);
     return hipGetLastError();
 }
 
-hipError_t launch_place_pass(const PassPair *pairs, int npairs, uint32_t max_cells, hipStream_t st)
+hipError_t launch_place_pass(const PassPair *pairs, int npairs, uint32_t max_cells, hipStream_t st,
+                             const uint32_t *begins, uint32_t total_blocks)
 {
     const uint32_t bpp = (max_cells + WBLOCK - 1) / WBLOCK;
-    const uint32_t blocks = bpp * (uint32_t)npairs;
+    const uint32_t blocks = begins ? total_blocks : bpp * (uint32_t)npairs;
     if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_place_pass, dim3(blocks), dim3(WBLOCK), 0, st, pairs, bpp);
+    hipLaunchKernelGGL(k_place_pass, dim3(blocks), dim3(WBLOCK), 0, st, pairs, bpp, begins, npairs);
     return hipGetLastError();
 }
 
 hipError_t launch_peak_finish(const PassPair *pairs, int npairs, uint32_t max_cells, hipStream_t st,
-                              bool f64)
+                              bool f64, const uint32_t *begins, uint32_t total_blocks)
 {
     const uint32_t bpp = (max_cells + WBLOCK - 1) / WBLOCK;
-    const uint32_t blocks = bpp * (uint32_t)npairs;
+    const uint32_t blocks = begins ? total_blocks : bpp * (uint32_t)npairs;
     if (blocks == 0) return hipSuccess;
-    if (f64) hipLaunchKernelGGL(k_peak_finish<true>, dim3(blocks), dim3(WBLOCK), 0, st, pairs, bpp);
-    else hipLaunchKernelGGL(k_peak_finish<false>, dim3(blocks), dim3(WBLOCK), 0, st, pairs, bpp);
+    if (f64)
+        hipLaunchKernelGGL(k_peak_finish<true>, dim3(blocks), dim3(WBLOCK), 0, st, pairs, bpp, begins, npairs);
+    else
+        hipLaunchKernelGGL(k_peak_finish<false>, dim3(blocks), dim3(WBLOCK), 0, st, pairs, bpp, begins, npairs);
     return hipGetLastError();
 }
 

@@ -18,10 +18,16 @@ bool correlate_generic(int n, int deform);
 // predictor deform)
 hipError_t launch_correlate(int n, int fmt, int variant, const PassPair *pairs, int npairs,
                             uint32_t max_cells, hipStream_t st, int deform = 0);
+// Block maps: the per-window and per-vector kernels run ceil(max_cells / 256) blocks for every
+// pair, or, with `begins` (device uint32 [npairs + 1]: pair p owns blocks begins[p] ..
+// begins[p + 1] - 1, 256 windows or vectors each), exactly the blocks each pair needs:
+// total_blocks = begins[npairs].
 // placement + mask (one thread per window), and sub-pixel fit + assembly of the raw field
-hipError_t launch_place_pass(const PassPair *pairs, int npairs, uint32_t max_cells, hipStream_t st);
+hipError_t launch_place_pass(const PassPair *pairs, int npairs, uint32_t max_cells, hipStream_t st,
+                             const uint32_t *begins = nullptr, uint32_t total_blocks = 0);
 hipError_t launch_peak_finish(const PassPair *pairs, int npairs, uint32_t max_cells, hipStream_t st,
-                              bool f64 = false);
+                              bool f64 = false, const uint32_t *begins = nullptr,
+                              uint32_t total_blocks = 0);
 // DFT-GEMM correlation on the matrix cores (precision 1 = fp16, 2 = bf16), n in {16, 32, 64}
 bool correlate_mf_supported(int n);
 hipError_t launch_correlate_mf(int n, int fmt, int precision, const PassPair *pairs, int npairs,
@@ -31,7 +37,8 @@ hipError_t launch_pass(int n, int fmt, int variant, const PassPair *pairs, int n
                        uint32_t max_cells, hipStream_t st);
 // both validation launches of one pass; summaries must be zeroed beforehand
 hipError_t launch_validate(const ValidatePair *jobs, int njobs, uint32_t max_cells,
-                           hipStream_t st);
+                           hipStream_t st, const uint32_t *begins = nullptr,
+                           uint32_t total_blocks = 0);
 hipError_t launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t st);
 // nframes > 1: a batch of frames (raw frames raw_frame_bytes apart, base images H*W apart,
 // tables sat_stride doubles apart), one launch per kernel

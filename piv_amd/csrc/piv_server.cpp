@@ -331,11 +331,18 @@ struct BatchPlan {
         int count = 0;
         uint32_t max_cells = 0;
         size_t pp_off = 0, vp_off = 0;
+        // block map of the per-window / per-vector kernels (uint32 [count + 1]): requests of
+        // different frame sizes get their own block counts instead of the largest one's
+        size_t bm_off = 0;
+        uint32_t blocks = 0;
     };
     std::vector<Step> steps;
     std::vector<char> host;  // descriptor bytes
     std::vector<int64_t> key;
 };
+
+// windows (vectors) per block of the per-window and per-vector kernels
+constexpr size_t kStepBlock = 256;
 
 void build_batch(Lane &L, BatchPlan &bp)
 {
@@ -390,6 +397,8 @@ void build_batch(Lane &L, BatchPlan &bp)
         doff = align_up(doff, 64);
         S.vp_off = doff;
         doff = align_up(doff + S.count * sizeof(ValidatePair), 64);
+        S.bm_off = doff;
+        doff = align_up(doff + (S.count + 1) * sizeof(uint32_t), 64);
     }
     bp.host.assign(doff, 0);
     // contents
@@ -398,6 +407,8 @@ void build_batch(Lane &L, BatchPlan &bp)
     for (size_t si = 0; si < maxsteps; ++si) {
         BatchPlan::Step &S = bp.steps[si];
         int vi = 0;
+        uint32_t *bm = reinterpret_cast<uint32_t *>(bp.host.data() + S.bm_off);
+        bm[0] = 0;
         for (BatchPlan::Group &g : S.groups) {
             int gi = 0;
             for (int k = 0; k < B; ++k) {
@@ -463,10 +474,14 @@ void build_batch(Lane &L, BatchPlan &bp)
                 v.median_eps = cf.median_eps;
                 v.max_displacement = (float)cf.windows[0] / 2.0f;
                 v.replace = cf.replace ? 1 : 0;
-                reinterpret_cast<ValidatePair *>(bp.host.data() + S.vp_off)[vi++] = v;
+                reinterpret_cast<ValidatePair *>(bp.host.data() + S.vp_off)[vi] = v;
+                bm[vi + 1] = bm[vi] + (uint32_t)((c + kStepBlock - 1) / kStepBlock);
+                ++vi;
             }
         }
     }
+    for (BatchPlan::Step &S : bp.steps)
+        S.blocks = reinterpret_cast<const uint32_t *>(bp.host.data() + S.bm_off)[S.count];
     // launch structure (the graph cache key): descriptor offsets follow from it
     bp.key.clear();
     bp.key.push_back((int64_t)maxsteps);
@@ -474,6 +489,7 @@ void build_batch(Lane &L, BatchPlan &bp)
     for (const BatchPlan::Step &S : bp.steps) {
         bp.key.push_back(S.count);
         bp.key.push_back(S.max_cells);
+        bp.key.push_back(S.blocks);
         for (const BatchPlan::Group &g : S.groups) {
             bp.key.push_back(g.n);
             bp.key.push_back(g.fmt);
@@ -493,16 +509,17 @@ hipError_t enqueue_batch(Lane &L, const BatchPlan &bp)
     for (const BatchPlan::Step &S : bp.steps) {
         if (e != hipSuccess) break;
         const auto *pp = reinterpret_cast<const PassPair *>(dd + S.pp_off);
-        e = launch_place_pass(pp, S.count, S.max_cells, st);
+        const auto *bm = reinterpret_cast<const uint32_t *>(dd + S.bm_off);
+        e = launch_place_pass(pp, S.count, S.max_cells, st, bm, S.blocks);
         for (const BatchPlan::Group &g : S.groups) {
             if (e != hipSuccess) break;
             e = launch_correlate(g.n, g.fmt, g.variant, reinterpret_cast<const PassPair *>(dd + g.pp_off), g.count,
                                  g.max_cells, st, g.deform);
         }
-        if (e == hipSuccess) e = launch_peak_finish(pp, S.count, S.max_cells, st);
+        if (e == hipSuccess) e = launch_peak_finish(pp, S.count, S.max_cells, st, false, bm, S.blocks);
         if (e == hipSuccess)
             e = launch_validate(reinterpret_cast<const ValidatePair *>(dd + S.vp_off), S.count,
-                                S.max_cells, st);
+                                S.max_cells, st, bm, S.blocks);
     }
     return e;
 }
