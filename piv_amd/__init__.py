@@ -802,25 +802,33 @@ class PivServer:
         _check(self._lib.piv_poll(self._h, arr, max_results, C.byref(cnt), int(wait_ms)))
         out = []
         release = self._lib.piv_result_release
-        for i in range(cnt.value):
-            r = arr[i]
-            field, summ, err = None, None, None
-            if r.status == L.OK:
-                summ = ValidationSummary._from_c(r.summary)
-                if copy:
-                    g = Grid._from_c(r.grid)
-                    n = g.nx * g.ny
-                    shp = (g.ny, g.nx)
-                    field = VectorField(
-                        g, np.ctypeslib.as_array(r.dx, (n,)).reshape(shp).copy(),
-                        np.ctypeslib.as_array(r.dy, (n,)).reshape(shp).copy(),
-                        np.ctypeslib.as_array(r.peak_ratio, (n,)).reshape(shp).copy(),
-                        np.ctypeslib.as_array(r.flags, (n,)).reshape(shp).copy())
-            else:
-                err = f"status {r.status}"
-            out.append(ServedResult(int(r.request_id), int(r.stream_id), field, summ,
-                                    float(r.submit_ms), float(r.done_ms), int(r.batch_pairs), err))
-            _check(release(self._h, r.request_id))
+        ids = [int(arr[i].request_id) for i in range(cnt.value)]
+        try:
+            for i in range(cnt.value):
+                r = arr[i]
+                field, summ, err = None, None, None
+                if r.status == L.OK:
+                    summ = ValidationSummary._from_c(r.summary)
+                    if copy:
+                        g = Grid._from_c(r.grid)
+                        n = g.nx * g.ny
+                        shp = (g.ny, g.nx)
+                        field = VectorField(
+                            g, np.ctypeslib.as_array(r.dx, (n,)).reshape(shp).copy(),
+                            np.ctypeslib.as_array(r.dy, (n,)).reshape(shp).copy(),
+                            np.ctypeslib.as_array(r.peak_ratio, (n,)).reshape(shp).copy(),
+                            np.ctypeslib.as_array(r.flags, (n,)).reshape(shp).copy())
+                else:
+                    err = f"status {r.status}"
+                out.append(ServedResult(int(r.request_id), int(r.stream_id), field, summ,
+                                        float(r.submit_ms), float(r.done_ms),
+                                        int(r.batch_pairs), err))
+        finally:
+            # every polled result goes back to the server, also when a conversion failed, so
+            # none keeps its request slot
+            rcs = [release(self._h, rid) for rid in ids]
+        for rc in rcs:
+            _check(rc)
         return out
 
     def pause(self, paused=True):

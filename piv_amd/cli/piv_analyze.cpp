@@ -23,6 +23,7 @@
 #include <cstring>
 #include <filesystem>
 #include <deque>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -34,11 +35,29 @@
 
 namespace {
 
+// Work that must stop before the process exits (background upload threads, writer threads),
+// run by die() in reverse order of registration.  die() is called from the main thread only.
+std::vector<std::function<void()>> g_cleanup;
+
 [[noreturn]] void die(const std::string &msg)
 {
     std::fprintf(stderr, "error: %s\n", msg.c_str());
+    static bool dying = false;
+    if (!dying) {  // a hook that fails does not run the hooks again
+        dying = true;
+        for (auto it = g_cleanup.rbegin(); it != g_cleanup.rend(); ++it) (*it)();
+    }
     std::exit(1);
 }
+
+// Runs `f` when the scope ends, also when an exception unwinds it.
+struct OnExit {
+    std::function<void()> f;
+    ~OnExit()
+    {
+        if (f) f();
+    }
+};
 
 void check(piv_status s, const char *what)
 {
@@ -240,7 +259,22 @@ std::string json_str(const std::string &s)
 
 }  // namespace
 
+int run(int argc, char **argv);
+
 int main(int argc, char **argv)
+{
+    // an exception unwinds run() (its upload and writer threads are stopped on the way) and
+    // ends the process with the tool's own error status
+    try {
+        return run(argc, argv);
+    } catch (const std::exception &e) {
+        die(std::string("unexpected error: ") + e.what());
+    } catch (...) {
+        die("unexpected error");
+    }
+}
+
+int run(int argc, char **argv)
 {
     const Args A = parse(argc, argv);
     const auto t0 = std::chrono::steady_clock::now();
@@ -371,10 +405,23 @@ int main(int argc, char **argv)
         // blocks until the first `b` bytes are in HBM
         void wait_bytes(size_t b)
         {
-            std::unique_lock<std::mutex> lk(mu);
-            cv.wait(lk, [&] { return !err.empty() || prefix == pieces || prefix * kPiece >= b; });
-            if (!err.empty()) die("upload: " + err);
+            std::string e;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return !err.empty() || prefix == pieces || prefix * kPiece >= b; });
+                e = err;
+            }
+            if (!e.empty()) die("upload: " + e);  // lock released: die() joins the workers
         }
+        // workers claim no further piece; the pieces in progress finish, then the threads end
+        void abort()
+        {
+            next.store(pieces);
+            for (auto &t : ts)
+                if (t.joinable()) t.join();
+            ts.clear();
+        }
+        ~ClipUpload() { abort(); }
         void finish()
         {
             wait_bytes(bytes);
@@ -383,6 +430,7 @@ int main(int argc, char **argv)
         }
     };
     ClipUpload up;
+    g_cleanup.push_back([&up] { up.abort(); });
     void *clip = nullptr;
     check(piv_device_alloc(A.device, in.frames * frame_bytes, &clip), "device memory for the clip");
     if (in.channels == 1) {
@@ -518,6 +566,8 @@ int main(int argc, char **argv)
         for (auto &t : writers) t.join();
         writers.clear();
     };
+    OnExit writers_guard{stop_writers};
+    g_cleanup.push_back(stop_writers);
 
     uint64_t valid = 0, vectors = 0;
     // seconds of the loop per phase: waiting for the clip's upload, preparation, analysis, flow

@@ -183,7 +183,7 @@ static piv_status frame_reserve(piv_analyzer *a, DevBuf &b, size_t bytes)
 }
 
 // PreparedFrame::new (preprocess.rs:137-153) on device samples.
-piv_status prepare_frame(piv_analyzer *a, piv_frame *f, const float *bg_dev)
+static piv_status prepare_frame_impl(piv_analyzer *a, piv_frame *f, const float *bg_dev)
 {
     const piv_config &c = a->cfg;
     const int H = f->H, W = f->W;
@@ -249,7 +249,8 @@ piv_status prepare_frame(piv_analyzer *a, piv_frame *f, const float *bg_dev)
 // Frames of one size and format at their own addresses (the requests of a serving batch),
 // prepared together: without conditioning, their tables come from one launch per kernel for
 // up to 64 frames (launch_sat_list); otherwise frame by frame.
-piv_status prepare_frames_list(piv_analyzer *a, piv_frame *const *fs, int count, const float *bg_dev)
+static piv_status prepare_frames_list_impl(piv_analyzer *a, piv_frame *const *fs, int count,
+                                           const float *bg_dev)
 {
     if (count <= 0) return PIV_OK;
     const piv_config &c = a->cfg;
@@ -260,7 +261,7 @@ piv_status prepare_frames_list(piv_analyzer *a, piv_frame *const *fs, int count,
         same = same && fs[i]->H == f0->H && fs[i]->W == f0->W && fs[i]->in_fmt == f0->in_fmt &&
                fs[i]->raw_pitch == f0->raw_pitch;
     if (cond || !same || !mask_needed(c, f0->in_fmt, false)) {
-        for (int i = 0; i < count; ++i) PIV_TRY(prepare_frame(a, fs[i], bg_dev));
+        for (int i = 0; i < count; ++i) PIV_TRY(prepare_frame_impl(a, fs[i], bg_dev));
         return PIV_OK;
     }
     const int H = f0->H, W = f0->W;
@@ -294,13 +295,12 @@ static piv_status batch_storage(piv_analyzer *a, size_t bytes, std::shared_ptr<D
     std::weak_ptr<BlockPool> wp = a->pool;
     const int dev = a->device;
     store.reset(new DevBuf, [wp, dev](DevBuf *b) {
-        // the frames' last user is done before the block is reused (hipFree would
-        // synchronise the device as well)
+        // the last frame of the batch is being released: every frame of it has waited for its
+        // own last use (piv_frame_release), so the block is free for reuse
         if (b->p) {
             int cur = dev;
             (void)hipGetDevice(&cur);
             if (cur != dev) (void)hipSetDevice(dev);
-            (void)hipDeviceSynchronize();
             if (auto pool = wp.lock()) pool->give(*b);
             if (cur != dev) (void)hipSetDevice(cur);
         }
@@ -382,8 +382,8 @@ static piv_status condition_frames_batch(piv_analyzer *a, piv_frame *const *fs, 
 // Preparation of `count` frames of one size and format (frames raw_frame_bytes apart): every
 // kernel is launched once for the whole set (per chunk when conditioning), into storage the
 // frames share.
-piv_status prepare_frames_batch(piv_analyzer *a, piv_frame *const *fs, int count,
-                                const float *bg_dev, long raw_frame_bytes)
+static piv_status prepare_frames_batch_impl(piv_analyzer *a, piv_frame *const *fs, int count,
+                                            const float *bg_dev, long raw_frame_bytes)
 {
     if (count <= 0) return PIV_OK;
     const piv_config &c = a->cfg;
@@ -449,11 +449,11 @@ struct FieldPtrs {
 
 // Runs every pass for npairs pairs.  Returns device pointers of each pass's raw and validated
 // fields (in the workspace unless `out` redirects the final validated field).
-piv_status run_batch(piv_analyzer *a, const piv_frame *const *fa, const piv_frame *const *fb,
-                     uint32_t npairs, const BatchOut &out, bool traced,
-                     std::vector<PassPlan> &plans, std::vector<FieldPtrs> &raw,
-                     std::vector<FieldPtrs> &val, std::vector<float *> &origins,
-                     std::vector<int32_t *> &peaks, uint32_t **summaries_dev)
+static piv_status run_batch_impl(piv_analyzer *a, const piv_frame *const *fa, const piv_frame *const *fb,
+                                 uint32_t npairs, const BatchOut &out, bool traced,
+                                 std::vector<PassPlan> &plans, std::vector<FieldPtrs> &raw,
+                                 std::vector<FieldPtrs> &val, std::vector<float *> &origins,
+                                 std::vector<int32_t *> &peaks, uint32_t **summaries_dev)
 {
     const int H = fa[0]->H, W = fa[0]->W;
     for (uint32_t i = 0; i < npairs; ++i) {
@@ -697,6 +697,48 @@ piv_status run_batch(piv_analyzer *a, const piv_frame *const *fa, const piv_fram
     return PIV_OK;
 }
 
+// The entry points below enqueue a frame operation and then record where on the stream its
+// frames were last used, also after a failure part-way (work may be in flight).
+
+piv_status prepare_frame(piv_analyzer *a, piv_frame *f, const float *bg_dev)
+{
+    const piv_status s = prepare_frame_impl(a, f, bg_dev);
+    const piv_frame *used = f;
+    const piv_status m = mark_use(a, a->stream, &used, 1);
+    return s != PIV_OK ? s : m;
+}
+
+piv_status prepare_frames_list(piv_analyzer *a, piv_frame *const *fs, int count, const float *bg_dev)
+{
+    if (count <= 0) return PIV_OK;
+    const piv_status s = prepare_frames_list_impl(a, fs, count, bg_dev);
+    const piv_status m = mark_use(a, a->stream, fs, (size_t)count);
+    return s != PIV_OK ? s : m;
+}
+
+piv_status prepare_frames_batch(piv_analyzer *a, piv_frame *const *fs, int count,
+                                const float *bg_dev, long raw_frame_bytes)
+{
+    if (count <= 0) return PIV_OK;
+    const piv_status s = prepare_frames_batch_impl(a, fs, count, bg_dev, raw_frame_bytes);
+    const piv_status m = mark_use(a, a->stream, fs, (size_t)count);
+    return s != PIV_OK ? s : m;
+}
+
+piv_status run_batch(piv_analyzer *a, const piv_frame *const *fa, const piv_frame *const *fb,
+                     uint32_t npairs, const BatchOut &out, bool traced,
+                     std::vector<PassPlan> &plans, std::vector<FieldPtrs> &raw,
+                     std::vector<FieldPtrs> &val, std::vector<float *> &origins,
+                     std::vector<int32_t *> &peaks, uint32_t **summaries_dev)
+{
+    if (npairs == 0) return fail(PIV_ERR_INVALID_ARGUMENT, "empty batch");
+    const piv_status s = run_batch_impl(a, fa, fb, npairs, out, traced, plans, raw, val, origins,
+                                        peaks, summaries_dev);
+    piv_status m = mark_use(a, a->stream, fa, npairs);
+    if (m == PIV_OK) m = mark_use(a, a->stream, fb, npairs);
+    return s != PIV_OK ? s : m;
+}
+
 void to_summary(const uint32_t *c, piv_summary *s)
 {
     s->no_signal = c[0];
@@ -726,6 +768,25 @@ piv_status frame_from_device(piv_analyzer *a, const void *pixels, int fmt, uint3
         return s;
     }
     *out = f;
+    return PIV_OK;
+}
+
+piv_status mark_use(piv_analyzer *a, hipStream_t st, const piv_frame *const *fs, size_t n)
+{
+    // on a stream the caller is capturing, a recorded event marks no point of execution: the
+    // frames keep their previous last use
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIP_TRY(hipStreamIsCapturing(st, &cs));
+    if (cs != hipStreamCaptureStatusNone) return PIV_OK;
+    std::shared_ptr<UseEvent> &ev = a->use_events[st];
+    if (!ev) {
+        auto e = std::make_shared<UseEvent>();
+        HIP_TRY(hipEventCreateWithFlags(&e->e, hipEventDisableTiming));
+        ev = e;
+    }
+    HIP_TRY(hipEventRecord(ev->e, st));
+    for (size_t i = 0; i < n; ++i)
+        if (fs[i]) fs[i]->last_use = ev;
     return PIV_OK;
 }
 
@@ -979,11 +1040,11 @@ void piv_frame_release(piv_frame *f)
     try {
         if (!f) return;
         (void)hipSetDevice(f->device);
+        // the frame's last user (its preparation or the last analysis that read it) is done
+        // before its memory, or its batch's, is reused
+        if (f->last_use && f->last_use->e) (void)hipEventSynchronize(f->last_use->e);
         if (f->raw.p || f->image.p || f->base.p || f->sat.p) {
             if (auto pool = f->pool.lock()) {
-                // the frame's last user is done before its memory is reused (hipFree would
-                // synchronise the device as well)
-                (void)hipDeviceSynchronize();
                 pool->give(f->raw);
                 pool->give(f->image);
                 pool->give(f->base);

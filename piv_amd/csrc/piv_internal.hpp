@@ -104,6 +104,16 @@ struct HostBuf {
 
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
+// An event recorded on one stream after each analyzer call that used a frame there: the
+// frame's memory may be reused once the event has completed.
+struct UseEvent {
+    hipEvent_t e = nullptr;
+    ~UseEvent()
+    {
+        if (e) (void)hipEventDestroy(e);
+    }
+};
+
 inline int bytes_per_px(int fmt) { return fmt == FMT_U8 ? 1 : (fmt == FMT_U16 ? 2 : 4); }
 
 piv_status grid_new(uint32_t h, uint32_t w, uint32_t window, float overlap, GridDev &g);
@@ -182,6 +192,9 @@ struct piv_frame {
     const void *raw_ptr = nullptr;  // device pointer of the original samples
     int raw_pitch = 0;
     bool has_bg = false;
+    // the last stream position that used the frame (preparation or analysis): release waits
+    // for it instead of for the whole device
+    mutable std::shared_ptr<pivi::UseEvent> last_use;
     ~piv_frame()
     {
         raw.release();
@@ -239,6 +252,8 @@ struct piv_analyzer {
     std::map<std::pair<int, int>, int> variants;
     // correlation precision (piv_analyzer_set_precision): 0 FP32 FFT, 1 fp16 / 2 bf16 DFT-GEMM
     int precision = 0;
+    // one use event per stream the analyzer has run frames on (UseEvent)
+    std::map<hipStream_t, std::shared_ptr<pivi::UseEvent>> use_events;
     // replay cache: the last call's descriptor bytes and the graph of its launch sequence
     std::vector<char> last_sig;
     hipGraphExec_t graph_exec = nullptr;
@@ -305,6 +320,9 @@ piv_status frame_from_device(piv_analyzer *a, const void *pixels, int fmt, uint3
                              piv_frame **out);
 piv_status check_pixels_args(const void *pixels, int fmt, uint32_t h, uint32_t w,
                              size_t row_stride);
+// Records the analyzer's use event of stream `st` after the work enqueued there and makes it
+// the last use of the `n` frames.
+piv_status mark_use(piv_analyzer *a, hipStream_t st, const piv_frame *const *fs, size_t n);
 
 }  // namespace pivi
 

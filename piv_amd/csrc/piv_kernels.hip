@@ -376,9 +376,13 @@ __global__ void k_base(const void *__restrict__ raw, int pitch, long raw_frame_b
 //                 then carries each column down the band.
 // The samples are read twice and the table written once; the f64 chains are serial per row
 // and per column whatever the split, so the parallelism comes from rows x strips x frames.
-// In both kernels lanes 0-31 run the sums and lanes 32-63 the squares of 32 rows (or
-// columns), so every lane carries one chain.
+// In both kernels lanes 0-31 run the sums and lanes 32-63 the squares of 32 rows (or of the
+// strip's columns), so every lane carries one chain.  (16-column strips, twice the strip
+// waves per frame, were measured too: a serving batch's few frames gained within the noise,
+// a 2000-frame video batch lost 15 %.)
 constexpr int SAT_SW = 32, SAT_BAND = 32, SATK_TW = 64;
+// lanes that load one strip column of a band (each loads every SAT_LPC-th row)
+constexpr int SAT_LPC = 64 / SAT_SW;
 
 // Frames of a launch at their own addresses (a serving batch's frames in their request
 // slots): frame blockIdx.y reads img[y] and writes the tables at sum[y] (entry (0, 0) of the
@@ -465,17 +469,22 @@ __device__ __forceinline__ void sat_strips_frame(const void *__restrict__ img, i
     // this lane's table: sums (lanes 0-31) or squares (32-63)
     double *tab = sq ? sumsq : sum;
     const long s = (long)sat_pitch(W);
+    // the column walk: lane (c, sq) carries strip column c < SAT_SW down the frame
+    const bool colw = c < SAT_SW;
     const int x = strip * SAT_SW + c;  // sample column of table column x + 1
-    if (x < W) tab[x + 1] = 0.0;       // row 0
+    if (colw && x < W) tab[x + 1] = 0.0;  // row 0
     if (strip == 0 && c == 0) tab[0] = 0.0;
     const double *crow = strip > 0 ? carry + ((long)(strip - 1) * 2 + sq) * H : nullptr;
-    // raw samples from clamped addresses, zeroed outside the frame when normalised into LDS
-    raw_t<FMT> pre[SAT_BAND / 2];
-    const int xc = min(x, W - 1);
+    // band loads: lane (lc, lg) fetches strip column lc of rows lg, lg + SAT_LPC, ..., as raw
+    // samples from clamped addresses, zeroed outside the frame when normalised into LDS
+    const int lc = lane % SAT_SW, lg = lane / SAT_SW;
+    const int xl = strip * SAT_SW + lc;
+    raw_t<FMT> pre[SAT_BAND / SAT_LPC];
+    const int xc = min(xl, W - 1);
     auto fetch = [&](int y0) {
 #pragma unroll
-        for (int i = 0; i < SAT_BAND / 2; ++i)
-            pre[i] = load_raw<FMT>(img, (long)min(y0 + 2 * i + sq, H - 1) * pitch + xc);
+        for (int i = 0; i < SAT_BAND / SAT_LPC; ++i)
+            pre[i] = load_raw<FMT>(img, (long)min(y0 + SAT_LPC * i + lg, H - 1) * pitch + xc);
     };
     auto fetch_carry = [&](int y) {
         if (!crow) return 0.0;
@@ -491,8 +500,9 @@ __device__ __forceinline__ void sat_strips_frame(const void *__restrict__ img, i
             for (int r = c; r < rows; r += 32) tab[(long)(y0 + r + 1) * s] = 0.0;  // column 0
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < SAT_BAND / 2; ++i)
-            tin[2 * i + sq][c] = (y0 + 2 * i + sq < H && x < W) ? normalise_raw<FMT>(pre[i]) : 0.0f;
+        for (int i = 0; i < SAT_BAND / SAT_LPC; ++i)
+            tin[SAT_LPC * i + lg][lc] =
+                (y0 + SAT_LPC * i + lg < H && xl < W) ? normalise_raw<FMT>(pre[i]) : 0.0f;
         __syncthreads();
         double acc = ci;
         if (y0 + SAT_BAND < H) {
@@ -513,7 +523,7 @@ __device__ __forceinline__ void sat_strips_frame(const void *__restrict__ img, i
         }
         __syncthreads();
         // lane (c, sq): column x down the band
-        if (x < W) {
+        if (colw && x < W) {
 #pragma unroll
             for (int r0 = 0; r0 < SAT_BAND; r0 += 8) {
                 double col[8];

@@ -65,6 +65,8 @@ struct SrvStream {
     size_t slot_dev = 0, slot_host = 0;
     std::vector<Slot *> free_slots;
     int in_use = 0;        // slots held by admitted requests not yet released
+    int in_flight = 0;     // admitted requests not yet finished (queued or on the device);
+                           //   polled results the client has not released are not counted
     std::mutex upload_mu;  // orders uploads + conditioning on the analyzer's stream
 };
 
@@ -230,7 +232,7 @@ void free_slot(piv_server *s, Slot *sl)
 
 // Drop cached free slots (largest streams first) until `need` more bytes fit the budget.
 // A stream gives up free slots only while it holds more of them than it has requests in
-// flight: a stream being served reuses a free slot for its next request as soon as one of
+// flight (queued or on the device; results held by the client count neither way): a stream being served reuses a free slot for its next request as soon as one of
 // its results is released, and taking those slots would only have it reallocate them
 // (hipFree / hipMalloc, each synchronising) a moment later.  Past that, the submission is
 // refused and retried by the client once results are released; slots left over from an
@@ -240,7 +242,7 @@ bool make_room(piv_server *s, size_t need)
     while (s->allocated + need > s->cfg.hbm_budget) {
         SrvStream *best = nullptr;
         for (SrvStream *st : s->streams)
-            if (st->free_slots.size() > (size_t)st->in_use &&
+            if (st->free_slots.size() > (size_t)st->in_flight &&
                 (!best || st->slot_dev > best->slot_dev))
                 best = st;
         if (!best) return false;
@@ -674,6 +676,7 @@ void scheduler_loop(piv_server *s)
             } else if (r->status != PIV_OK) {
                 // its preparation failed: the request completes with that error
                 r->t_done = s->now_ms();
+                r->st->in_flight -= 1;
                 s->done.push_back(r);
                 it = s->queue.erase(it);
                 failed = true;
@@ -700,6 +703,7 @@ void scheduler_loop(piv_server *s)
             for (Request *r : L.batch) {
                 r->status = st;
                 r->t_done = s->now_ms();
+                r->st->in_flight -= 1;
                 s->done.push_back(r);
             }
             L.batch.clear();
@@ -750,6 +754,7 @@ void completer_loop(piv_server *s)
         lk.lock();
         s->inflight.pop_front();
         for (int k = 0; k < B; ++k) {
+            L.batch[k]->st->in_flight -= 1;
             s->done.push_back(L.batch[k]);
             s->stats.completed += 1;
         }
@@ -951,6 +956,7 @@ piv_status submit_impl(piv_server *s, uint32_t stream_id, const void *pa, const 
         PIV_TRY(new_slot(s, st, &sl));
     }
     st->in_use += 1;
+    st->in_flight += 1;
     lk.unlock();
     piv_status r = PIV_OK;
     {
@@ -1032,6 +1038,7 @@ piv_status submit_impl(piv_server *s, uint32_t stream_id, const void *pa, const 
     if (r != PIV_OK) {
         st->free_slots.push_back(sl);
         st->in_use -= 1;
+        st->in_flight -= 1;
         return r;
     }
     auto *req = new Request();

@@ -306,6 +306,52 @@ def test_free_slots_move_between_streams_only_when_idle(piv):
     srv.close()
 
 
+def test_unreleased_results_do_not_pin_free_slots(piv):
+    """A client that polls results through the C ABI and keeps some of them unreleased: those
+    results hold their slots, but they are not in flight, so the stream's spare cached slots
+    still go to another stream over the budget."""
+    import ctypes as C
+
+    from piv_amd import _lib as L
+
+    lib = L.load()
+    cfg = piv.PivConfig(windows=[32], final_iterations=1)
+    raw = piv.PreprocessConfig.raw()
+    a, b = S.particle_frames(256, 256, 6, field=S.uniform_field(1.0, 0.5))
+    probe = piv.PivServer()
+    psid = probe.add_stream(cfg, 256, 256, np.uint8, raw)
+    probe.submit(psid, a, b)
+    probe.poll(1, wait_ms=5000)
+    one = probe.stats()["hbm_peak"]
+    probe.close()
+
+    srv = piv.PivServer(hbm_budget=4 * one)
+    sa = srv.add_stream(cfg, 256, 256, np.uint8, raw)
+    sb = srv.add_stream(cfg, 256, 256, np.uint8, raw)
+    srv.pause(True)
+    ids = [srv.submit(sa, a, b) for _ in range(4)]  # the whole budget, in stream a's slots
+    srv.pause(False)
+    arr = (L.piv_result * 8)()
+    got = []
+    while len(got) < 4:
+        cnt = C.c_uint32()
+        assert lib.piv_poll(srv._h, arr, 8, C.byref(cnt), 5000) == 0
+        assert cnt.value > 0, "no result within 5 s"
+        got += [arr[i].request_id for i in range(cnt.value)]
+    assert sorted(got) == sorted(ids)
+    for rid in ids[:2]:  # two slots back to stream a's cache; two results kept
+        assert lib.piv_result_release(srv._h, rid) == 0
+    rid = srv.submit(sb, a, b)  # admitted from stream a's spare cached slots
+    res = srv.poll(1, wait_ms=5000)
+    assert res and res[0].request_id == rid and res[0].error is None
+    for r in ids[2:]:
+        assert lib.piv_result_release(srv._h, r) == 0
+    st = srv.stats()
+    assert st["rejected"] == 0
+    assert st["hbm_peak"] <= 4 * one
+    srv.close()
+
+
 def test_large_host_pairs_staged_with_helpers(piv):
     """Host pairs of 3 MiB and more are staged by the submitting thread and the server's
     staging helpers together, in row chunks of about 1 MiB (the last one partial here), from
