@@ -4,5 +4,5 @@
 #pragma once
 
 #define PIV_FFT_SIZES(X) \
-    X(8) X(10) X(12) X(14) X(16) X(20) X(24) X(28) X(32) X(40) X(48) X(56) X(64) X(96)
+    X(8) X(10) X(12) X(14) X(16) X(20) X(24) X(28) X(32) X(40) X(48) X(56) X(64) X(96) X(128)
 #define PIV_MF_SIZES(X) X(16) X(32) X(64)

@@ -1,8 +1,9 @@
 // Correlation pass for window sizes without a register-resident FFT kernel: any even n up to
 // NMAX (128), every window of every pair of a pass in one launch, one 256-lane workgroup per
 // window.  The reference accepts every even n >= 8 (multipass.rs:96-100; realfft plans any
-// length, correlate.rs:53-65); sizes of the form 2^a 3^b 5^c 7^d up to 64 and 96 use
-// k_correlate<N>, everything else comes here, and so do deformed passes of windows above 64.
+// length, correlate.rs:53-65); the sizes with an instantiated k_correlate<N> (8, 10, 12, 14,
+// 16, 20, 24, 28, 32, 40, 48, 56, 64, 96) use it, every other size comes here, and so do
+// deformed passes of windows above 64.
 //
 // Per window (correlate.rs:99-139):
 //   1. the window pair as one complex plane z = a + i b in LDS (row stride n + 1), sampled as
