@@ -160,6 +160,12 @@ void piv_frame_release(piv_frame *f);
 /* PreparedFrame's public `image` (and the background-subtracted base behind `stats`), as
  * host float [h*w]; either pointer may be NULL. */
 piv_status piv_frame_download(const piv_frame *f, float *image, float *base);
+/* Whether the frame's signal-mask tables (IntegralImage, preprocess.rs:84-124) exist on the
+ * device: *built = 0 when the frame has no tables (the mask needs none) or when they were
+ * screened out.  No counterpart in the reference (a diagnostic): with the default mask a u16
+ * frame without conditioning gets tables only if it holds an aligned all-zero tile of half
+ * the analyzer's smallest window, since only all-zero windows can depend on them. */
+piv_status piv_frame_tables_built(const piv_frame *f, int *built);
 
 /* BackgroundEstimator (preprocess.rs:31-72): per-pixel min or mean over sample frames, on
  * the analyzer's device.  add() takes HOST samples (on_device = 0) or DEVICE samples

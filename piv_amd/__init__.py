@@ -545,6 +545,13 @@ class PreparedFrame:
     def image(self):
         return self.download()[0]
 
+    def tables_built(self):
+        """Whether the frame's signal-mask tables exist on the device (False when the mask
+        needs none or when screening found that no window can depend on them)."""
+        v = C.c_int(0)
+        _check(self._lib.piv_frame_tables_built(self._h, C.byref(v)))
+        return bool(v.value)
+
     def close(self):
         if self._h:
             self._lib.piv_frame_release(self._h)

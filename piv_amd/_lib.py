@@ -111,6 +111,7 @@ _SIGS = {
                                   C.c_size_t, C.c_void_p, C.POINTER(C.c_void_p)], C.c_int),
     "piv_frame_release": ([C.c_void_p], None),
     "piv_frame_download": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "piv_frame_tables_built": ([C.c_void_p, C.POINTER(C.c_int)], C.c_int),
     "piv_analyze_pair": ([C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(piv_field_out)], C.c_int),
     "piv_analyze_traced": ([C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(piv_field_out),
                             C.POINTER(piv_pass_trace), C.c_uint32], C.c_int),

@@ -395,7 +395,25 @@ __device__ __forceinline__ void find_peak_plane(const float *pr, int lane, int t
     for (int x = 0; x < N; ++x) zr[x] = pr[lane * P + x];
     float best = -INFINITY;
     int bidx = 0;
-    {
+    if constexpr (C::IN_WAVE && RAW_MAX) {
+        // finite plane: the row maxima by a max3 tree, the window maximum across the rows,
+        // then the first row holding it and the first column of that row holding it (ballots
+        // of exact equality), which is the first maximum in row-major order
+        float rm = zr[0];
+#pragma unroll
+        for (int x = 1; x < N; x += 2) rm = x + 1 < N ? max3_finite(rm, zr[x], zr[x + 1]) : max_finite(rm, zr[x]);
+        float b = rm;
+#pragma unroll
+        for (int m = 1; m < N; m <<= 1) b = max_finite(b, __shfl_xor(b, m, N));
+        best = b;
+        const int base = tid - lane;
+        const unsigned long long full = N == 64 ? ~0ull : ((1ull << (N & 63)) - 1);
+        const unsigned long long rows = (__ballot(rm == b) >> base) & full;
+        const int py = rows ? __builtin_ctzll(rows) : 0;
+        const float v = pr[py * P + lane];  // lane = column of row py
+        const unsigned long long cols = (__ballot(v == b) >> base) & full;
+        bidx = py * N + (cols ? __builtin_ctzll(cols) : 0);
+    } else {
         float rb = -INFINITY;
         int ri = 0;
 #pragma unroll
@@ -643,12 +661,37 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
     // ---- fused ingest + mean removal --------------------------------------------------
     float zr[N], zi[N];
     ingest<N, FMT, DEF>(pp, sig, ax, ay, bx, by, lane, red, pr, pim, zr, zi);
+    // Window energies.  Windows tiling the wave (N | 64) sum (a - m)^2 over the lane's column
+    // here (m the approximate mean), with the exact-mean correction from the column sums the
+    // column transform leaves in bin 0: two fma per sample instead of four per spectral bin.
+    // Other sizes sum |2A|^2 and |2B|^2 over the spectrum (Parseval): their window sums go
+    // through LDS, and two more of them next to the transforms cost registers.
+    constexpr bool SPATIAL_E = C::IN_WAVE;
+    float qa = 0.0f, qb = 0.0f;
+    if constexpr (SPATIAL_E) {
+#pragma unroll
+        for (int r = 0; r < N; ++r) {
+            qa = __builtin_fmaf(zr[r], zr[r], qa);
+            qb = __builtin_fmaf(zi[r], zi[r], qb);
+        }
+        // summed here, before the transform: the opaque use keeps the compiler from sinking
+        // the sums past it, which would keep a copy of every sample live through the transform
+        asm volatile("" : "+v"(qa), "+v"(qb));
+    }
 
     // ---- forward transform of z = a + i b ----------------------------------------------
     // Transposes go through LDS.  A window inside one wave uses a single float plane and
     // moves real and imaginary parts in two passes (the barriers of a one-wave workgroup cost
     // nothing), halving the LDS footprint, which bounds occupancy at N = 32 and 64.
     pivfft::fft<N, -1>(zr, zi);  // along rows r; lane = column
+    // sum (a - mean)^2 = sum (a - m)^2 - (sum (a - m))^2 / N^2, in raw units squared
+    float ea32 = 0.0f, eb32 = 0.0f;
+    if constexpr (SPATIAL_E) {
+        constexpr float inv_n2 = 1.0f / (float)(N * N);
+        const float wsa = window_sum<N>(zr[0], red, lane), wsb = window_sum<N>(zi[0], red, lane);
+        ea32 = window_sum<N>(qa, red, lane) - wsa * wsa * inv_n2;
+        eb32 = window_sum<N>(qb, red, lane) - wsb * wsb * inv_n2;
+    }
     if constexpr (C::ONE_WAVE) {
 #pragma unroll
         for (int r = 0; r < N; ++r) pr[r * P + lane] = zr[r];
@@ -677,9 +720,9 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
     }
     pivfft::fft<N, -1>(zr, zi);  // lane ky holds Z[ky][kx]
 
-    // ---- cross spectrum conj(FA) FB, DC cleared; energies by Parseval ---------------------
+    // ---- cross spectrum conj(FA) FB, DC cleared -------------------------------------------
     // 2A = Z[k] + conj(Z[-k]),  2B = (Z[k] - conj(Z[-k])) / i,  X' = conj(2A) 2B = 4 X
-    float ea2 = 0.0f, eb2 = 0.0f;
+    float ea2 = 0.0f, eb2 = 0.0f;  // Parseval energies (sizes not tiling the wave)
     {
         const int mrow = (N - lane) % N;
         if constexpr (C::ONE_WAVE) {
@@ -699,8 +742,10 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
                     const float a2r = zr[x] + qr, a2i = zi[x] - qi;
                     const float b2r = zi[x] + qi, b2i = qr - zr[x];
                     const bool dc = (x == 0) && (lane == 0);
-                    ea2 = dc ? ea2 : __builtin_fmaf(a2r, a2r, __builtin_fmaf(a2i, a2i, ea2));
-                    eb2 = dc ? eb2 : __builtin_fmaf(b2r, b2r, __builtin_fmaf(b2i, b2i, eb2));
+                    if constexpr (!SPATIAL_E) {
+                        ea2 = dc ? ea2 : __builtin_fmaf(a2r, a2r, __builtin_fmaf(a2i, a2i, ea2));
+                        eb2 = dc ? eb2 : __builtin_fmaf(b2r, b2r, __builtin_fmaf(b2i, b2i, eb2));
+                    }
                     const float xr = __builtin_fmaf(a2r, b2r, a2i * b2i);
                     const float xi = __builtin_fmaf(a2r, b2i, -(a2i * b2r));
                     zr[x] = dc ? 0.0f : xr;
@@ -709,8 +754,10 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
                 if (mx != x) {
                     const float a2r = zr[mx] + qr2, a2i = zi[mx] - qi2;
                     const float b2r = zi[mx] + qi2, b2i = qr2 - zr[mx];
-                    ea2 = __builtin_fmaf(a2r, a2r, __builtin_fmaf(a2i, a2i, ea2));
-                    eb2 = __builtin_fmaf(b2r, b2r, __builtin_fmaf(b2i, b2i, eb2));
+                    if constexpr (!SPATIAL_E) {
+                        ea2 = __builtin_fmaf(a2r, a2r, __builtin_fmaf(a2i, a2i, ea2));
+                        eb2 = __builtin_fmaf(b2r, b2r, __builtin_fmaf(b2i, b2i, eb2));
+                    }
                     zr[mx] = __builtin_fmaf(a2r, b2r, a2i * b2i);
                     zi[mx] = __builtin_fmaf(a2r, b2i, -(a2i * b2r));
                 }
@@ -731,8 +778,10 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
                 const float a2r = zr[x] + qr, a2i = zi[x] - qi;
                 const float b2r = zi[x] + qi, b2i = qr - zr[x];
                 const bool dc = (x == 0) && (lane == 0);
-                ea2 = dc ? ea2 : __builtin_fmaf(a2r, a2r, __builtin_fmaf(a2i, a2i, ea2));
-                eb2 = dc ? eb2 : __builtin_fmaf(b2r, b2r, __builtin_fmaf(b2i, b2i, eb2));
+                if constexpr (!SPATIAL_E) {
+                    ea2 = dc ? ea2 : __builtin_fmaf(a2r, a2r, __builtin_fmaf(a2i, a2i, ea2));
+                    eb2 = dc ? eb2 : __builtin_fmaf(b2r, b2r, __builtin_fmaf(b2i, b2i, eb2));
+                }
                 const float xr = __builtin_fmaf(a2r, b2r, a2i * b2i);
                 const float xi = __builtin_fmaf(a2r, b2i, -(a2i * b2r));
                 zr[x] = dc ? 0.0f : xr;
@@ -740,10 +789,13 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
             }
         }
     }
-    // sum_k |2A_k|^2 = 4 N^2 sum (a - mean)^2  (Parseval), in raw units squared
-    const double norm = 1.0 / (4.0 * (double)(N * N));
-    const double ea = (double)window_sum<N>(ea2, red, lane) * norm;
-    const double eb = (double)window_sum<N>(eb2, red, lane) * norm;
+    if constexpr (!SPATIAL_E) {
+        // sum_k |2A_k|^2 = 4 N^2 sum (a - mean)^2  (Parseval)
+        constexpr float inv4n2 = 1.0f / (4.0f * (float)(N * N));
+        ea32 = window_sum<N>(ea2, red, lane) * inv4n2;
+        eb32 = window_sum<N>(eb2, red, lane) * inv4n2;
+    }
+    const double ea = (double)ea32, eb = (double)eb32;
     const double floor_e = energy_floor<FMT>();
     const bool has_peak = sig && !(ea <= floor_e || eb <= floor_e);
 

@@ -139,6 +139,109 @@ __device__ __forceinline__ void twiddle(float &re, float &im, int t)
     im = __builtin_fmaf(r, s, im * c);
 }
 
+// Twiddles with their scale folded out (the FMA form of a twiddled butterfly):
+//   W = m (1 + i q)  with m = cos, q = tan   when |cos| >= |sin|  (form 1),
+//   W = m (q + i)    with m = sin, q = cot   otherwise            (form 2),
+// so v W = m y with y = v (1 + i q) or v (q + i), two fma per complex value, |q| <= 1.  The
+// factor m is then applied inside the butterfly's own additions (a + m y is one fma), so a
+// twiddle costs two instructions instead of four.  Exact rotations (1, -1, +-i) keep form 0
+// with m = 1.  k4[t] = m(3t) / m(t) lets the radix-4 butterfly fold input 3 onto input 1.
+// Values are computed in double and rounded once.
+template <int N, int S>
+struct TwFold {
+    float m[N], q[N], k4[N];
+    int form[N];
+    constexpr TwFold() : m(), q(), k4(), form()
+    {
+        double md[N] = {};
+        for (int t = 0; t < N; ++t) {
+            double c = 0, s = 0;
+            cos_sin_2pi(t, N, c, s);
+            if (S < 0) s = -s;
+            if (t == 0 || 4 * t == N || 2 * t == N || 4 * t == 3 * N) {
+                form[t] = 0;
+                md[t] = 1.0;
+                q[t] = 0.0f;
+            } else if ((c < 0 ? -c : c) >= (s < 0 ? -s : s)) {
+                form[t] = 1;
+                md[t] = c;
+                q[t] = (float)(s / c);
+            } else {
+                form[t] = 2;
+                md[t] = s;
+                q[t] = (float)(c / s);
+            }
+            m[t] = (float)md[t];
+        }
+        for (int t = 0; t < N; ++t) k4[t] = (float)(md[(3 * t) % N] / md[t]);
+    }
+};
+
+template <int N, int S>
+struct TwF {
+    static constexpr TwFold<N, S> f{};
+};
+
+// (re, im) := (re, im) W_N^t / m(t) with W = exp(S 2 pi i t / N); returns m(t)
+template <int N, int S>
+__device__ __forceinline__ float twiddle_folded(float &re, float &im, int t)
+{
+    t %= N;
+    if (TwF<N, S>::f.form[t] == 0) {
+        twiddle<N, S>(re, im, t);
+        return 1.0f;
+    }
+    const float q = TwF<N, S>::f.q[t], r = re;
+    if (TwF<N, S>::f.form[t] == 1) {  // (1 + i q)
+        re = __builtin_fmaf(-q, im, r);
+        im = __builtin_fmaf(q, r, im);
+    } else {  // (q + i)
+        re = __builtin_fmaf(q, r, -im);
+        im = __builtin_fmaf(q, im, r);
+    }
+    return TwF<N, S>::f.m[t];
+}
+
+// Radix-4 butterfly on inputs twiddled by W_N^{r t} (r = 0..3), the twiddle scales folded
+// into the butterfly's additions: 22 instructions instead of 28 with three general twiddles.
+template <int N, int S>
+__device__ __forceinline__ void dft4_twiddled(float (&vr)[4], float (&vi)[4], int t)
+{
+    t %= N;
+    const float m1 = twiddle_folded<N, S>(vr[1], vi[1], t);
+    const float m2 = twiddle_folded<N, S>(vr[2], vi[2], 2 * t);
+    (void)twiddle_folded<N, S>(vr[3], vi[3], 3 * t);
+    const float k = TwF<N, S>::f.k4[t];  // m(3t) / m(t)
+    const float ar = __builtin_fmaf(m2, vr[2], vr[0]), ai = __builtin_fmaf(m2, vi[2], vi[0]);
+    const float br = __builtin_fmaf(-m2, vr[2], vr[0]), bi = __builtin_fmaf(-m2, vi[2], vi[0]);
+    // v1 + v3 = m1 (y1 + k y3), v1 - v3 = m1 (y1 - k y3)
+    const float cr = __builtin_fmaf(k, vr[3], vr[1]), ci = __builtin_fmaf(k, vi[3], vi[1]);
+    const float er = __builtin_fmaf(-k, vr[3], vr[1]), ei = __builtin_fmaf(-k, vi[3], vi[1]);
+    // W_4 = S i
+    const float fr = S < 0 ? ei : -ei;
+    const float fi = S < 0 ? -er : er;
+    vr[0] = __builtin_fmaf(m1, cr, ar);
+    vi[0] = __builtin_fmaf(m1, ci, ai);
+    vr[1] = __builtin_fmaf(m1, fr, br);
+    vi[1] = __builtin_fmaf(m1, fi, bi);
+    vr[2] = __builtin_fmaf(-m1, cr, ar);
+    vi[2] = __builtin_fmaf(-m1, ci, ai);
+    vr[3] = __builtin_fmaf(-m1, fr, br);
+    vi[3] = __builtin_fmaf(-m1, fi, bi);
+}
+
+// Radix-2 butterfly on inputs twiddled by W_N^{r t}: 6 instructions instead of 8.
+template <int N, int S>
+__device__ __forceinline__ void dft2_twiddled(float (&vr)[2], float (&vi)[2], int t)
+{
+    const float m1 = twiddle_folded<N, S>(vr[1], vi[1], t);
+    const float ar = vr[0], ai = vi[0];
+    vr[0] = __builtin_fmaf(m1, vr[1], ar);
+    vi[0] = __builtin_fmaf(m1, vi[1], ai);
+    vr[1] = __builtin_fmaf(-m1, vr[1], ar);
+    vi[1] = __builtin_fmaf(-m1, vi[1], ai);
+}
+
 template <int R, int S>
 __device__ __forceinline__ void dft(float (&vr)[R], float (&vi)[R])
 {
@@ -238,9 +341,15 @@ __device__ __forceinline__ void stockham(float (&xr)[N], float (&xi)[N])
                 vi[r] = xi[j + r * (N / R)];
             }
             const int jm = j % NS;
+            if constexpr (R == 4) {
+                dft4_twiddled<N, S>(vr, vi, jm * (N / (NS * R)));
+            } else if constexpr (R == 2) {
+                dft2_twiddled<N, S>(vr, vi, jm * (N / (NS * R)));
+            } else {
 #pragma unroll
-            for (int r = 1; r < R; ++r) twiddle<N, S>(vr[r], vi[r], r * jm * (N / (NS * R)));
-            dft<R, S>(vr, vi);
+                for (int r = 1; r < R; ++r) twiddle<N, S>(vr[r], vi[r], r * jm * (N / (NS * R)));
+                dft<R, S>(vr, vi);
+            }
             const int d = (j / NS) * NS * R + jm;
 #pragma unroll
             for (int r = 0; r < R; ++r) {

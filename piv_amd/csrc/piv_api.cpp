@@ -6,6 +6,8 @@
 // kernels; the validated field stays in HBM as the next pass's predictor.  All per-pair
 // descriptors of a call are uploaded with a single host-to-device copy before the first
 // launch, and a call whose descriptors equal the previous call's replays its captured graph.
+#include <cstdlib>
+
 #include "piv_internal.hpp"
 
 using namespace piv;
@@ -129,6 +131,41 @@ bool mask_needed(const piv_config &c, int in_fmt, bool has_bg)
     return true;
 }
 
+int screen_tile(const piv_analyzer *a, int in_fmt, bool cond, int H, int W)
+{
+    const piv_config &c = a->cfg;
+    if (in_fmt != FMT_U16 || cond || !(c.min_mean <= 0.0f) || !(c.min_std <= 0.0f)) return 0;
+    if (const char *env = std::getenv("PIV_NO_TABLE_SCREEN"))
+        if (env[0] == '1') return 0;
+    if ((H + 31) / 32 > SCREEN_MAX_BANDS) return 0;
+    // the tables' rounding error on any box, (4 (W + H) + 3) 2^-53 H W for samples <= 1,
+    // must stay below the smallest non-zero sample 1/65535
+    if ((4.0 * ((double)W + H) + 3.0) * (double)H * (double)W * 0x1p-53 >= 0x1p-16) return 0;
+    int n = 1 << 30;
+    for (int p : a->passes) n = std::min(n, p);
+    int t = 1;
+    while (t * 2 <= n / 2 && t < 32) t *= 2;
+    return t;
+}
+
+piv_status screen_for(piv_analyzer *a, const piv_frame *const *fs, size_t count, int n,
+                      hipStream_t st)
+{
+    int need = 1;
+    while (need * 2 <= n / 2 && need < 32) need *= 2;
+    for (size_t i = 0; i < count; ++i) {
+        const piv_frame *f = fs[i];
+        if (!f->has_sat || f->screen_t <= need) continue;
+        PIV_TRY(a->sat_carry.reserve(sat_carry_bytes(f->H, f->W, 1)));
+        const void *img = f->raw_ptr;
+        double *sum = const_cast<double *>(f->sat_ptr);
+        HIP_TRY(launch_sat_list(f->in_fmt, &img, &sum, 1, f->raw_pitch, f->H, f->W,
+                                a->sat_carry.as<double>(), st, need));
+        f->screen_t = need;
+    }
+    return PIV_OK;
+}
+
 bool prepare_launches(const piv_config &c, int in_fmt, bool has_bg)
 {
     return c.smooth_sigma > 0.0f || c.normalize_radius > 0 || has_bg || mask_needed(c, in_fmt, has_bg);
@@ -207,8 +244,10 @@ static piv_status prepare_frame_impl(piv_analyzer *a, piv_frame *f, const float 
         // from the samples and no base image is kept (piv_frame_download recomputes it)
         PIV_TRY(a->sat_carry.reserve(sat_carry_bytes(H, W, 1)));
         double *carry = a->sat_carry.as<double>();
+        f->screen_t = cond ? 0 : screen_tile(a, f->in_fmt, false, H, W);
         if (cond) HIP_TRY(launch_sat(f->base_ptr, H, W, sum, sq, carry, st));
-        else HIP_TRY(launch_sat_raw(f->in_fmt, f->raw_ptr, f->raw_pitch, 0, H, W, sum, sq, carry, st));
+        else HIP_TRY(launch_sat_raw(f->in_fmt, f->raw_ptr, f->raw_pitch, 0, H, W, sum, sq, carry, st,
+                                    1, 0, f->screen_t));
         f->has_sat = true;
     }
     if (!cond) {
@@ -266,6 +305,7 @@ static piv_status prepare_frames_list_impl(piv_analyzer *a, piv_frame *const *fs
     }
     const int H = f0->H, W = f0->W;
     const int64_t cells = sat_table_elems(H, W);
+    const int t = screen_tile(a, f0->in_fmt, false, H, W);
     std::vector<const void *> imgs(count);
     std::vector<double *> sums(count);
     for (int i = 0; i < count; ++i) {
@@ -275,6 +315,7 @@ static piv_status prepare_frames_list_impl(piv_analyzer *a, piv_frame *const *fs
         imgs[i] = f->raw_ptr;
         f->sat_ptr = sums[i];
         f->has_sat = true;
+        f->screen_t = t;
         f->has_bg = false;
         f->cells = cells;
         f->base_ptr = nullptr;
@@ -284,7 +325,7 @@ static piv_status prepare_frames_list_impl(piv_analyzer *a, piv_frame *const *fs
     }
     PIV_TRY(a->sat_carry.reserve(sat_carry_bytes(H, W, std::min(count, 64))));
     HIP_TRY(launch_sat_list(f0->in_fmt, imgs.data(), sums.data(), count, f0->raw_pitch, H, W,
-                            a->sat_carry.as<double>(), a->stream));
+                            a->sat_carry.as<double>(), a->stream, t));
     return PIV_OK;
 }
 
@@ -401,14 +442,16 @@ static piv_status prepare_frames_batch_impl(piv_analyzer *a, piv_frame *const *f
         const size_t need = (size_t)count * cells * 2 * sizeof(double);
         PIV_TRY(batch_storage(a, need, store));
         double *sat = store->as<double>() + SAT_LEAD;  // entry (0, 0) of frame 0's sums
+        const int t = screen_tile(a, f0->in_fmt, false, H, W);
         PIV_TRY(a->sat_carry.reserve(sat_carry_bytes(H, W, count)));
         HIP_TRY(launch_sat_raw(f0->in_fmt, f0->raw_ptr, f0->raw_pitch, raw_frame_bytes, H, W, sat,
-                               sat + cells, a->sat_carry.as<double>(), a->stream, count, 2 * cells));
+                               sat + cells, a->sat_carry.as<double>(), a->stream, count, 2 * cells, t));
         for (int i = 0; i < count; ++i) {
             piv_frame *f = fs[i];
             f->base_ptr = nullptr;
             f->sat_ptr = sat + (size_t)i * 2 * cells;
             f->has_sat = true;
+            f->screen_t = t;
             f->batch = store;
         }
     }
@@ -470,6 +513,12 @@ static piv_status run_batch_impl(piv_analyzer *a, const piv_frame *const *fa, co
     }
     PIV_TRY(plan_passes(a, H, W, plans));
     const size_t np = plans.size();
+    {
+        int nmin = 1 << 30;
+        for (const PassPlan &pl : plans) nmin = std::min(nmin, pl.n);
+        PIV_TRY(screen_for(a, fa, npairs, nmin, a->stream));
+        PIV_TRY(screen_for(a, fb, npairs, nmin, a->stream));
+    }
     // ---- workspace layout ----
     size_t off = 0;
     std::vector<size_t> o_raw(np), o_val(np), o_snap(np), o_org(np), o_sig(np), o_rec(np), o_pk(np);
@@ -559,6 +608,7 @@ static piv_status run_batch_impl(piv_analyzer *a, const piv_frame *const *fa, co
             d.sat_a = A->has_sat ? A->sat_ptr : nullptr;
             d.sat_b = B->has_sat ? B->sat_ptr : nullptr;
             d.sat_cells = A->cells;
+            d.screened = screened_bits(A, B);
             if (i > 0) {
                 d.prev_dx = val[i - 1].dx + (size_t)k * plans[i - 1].cells;
                 d.prev_dy = val[i - 1].dy + (size_t)k * plans[i - 1].cells;
@@ -625,35 +675,57 @@ static piv_status run_batch_impl(piv_analyzer *a, const piv_frame *const *fa, co
                        reinterpret_cast<const char *>(pv) + sizeof pv);
         }
     }
-    auto enqueue = [&]() -> piv_status {
-        HIP_TRY(launch_zero_u32(sum_dev, (uint32_t)(np * npairs * 5), st));
+    // The pass chain of pairs [k0, k0 + cnt) on stream s.
+    auto chain = [&](uint32_t k0, uint32_t cnt, hipStream_t s) -> piv_status {
         for (size_t i = 0; i < np; ++i) {
             const PassPlan &pl = plans[i];
-            const uint32_t total = (uint32_t)(pl.cells * npairs);
+            const uint32_t total = (uint32_t)(pl.cells * cnt);
             const double cbytes = (double)total * (2.0 * pl.n * pl.n * 4.0 + 49.0);
-            const PassPair *ppd = reinterpret_cast<PassPair *>(dd + o_pp[i]);
+            const PassPair *ppd = reinterpret_cast<PassPair *>(dd + o_pp[i]) + k0;
             // placement reads the predictor field (8 B/vector of the previous grid, bilinear
             // taps mostly cached) and writes origins + mask (17 B/window)
             HIP_TRY(timed(a, "place", 0, (double)total * 17.0, total, [&] {
-                return launch_place_pass(ppd, (int)npairs, (uint32_t)pl.cells, st);
+                return launch_place_pass(ppd, (int)cnt, (uint32_t)pl.cells, s);
             }));
             const int mp = mf_precision(a, pl.n);
             const int def = i > 0 ? (int)a->cfg.deformation : 0;
             HIP_TRY(timed(a, mp ? "correlate_mf" : "correlate", pl.n, cbytes, total, [&] {
-                return mp ? launch_correlate_mf(pl.n, fa[0]->fmt, mp, ppd, (int)npairs,
-                                                (uint32_t)pl.cells, st, def)
+                return mp ? launch_correlate_mf(pl.n, fa[0]->fmt, mp, ppd, (int)cnt,
+                                                (uint32_t)pl.cells, s, def)
                           : launch_correlate(pl.n, fa[0]->fmt, variant_of(a, pl.n, fa[0]->fmt),
-                                             ppd, (int)npairs, (uint32_t)pl.cells, st, def);
+                                             ppd, (int)cnt, (uint32_t)pl.cells, s, def);
             }));
             // the finishing kernel reads peak records + origins (48 B) and writes 13 B/window
             HIP_TRY(timed(a, "finish", 0, (double)total * 61.0, total, [&] {
-                return launch_peak_finish(ppd, (int)npairs, (uint32_t)pl.cells, st, mp != 0);
+                return launch_peak_finish(ppd, (int)cnt, (uint32_t)pl.cells, s, mp != 0);
             }));
             // validation reads the raw field (13 B/vector) and writes the validated one (13 B)
             HIP_TRY(timed(a, "validate", 0, (double)total * 26.0, total, [&] {
-                return launch_validate(reinterpret_cast<ValidatePair *>(dd + o_vp[i]), (int)npairs,
-                                       (uint32_t)pl.cells, st);
+                return launch_validate(reinterpret_cast<ValidatePair *>(dd + o_vp[i]) + k0,
+                                       (int)cnt, (uint32_t)pl.cells, s);
             }));
+        }
+        return PIV_OK;
+    };
+    // Lanes: groups of at least 4 pairs each; profiled calls run one chain on `st` so that
+    // every kernel is bracketed by events on one stream.
+    const uint32_t lanes =
+        a->prof_on ? 1u : std::max(1u, std::min((uint32_t)a->lanes, npairs / 4));
+    auto enqueue = [&]() -> piv_status {
+        HIP_TRY(launch_zero_u32(sum_dev, (uint32_t)(np * npairs * 5), st));
+        if (lanes == 1) return chain(0, npairs, st);
+        // fork: every helper stream starts after the work already on `st`
+        HIP_TRY(hipEventRecord(a->ev_fork, st));
+        for (uint32_t l = 1; l < lanes; ++l) HIP_TRY(hipStreamWaitEvent(a->side[l - 1], a->ev_fork, 0));
+        for (uint32_t l = lanes; l-- > 0;) {
+            const uint32_t k0 = (uint32_t)((uint64_t)npairs * l / lanes);
+            const uint32_t k1 = (uint32_t)((uint64_t)npairs * (l + 1) / lanes);
+            PIV_TRY(chain(k0, k1 - k0, l ? a->side[l - 1] : st));
+        }
+        // join: `st` continues only after every helper stream's chain
+        for (uint32_t l = 1; l < lanes; ++l) {
+            HIP_TRY(hipEventRecord(a->ev_join[l - 1], a->side[l - 1]));
+            HIP_TRY(hipStreamWaitEvent(st, a->ev_join[l - 1], 0));
         }
         return PIV_OK;
     };
@@ -885,6 +957,18 @@ piv_status piv_analyzer_create(const piv_config *cfg, int device, piv_analyzer *
         for (uint32_t i = 1; i < cfg->final_iterations; ++i)
             a->passes.push_back((int)cfg->windows[cfg->num_windows - 1]);
         hipError_t e = hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking);
+        // batch lanes (PIV_BATCH_LANES = 1..4 overrides the default of 2): helper streams at
+        // the default priority and the events that fork them from and join them into `stream`
+        if (const char *env = std::getenv("PIV_BATCH_LANES")) {
+            const int v = std::atoi(env);
+            if (v >= 1 && v <= piv_analyzer::MAX_LANES) a->lanes = v;
+        }
+        if (e == hipSuccess && a->lanes > 1)
+            e = hipEventCreateWithFlags(&a->ev_fork, hipEventDisableTiming);
+        for (int i = 0; e == hipSuccess && i < a->lanes - 1; ++i) {
+            e = hipStreamCreateWithFlags(&a->side[i], hipStreamNonBlocking);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&a->ev_join[i], hipEventDisableTiming);
+        }
         if (e != hipSuccess) {
             delete a;
             return fail(PIV_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
@@ -1096,6 +1180,26 @@ piv_status piv_frame_download(const piv_frame *f, float *image, float *base)
     });
 }
 
+piv_status piv_frame_tables_built(const piv_frame *f, int *built)
+{
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!f || !built) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+        *built = 0;
+        if (!f->has_sat) return PIV_OK;
+        if (f->screen_t == 0) {
+            *built = 1;
+            return PIV_OK;
+        }
+        PIV_TRY(set_device(f->device));
+        // the state word is written by the preparation (or a re-screen) on the analyzer's stream
+        if (f->last_use && f->last_use->e) HIP_TRY(hipEventSynchronize(f->last_use->e));
+        uint32_t state = 0;
+        HIP_TRY(hipMemcpy(&state, sat_state_word(f->sat_ptr), sizeof state, hipMemcpyDeviceToHost));
+        *built = state != 0 ? 1 : 0;
+        return PIV_OK;
+    });
+}
+
 static piv_status analyze_impl(piv_analyzer *a, const piv_frame *fa, const piv_frame *fb,
                                piv_field_out *out, piv_pass_trace *traces, uint32_t num_traces)
 {
@@ -1295,6 +1399,8 @@ piv_status piv_correlate_windows(piv_analyzer *a, const piv_frame *fa, const piv
         float *o_dx = a->stage_out.as<float>(), *o_dy = o_dx + count, *o_ppr = o_dy + count;
         int32_t *o_pk = reinterpret_cast<int32_t *>(o_ppr + count);
         uint8_t *o_fl = reinterpret_cast<uint8_t *>(o_pk + count);
+        PIV_TRY(screen_for(a, &fa, 1, (int)window, a->stream));
+        PIV_TRY(screen_for(a, &fb, 1, (int)window, a->stream));
         PassPair d{};
         d.img_a = fa->img;
         d.img_b = fb->img;
@@ -1302,6 +1408,7 @@ piv_status piv_correlate_windows(piv_analyzer *a, const piv_frame *fa, const piv
         d.sat_a = fa->has_sat ? fa->sat_ptr : nullptr;
         d.sat_b = fb->has_sat ? fb->sat_ptr : nullptr;
         d.sat_cells = fa->cells;
+        d.screened = screened_bits(fa, fb);
         d.origins = d_org;
         d.origins_given = 1;
         d.sig = reinterpret_cast<uint8_t *>(din + o_sig);

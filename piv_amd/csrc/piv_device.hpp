@@ -67,7 +67,33 @@ struct PassPair {
     float min_mean, min_std;
     int32_t deform;           // window deformation from the predictor (passes with has_prev):
                               //   0 none (rigid offset), 1 bilinear, 2 bicubic resampling
+    int32_t screened;         // bit 0 (frame A), bit 1 (frame B): the frame's tables were
+                              //   screened; its state word (sat_state) says whether they
+                              //   were built, and unbuilt tables pass the (default) mask
 };
+
+// Screened mask tables (u16 frames without conditioning, masks with min_mean <= 0 and
+// min_std <= 0): the SAT_LEAD entries before entry (0, 0) of a frame's sums hold a u32 state
+// word (byte 0: 1 if the tables were built) and one flag byte per 32-row band (from byte
+// SCREEN_FLAGS: 1 if the band holds an aligned all-zero tile).
+constexpr int SCREEN_FLAGS = 8;
+constexpr int SCREEN_MAX_BANDS = SAT_LEAD * 8 - SCREEN_FLAGS;
+__host__ __device__ inline uint32_t *sat_state_word(double *sum)
+{
+    return reinterpret_cast<uint32_t *>(sum - SAT_LEAD);
+}
+__host__ __device__ inline const uint32_t *sat_state_word(const double *sum)
+{
+    return reinterpret_cast<const uint32_t *>(sum - SAT_LEAD);
+}
+__host__ __device__ inline uint8_t *screen_flags(double *sum)
+{
+    return reinterpret_cast<uint8_t *>(sum - SAT_LEAD) + SCREEN_FLAGS;
+}
+__host__ __device__ inline const uint8_t *screen_flags(const double *sum)
+{
+    return reinterpret_cast<const uint8_t *>(sum - SAT_LEAD) + SCREEN_FLAGS;
+}
 
 // One image pair's validation job (piv-core/src/validate.rs:51-130).
 struct ValidatePair {

@@ -184,6 +184,9 @@ struct piv_frame {
     DevBuf sat;                // owned f64 SAT of base (sum, then sum of squares)
     int64_t cells = 0;
     bool has_sat = false;
+    // > 0: the tables were screened with this tile size (see screen_tile); windows of size n
+    // may rely on the screen when screen_t <= n / 2 (re-screened with a smaller tile otherwise)
+    mutable int screen_t = 0;
     float *base_ptr = nullptr;        // background-subtracted base: own `base` or batch storage
     const double *sat_ptr = nullptr;  // SAT (sum, then sum of squares): own `sat` or batch storage
     std::shared_ptr<DevBuf> batch;    // storage shared by the frames of one batch preparation
@@ -223,6 +226,14 @@ struct piv_analyzer {
     piv_config cfg{};
     int device = 0;
     hipStream_t stream = nullptr;
+    // A batch's pairs are independent: run_batch splits them into `lanes` groups whose pass
+    // chains run side by side, on `stream` and on the helper streams, forked from and joined
+    // back into `stream` by events (so one group's short per-vector kernels overlap another
+    // group's correlation).  Callers only ever see `stream`.
+    static constexpr int MAX_LANES = 4;
+    int lanes = 2;
+    hipStream_t side[MAX_LANES - 1] = {};
+    hipEvent_t ev_fork = nullptr, ev_join[MAX_LANES - 1] = {};
     std::mutex mu;
     std::vector<int> passes;
     std::vector<float> taps;
@@ -284,6 +295,11 @@ struct piv_analyzer {
         post_keys.release();
         post_out.release();
         d_taps.release();
+        for (int i = 0; i < MAX_LANES - 1; ++i) {
+            if (side[i]) (void)hipStreamDestroy(side[i]);
+            if (ev_join[i]) (void)hipEventDestroy(ev_join[i]);
+        }
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -304,6 +320,18 @@ int mf_precision(const piv_analyzer *a, int n);
 piv_status check_config(const piv_config &c);
 // whether the signal mask can change a result (else its tables are not built)
 bool mask_needed(const piv_config &c, int in_fmt, bool has_bg);
+// Tile size with which an unconditioned frame's tables may be screened for the analyzer's
+// passes (the largest power of two <= n / 2 for its smallest window n), or 0 where the tables
+// must be built unconditionally.
+int screen_tile(const piv_analyzer *a, int in_fmt, bool cond, int H, int W);
+// Re-screens frames whose screen tile is too large for windows of size n.
+piv_status screen_for(piv_analyzer *a, const piv_frame *const *fs, size_t count, int n,
+                      hipStream_t st);
+// PassPair::screened bits of a pair of frames
+inline int32_t screened_bits(const piv_frame *a, const piv_frame *b)
+{
+    return (a->has_sat && a->screen_t > 0 ? 1 : 0) | (b->has_sat && b->screen_t > 0 ? 2 : 0);
+}
 // device address of this GPU's workspace for piv_analyze_batch_noalloc, taken once per device
 // (device `dev` current); later calls return the stored address or failure
 piv_status noalloc_init(int dev);

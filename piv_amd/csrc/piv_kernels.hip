@@ -440,19 +440,87 @@ __device__ __forceinline__ void sat_carry_frame(const void *__restrict__ img, in
     }
 }
 
+// ---- screened tables --------------------------------------------------------------------
+// With the default mask (min_mean <= 0, min_std <= 0) a u16 frame's tables only decide windows
+// whose n x n box is all zero: any other box sums to at least 1/65535, above the tables'
+// rounding error on frames up to about 2580^2 (the host checks the bound), so its mean is
+// positive and its standard deviation never negative.  Every n x n box contains an aligned
+// t x t tile for t <= n / 2, so a frame without an all-zero aligned tile needs no tables.
+// k_sat_screen flags each 32-row band holding such a tile (full tiles only); the table
+// kernels then build the tables only for flagged frames, and record whether they did in the
+// frame's state word for the placement kernel.
+
+// one 32-row band of a frame: flag = 1 if it holds an aligned all-zero t x t tile (t | 32)
+template <int FMT>
+__device__ __forceinline__ void screen_band(const void *__restrict__ img, int pitch, int H, int W,
+                                            int t, uint8_t *__restrict__ flag)
+{
+    __shared__ int s_zero;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) s_zero = 0;
+    __syncthreads();
+    const int y0 = blockIdx.x * SAT_BAND;
+    const int ntr = max(0, min(SAT_BAND, H - y0)) / t;  // full tile rows of the band
+    const unsigned long long gmask = t >= 64 ? ~0ull : ((1ull << t) - 1);
+    bool zero = false;
+    for (int cb = wave * 64; cb < W; cb += 256) {  // 64-column chunks (tiles never straddle)
+        const int x = cb + lane;
+        const bool inx = x < W;
+        const long xc = min(x, W - 1);
+        for (int j = 0; j < ntr; ++j) {
+            uint32_t acc = 0;
+            for (int r = 0; r < t; ++r) acc |= (uint32_t)load_raw<FMT>(img, (long)(y0 + j * t + r) * pitch + xc);
+            // columns past the frame count as non-zero: tiles crossing the edge are not full
+            const unsigned long long nz = __ballot(acc != 0 || !inx);
+            for (int g = 0; g < 64; g += t) zero = zero || ((nz >> g) & gmask) == 0;
+        }
+    }
+    if (zero) s_zero = 1;
+    __syncthreads();
+    if (tid == 0) *flag = (uint8_t)s_zero;
+}
+
+template <int FMT>
+__global__ __launch_bounds__(256) void k_sat_screen(const void *__restrict__ img, int pitch,
+                                                    long frame_bytes, int H, int W, int t,
+                                                    double *__restrict__ sum0, long sat_stride)
+{
+    screen_band<FMT>(static_cast<const char *>(img) + (long)blockIdx.y * frame_bytes, pitch, H, W, t,
+                     screen_flags(sum0 + (long)blockIdx.y * sat_stride) + blockIdx.x);
+}
+
+template <int FMT>
+__global__ __launch_bounds__(256) void k_sat_screen_list(SatList list, int pitch, int H, int W, int t)
+{
+    screen_band<FMT>(list.img[blockIdx.y], pitch, H, W, t, screen_flags(list.sum[blockIdx.y]) + blockIdx.x);
+}
+
+// 64-lane table blocks: does the screen require this frame's tables?
+__device__ __forceinline__ bool screen_needs(const double *sum, int H)
+{
+    const int nbands = (H + SAT_BAND - 1) / SAT_BAND, l = threadIdx.x;
+    const uint8_t *f = screen_flags(sum);
+    const bool any = (l < nbands && f[l] != 0) || (l + 64 < nbands && f[l + 64] != 0);
+    return __ballot(any) != 0;
+}
+
 template <int FMT>
 __global__ __launch_bounds__(64) void k_sat_carry(const void *__restrict__ img, int pitch,
                                                   long frame_bytes, int H, int W,
-                                                  double *__restrict__ carry, int nb)
+                                                  double *__restrict__ carry, int nb,
+                                                  const double *__restrict__ sum0, long sat_stride,
+                                                  int screened)
 {
+    if (screened && !screen_needs(sum0 + (long)blockIdx.y * sat_stride, H)) return;
     sat_carry_frame<FMT>(static_cast<const char *>(img) + (long)blockIdx.y * frame_bytes, pitch, H,
                          W, carry + (long)blockIdx.y * nb * 2 * H, nb);
 }
 
 template <int FMT>
 __global__ __launch_bounds__(64) void k_sat_carry_list(SatList list, int pitch, int H, int W,
-                                                       double *__restrict__ carry, int nb)
+                                                       double *__restrict__ carry, int nb, int screened)
 {
+    if (screened && !screen_needs(list.sum[blockIdx.y], H)) return;
     sat_carry_frame<FMT>(list.img[blockIdx.y], pitch, H, W, carry + (long)blockIdx.y * nb * 2 * H, nb);
 }
 
@@ -541,24 +609,37 @@ __device__ __forceinline__ void sat_strips_frame(const void *__restrict__ img, i
     }
 }
 
+// screened frames: every strip block records in the state word whether the tables are built
+// (the same value from every block)
+__device__ __forceinline__ bool screen_strips(double *sum, int H)
+{
+    const bool need = screen_needs(sum, H);
+    if (threadIdx.x == 0) *sat_state_word(sum) = need ? 1u : 0u;
+    return need;
+}
+
 template <int FMT>
 __global__ __launch_bounds__(64) void k_sat_strips(const void *__restrict__ img, int pitch,
                                                    long frame_bytes, int H, int W,
                                                    const double *__restrict__ carry, int nb,
                                                    double *__restrict__ sum,
-                                                   double *__restrict__ sumsq, long sat_stride)
+                                                   double *__restrict__ sumsq, long sat_stride,
+                                                   int screened)
 {
     const long y = blockIdx.y;
+    if (screened && !screen_strips(sum + y * sat_stride, H)) return;
     sat_strips_frame<FMT>(static_cast<const char *>(img) + y * frame_bytes, pitch, H, W,
                           carry + y * nb * 2 * H, nb, sum + y * sat_stride, sumsq + y * sat_stride);
 }
 
 template <int FMT>
 __global__ __launch_bounds__(64) void k_sat_strips_list(SatList list, int pitch, int H, int W,
-                                                        const double *__restrict__ carry, int nb)
+                                                        const double *__restrict__ carry, int nb,
+                                                        int screened)
 {
     const long y = blockIdx.y;
     double *sum = list.sum[y];
+    if (screened && !screen_strips(sum, H)) return;
     sat_strips_frame<FMT>(list.img[y], pitch, H, W, carry + y * nb * 2 * H, nb, sum,
                           sum + sat_table_elems(H, W));
 }
@@ -715,12 +796,17 @@ __global__ __launch_bounds__(WBLOCK) void k_place_pass(const PassPair *__restric
     bool sig = true;
     if (pp.use_mask) {
         float m, sd;
-        sat_stats(pp.sat_a, pp.sat_a + pp.sat_cells, pp.W, (int)roundf(o.x), (int)roundf(o.y), n,
-                  n, m, sd);
-        if (!(m >= pp.min_mean && sd >= pp.min_std)) sig = false;
-        sat_stats(pp.sat_b, pp.sat_b + pp.sat_cells, pp.W, (int)roundf(o.z), (int)roundf(o.w), n,
-                  n, m, sd);
-        if (!(m >= pp.min_mean && sd >= pp.min_std)) sig = false;
+        // a screened frame whose tables were not built has no all-zero box: it passes
+        if (!(pp.screened & 1) || *sat_state_word(pp.sat_a) != 0) {
+            sat_stats(pp.sat_a, pp.sat_a + pp.sat_cells, pp.W, (int)roundf(o.x), (int)roundf(o.y),
+                      n, n, m, sd);
+            if (!(m >= pp.min_mean && sd >= pp.min_std)) sig = false;
+        }
+        if (!(pp.screened & 2) || *sat_state_word(pp.sat_b) != 0) {
+            sat_stats(pp.sat_b, pp.sat_b + pp.sat_cells, pp.W, (int)roundf(o.z), (int)roundf(o.w),
+                      n, n, m, sd);
+            if (!(m >= pp.min_mean && sd >= pp.min_std)) sig = false;
+        }
     }
     pp.sig[k] = sig ? 1 : 0;
 }
@@ -934,28 +1020,38 @@ size_t sat_carry_bytes(int H, int W, int nframes)
 
 hipError_t launch_sat_raw(int fmt, const void *raw, int pitch, long raw_frame_bytes, int H, int W,
                           double *sum, double *sq, double *carry, hipStream_t st, int nframes,
-                          long sat_stride)
+                          long sat_stride, int screen_t)
 {
     const int nb = sat_boundaries(W);
+    const int scr = screen_t > 0 ? 1 : 0;
+    if (scr && (fmt != FMT_U16 || screen_t > SAT_BAND || SAT_BAND % screen_t != 0 ||
+                nblk(H, SAT_BAND) > (unsigned)SCREEN_MAX_BANDS))
+        return hipErrorInvalidValue;
     for (int f0 = 0; f0 < nframes; f0 += MAX_GRID_Y) {
         const unsigned nf = (unsigned)min(MAX_GRID_Y, nframes - f0);
         const void *r = static_cast<const char *>(raw) + (long)f0 * raw_frame_bytes;
         double *su = sum + (long)f0 * sat_stride, *sqq = sq + (long)f0 * sat_stride;
+        if (scr) {
+            hipLaunchKernelGGL(k_sat_screen<FMT_U16>, dim3(nblk(H, SAT_BAND), nf), dim3(256), 0, st, r,
+                               pitch, raw_frame_bytes, H, W, screen_t, su, sat_stride);
+            const hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+        }
         if (nb > 0) {
             const dim3 grid(nblk(H, SAT_BAND), nf);
             switch (fmt) {
-            case FMT_U8: hipLaunchKernelGGL(k_sat_carry<FMT_U8>, grid, dim3(64), 0, st, r, pitch, raw_frame_bytes, H, W, carry, nb); break;
-            case FMT_U16: hipLaunchKernelGGL(k_sat_carry<FMT_U16>, grid, dim3(64), 0, st, r, pitch, raw_frame_bytes, H, W, carry, nb); break;
-            default: hipLaunchKernelGGL(k_sat_carry<FMT_F32>, grid, dim3(64), 0, st, r, pitch, raw_frame_bytes, H, W, carry, nb); break;
+            case FMT_U8: hipLaunchKernelGGL(k_sat_carry<FMT_U8>, grid, dim3(64), 0, st, r, pitch, raw_frame_bytes, H, W, carry, nb, su, sat_stride, scr); break;
+            case FMT_U16: hipLaunchKernelGGL(k_sat_carry<FMT_U16>, grid, dim3(64), 0, st, r, pitch, raw_frame_bytes, H, W, carry, nb, su, sat_stride, scr); break;
+            default: hipLaunchKernelGGL(k_sat_carry<FMT_F32>, grid, dim3(64), 0, st, r, pitch, raw_frame_bytes, H, W, carry, nb, su, sat_stride, scr); break;
             }
             const hipError_t e = hipGetLastError();
             if (e != hipSuccess) return e;
         }
         const dim3 grid(nblk(W, SAT_SW), nf);
         switch (fmt) {
-        case FMT_U8: hipLaunchKernelGGL(k_sat_strips<FMT_U8>, grid, dim3(64), 0, st, r, pitch, raw_frame_bytes, H, W, carry, nb, su, sqq, sat_stride); break;
-        case FMT_U16: hipLaunchKernelGGL(k_sat_strips<FMT_U16>, grid, dim3(64), 0, st, r, pitch, raw_frame_bytes, H, W, carry, nb, su, sqq, sat_stride); break;
-        default: hipLaunchKernelGGL(k_sat_strips<FMT_F32>, grid, dim3(64), 0, st, r, pitch, raw_frame_bytes, H, W, carry, nb, su, sqq, sat_stride); break;
+        case FMT_U8: hipLaunchKernelGGL(k_sat_strips<FMT_U8>, grid, dim3(64), 0, st, r, pitch, raw_frame_bytes, H, W, carry, nb, su, sqq, sat_stride, scr); break;
+        case FMT_U16: hipLaunchKernelGGL(k_sat_strips<FMT_U16>, grid, dim3(64), 0, st, r, pitch, raw_frame_bytes, H, W, carry, nb, su, sqq, sat_stride, scr); break;
+        default: hipLaunchKernelGGL(k_sat_strips<FMT_F32>, grid, dim3(64), 0, st, r, pitch, raw_frame_bytes, H, W, carry, nb, su, sqq, sat_stride, scr); break;
         }
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
@@ -964,9 +1060,13 @@ hipError_t launch_sat_raw(int fmt, const void *raw, int pitch, long raw_frame_by
 }
 
 hipError_t launch_sat_list(int fmt, const void *const *imgs, double *const *sums, int nframes,
-                           int pitch, int H, int W, double *carry, hipStream_t st)
+                           int pitch, int H, int W, double *carry, hipStream_t st, int screen_t)
 {
     const int nb = sat_boundaries(W);
+    const int scr = screen_t > 0 ? 1 : 0;
+    if (scr && (fmt != FMT_U16 || screen_t > SAT_BAND || SAT_BAND % screen_t != 0 ||
+                nblk(H, SAT_BAND) > (unsigned)SCREEN_MAX_BANDS))
+        return hipErrorInvalidValue;
     for (int f0 = 0; f0 < nframes; f0 += SAT_LIST_MAX) {
         const int nf = min(SAT_LIST_MAX, nframes - f0);
         SatList list{};
@@ -974,21 +1074,27 @@ hipError_t launch_sat_list(int fmt, const void *const *imgs, double *const *sums
             list.img[i] = imgs[f0 + i];
             list.sum[i] = sums[f0 + i];
         }
+        if (scr) {
+            hipLaunchKernelGGL(k_sat_screen_list<FMT_U16>, dim3(nblk(H, SAT_BAND), (unsigned)nf), dim3(256), 0,
+                               st, list, pitch, H, W, screen_t);
+            const hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+        }
         if (nb > 0) {
             const dim3 grid(nblk(H, SAT_BAND), (unsigned)nf);
             switch (fmt) {
-            case FMT_U8: hipLaunchKernelGGL(k_sat_carry_list<FMT_U8>, grid, dim3(64), 0, st, list, pitch, H, W, carry, nb); break;
-            case FMT_U16: hipLaunchKernelGGL(k_sat_carry_list<FMT_U16>, grid, dim3(64), 0, st, list, pitch, H, W, carry, nb); break;
-            default: hipLaunchKernelGGL(k_sat_carry_list<FMT_F32>, grid, dim3(64), 0, st, list, pitch, H, W, carry, nb); break;
+            case FMT_U8: hipLaunchKernelGGL(k_sat_carry_list<FMT_U8>, grid, dim3(64), 0, st, list, pitch, H, W, carry, nb, scr); break;
+            case FMT_U16: hipLaunchKernelGGL(k_sat_carry_list<FMT_U16>, grid, dim3(64), 0, st, list, pitch, H, W, carry, nb, scr); break;
+            default: hipLaunchKernelGGL(k_sat_carry_list<FMT_F32>, grid, dim3(64), 0, st, list, pitch, H, W, carry, nb, scr); break;
             }
             const hipError_t e = hipGetLastError();
             if (e != hipSuccess) return e;
         }
         const dim3 grid(nblk(W, SAT_SW), (unsigned)nf);
         switch (fmt) {
-        case FMT_U8: hipLaunchKernelGGL(k_sat_strips_list<FMT_U8>, grid, dim3(64), 0, st, list, pitch, H, W, carry, nb); break;
-        case FMT_U16: hipLaunchKernelGGL(k_sat_strips_list<FMT_U16>, grid, dim3(64), 0, st, list, pitch, H, W, carry, nb); break;
-        default: hipLaunchKernelGGL(k_sat_strips_list<FMT_F32>, grid, dim3(64), 0, st, list, pitch, H, W, carry, nb); break;
+        case FMT_U8: hipLaunchKernelGGL(k_sat_strips_list<FMT_U8>, grid, dim3(64), 0, st, list, pitch, H, W, carry, nb, scr); break;
+        case FMT_U16: hipLaunchKernelGGL(k_sat_strips_list<FMT_U16>, grid, dim3(64), 0, st, list, pitch, H, W, carry, nb, scr); break;
+        default: hipLaunchKernelGGL(k_sat_strips_list<FMT_F32>, grid, dim3(64), 0, st, list, pitch, H, W, carry, nb, scr); break;
         }
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;

@@ -51,13 +51,16 @@ hipError_t launch_sat(const float *img, int H, int W, double *sum, double *sq, d
 // apart: the same additions in the same order as launch_sat on the normalised image
 hipError_t launch_sat_raw(int fmt, const void *raw, int pitch, long raw_frame_bytes, int H, int W,
                           double *sum, double *sq, double *carry, hipStream_t st, int nframes = 1,
-                          long sat_stride = 0);
+                          long sat_stride = 0, int screen_t = 0);
+// screen_t > 0 (u16 frames, default mask): screen the frames for aligned all-zero
+// screen_t x screen_t tiles first (screen_t divides 32) and build only the tables of frames
+// holding one, recording in each frame's state word whether its tables were built
 size_t sat_carry_bytes(int H, int W, int nframes);
 // SAT straight from the samples of frames at their own addresses (imgs[i], tables with entry
 // (0, 0) of the sums at sums[i], squares sat_table_elems(H, W) later), up to 64 frames per
 // launch; `carry` is scratch of sat_carry_bytes(H, W, min(nframes, 64))
 hipError_t launch_sat_list(int fmt, const void *const *imgs, double *const *sums, int nframes,
-                           int pitch, int H, int W, double *carry, hipStream_t st);
+                           int pitch, int H, int W, double *carry, hipStream_t st, int screen_t = 0);
 // nframes > 1: a batch of images H*W apart (tables sat_stride doubles apart)
 hipError_t launch_blur(const float *in, int H, int W, const float *taps, int r, float *tmp,
                        float *out, hipStream_t st, int nframes = 1);

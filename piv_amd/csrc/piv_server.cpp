@@ -431,6 +431,9 @@ void build_batch(Lane &L, BatchPlan &bp)
                 d.sat_a = A->has_sat ? A->sat_ptr : nullptr;
                 d.sat_b = Bf->has_sat ? Bf->sat_ptr : nullptr;
                 d.sat_cells = A->cells;
+                // the stream's own analyzer prepared both frames for these passes, so their
+                // screen tiles fit every window of the plan
+                d.screened = screened_bits(A, Bf);
                 if (si > 0) {
                     const size_t pc = st->plans[si - 1].cells;
                     d.prev_dx = sl->ws[si - 1].val;

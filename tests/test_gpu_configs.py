@@ -180,20 +180,22 @@ def test_background_and_cli_mask(piv, oracle):
     full_parity(oracle, res, pa, pb, cfg)
 
 
-def test_batch_equals_single_bitwise(piv):
+@pytest.mark.parametrize("npairs", [3, 9])
+def test_batch_equals_single_bitwise(piv, npairs):
     """The batched path (one launch per pass for several pairs, device outputs) gives every
-    pair exactly what a single-pair call gives (multipass.rs: pairs are independent)."""
+    pair exactly what a single-pair call gives (multipass.rs: pairs are independent).  Nine
+    pairs run as two groups on two streams."""
     cfg = piv.PivConfig(windows=[64, 32, 16], final_iterations=1)
     an = piv.PivAnalyzer(cfg, piv.PreprocessConfig.raw())
-    pairs = [S.particle_frames(256, 320, 300 + i, field=S.uniform_field(1.0 + i, -0.5))
-             for i in range(3)]
+    pairs = [S.particle_frames(256, 320, 300 + i, field=S.uniform_field(1.0 + 0.4 * i, -0.5))
+             for i in range(npairs)]
     A = piv.DeviceArray.from_numpy(np.stack([p[0] for p in pairs]))
     B = piv.DeviceArray.from_numpy(np.stack([p[1] for p in pairs]))
-    fa = [piv.PreparedFrame.from_device(an, A.slice(i)) for i in range(3)]
-    fb = [piv.PreparedFrame.from_device(an, B.slice(i)) for i in range(3)]
+    fa = [piv.PreparedFrame.from_device(an, A.slice(i)) for i in range(npairs)]
+    fb = [piv.PreparedFrame.from_device(an, B.slice(i)) for i in range(npairs)]
     g = an.output_grid(256, 320)
-    dx, dy, pr = (piv.DeviceArray((3, g.ny, g.nx), np.float32) for _ in range(3))
-    fl = piv.DeviceArray((3, g.ny, g.nx), np.uint8)
+    dx, dy, pr = (piv.DeviceArray((npairs, g.ny, g.nx), np.float32) for _ in range(3))
+    fl = piv.DeviceArray((npairs, g.ny, g.nx), np.uint8)
     sums = an.analyze_batch_device(fa, fb, dx, dy, pr, fl)
     hdx, hdy, hfl = dx.numpy(), dy.numpy(), fl.numpy()
     for i, (a, b) in enumerate(pairs):
@@ -202,6 +204,45 @@ def test_batch_equals_single_bitwise(piv):
         assert np.array_equal(bits(r.field.dy), bits(hdy[i]))
         assert np.array_equal(r.field.flags, hfl[i])
         assert r.summary == sums[i]
+
+
+@pytest.mark.parametrize("lanes", ["1", "3", "4"])
+def test_batch_lanes_bitwise(piv, monkeypatch, lanes):
+    """A batch split over 1, 3 or 4 stream lanes (groups of pairs whose pass chains run side by
+    side) gives bitwise the default two-lane result, eager and replayed, with uneven groups."""
+    cfg = piv.PivConfig(windows=[48, 24], final_iterations=1)
+    h, w, n = 200, 240, 17
+    pairs = [S.particle_frames(h, w, 700 + i, field=S.uniform_field(-1.5 + 0.2 * i, 0.6),
+                               bits=12, container=np.uint16, amp_scale=16.0) for i in range(n)]
+    A = np.stack([p[0] for p in pairs])
+    B = np.stack([p[1] for p in pairs])
+
+    def run(env):
+        if env is None:
+            monkeypatch.delenv("PIV_BATCH_LANES", raising=False)
+        else:
+            monkeypatch.setenv("PIV_BATCH_LANES", env)
+        an = piv.PivAnalyzer(cfg, piv.PreprocessConfig.raw())
+        dA, dB = piv.DeviceArray.from_numpy(A), piv.DeviceArray.from_numpy(B)
+        fa = [piv.PreparedFrame.from_device(an, dA.slice(i)) for i in range(n)]
+        fb = [piv.PreparedFrame.from_device(an, dB.slice(i)) for i in range(n)]
+        g = an.output_grid(h, w)
+        dx, dy, pr = (piv.DeviceArray((n, g.ny, g.nx), np.float32) for _ in range(3))
+        fl = piv.DeviceArray((n, g.ny, g.nx), np.uint8)
+        outs = []
+        for _ in range(3):  # eager, captured, replayed
+            for d in (dx, dy, pr):
+                d.upload(np.full(d.shape, np.nan, np.float32))
+            sums = an.analyze_batch_device(fa, fb, dx, dy, pr, fl)
+            outs.append((bits(dx.numpy()), bits(dy.numpy()), bits(pr.numpy()), fl.numpy(), sums))
+        return outs
+
+    ref = run(None)[0]
+    for k, o in enumerate(run(lanes)):
+        for x, y in zip(o[:4], ref[:4]):
+            assert np.array_equal(x, y), f"lanes {lanes}, call {k}"
+        assert o[4] == ref[4]
+    monkeypatch.delenv("PIV_BATCH_LANES", raising=False)
 
 
 def test_batch_replay_matches_eager(piv):
