@@ -450,6 +450,48 @@ __device__ __forceinline__ void sat_carry_frame(const void *__restrict__ img, in
 // kernels then build the tables only for flagged frames, and record whether they did in the
 // frame's state word for the placement kernel.
 
+// u16 frames whose rows are whole 16 B lines (W and the pitch multiples of 8 samples, the base
+// 16 B aligned): each lane ORs 8 samples per load over a tile row's T rows; a tile of T = 4
+// is half a lane's samples, of T >= 8 the samples of T / 8 neighbouring lanes.
+template <int T>
+__device__ __forceinline__ bool screen_band_wide(const uint16_t *__restrict__ img, int pitch, int W,
+                                                 int y0, int ntr, int lane, int wave)
+{
+    const int ng = W / 8;  // 8-sample groups per row
+    bool zero = false;
+    for (int gb = wave * 64; gb < ng; gb += 256) {  // 64 groups per wave (tiles never straddle)
+        const int gi = gb + lane;
+        const bool ing = gi < ng;
+        const long gc = (long)min(gi, ng - 1) * 8;
+        for (int j = 0; j < ntr; ++j) {
+            uint32_t a0 = 0, a1 = 0;  // samples 0-3 and 4-7 of the lane's group
+#pragma unroll
+            for (int r0 = 0; r0 < T; r0 += 8) {
+                uint4 v[T < 8 ? T : 8];
+#pragma unroll
+                for (int r = 0; r < (T < 8 ? T : 8); ++r)
+                    v[r] = *reinterpret_cast<const uint4 *>(img + (long)(y0 + j * T + r0 + r) * pitch + gc);
+#pragma unroll
+                for (int r = 0; r < (T < 8 ? T : 8); ++r) {
+                    a0 |= v[r].x | v[r].y;
+                    a1 |= v[r].z | v[r].w;
+                }
+            }
+            if constexpr (T == 4) {
+                zero = zero || __ballot(ing && (a0 == 0 || a1 == 0)) != 0;
+            } else {
+                // lanes whose 8 samples are all zero; groups past the frame count as non-zero
+                const unsigned long long z = __ballot(ing && (a0 | a1) == 0);
+                unsigned long long m = z;
+                if constexpr (T >= 16) m &= (z >> 1) & 0x5555555555555555ull;
+                if constexpr (T >= 32) m &= (z >> 2) & (z >> 3) & 0x1111111111111111ull;
+                zero = zero || m != 0;
+            }
+        }
+    }
+    return zero;
+}
+
 // one 32-row band of a frame: flag = 1 if it holds an aligned all-zero t x t tile (t | 32)
 template <int FMT>
 __device__ __forceinline__ void screen_band(const void *__restrict__ img, int pitch, int H, int W,
@@ -463,6 +505,20 @@ __device__ __forceinline__ void screen_band(const void *__restrict__ img, int pi
     const int ntr = max(0, min(SAT_BAND, H - y0)) / t;  // full tile rows of the band
     const unsigned long long gmask = t >= 64 ? ~0ull : ((1ull << t) - 1);
     bool zero = false;
+    if (FMT == FMT_U16 && W % 8 == 0 && pitch % 8 == 0 &&
+        (reinterpret_cast<uintptr_t>(img) & 15) == 0) {
+        const uint16_t *p = static_cast<const uint16_t *>(img);
+        switch (t) {
+        case 4: zero = screen_band_wide<4>(p, pitch, W, y0, ntr, lane, wave); break;
+        case 8: zero = screen_band_wide<8>(p, pitch, W, y0, ntr, lane, wave); break;
+        case 16: zero = screen_band_wide<16>(p, pitch, W, y0, ntr, lane, wave); break;
+        default: zero = screen_band_wide<32>(p, pitch, W, y0, ntr, lane, wave); break;
+        }
+        if (zero) s_zero = 1;
+        __syncthreads();
+        if (tid == 0) *flag = (uint8_t)s_zero;
+        return;
+    }
     for (int cb = wave * 64; cb < W; cb += 256) {  // 64-column chunks (tiles never straddle)
         const int x = cb + lane;
         const bool inx = x < W;

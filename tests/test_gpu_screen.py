@@ -102,6 +102,29 @@ def test_screen_decides_which_frames_get_tables(piv, monkeypatch):
     assert an.prepare(a).tables_built()
 
 
+@pytest.mark.parametrize("windows,t", [([16, 8], 4), ([24, 12], 4), ([48, 24], 8), ([32], 16),
+                                       ([64], 32), ([128, 64], 32)])
+@pytest.mark.parametrize("w", [512, 520, 517])
+def test_screen_tile_sizes(piv, windows, t, w):
+    """Tile t (the largest power of two <= n / 2 of the smallest window n) on frames whose
+    width is a multiple of 8 samples (16 B loads) or not: an aligned all-zero t x t tile gets
+    tables, the same patch moved by one pixel (no aligned t x t tile inside it) does not, at
+    the frame's last full tile column too."""
+    an = piv.PivAnalyzer(piv.PivConfig(windows=windows, final_iterations=1),
+                         piv.PreprocessConfig.raw())
+    h = 136
+    a, _ = _pair(h, w, 7)
+    assert not an.prepare(a).tables_built()
+    last = (w // t - 1) * t
+    for x0 in (t, 3 * t, last):
+        z = a.copy()
+        z[2 * t:3 * t, x0:x0 + t] = 0
+        assert an.prepare(z).tables_built(), (x0, "aligned")
+        z = a.copy()
+        z[2 * t + 1:3 * t + 1, x0 + 1:x0 + t + 1] = 0
+        assert not an.prepare(z).tables_built(), (x0, "shifted")
+
+
 @pytest.mark.parametrize("windows", [[32, 16], [48, 24], [64, 32, 16]])
 def test_blank_regions_against_oracle(piv, oracle, monkeypatch, windows):
     """Frames with all-zero regions: every pass replayed through the oracle (integer peaks,
