@@ -225,27 +225,65 @@ static piv_status prepare_frame_impl(piv_analyzer *a, piv_frame *f, const float 
     const piv_config &c = a->cfg;
     const int H = f->H, W = f->W;
     const size_t n = (size_t)H * W;
-    const bool cond = c.smooth_sigma > 0.0f || c.normalize_radius > 0 || bg_dev != nullptr;
+    const bool blur = c.smooth_sigma > 0.0f, norm = c.normalize_radius > 0;
+    const bool cond = blur || norm || bg_dev != nullptr;
     const bool mask = mask_needed(c, f->in_fmt, bg_dev != nullptr);
+    // the blur reads the samples itself (base and blur in one pass)
+    const bool fused = blur && blur_fusable(a->taps_r);
+    // The base image (background-subtracted samples) is materialised where something reads
+    // it: with a background (mask tables, downloads, the image itself when nothing else is
+    // configured) or for a conditioning step that reads an image.  Without a background and
+    // with the fused blur the tables read the samples, as without conditioning.
+    const bool mat_base = cond && (bg_dev != nullptr || !fused);
     f->has_bg = bg_dev != nullptr;
     f->cells = sat_table_elems(H, W);
     hipStream_t st = a->stream;
     f->base_ptr = nullptr;
-    if (cond) {
+    if (mat_base) {
         PIV_TRY(frame_reserve(a, f->base, n * sizeof(float)));
         f->base_ptr = f->base.as<float>();
-        HIP_TRY(launch_base(f->in_fmt, f->raw_ptr, f->raw_pitch, bg_dev, H, W, f->base_ptr, st));
     }
+    // ---- base image and blur ----
+    const float *cur = f->base_ptr;
+    float *image = nullptr;
+    if (blur || norm) {
+        PIV_TRY(frame_reserve(a, f->image, n * sizeof(float)));
+        image = f->image.as<float>();
+    }
+    if (fused) {
+        float *dst = image;
+        if (norm) {
+            PIV_TRY(a->prep_tmp2.reserve(n * sizeof(float)));
+            dst = a->prep_tmp2.as<float>();
+        }
+        HIP_TRY(launch_blur_fused(f->in_fmt, f->raw_ptr, f->raw_pitch, bg_dev, H, W,
+                                  a->d_taps.as<float>(), a->taps_r, f->base_ptr, dst, st));
+        cur = dst;
+    } else if (mat_base) {
+        HIP_TRY(launch_base(f->in_fmt, f->raw_ptr, f->raw_pitch, bg_dev, H, W, f->base_ptr, st));
+        if (blur) {
+            float *dst = image;
+            if (norm) {
+                PIV_TRY(a->prep_tmp2.reserve(n * sizeof(float)));
+                dst = a->prep_tmp2.as<float>();
+            }
+            PIV_TRY(a->prep_tmp.reserve(n * sizeof(float)));
+            HIP_TRY(launch_blur(cur, H, W, a->d_taps.as<float>(), a->taps_r, a->prep_tmp.as<float>(),
+                                dst, st));
+            cur = dst;
+        }
+    }
+    // ---- mask tables of the base image ----
     if (mask) {
         PIV_TRY(frame_reserve(a, f->sat, (size_t)f->cells * 2 * sizeof(double)));
         double *sum = f->sat.as<double>() + SAT_LEAD, *sq = sum + f->cells;
         f->sat_ptr = sum;
-        // without conditioning the base image is the normalised raw frame: the table is summed
-        // from the samples and no base image is kept (piv_frame_download recomputes it)
+        // without a materialised base image the base is the normalised raw frame: the table is
+        // summed from the samples (piv_frame_download recomputes the base)
         PIV_TRY(a->sat_carry.reserve(sat_carry_bytes(H, W, 1)));
         double *carry = a->sat_carry.as<double>();
         f->screen_t = cond ? 0 : screen_tile(a, f->in_fmt, false, H, W);
-        if (cond) HIP_TRY(launch_sat(f->base_ptr, H, W, sum, sq, carry, st));
+        if (mat_base) HIP_TRY(launch_sat(f->base_ptr, H, W, sum, sq, carry, st));
         else HIP_TRY(launch_sat_raw(f->in_fmt, f->raw_ptr, f->raw_pitch, 0, H, W, sum, sq, carry, st,
                                     1, 0, f->screen_t));
         f->has_sat = true;
@@ -256,31 +294,18 @@ static piv_status prepare_frame_impl(piv_analyzer *a, piv_frame *f, const float 
         f->pitch = f->raw_pitch;
         return PIV_OK;
     }
-    PIV_TRY(frame_reserve(a, f->image, n * sizeof(float)));
-    const float *cur = f->base_ptr;
-    if (c.smooth_sigma > 0.0f) {
-        PIV_TRY(a->prep_tmp.reserve(n * sizeof(float)));
-        float *dst = c.normalize_radius > 0 ? nullptr : f->image.as<float>();
-        if (!dst) {
-            PIV_TRY(a->prep_tmp2.reserve(n * sizeof(float)));
-            dst = a->prep_tmp2.as<float>();
-        }
-        HIP_TRY(launch_blur(cur, H, W, a->d_taps.as<float>(), a->taps_r, a->prep_tmp.as<float>(),
-                            dst, st));
-        cur = dst;
-    }
-    if (c.normalize_radius > 0) {
+    // ---- local normalisation ----
+    if (norm) {
         PIV_TRY(a->prep_sat.reserve((size_t)f->cells * 2 * sizeof(double)));
         double *s = a->prep_sat.as<double>() + SAT_LEAD;
         PIV_TRY(a->sat_carry.reserve(sat_carry_bytes(H, W, 1)));
         HIP_TRY(launch_sat(cur, H, W, s, s + f->cells, a->sat_carry.as<double>(), st));
         HIP_TRY(launch_local_norm(cur, H, W, (int)c.normalize_radius, c.normalize_eps, s,
-                                  s + f->cells, f->image.as<float>(), st));
-    } else if (cur != f->image.as<float>()) {
-        HIP_TRY(hipMemcpyAsync(f->image.p, cur, n * sizeof(float), hipMemcpyDeviceToDevice, st));
+                                  s + f->cells, image, st));
     }
     f->fmt = FMT_F32;
-    f->img = f->image.p;
+    // with only a background subtracted the base image is the correlation's image
+    f->img = (blur || norm) ? static_cast<const void *>(image) : static_cast<const void *>(f->base_ptr);
     f->pitch = W;
     return PIV_OK;
 }
@@ -366,15 +391,22 @@ static piv_status condition_frames_batch(piv_analyzer *a, piv_frame *const *fs, 
     const size_t n = (size_t)H * W;
     const int64_t cells = sat_table_elems(H, W);
     const bool blur = c.smooth_sigma > 0.0f, norm = c.normalize_radius > 0;
+    // as prepare_frame_impl: the base image is materialised where something reads it, the
+    // blur reads the samples itself
+    const bool fused = blur && blur_fusable(a->taps_r);
+    const bool mat_base = bg_dev != nullptr || !fused;
+    const bool has_image = blur || norm;
     const size_t img_bytes = align_up((size_t)count * n * sizeof(float), 256);
+    const size_t base_bytes = mat_base ? img_bytes : 0, image_bytes = has_image ? img_bytes : 0;
     const size_t sat_bytes = mask ? (size_t)count * cells * 2 * sizeof(double) : 0;
     std::shared_ptr<DevBuf> store;
-    PIV_TRY(batch_storage(a, 2 * img_bytes + sat_bytes, store));
-    float *base = store->as<float>();
-    float *image = reinterpret_cast<float *>(store->as<char>() + img_bytes);
-    double *sat = mask ? reinterpret_cast<double *>(store->as<char>() + 2 * img_bytes) + SAT_LEAD : nullptr;
+    PIV_TRY(batch_storage(a, base_bytes + image_bytes + sat_bytes, store));
+    float *base = mat_base ? store->as<float>() : nullptr;
+    float *image = has_image ? reinterpret_cast<float *>(store->as<char>() + base_bytes) : nullptr;
+    double *sat = mask ? reinterpret_cast<double *>(store->as<char>() + base_bytes + image_bytes) + SAT_LEAD
+                       : nullptr;
     const int k_max = std::min(CHUNK, count);
-    if (blur) PIV_TRY(a->prep_tmp.reserve((size_t)k_max * n * sizeof(float)));
+    if (blur && !fused) PIV_TRY(a->prep_tmp.reserve((size_t)k_max * n * sizeof(float)));
     if (blur && norm) PIV_TRY(a->prep_tmp2.reserve((size_t)k_max * n * sizeof(float)));
     if (norm) PIV_TRY(a->prep_sat.reserve((size_t)k_max * cells * 2 * sizeof(double)));
     if (mask || norm) PIV_TRY(a->sat_carry.reserve(sat_carry_bytes(H, W, k_max)));
@@ -383,37 +415,47 @@ static piv_status condition_frames_batch(piv_analyzer *a, piv_frame *const *fs, 
     for (int i0 = 0; i0 < count; i0 += CHUNK) {
         const int k = std::min(CHUNK, count - i0);
         const void *raw = static_cast<const char *>(f0->raw_ptr) + (long)i0 * raw_frame_bytes;
-        float *b0 = base + (size_t)i0 * n, *img0 = image + (size_t)i0 * n;
-        HIP_TRY(launch_base(f0->in_fmt, raw, f0->raw_pitch, bg_dev, H, W, b0, st, k, raw_frame_bytes));
+        float *b0 = mat_base ? base + (size_t)i0 * n : nullptr;
+        float *img0 = has_image ? image + (size_t)i0 * n : nullptr;
+        const float *cur = b0;
+        if (fused) {
+            float *dst = norm ? a->prep_tmp2.as<float>() : img0;
+            HIP_TRY(launch_blur_fused(f0->in_fmt, raw, f0->raw_pitch, bg_dev, H, W, a->d_taps.as<float>(),
+                                      a->taps_r, b0, dst, st, k, raw_frame_bytes));
+            cur = dst;
+        } else {
+            HIP_TRY(launch_base(f0->in_fmt, raw, f0->raw_pitch, bg_dev, H, W, b0, st, k, raw_frame_bytes));
+            if (blur) {
+                float *dst = norm ? a->prep_tmp2.as<float>() : img0;
+                HIP_TRY(launch_blur(cur, H, W, a->d_taps.as<float>(), a->taps_r, a->prep_tmp.as<float>(), dst,
+                                    st, k));
+                cur = dst;
+            }
+        }
         if (mask) {
             double *s0 = sat + (size_t)i0 * 2 * cells;
-            HIP_TRY(launch_sat(b0, H, W, s0, s0 + cells, carry, st, k, 2 * cells));
-        }
-        const float *cur = b0;
-        if (blur) {
-            float *dst = norm ? a->prep_tmp2.as<float>() : img0;
-            HIP_TRY(launch_blur(cur, H, W, a->d_taps.as<float>(), a->taps_r, a->prep_tmp.as<float>(), dst,
-                                st, k));
-            cur = dst;
+            if (mat_base) HIP_TRY(launch_sat(b0, H, W, s0, s0 + cells, carry, st, k, 2 * cells));
+            else HIP_TRY(launch_sat_raw(f0->in_fmt, raw, f0->raw_pitch, raw_frame_bytes, H, W, s0, s0 + cells,
+                                        carry, st, k, 2 * cells));
         }
         if (norm) {
             double *ns = a->prep_sat.as<double>() + SAT_LEAD;
             HIP_TRY(launch_sat(cur, H, W, ns, ns + cells, carry, st, k, 2 * cells));
             HIP_TRY(launch_local_norm(cur, H, W, (int)c.normalize_radius, c.normalize_eps, ns, ns + cells,
                                       img0, st, k, 2 * cells));
-        } else if (cur != img0) {
-            HIP_TRY(hipMemcpyAsync(img0, cur, (size_t)k * n * sizeof(float), hipMemcpyDeviceToDevice, st));
         }
     }
     for (int i = 0; i < count; ++i) {
         piv_frame *f = fs[i];
         f->has_bg = bg_dev != nullptr;
         f->cells = cells;
-        f->base_ptr = base + (size_t)i * n;
+        f->base_ptr = mat_base ? base + (size_t)i * n : nullptr;
         f->has_sat = mask;
+        f->screen_t = 0;
         f->sat_ptr = mask ? sat + (size_t)i * 2 * cells : nullptr;
         f->fmt = FMT_F32;
-        f->img = image + (size_t)i * n;
+        // with only a background subtracted the base image is the correlation's image
+        f->img = has_image ? image + (size_t)i * n : base + (size_t)i * n;
         f->pitch = W;
         f->batch = store;
     }

@@ -700,6 +700,72 @@ __global__ __launch_bounds__(64) void k_sat_strips_list(SatList list, int pitch,
                           sum + sat_table_elems(H, W));
 }
 
+// gaussian_blur (preprocess.rs:161-195) fused with the base image (PreparedFrame::new,
+// preprocess.rs:137-153: the normalised samples, minus the background, clamped at 0): a
+// workgroup holds the base samples of a BT_H x BT_W tile of outputs and its halo in LDS, with
+// the halo's coordinates clamped into the frame as the reference clamps its taps, blurs the
+// rows of the tile and halo rows, then the columns.  Taps are summed in kernel order with a
+// rounded product then a rounded sum, as the two separate passes do, so the image is the same
+// bit for bit; the intermediate row-blurred image never leaves LDS.  base_out (optional)
+// receives the tile's base samples.
+constexpr int BT_W = 64, BT_H = 16, BT_RMAX = 16;
+
+template <int FMT>
+__global__ __launch_bounds__(256) void k_blur_tiled(const void *__restrict__ raw, int pitch,
+                                                    long raw_frame_bytes, const float *__restrict__ bg,
+                                                    int H, int W, const float *__restrict__ taps,
+                                                    int r, float *__restrict__ base_out,
+                                                    float *__restrict__ out)
+{
+    __shared__ float sb[(BT_H + 2 * BT_RMAX) * (BT_W + 2 * BT_RMAX)];  // base, tile + halo
+    __shared__ float sr[(BT_H + 2 * BT_RMAX) * BT_W];                  // row-blurred
+    __shared__ float stp[2 * BT_RMAX + 1];
+    const long fy = blockIdx.z;  // frame of a batch
+    raw = static_cast<const char *>(raw) + fy * raw_frame_bytes;
+    out += fy * (long)H * W;
+    if (base_out) base_out += fy * (long)H * W;
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x * BT_W, y0 = blockIdx.y * BT_H;
+    const int tw = BT_W + 2 * r, th = BT_H + 2 * r, nt = 2 * r + 1;
+    for (int i = tid; i < nt; i += 256) stp[i] = taps[i];
+    for (int i = tid; i < th * tw; i += 256) {
+        const int ly = i / tw, lx = i - ly * tw;
+        const int gy = min(max(y0 - r + ly, 0), H - 1), gx = min(max(x0 - r + lx, 0), W - 1);
+        float v = load_pixel<FMT>(raw, (long)gy * pitch + gx);
+        if (bg) {
+            v = v - bg[(long)gy * W + gx];
+            v = v > 0.0f ? v : 0.0f;  // f32::max(v, 0.0)
+        }
+        sb[i] = v;
+    }
+    __syncthreads();
+    if (base_out) {
+        for (int i = tid; i < BT_H * BT_W; i += 256) {
+            const int ly = i / BT_W, lx = i - ly * BT_W;
+            if (y0 + ly < H && x0 + lx < W)
+                base_out[(long)(y0 + ly) * W + x0 + lx] = sb[(ly + r) * tw + lx + r];
+        }
+    }
+    // rows: output column x reads base columns clamp(x + j - r), halo column x - x0 + j
+    for (int i = tid; i < th * BT_W; i += 256) {
+        const int ly = i / BT_W, lx = i - ly * BT_W;
+        const float *row = sb + ly * tw + lx;
+        float acc = 0.0f;
+        for (int j = 0; j < nt; ++j) acc += stp[j] * row[j];
+        sr[i] = acc;
+    }
+    __syncthreads();
+    // columns: output row y reads row-blurred rows clamp(y + j - r), halo row y - y0 + j
+    for (int i = tid; i < BT_H * BT_W; i += 256) {
+        const int ly = i / BT_W, lx = i - ly * BT_W;
+        if (y0 + ly >= H || x0 + lx >= W) continue;
+        const float *col = sr + ly * BT_W + lx;
+        float acc = 0.0f;
+        for (int j = 0; j < nt; ++j) acc += stp[j] * col[j * BT_W];
+        out[(long)(y0 + ly) * W + x0 + lx] = acc;
+    }
+}
+
 // gaussian_blur rows: sum over taps in kernel order, clamped edges
 __global__ void k_blur_rows(const float *__restrict__ in, int H, int W,
                             const float *__restrict__ taps, int r, float *__restrict__ out)
@@ -1060,6 +1126,29 @@ hipError_t launch_base(int fmt, const void *raw, int pitch, const float *bg, int
         case FMT_U8: hipLaunchKernelGGL(k_base<FMT_U8>, grid, dim3(256), 0, st, r, pitch, raw_frame_bytes, bg, H, W, o); break;
         case FMT_U16: hipLaunchKernelGGL(k_base<FMT_U16>, grid, dim3(256), 0, st, r, pitch, raw_frame_bytes, bg, H, W, o); break;
         default: hipLaunchKernelGGL(k_base<FMT_F32>, grid, dim3(256), 0, st, r, pitch, raw_frame_bytes, bg, H, W, o); break;
+        }
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+bool blur_fusable(int r) { return r >= 1 && r <= BT_RMAX; }
+
+hipError_t launch_blur_fused(int fmt, const void *raw, int pitch, const float *bg, int H, int W,
+                             const float *taps, int r, float *base_out, float *out, hipStream_t st,
+                             int nframes, long raw_frame_bytes)
+{
+    if (!blur_fusable(r)) return hipErrorInvalidValue;
+    for (int f0 = 0; f0 < nframes; f0 += MAX_GRID_Y) {
+        const dim3 grid(nblk(W, BT_W), nblk(H, BT_H), (unsigned)min(MAX_GRID_Y, nframes - f0));
+        const void *rw = static_cast<const char *>(raw) + (long)f0 * raw_frame_bytes;
+        const long o = (long)f0 * H * W;
+        float *bo = base_out ? base_out + o : nullptr;
+        switch (fmt) {
+        case FMT_U8: hipLaunchKernelGGL(k_blur_tiled<FMT_U8>, grid, dim3(256), 0, st, rw, pitch, raw_frame_bytes, bg, H, W, taps, r, bo, out + o); break;
+        case FMT_U16: hipLaunchKernelGGL(k_blur_tiled<FMT_U16>, grid, dim3(256), 0, st, rw, pitch, raw_frame_bytes, bg, H, W, taps, r, bo, out + o); break;
+        default: hipLaunchKernelGGL(k_blur_tiled<FMT_F32>, grid, dim3(256), 0, st, rw, pitch, raw_frame_bytes, bg, H, W, taps, r, bo, out + o); break;
         }
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;

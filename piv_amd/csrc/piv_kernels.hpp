@@ -61,6 +61,14 @@ size_t sat_carry_bytes(int H, int W, int nframes);
 // launch; `carry` is scratch of sat_carry_bytes(H, W, min(nframes, 64))
 hipError_t launch_sat_list(int fmt, const void *const *imgs, double *const *sums, int nframes,
                            int pitch, int H, int W, double *carry, hipStream_t st, int screen_t = 0);
+// The base image (normalised samples, minus the background when bg is given, clamped at 0)
+// and its Gaussian blur in one pass: `out` receives the blurred image, `base_out` (optional)
+// the base; bit-identical to launch_base followed by launch_blur.  Radius 1..16 (blur_fusable).
+// nframes > 1: frames raw_frame_bytes apart, images H*W apart.
+bool blur_fusable(int r);
+hipError_t launch_blur_fused(int fmt, const void *raw, int pitch, const float *bg, int H, int W,
+                             const float *taps, int r, float *base_out, float *out, hipStream_t st,
+                             int nframes = 1, long raw_frame_bytes = 0);
 // nframes > 1: a batch of images H*W apart (tables sat_stride doubles apart)
 hipError_t launch_blur(const float *in, int H, int W, const float *taps, int r, float *tmp,
                        float *out, hipStream_t st, int nframes = 1);
