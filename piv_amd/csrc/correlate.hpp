@@ -356,6 +356,117 @@ __device__ __forceinline__ void ingest(const PassPair &pp, bool sig, float ax, f
     }
 }
 
+// DIF half of an N-point inverse: lane half hp transforms v[n] + (-1)^hp v[n + N/2], the odd
+// half twiddled by e^{+2 pi i n / N} (tw[hp * N/2 + n]), as an N/2-point inverse; the outputs are
+// the transform's entries 2k + hp.
+template <int N>
+__device__ __forceinline__ void dif_half_inverse(const float (&vr)[N], const float (&vi)[N], int hp,
+                                                 const float2 *tw, float (&ur)[N / 2],
+                                                 float (&ui)[N / 2])
+{
+    constexpr int H2 = N / 2;
+    const float sg = hp ? -1.0f : 1.0f;
+#pragma unroll
+    for (int n = 0; n < H2; ++n) {
+        const float ar = __builtin_fmaf(sg, vr[n + H2], vr[n]);
+        const float ai = __builtin_fmaf(sg, vi[n + H2], vi[n]);
+        if (n == 0) {
+            ur[n] = ar;
+            ui[n] = ai;
+        } else {
+            const float2 w = tw[hp * H2 + n];
+            ur[n] = __builtin_fmaf(ar, w.x, -(ai * w.y));
+            ui[n] = __builtin_fmaf(ar, w.y, ai * w.x);
+        }
+    }
+    pivfft::fft<H2, 1>(ur, ui);
+}
+
+// The inverse 2-D transform of a window's Hermitian cross spectrum (the correlation is real)
+// for windows inside one wave, every lane running two N/2-point transforms.  Lane ky holds
+// spectrum row ky in (zr, zi); lane (t, hp) = t + hp N/2 of the window takes part in:
+//   1. the columns of the half spectrum, inverted along ky: columns 1 .. N/2-1, and columns 0
+//      and N/2 packed as one complex transform (both are real after it, the spectrum being
+//      Hermitian along ky in them); column t by lanes (t, 0) and (t, 1), one decimation-in-
+//      frequency half each, giving the column's rows 2k + hp;
+//   2. rows t and t + N/2 packed as one complex transform (both are real after it), their
+//      missing half spectrum the conjugate mirror G[y][N - kx] = conj(G[y][kx]), inverted along
+//      kx by the same two lanes, giving the rows' columns 2k + hp.
+// The two LDS exchanges move real and imaginary parts through the one plane in two rounds;
+// the plane then receives the correlation scaled by `scale`.
+template <int N>
+__device__ __forceinline__ void inverse_half_spectrum(float (&zr)[N], float (&zi)[N], float *pr,
+                                                      int lane, const float2 *tw, float scale)
+{
+    // odd stride of the half-width plane: conflict-free rows and columns
+    constexpr int H2 = N / 2, HS = H2 | 1, P = KCfg<N>::P;
+    static_assert(N % 2 == 0, "even window");
+    const int hp = lane / H2, t = lane - hp * H2;
+    // ---- 1. columns of the half spectrum: slot 0 = X[ky][0] + i X[ky][N/2] ----
+    const float s0r = zr[0] - zi[H2], s0i = zi[0] + zr[H2];
+    float vr[N], vi[N];
+    __syncthreads();
+    pr[lane * HS] = s0r;
+#pragma unroll
+    for (int s = 1; s < H2; ++s) pr[lane * HS + s] = zr[s];
+    __syncthreads();
+#pragma unroll
+    for (int ky = 0; ky < N; ++ky) vr[ky] = pr[ky * HS + t];
+    __syncthreads();
+    pr[lane * HS] = s0i;
+#pragma unroll
+    for (int s = 1; s < H2; ++s) pr[lane * HS + s] = zi[s];
+    __syncthreads();
+#pragma unroll
+    for (int ky = 0; ky < N; ++ky) vi[ky] = pr[ky * HS + t];
+    float ur[H2], ui[H2];
+    dif_half_inverse<N>(vr, vi, hp, tw, ur, ui);  // G[2k + hp][t] (t = 0: G[.][0] + i G[.][N/2])
+    // ---- 2. row pairs (t, t + N/2): plane row y, slot s = G[y][s] ----
+    float g0r[H2], g1r[H2], g0i[H2], g1i[H2];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < H2; ++k) pr[(2 * k + hp) * HS + t] = ur[k];
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < H2; ++s) {
+        g0r[s] = pr[t * HS + s];
+        g1r[s] = pr[(t + H2) * HS + s];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < H2; ++k) pr[(2 * k + hp) * HS + t] = ui[k];
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < H2; ++s) {
+        g0i[s] = pr[t * HS + s];
+        g1i[s] = pr[(t + H2) * HS + s];
+    }
+    // R[kx] = G[t][kx] + i G[t + N/2][kx]; G[y][0] and G[y][N/2] are the real and imaginary
+    // parts of slot 0
+#pragma unroll
+    for (int n = 0; n < H2; ++n) {
+        if (n == 0) {
+            vr[0] = g0r[0];
+            vi[0] = g1r[0];
+            vr[H2] = g0i[0];
+            vi[H2] = g1i[0];
+        } else {
+            const int k = H2 - n;
+            vr[n] = g0r[n] - g1i[n];
+            vi[n] = g0i[n] + g1r[n];
+            vr[n + H2] = g0r[k] + g1i[k];
+            vi[n + H2] = g1r[k] - g0i[k];
+        }
+    }
+    dif_half_inverse<N>(vr, vi, hp, tw, ur, ui);  // c[t][2k + hp] + i c[t + N/2][2k + hp]
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < H2; ++k) {
+        pr[t * P + 2 * k + hp] = ur[k] * scale;
+        pr[(t + H2) * P + 2 * k + hp] = ui[k] * scale;
+    }
+}
+
 // IEEE max of finite values without the operand canonicalisation the compiler adds to fmaxf
 // when it cannot prove an operand canonical (values loaded from LDS); used only where every
 // value is finite.
@@ -617,7 +728,10 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
     // N = 16 the shorter transform saves less than the twiddles cost): entry (p, n) = 1 for the
     // even half (p = 0), e^{+2 pi i n / N} for the odd half (p = 1), n < N / 2
     constexpr bool SPLIT_INV = C::ONE_WAVE && N >= 24;
-    __shared__ float2 s_tw[SPLIT_INV ? N : 1];
+    // one-wave windows: the inverse by half-spectrum columns and row pairs, every lane running
+    // two N/2-point transforms (inverse_half_spectrum)
+    constexpr bool HALF_INV = C::ONE_WAVE && N >= 16;
+    __shared__ float2 s_tw[(SPLIT_INV || HALF_INV) ? N : 1];
 
     // ---- XCD-aware block order: XCD x (blocks L = x, x+8, ...) gets a contiguous range ----
     const uint32_t L = blockIdx.x;
@@ -628,7 +742,7 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
     const int tid = threadIdx.x;
     const int wl = tid / N;
     const int lane = tid - wl * N;
-    if constexpr (SPLIT_INV) {
+    if constexpr (SPLIT_INV || HALF_INV) {
         if (tid < N) {
             const int hp = tid / H2, n = tid - hp * H2;
             s_tw[tid] = hp ? make_float2(pivfft::Tw<N>::t.c[n], pivfft::Tw<N>::t.s[n])
@@ -799,6 +913,12 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
     const double floor_e = energy_floor<FMT>();
     const bool has_peak = sig && !(ea <= floor_e || eb <= floor_e);
 
+    // correlate.rs:134-137: corr = C / (n^2 sqrt(ea eb)), the 1/4 undoes X' = 4 X
+    const float denom = (float)((double)(N * N) * sqrt(ea * eb));
+    const float scale = 0.25f / denom;
+    if constexpr (HALF_INV) {
+        inverse_half_spectrum<N>(zr, zi, pr, lane, s_tw, scale);
+    } else {
     // ---- Hermitian inverse ------------------------------------------------------------
     // Rows ky = 0..N/2 are inverted along kx (Y[ky][x]); the other rows are conjugates.
     // Then column pairs (2j, 2j+1) are inverted along ky as one complex transform
@@ -862,9 +982,6 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
             if ((ky & 7) == 7) __builtin_amdgcn_sched_barrier(0);
         }
     }
-    // correlate.rs:134-137: corr = C / (n^2 sqrt(ea eb)), the 1/4 undoes X' = 4 X
-    const float denom = (float)((double)(N * N) * sqrt(ea * eb));
-    const float scale = 0.25f / denom;
     const int c0 = 2 * jj, c1 = 2 * jj + 1;
     if constexpr (SPLIT_INV) {
         // Column pair jj by two lanes (decimation in frequency): lane half hp transforms
@@ -903,6 +1020,7 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
                 pr[y * P + c1] = zi[y] * scale;
             }
         }
+    }
     }
     __syncthreads();
 
