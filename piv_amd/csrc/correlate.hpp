@@ -841,14 +841,19 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
         const int mrow = (N - lane) % N;
         if constexpr (C::ONE_WAVE) {
             // The partner lane (row -ky) supplies Z[-ky][-kx] by shuffle.  Columns x and N-x
-            // are done together so both originals are read before either is overwritten.
+            // are done together so both originals are read before either is overwritten.  The
+            // half-spectrum inverse reads columns 0 .. N/2 only: with it the columns past N/2
+            // are not formed, and their Parseval energies are those of columns 1 .. N/2-1
+            // counted twice (|2A(-k)| = |2A(k)|).
             const int src = (tid - lane) + mrow;  // partner lane in this wave
+            float ea2d = 0.0f, eb2d = 0.0f;      // Parseval terms counted twice
 #pragma unroll
             for (int x = 0; x <= H2; ++x) {
                 const int mx = (N - x) % N;
+                const bool mirror = !HALF_INV && mx != x;
                 const float qr = __shfl(zr[mx], src, 64), qi = __shfl(zi[mx], src, 64);
                 float qr2 = 0.0f, qi2 = 0.0f;
-                if (mx != x) {
+                if (mirror) {
                     qr2 = __shfl(zr[x], src, 64);
                     qi2 = __shfl(zi[x], src, 64);
                 }
@@ -857,15 +862,20 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
                     const float b2r = zi[x] + qi, b2i = qr - zr[x];
                     const bool dc = (x == 0) && (lane == 0);
                     if constexpr (!SPATIAL_E) {
-                        ea2 = dc ? ea2 : __builtin_fmaf(a2r, a2r, __builtin_fmaf(a2i, a2i, ea2));
-                        eb2 = dc ? eb2 : __builtin_fmaf(b2r, b2r, __builtin_fmaf(b2i, b2i, eb2));
+                        if (HALF_INV && x != 0 && x != H2) {
+                            ea2d = __builtin_fmaf(a2r, a2r, __builtin_fmaf(a2i, a2i, ea2d));
+                            eb2d = __builtin_fmaf(b2r, b2r, __builtin_fmaf(b2i, b2i, eb2d));
+                        } else {
+                            ea2 = dc ? ea2 : __builtin_fmaf(a2r, a2r, __builtin_fmaf(a2i, a2i, ea2));
+                            eb2 = dc ? eb2 : __builtin_fmaf(b2r, b2r, __builtin_fmaf(b2i, b2i, eb2));
+                        }
                     }
                     const float xr = __builtin_fmaf(a2r, b2r, a2i * b2i);
                     const float xi = __builtin_fmaf(a2r, b2i, -(a2i * b2r));
                     zr[x] = dc ? 0.0f : xr;
                     zi[x] = dc ? 0.0f : xi;
                 }
-                if (mx != x) {
+                if (mirror) {
                     const float a2r = zr[mx] + qr2, a2i = zi[mx] - qi2;
                     const float b2r = zi[mx] + qi2, b2i = qr2 - zr[mx];
                     if constexpr (!SPATIAL_E) {
@@ -876,6 +886,10 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
                     zi[mx] = __builtin_fmaf(a2r, b2i, -(a2i * b2r));
                 }
                 if ((x & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+            }
+            if constexpr (!SPATIAL_E && HALF_INV) {
+                ea2 = __builtin_fmaf(2.0f, ea2d, ea2);
+                eb2 = __builtin_fmaf(2.0f, eb2d, eb2);
             }
         } else {
             __syncthreads();
