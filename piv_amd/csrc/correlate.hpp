@@ -2,7 +2,9 @@
 //
 // One launch covers every window of every pair of a pass.  Work decomposition:
 //   - a workgroup holds WPB windows of ONE pair, so the pair descriptor and the image base
-//     pointers are wave-uniform (scalar registers, SGPR-based global loads);
+//     pointers are wave-uniform (scalar registers, SGPR-based global loads): one wave of
+//     64 / N windows when N divides 64, three waves of 192 / N windows at N = 12, 24 and 48
+//     (packed: some windows cross a wave boundary, and no lane idles), else one window;
 //   - one lane per window line: lane c first owns column c of the window, later row c;
 //   - workgroup ids are remapped so that each of the 8 XCDs works on a contiguous range of
 //     windows (consecutive windows share most of their pixels: one XCD's L2 serves them).
@@ -14,10 +16,12 @@
 //      1/255 or 1/65535, which only rescales the energy threshold);
 //   3. z = a + i b, 2-D FFT: column FFT in registers, LDS transpose, row FFT;
 //   4. cross spectrum conj(FA) FB from Z[k] and Z[-k], with the DC bin cleared (the mean
-//      removal of load_window) and the window energies from Parseval;
-//   5. Hermitian inverse: row IFFT of rows 0..N/2, then column-pair IFFTs (two real
-//      columns per complex transform, split over two lanes for in-wave windows of N >= 32),
-//      scaled into an LDS correlation plane;
+//      removal of load_window); the window energies from the samples (N dividing 64) or
+//      from Parseval;
+//   5. Hermitian inverse (windows of at most 64 lanes, N >= 16): the half spectrum's columns,
+//      then row pairs, each transform split over two lanes; else row IFFTs of rows 0..N/2
+//      and column-pair IFFTs (two real columns per complex transform); scaled into an LDS
+//      correlation plane;
 //   6. find_peak (correlate.rs:215-303): first row-major argmax, footprint half width,
 //      highest local maximum outside it; the peak record goes to the finishing kernel
 //      (one thread per window), which does the 3-point fit, the peak ratio and assembly.
@@ -31,20 +35,31 @@
 
 namespace piv {
 
+constexpr int gcd_c(int a, int b) { return b == 0 ? a : gcd_c(b, a % b); }
+
 template <int N>
 struct KCfg {
     // N divides 64: windows tile a wave exactly, and butterfly reductions over a window's
     // lanes (power-of-two widths) apply
     static constexpr bool IN_WAVE = (64 % N == 0);
-    // windows per workgroup: as many whole windows as one wave holds (N <= 64; 2 at N = 24,
-    // 5 at N = 12, one at N = 48, the rest of the wave idle), else one window of N lanes
-    static constexpr int WPB = N <= 64 ? 64 / N : 1;
+    // the least common multiple of N and the wave width
+    static constexpr int LCM = N / gcd_c(N, 64) * 64;
+    // N below 64 not dividing it, whose windows tile at most four waves (12, 24 and 48: three
+    // waves): the workgroup packs LCM / N windows back to back, some across a wave boundary,
+    // instead of leaving a quarter of every wave idle
+    static constexpr bool PACKED = N < 64 && !IN_WAVE && LCM <= 256;
+    // windows per workgroup: packed, or as many whole windows as one wave holds (N <= 64;
+    // one at N = 40 and 56, the rest of the wave idle), else one window of N lanes
+    static constexpr int WPB = PACKED ? LCM / N : (N <= 64 ? 64 / N : 1);
     static constexpr int BLOCK = WPB * N;
     static constexpr int P = N + 1;   // odd LDS row stride: conflict-free rows and columns
     static constexpr int PL = N * P;
     // every window inside one one-wave workgroup: cross-lane exchanges may use shuffles and
     // ballots, and workgroup barriers cost nothing
-    static constexpr bool ONE_WAVE = N <= 64;
+    static constexpr bool ONE_WAVE = N <= 64 && !PACKED;
+    // at most one wave's width per window (one-wave and packed workgroups): one LDS plane per
+    // window, transposes in two passes (real, then imaginary parts), the half-spectrum inverse
+    static constexpr bool NARROW = N <= 64;
 };
 
 // Sum over the N lanes of one window (all lanes receive the total).
@@ -720,17 +735,17 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
     constexpr int WPB = C::WPB, P = C::P, PL = C::PL, H2 = N / 2;
     __shared__ float s_re[WPB * PL];
     // a second plane: windows of more than one wave, and the deformed loader (window B)
-    __shared__ float s_im[(C::ONE_WAVE && DEF == 0) ? 1 : WPB * PL];
+    __shared__ float s_im[(C::NARROW && DEF == 0) ? 1 : WPB * PL];
     __shared__ float s_red[WPB * N];
     __shared__ float s_pkv[WPB * N];
     __shared__ int s_pki[WPB * N];
-    // inverse column pass split over two lanes per column pair (one-wave windows, N >= 24; at
-    // N = 16 the shorter transform saves less than the twiddles cost): entry (p, n) = 1 for the
-    // even half (p = 0), e^{+2 pi i n / N} for the odd half (p = 1), n < N / 2
-    constexpr bool SPLIT_INV = C::ONE_WAVE && N >= 24;
-    // one-wave windows: the inverse by half-spectrum columns and row pairs, every lane running
+    // inverse column pass split over two lanes per column pair (one-plane windows, N >= 24;
+    // at N = 16 the shorter transform saves less than the twiddles cost): entry (p, n) = 1 for
+    // the even half (p = 0), e^{+2 pi i n / N} for the odd half (p = 1), n < N / 2
+    constexpr bool SPLIT_INV = C::NARROW && N >= 24;
+    // one-plane windows: the inverse by half-spectrum columns and row pairs, every lane running
     // two N/2-point transforms (inverse_half_spectrum)
-    constexpr bool HALF_INV = C::ONE_WAVE && N >= 16;
+    constexpr bool HALF_INV = C::NARROW && N >= 16;
     __shared__ float2 s_tw[(SPLIT_INV || HALF_INV) ? N : 1];
 
     // ---- XCD-aware block order: XCD x (blocks L = x, x+8, ...) gets a contiguous range ----
@@ -755,7 +770,7 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
     const uint32_t k = blk * WPB + wl;
     const bool active = k < cells;
     float *pr = s_re + wl * PL;
-    float *pim = (C::ONE_WAVE && DEF == 0) ? s_im : s_im + wl * PL;
+    float *pim = (C::NARROW && DEF == 0) ? s_im : s_im + wl * PL;
     float *red = s_red + wl * N;
     float *pkv = s_pkv + wl * N;
     int *pki = s_pki + wl * N;
@@ -794,9 +809,10 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
     }
 
     // ---- forward transform of z = a + i b ----------------------------------------------
-    // Transposes go through LDS.  A window inside one wave uses a single float plane and
-    // moves real and imaginary parts in two passes (the barriers of a one-wave workgroup cost
-    // nothing), halving the LDS footprint, which bounds occupancy at N = 32 and 64.
+    // Transposes go through LDS.  A window of at most one wave's width uses a single float
+    // plane and moves real and imaginary parts in two passes (the barriers of a one-wave
+    // workgroup cost nothing), halving the LDS footprint, which bounds occupancy at N = 32,
+    // 48 and 64.
     pivfft::fft<N, -1>(zr, zi);  // along rows r; lane = column
     // sum (a - mean)^2 = sum (a - m)^2 - (sum (a - m))^2 / N^2, in raw units squared
     float ea32 = 0.0f, eb32 = 0.0f;
@@ -806,7 +822,7 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
         ea32 = window_sum<N>(qa, red, lane) - wsa * wsa * inv_n2;
         eb32 = window_sum<N>(qb, red, lane) - wsb * wsb * inv_n2;
     }
-    if constexpr (C::ONE_WAVE) {
+    if constexpr (C::NARROW) {
 #pragma unroll
         for (int r = 0; r < N; ++r) pr[r * P + lane] = zr[r];
         __syncthreads();
@@ -891,6 +907,61 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
                 ea2 = __builtin_fmaf(2.0f, ea2d, ea2);
                 eb2 = __builtin_fmaf(2.0f, eb2d, eb2);
             }
+        } else if constexpr (C::PACKED) {
+            // A window may cross a wave boundary: Z[-ky][-kx] comes through LDS.  Only the
+            // columns the inverse reads are formed (0 .. N/2 with the half-spectrum inverse,
+            // the energies as above), and only their partner columns N - x are exchanged: with
+            // the half-spectrum inverse in one round (row ky of the plane holds Re Z[ky][m] in
+            // slot m - N/2 and Im Z[ky][m] in slot m for m = N/2 .. N-1, Re Z[ky][0] in slot N,
+            // and the window's reduction row holds Im Z[ky][0]), else real parts, then
+            // imaginary parts.
+            constexpr int XN = HALF_INV ? H2 + 1 : N;
+            float ea2d = 0.0f, eb2d = 0.0f;  // Parseval terms counted twice
+            auto cross = [&](int x, float qr, float qi) {
+                const float a2r = zr[x] + qr, a2i = zi[x] - qi;
+                const float b2r = zi[x] + qi, b2i = qr - zr[x];
+                const bool dc = (x == 0) && (lane == 0);
+                if (HALF_INV && x != 0 && x != H2) {
+                    ea2d = __builtin_fmaf(a2r, a2r, __builtin_fmaf(a2i, a2i, ea2d));
+                    eb2d = __builtin_fmaf(b2r, b2r, __builtin_fmaf(b2i, b2i, eb2d));
+                } else {
+                    ea2 = dc ? ea2 : __builtin_fmaf(a2r, a2r, __builtin_fmaf(a2i, a2i, ea2));
+                    eb2 = dc ? eb2 : __builtin_fmaf(b2r, b2r, __builtin_fmaf(b2i, b2i, eb2));
+                }
+                const float xr = __builtin_fmaf(a2r, b2r, a2i * b2i);
+                const float xi = __builtin_fmaf(a2r, b2i, -(a2i * b2r));
+                zr[x] = dc ? 0.0f : xr;
+                zi[x] = dc ? 0.0f : xi;
+            };
+            __syncthreads();
+            if constexpr (HALF_INV) {
+#pragma unroll
+                for (int m = H2; m < N; ++m) {
+                    pr[lane * P + m - H2] = zr[m];
+                    pr[lane * P + m] = zi[m];
+                }
+                pr[lane * P + N] = zr[0];
+                red[lane] = zi[0];
+                __syncthreads();
+                cross(0, pr[mrow * P + N], red[mrow]);
+#pragma unroll
+                for (int x = 1; x < XN; ++x) cross(x, pr[mrow * P + H2 - x], pr[mrow * P + N - x]);
+                ea2 = __builtin_fmaf(2.0f, ea2d, ea2);
+                eb2 = __builtin_fmaf(2.0f, eb2d, eb2);
+            } else {
+                float qr[XN];
+#pragma unroll
+                for (int x = 0; x < XN; ++x) pr[lane * P + x] = zr[x];
+                __syncthreads();
+#pragma unroll
+                for (int x = 0; x < XN; ++x) qr[x] = pr[mrow * P + (N - x) % N];
+                __syncthreads();
+#pragma unroll
+                for (int x = 0; x < XN; ++x) pr[lane * P + x] = zi[x];
+                __syncthreads();
+#pragma unroll
+                for (int x = 0; x < XN; ++x) cross(x, qr[x], pr[mrow * P + (N - x) % N]);
+            }
         } else {
             __syncthreads();
 #pragma unroll
@@ -939,7 +1010,7 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
     //   V[ky] = Y[ky][2j] + i Y[ky][2j+1]  ->  C[.][2j] + i C[.][2j+1].
     pivfft::fft<N, 1>(zr, zi);  // lane ky (<= N/2) holds Y[ky][x]
     const int jj = lane < H2 ? lane : lane - H2;
-    if constexpr (C::ONE_WAVE) {
+    if constexpr (C::NARROW) {
         // Row owners write V's real parts (virtual rows ky and N-ky of the half plane), keep
         // V's imaginary parts in zi, then the column-pair lanes gather them in two passes.
         constexpr int HW = H2 + 1;  // odd row stride of the half-width plane
@@ -1054,17 +1125,17 @@ hipError_t launch_corr_t(const PassPair *pairs, int npairs, uint32_t max_cells, 
 }
 
 // deform != 0 (a deformed pass) runs the default variant's occupancy with the deformed loader;
-// only one-wave sizes (N <= 64) have deformed instances here (larger windows deform in the
+// only sizes N <= 64 have deformed instances here (larger windows deform in the
 // generic-size kernel)
 template <int N, int FMT>
 hipError_t launch_corr_v(int variant, int deform, const PassPair *pairs, int npairs,
                          uint32_t max_cells, hipStream_t st)
 {
     constexpr VariantSet vs = corr_variants(N);
-    if constexpr (KCfg<N>::ONE_WAVE) {
+    if constexpr (KCfg<N>::NARROW) {
         // the deformed loader's second LDS plane bounds the workgroups per CU (160 KiB)
         constexpr int lds = 2 * KCfg<N>::WPB * KCfg<N>::PL * 4;
-        constexpr int occ_lds = (163840 / lds + 3) / 4;
+        constexpr int occ_lds = ((163840 / lds) * ((KCfg<N>::BLOCK + 63) / 64) + 3) / 4;
         constexpr int occ = vs.occ[0] < occ_lds ? vs.occ[0] : (occ_lds < 1 ? 1 : occ_lds);
         if (deform == 1) return launch_corr_t<N, FMT, occ, 1>(pairs, npairs, max_cells, st);
         if (deform == 2) return launch_corr_t<N, FMT, occ, 2>(pairs, npairs, max_cells, st);

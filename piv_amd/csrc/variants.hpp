@@ -23,7 +23,7 @@ constexpr VariantSet corr_variants(int n)
     case 12: return {3, {6, 4, 8}};
     case 20: case 24: case 28: return {3, {4, 3, 5}};
     case 32: case 40: return {2, {3, 4}};
-    case 48: return {3, {2, 3, 4}};
+    case 48: return {2, {2, 3}};  // three-wave workgroups of 40 KB LDS: at most 3 per SIMD
     case 56: return {2, {2, 3}};
     case 64: return {2, {2, 1}};
     default: return {1, {1, 0, 0}};  // 96 and the generic-size kernel
