@@ -3,8 +3,9 @@
 // One launch covers every window of every pair of a pass.  Work decomposition:
 //   - a workgroup holds WPB windows of ONE pair, so the pair descriptor and the image base
 //     pointers are wave-uniform (scalar registers, SGPR-based global loads): one wave of
-//     64 / N windows when N divides 64, three waves of 192 / N windows at N = 12, 24 and 48
-//     (packed: some windows cross a wave boundary, and no lane idles), else one window;
+//     64 / N windows when N divides 64, three waves of 192 / N windows at N = 24 and 48
+//     (packed: some windows cross a wave boundary, and no lane idles), else as many whole
+//     windows as one wave holds;
 //   - one lane per window line: lane c first owns column c of the window, later row c;
 //   - workgroup ids are remapped so that each of the 8 XCDs works on a contiguous range of
 //     windows (consecutive windows share most of their pixels: one XCD's L2 serves them).
@@ -44,10 +45,12 @@ struct KCfg {
     static constexpr bool IN_WAVE = (64 % N == 0);
     // the least common multiple of N and the wave width
     static constexpr int LCM = N / gcd_c(N, 64) * 64;
-    // N below 64 not dividing it, whose windows tile at most four waves (12, 24 and 48: three
-    // waves): the workgroup packs LCM / N windows back to back, some across a wave boundary,
-    // instead of leaving a quarter of every wave idle
-    static constexpr bool PACKED = N < 64 && !IN_WAVE && LCM <= 256;
+    // N below 64 not dividing it, whose one-wave workgroups leave a quarter of the wave idle
+    // and whose windows tile at most four waves (24 and 48: three waves): the workgroup packs
+    // LCM / N windows back to back, some across a wave boundary.  Elsewhere the workgroup
+    // barriers cost more than the idle lanes (measured: 12 packed in three waves, 10, 20 and
+    // 40 in five, 14 and 28 in seven are all slower): those sizes keep one-wave workgroups.
+    static constexpr bool PACKED = N < 64 && !IN_WAVE && LCM <= 256 && 64 % N >= 16;
     // windows per workgroup: packed, or as many whole windows as one wave holds (N <= 64;
     // one at N = 40 and 56, the rest of the wave idle), else one window of N lanes
     static constexpr int WPB = PACKED ? LCM / N : (N <= 64 ? 64 / N : 1);
@@ -1133,9 +1136,14 @@ hipError_t launch_corr_v(int variant, int deform, const PassPair *pairs, int npa
 {
     constexpr VariantSet vs = corr_variants(N);
     if constexpr (KCfg<N>::NARROW) {
-        // the deformed loader's second LDS plane bounds the workgroups per CU (160 KiB)
-        constexpr int lds = 2 * KCfg<N>::WPB * KCfg<N>::PL * 4;
-        constexpr int occ_lds = ((163840 / lds) * ((KCfg<N>::BLOCK + 63) / 64) + 3) / 4;
+        // the deformed loader's second LDS plane bounds the workgroups per CU (160 KiB of LDS,
+        // 32 waves): the kernel's shared arrays, two planes, the three per-lane rows and the
+        // twiddles
+        using C = KCfg<N>;
+        constexpr int lds = 2 * C::WPB * C::PL * 4 + 3 * C::WPB * N * 4 + N * 8;
+        constexpr int wpg = (C::BLOCK + 63) / 64;
+        constexpr int wgs = 163840 / lds < 32 / wpg ? 163840 / lds : 32 / wpg;
+        constexpr int occ_lds = (wgs * wpg + 3) / 4;
         constexpr int occ = vs.occ[0] < occ_lds ? vs.occ[0] : (occ_lds < 1 ? 1 : occ_lds);
         if (deform == 1) return launch_corr_t<N, FMT, occ, 1>(pairs, npairs, max_cells, st);
         if (deform == 2) return launch_corr_t<N, FMT, occ, 2>(pairs, npairs, max_cells, st);
