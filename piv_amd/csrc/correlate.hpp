@@ -95,6 +95,49 @@ __device__ __forceinline__ float raw_sample(const void *__restrict__ base, uint3
     }
 }
 
+// A frame as a raw buffer resource for the interior loaders: a row's offset is wave-uniform
+// (the scalar offset of the load), the lane's column is the vector offset, and the sample to
+// its right is the instruction's immediate offset, so a row costs no vector address
+// arithmetic.  Offsets are 32-bit: frames whose extent is 2^31 bytes or more take the
+// clamped loader (frame_buf_ok).
+template <int FMT>
+constexpr uint32_t sample_bytes()
+{
+    return FMT == FMT_U8 ? 1u : (FMT == FMT_U16 ? 2u : 4u);
+}
+
+template <int FMT>
+__device__ __forceinline__ uint64_t frame_extent_bytes(int H, int W, int pitch)
+{
+    return ((uint64_t)(H - 1) * (uint64_t)pitch + (uint64_t)W) * sample_bytes<FMT>();
+}
+
+template <int FMT>
+__device__ __forceinline__ bool frame_buf_ok(int H, int W, int pitch)
+{
+    return frame_extent_bytes<FMT>(H, W, pitch) < (1ull << 31);
+}
+
+template <int FMT>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t frame_buf(const void *img, int H, int W, int pitch)
+{
+    // word 3: the untyped-access format of gfx9 buffer resources
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(img), (short)0,
+                                             (int)frame_extent_bytes<FMT>(H, W, pitch), 0x00020000);
+}
+
+template <int FMT>
+__device__ __forceinline__ float buf_sample(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff)
+{
+    if constexpr (FMT == FMT_U8) {
+        return (float)__builtin_amdgcn_raw_buffer_load_b8(rs, (int)voff, (int)soff, 0);
+    } else if constexpr (FMT == FMT_U16) {
+        return (float)__builtin_amdgcn_raw_buffer_load_b16(rs, (int)voff, (int)soff, 0);
+    } else {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, (int)voff, (int)soff, 0));
+    }
+}
+
 // correlate.rs:182-202: window column c, sampled at fractional origin (x, y) in raw units.
 // The integer path (tx, ty < 1e-4) copies samples without interpolation, as the reference.
 // Rows and columns are clamped into the frame: a window exactly as wide (tall) as the frame
@@ -132,32 +175,29 @@ __device__ __forceinline__ bool interior(float x, float y, int W, int H)
     return x >= 0.0f && y >= 0.0f && (int)x + N + 1 <= W && (int)y + N + 1 <= H;
 }
 
-// load_column_raw for a window whose footprint lies inside the frame: no clamping, and
-// for integer samples the integer path is taken by zeroing the weights (0 * (p1 - p0) + p0
-// is exactly p0 for finite samples), so each row is one address add, two loads and two
-// lerps.
+// load_column_raw for a window whose footprint lies inside the frame (of an extent below
+// 2^31 bytes): no clamping, and for integer samples the integer path is taken by zeroing the
+// weights (0 * (p1 - p0) + p0 is exactly p0 for finite samples), so each row is two buffer
+// loads and two lerps.
 template <int N, int FMT>
-__device__ __forceinline__ void load_column_fast(const void *__restrict__ img, int pitch, float x,
+__device__ __forceinline__ void load_column_fast(__amdgpu_buffer_rsrc_t rs, int pitch, float x,
                                                  float y, int c, float (&col)[N])
 {
+    constexpr uint32_t BPS = sample_bytes<FMT>();
     const int x0 = (int)floorf(x), y0 = (int)floorf(y);
     float tx = x - (float)x0, ty = y - (float)y0;
     const bool ints = tx < 1e-4f && ty < 1e-4f;
-    const uint32_t o = (uint32_t)y0 * (uint32_t)pitch + (uint32_t)(x0 + c);
+    const uint32_t vo = ((uint32_t)y0 * (uint32_t)pitch + (uint32_t)(x0 + c)) * BPS;
+    const uint32_t rb = (uint32_t)pitch * BPS;  // row bytes (wave-uniform)
     if constexpr (FMT != FMT_F32) {
         tx = ints ? 0.0f : tx;
         ty = ints ? 0.0f : ty;
     }
-    using T = std::conditional_t<FMT == FMT_U8, uint8_t, std::conditional_t<FMT == FMT_U16, uint16_t, float>>;
-    const __attribute__((address_space(1))) T *base = (const __attribute__((address_space(1))) T *)img;
     float hprev = 0.0f;
 #pragma unroll
     for (int r = 0; r <= N; ++r) {
-        // 64-bit pointer arithmetic on a 32-bit row offset: the +1 sample folds into the
-        // load's immediate offset
-        const __attribute__((address_space(1))) T *q = base + (o + (uint32_t)r * (uint32_t)pitch);
-        const float p0 = (float)q[0];
-        const float p1 = (float)q[1];
+        const float p0 = buf_sample<FMT>(rs, vo, (uint32_t)r * rb);
+        const float p1 = buf_sample<FMT>(rs, vo + BPS, (uint32_t)r * rb);
         float h;
         if constexpr (FMT == FMT_F32) h = ints ? p0 : __builtin_fmaf(tx, p1 - p0, p0);
         else h = __builtin_fmaf(tx, p1 - p0, p0);
@@ -179,16 +219,16 @@ __device__ __forceinline__ bool integer_origin(float x, float y)
 
 // load_column_fast for integer origins: one sample per row, no interpolation.
 template <int N, int FMT>
-__device__ __forceinline__ void load_column_int(const void *__restrict__ img, int pitch, float x,
+__device__ __forceinline__ void load_column_int(__amdgpu_buffer_rsrc_t rs, int pitch, float x,
                                                 float y, int c, float (&col)[N])
 {
+    constexpr uint32_t BPS = sample_bytes<FMT>();
     const int x0 = (int)floorf(x), y0 = (int)floorf(y);
-    const uint32_t o = (uint32_t)y0 * (uint32_t)pitch + (uint32_t)(x0 + c);
-    using T = std::conditional_t<FMT == FMT_U8, uint8_t, std::conditional_t<FMT == FMT_U16, uint16_t, float>>;
-    const __attribute__((address_space(1))) T *base = (const __attribute__((address_space(1))) T *)img;
+    const uint32_t vo = ((uint32_t)y0 * (uint32_t)pitch + (uint32_t)(x0 + c)) * BPS;
+    const uint32_t rb = (uint32_t)pitch * BPS;
 #pragma unroll
     for (int r = 0; r < N; ++r) {
-        col[r] = (float)base[o + (uint32_t)r * (uint32_t)pitch];
+        col[r] = buf_sample<FMT>(rs, vo, (uint32_t)r * rb);
         if ((r & 31) == 31) __builtin_amdgcn_sched_barrier(0);
     }
 }
@@ -328,7 +368,9 @@ __device__ __forceinline__ void ingest(const PassPair &pp, bool sig, float ax, f
                                        float (&zr)[N], float (&zi)[N])
 {
     // every window of the wave keeps its (N+1)^2 footprint inside the frame: no clamping
-    const bool fast = __all(!sig || (interior<N>(ax, ay, pp.W, pp.H) && interior<N>(bx, by, pp.W, pp.H)));
+    // (buffer loads: frames of an extent below 2^31 bytes)
+    const bool fast = frame_buf_ok<FMT>(pp.H, pp.W, pp.pitch) &&
+                      __all(!sig || (interior<N>(ax, ay, pp.W, pp.H) && interior<N>(bx, by, pp.W, pp.H)));
     // every window of the wave at integer origins (a first pass without predictor)
     const bool ints = __all(!sig || (integer_origin(ax, ay) && integer_origin(bx, by)));
     // DEF: the launch's deformation mode (a pass with a predictor); it is a template
@@ -336,12 +378,16 @@ __device__ __forceinline__ void ingest(const PassPair &pp, bool sig, float ax, f
     if (DEF != 0 && pp.has_prev) {
         load_columns_deformed<N, FMT, DEF>(pp, sig, ax, ay, bx, by, lane, pl, pl2, zr, zi);
     } else if (sig) {
-        if (fast && ints) {
-            load_column_int<N, FMT>(pp.img_a, pp.pitch, ax, ay, lane, zr);
-            load_column_int<N, FMT>(pp.img_b, pp.pitch, bx, by, lane, zi);
-        } else if (fast) {
-            load_column_fast<N, FMT>(pp.img_a, pp.pitch, ax, ay, lane, zr);
-            load_column_fast<N, FMT>(pp.img_b, pp.pitch, bx, by, lane, zi);
+        if (fast) {
+            const __amdgpu_buffer_rsrc_t ra = frame_buf<FMT>(pp.img_a, pp.H, pp.W, pp.pitch);
+            const __amdgpu_buffer_rsrc_t rb = frame_buf<FMT>(pp.img_b, pp.H, pp.W, pp.pitch);
+            if (ints) {
+                load_column_int<N, FMT>(ra, pp.pitch, ax, ay, lane, zr);
+                load_column_int<N, FMT>(rb, pp.pitch, bx, by, lane, zi);
+            } else {
+                load_column_fast<N, FMT>(ra, pp.pitch, ax, ay, lane, zr);
+                load_column_fast<N, FMT>(rb, pp.pitch, bx, by, lane, zi);
+            }
         } else {
             load_column_raw<N, FMT>(pp.img_a, pp.pitch, pp.H, pp.W, ax, ay, lane, zr);
             load_column_raw<N, FMT>(pp.img_b, pp.pitch, pp.H, pp.W, bx, by, lane, zi);
