@@ -6,6 +6,7 @@ oracle's restatement (tests/test_oracle_deform.py pins the oracle):
     (integer peaks identical, displacements within 1e-4 px, validation bit-exact), then the
     final field against an independent oracle run (flags identical, within 1e-3 px); the
     error against the generating field falls below the rigid method's;
+  - packed 48 / 24 px windows (u8 and u16), every pass replayed through the oracle;
   - the batched, allocation-free and served paths give the single-pair result bitwise;
   - DFT-GEMM correlation (fp16 / bf16) with deformation holds RMS <= 0.02 px against the FP64
     oracle with the same deformation."""
@@ -120,6 +121,20 @@ def test_vortex_1024_stage_parity(piv, oracle, vortex, mode):
     e_def, e_rig = truth_rms(bench, res.field, g, field), truth_rms(bench, r0.field, g, field)
     print(f"{mode}: rms vs truth {e_def:.4f} px, rigid {e_rig:.4f} px")
     assert e_def < 0.9 * e_rig
+
+
+@pytest.mark.parametrize("mode", ["bilinear", "bicubic"])
+@pytest.mark.parametrize("container", [np.uint8, np.uint16])
+def test_packed_windows_stage_parity(piv, oracle, mode, container):
+    """Windows of 48 and 24 px run in packed three-wave workgroups (four and eight windows,
+    some across a wave boundary): their deformed passes against the oracle, u8 and 12-bit
+    u16 samples."""
+    h, w = 256, 320
+    frames, _ = S.video_frames(h, w, 2, 13, bits=12 if container is np.uint16 else 8,
+                               container=container)
+    cfg = piv.PivConfig(windows=[48, 24], overlap=0.5, final_iterations=2, deformation=mode)
+    an = piv.PivAnalyzer(cfg, piv.PreprocessConfig.raw())
+    deformed_stage_parity(piv, oracle, an, cfg, frames[0], frames[1], mode)
 
 
 @pytest.mark.parametrize("mode", ["bilinear", "bicubic"])
