@@ -404,7 +404,7 @@ def video_bench(total_pairs=1999, distinct=32, batch=256, seed=31):
                         "50%% overlap, RAW preprocessing, frames resident in HBM" % (nf, total_pairs),
             "pairs_per_s": round(total_pairs / el, 1), "seconds": round(el, 4),
             "prepare_ms": round((t1 - t0) * 1e3, 2), "analyze_ms": round((t2 - t1) * 1e3, 2),
-            "batch_pairs": batch, "autotune_occupancy": tune,
+            "batch_pairs": batch, "autotune_occupancy": tune, "batch_lanes": an.batch_lanes,
             "kernel_avg_ms": {f'{s["name"]}{s["window"] or ""}': round(s["total_ms"] / s["launches"], 4)
                               for s in stats},
             "roofline": {"kernel": f"k_correlate<{dom['window']}>", "achieved_GBps": round(achieved, 1),
@@ -826,7 +826,8 @@ def main():
     if args.only_batched:
         print(json.dumps({"fields_per_s": fields_per_s, "ms_per_step": ms_per_step,
                           "ms_per_step_with_kernel_events": elapsed_ev / args.steps * 1e3,
-                          "kernel_avg_ms": kernels, "autotune": tune}), flush=True)
+                          "kernel_avg_ms": kernels, "autotune": tune,
+                          "batch_lanes": an.batch_lanes}), flush=True)
         return
 
     # single-request latency (one pair per call, arrival to result on the host)
@@ -901,6 +902,7 @@ def main():
         "rms_vectors": nvec,
         "kernel_avg_ms": kernels,
         "autotune": tune,
+        "batch_lanes": an.batch_lanes,
         "roofline": {"bound": "valu", "kernel": f"k_correlate<{dom['window']}>",
                      "achieved": round(tflops, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(tflops / FP32_PEAK_TFLOPS, 4),

@@ -396,6 +396,15 @@ piv_status piv_autotune(piv_analyzer *a, const piv_frame *const *frames_a,
                         const piv_frame *const *frames_b, uint32_t npairs, uint32_t repeats,
                         piv_tune_entry *out, uint32_t capacity, uint32_t *count);
 
+/* Batch lanes: piv_analyze_batch* split a batch's pairs into this many groups (1..4, at least
+ * 4 pairs each) whose pass chains run side by side on helper streams, so that one group's
+ * short per-vector kernels and launch tails overlap another group's correlation.  Results do
+ * not depend on it.  The default is 2, or PIV_BATCH_LANES in the environment at analyzer
+ * creation, which also pins it; otherwise piv_autotune ends by timing the caller's batch
+ * (graph replays, two sweeps) at each count and keeps the fastest. */
+piv_status piv_batch_lanes_set(piv_analyzer *a, uint32_t lanes);
+piv_status piv_batch_lanes_get(const piv_analyzer *a, uint32_t *lanes);
+
 /* Frame source channel selection on decoded clips (piv-io/src/decode.rs:12-26,124-133): one
  * plane of `count` pixels of interleaved samples, device to device: out[i] = pixels[i *
  * channels + plane].  The green plane of an RGB clip is ffmpeg's `format=gbrp,extractplanes=g`

@@ -399,9 +399,22 @@ class PivAnalyzer:
                                  float(getattr(o, k))) for k, _ in o._fields_})
                 for o in out[:n]]
 
+    @property
+    def batch_lanes(self):
+        """Groups of pairs whose pass chains a batch runs side by side (1..4; results do not
+        depend on it): 2, PIV_BATCH_LANES, or what autotune() or the caller last chose."""
+        v = C.c_uint32(0)
+        _check(self._lib.piv_batch_lanes_get(self._h, C.byref(v)))
+        return int(v.value)
+
+    @batch_lanes.setter
+    def batch_lanes(self, lanes):
+        _check(self._lib.piv_batch_lanes_set(self._h, int(lanes)))
+
     def autotune(self, frames_a, frames_b, repeats=3):
         """Time every correlation-kernel variant on this batch and keep the fastest per window
-        size (variants give bit-identical results).  Returns one dict per window size."""
+        size (variants give bit-identical results), then the batch lanes (batch_lanes) unless
+        PIV_BATCH_LANES pins them.  Returns one dict per window size."""
         n = len(frames_a)
         A = (C.c_void_p * n)(*[f._h.value for f in frames_a])
         B = (C.c_void_p * n)(*[f._h.value for f in frames_b])

@@ -160,6 +160,8 @@ _SIGS = {
     "piv_analyzer_set_precision": ([C.c_void_p, C.c_int], C.c_int),
     "piv_kernel_variant_count": ([C.c_uint32], C.c_uint32),
     "piv_kernel_variant_set": ([C.c_void_p, C.c_uint32, C.c_int, C.c_uint32], C.c_int),
+    "piv_batch_lanes_set": ([C.c_void_p, C.c_uint32], C.c_int),
+    "piv_batch_lanes_get": ([C.c_void_p, C.POINTER(C.c_uint32)], C.c_int),
     "piv_autotune": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
                       C.c_uint32, C.POINTER(C.c_uint32)], C.c_int),
     "piv_write_pair_npz": ([C.c_char_p, C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_double,

@@ -703,10 +703,14 @@ static piv_status run_batch_impl(piv_analyzer *a, const piv_frame *const *fa, co
     // to the previous call's reuses the descriptors already on the device and replays the
     // captured launch sequence as one graph (kernel nodes only).  Profiled calls always launch
     // eagerly so every kernel can be bracketed by events.
+    // Lanes: groups of at least 4 pairs each; profiled calls run one chain on `st` so that
+    // every kernel is bracketed by events on one stream.
+    const uint32_t lanes =
+        a->prof_on ? 1u : std::max(1u, std::min((uint32_t)a->lanes, npairs / 4));
     std::vector<char> sig(hbuf);
     {
-        const uintptr_t extra[6] = {(uintptr_t)dd, (uintptr_t)dsz, (uintptr_t)np, (uintptr_t)npairs,
-                                    (uintptr_t)fa[0]->fmt, (uintptr_t)st};
+        const uintptr_t extra[7] = {(uintptr_t)dd, (uintptr_t)dsz, (uintptr_t)np, (uintptr_t)npairs,
+                                    (uintptr_t)fa[0]->fmt, (uintptr_t)st, (uintptr_t)lanes};
         sig.insert(sig.end(), reinterpret_cast<const char *>(extra),
                    reinterpret_cast<const char *>(extra) + sizeof extra);
         for (const PassPlan &pl : plans) {
@@ -749,10 +753,6 @@ static piv_status run_batch_impl(piv_analyzer *a, const piv_frame *const *fa, co
         }
         return PIV_OK;
     };
-    // Lanes: groups of at least 4 pairs each; profiled calls run one chain on `st` so that
-    // every kernel is bracketed by events on one stream.
-    const uint32_t lanes =
-        a->prof_on ? 1u : std::max(1u, std::min((uint32_t)a->lanes, npairs / 4));
     auto enqueue = [&]() -> piv_status {
         HIP_TRY(launch_zero_u32(sum_dev, (uint32_t)(np * npairs * 5), st));
         if (lanes == 1) return chain(0, npairs, st);
@@ -999,15 +999,18 @@ piv_status piv_analyzer_create(const piv_config *cfg, int device, piv_analyzer *
         for (uint32_t i = 1; i < cfg->final_iterations; ++i)
             a->passes.push_back((int)cfg->windows[cfg->num_windows - 1]);
         hipError_t e = hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking);
-        // batch lanes (PIV_BATCH_LANES = 1..4 overrides the default of 2): helper streams at
-        // the default priority and the events that fork them from and join them into `stream`
+        // batch lanes (PIV_BATCH_LANES = 1..4 pins the count, else the default of 2 until the
+        // autotuner or the caller picks one): helper streams at the default priority for the
+        // most lanes, and the events that fork them from and join them into `stream`
         if (const char *env = std::getenv("PIV_BATCH_LANES")) {
             const int v = std::atoi(env);
-            if (v >= 1 && v <= piv_analyzer::MAX_LANES) a->lanes = v;
+            if (v >= 1 && v <= piv_analyzer::MAX_LANES) {
+                a->lanes = v;
+                a->lanes_pinned = true;
+            }
         }
-        if (e == hipSuccess && a->lanes > 1)
-            e = hipEventCreateWithFlags(&a->ev_fork, hipEventDisableTiming);
-        for (int i = 0; e == hipSuccess && i < a->lanes - 1; ++i) {
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&a->ev_fork, hipEventDisableTiming);
+        for (int i = 0; e == hipSuccess && i < piv_analyzer::MAX_LANES - 1; ++i) {
             e = hipStreamCreateWithFlags(&a->side[i], hipStreamNonBlocking);
             if (e == hipSuccess) e = hipEventCreateWithFlags(&a->ev_join[i], hipEventDisableTiming);
         }
@@ -1757,6 +1760,72 @@ piv_status piv_kernel_variant_set(piv_analyzer *a, uint32_t window, piv_pixel_fo
     });
 }
 
+namespace pivi {
+
+// Batch lanes for the caller's batch: the whole batch timed at each lane count by events on
+// the analysis stream (which joins every lane) over `repeats` graph replays, after an eager
+// and a capturing call; two sweeps, each count keeping its faster one.  Counts that the
+// batch's size reduces to a smaller effective count (groups of at least 4 pairs) are skipped.
+static piv_status tune_lanes(piv_analyzer *a, const piv_frame *const *fa,
+                             const piv_frame *const *fb, uint32_t npairs, uint32_t repeats)
+{
+    constexpr int ML = piv_analyzer::MAX_LANES;
+    const int saved = a->lanes;
+    const bool saved_prof = a->prof_on;
+    double ms[ML + 1];
+    for (double &m : ms) m = 1e30;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    hipError_t he = hipEventCreate(&e1);
+    if (he != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        HIP_TRY(he);
+    }
+    piv_status s = PIV_OK;
+    a->prof_on = false;
+    for (int sweep = 0; sweep < 2 && s == PIV_OK; ++sweep)
+    for (int l = 1; l <= ML && s == PIV_OK; ++l) {
+        if (l > 1 && npairs / 4 < (uint32_t)l) continue;  // same chains as a smaller count
+        a->lanes = l;
+        std::vector<PassPlan> pl;
+        std::vector<FieldPtrs> raw, val;
+        std::vector<float *> org;
+        std::vector<int32_t *> pk;
+        uint32_t *sums = nullptr;
+        for (int k = 0; k < 2 && s == PIV_OK; ++k)  // eager, then captured
+            s = run_batch(a, fa, fb, npairs, BatchOut{}, false, pl, raw, val, org, pk, &sums);
+        if (s != PIV_OK) break;
+        if (hipEventRecord(e0, a->stream) != hipSuccess) {
+            s = fail(PIV_ERR_DEVICE, "autotune event failed");
+            break;
+        }
+        for (uint32_t r = 0; r < repeats && s == PIV_OK; ++r)
+            s = run_batch(a, fa, fb, npairs, BatchOut{}, false, pl, raw, val, org, pk, &sums);
+        if (s != PIV_OK) break;
+        float t = 0.0f;
+        if (hipEventRecord(e1, a->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&t, e0, e1) != hipSuccess) {
+            s = fail(PIV_ERR_DEVICE, "autotune run failed");
+            break;
+        }
+        ms[l] = std::min(ms[l], (double)t / (double)repeats);
+    }
+    a->prof_on = saved_prof;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (s != PIV_OK) {
+        a->lanes = saved;
+        return s;
+    }
+    int best = 1;
+    for (int l = 2; l <= ML; ++l)
+        if (ms[l] < ms[best]) best = l;
+    a->lanes = best;
+    return PIV_OK;
+}
+
+}  // namespace pivi
+
 piv_status piv_autotune(piv_analyzer *a, const piv_frame *const *fa, const piv_frame *const *fb,
                         uint32_t npairs, uint32_t repeats, piv_tune_entry *out, uint32_t capacity,
                         uint32_t *count)
@@ -1846,6 +1915,29 @@ piv_status piv_autotune(piv_analyzer *a, const piv_frame *const *fa, const piv_f
             ++i;
         }
         *count = i;
+        if (!a->lanes_pinned) PIV_TRY(tune_lanes(a, fa, fb, npairs, repeats));
+        return PIV_OK;
+    });
+}
+
+piv_status piv_batch_lanes_set(piv_analyzer *a, uint32_t lanes)
+{
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!a) return fail(PIV_ERR_INVALID_ARGUMENT, "null analyzer");
+        if (lanes < 1 || lanes > (uint32_t)piv_analyzer::MAX_LANES)
+            return fail(PIV_ERR_INVALID_ARGUMENT, "batch lanes %u outside 1..%d", lanes,
+                        piv_analyzer::MAX_LANES);
+        std::lock_guard<std::mutex> lk(a->mu);
+        a->lanes = (int)lanes;
+        return PIV_OK;
+    });
+}
+
+piv_status piv_batch_lanes_get(const piv_analyzer *a, uint32_t *lanes)
+{
+    return pivi::guard(__func__, [&]() -> piv_status {
+        if (!a || !lanes) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
+        *lanes = (uint32_t)a->lanes;
         return PIV_OK;
     });
 }

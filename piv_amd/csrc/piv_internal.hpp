@@ -232,6 +232,7 @@ struct piv_analyzer {
     // group's correlation).  Callers only ever see `stream`.
     static constexpr int MAX_LANES = 4;
     int lanes = 2;
+    bool lanes_pinned = false;  // PIV_BATCH_LANES set: the autotuner keeps `lanes`
     hipStream_t side[MAX_LANES - 1] = {};
     hipEvent_t ev_fork = nullptr, ev_join[MAX_LANES - 1] = {};
     std::mutex mu;

@@ -245,6 +245,52 @@ def test_batch_lanes_bitwise(piv, monkeypatch, lanes):
     monkeypatch.delenv("PIV_BATCH_LANES", raising=False)
 
 
+def test_batch_lanes_api_and_autotune(piv, monkeypatch):
+    """Lane counts set through the API (batch_lanes, 1..4) give bitwise the two-lane result,
+    eager and replayed, on a batch whose groups are uneven; autotune() keeps a count in 1..4
+    and the result does not change; counts outside 1..4 are refused."""
+    monkeypatch.delenv("PIV_BATCH_LANES", raising=False)
+    cfg = piv.PivConfig(windows=[32, 16], final_iterations=1)
+    h, w, n = 160, 192, 18
+    pairs = [S.particle_frames(h, w, 800 + i, field=S.uniform_field(1.1 - 0.1 * i, -0.4))
+             for i in range(n)]
+    an = piv.PivAnalyzer(cfg, piv.PreprocessConfig.raw())
+    assert an.batch_lanes == 2
+    dA = piv.DeviceArray.from_numpy(np.stack([p[0] for p in pairs]))
+    dB = piv.DeviceArray.from_numpy(np.stack([p[1] for p in pairs]))
+    fa = [piv.PreparedFrame.from_device(an, dA.slice(i)) for i in range(n)]
+    fb = [piv.PreparedFrame.from_device(an, dB.slice(i)) for i in range(n)]
+    g = an.output_grid(h, w)
+    dx, dy, pr = (piv.DeviceArray((n, g.ny, g.nx), np.float32) for _ in range(3))
+    fl = piv.DeviceArray((n, g.ny, g.nx), np.uint8)
+
+    def run():
+        outs = []
+        for _ in range(3):  # eager, captured, replayed
+            for d in (dx, dy, pr):
+                d.upload(np.full(d.shape, np.nan, np.float32))
+            sums = an.analyze_batch_device(fa, fb, dx, dy, pr, fl)
+            outs.append((bits(dx.numpy()), bits(dy.numpy()), bits(pr.numpy()), fl.numpy(), sums))
+        return outs
+
+    ref = run()[0]
+    for lanes in (1, 3, 4, 2):
+        an.batch_lanes = lanes
+        assert an.batch_lanes == lanes
+        for k, o in enumerate(run()):
+            for x, y in zip(o[:4], ref[:4]):
+                assert np.array_equal(x, y), f"lanes {lanes}, call {k}"
+            assert o[4] == ref[4]
+    an.autotune(fa, fb, repeats=2)
+    assert 1 <= an.batch_lanes <= 4
+    for k, o in enumerate(run()):
+        for x, y in zip(o[:4], ref[:4]):
+            assert np.array_equal(x, y), f"autotuned lanes {an.batch_lanes}, call {k}"
+    for bad in (0, 5):
+        with pytest.raises(piv.PivError):
+            an.batch_lanes = bad
+
+
 def test_batch_replay_matches_eager(piv):
     """Repeated batch calls with identical frames and outputs replay one captured graph of the
     pass chain.  Every replay, with other eager work and synchronisations between replays,
