@@ -958,19 +958,18 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
             }
         } else if constexpr (C::PACKED) {
             // A window may cross a wave boundary: Z[-ky][-kx] comes through LDS.  Only the
-            // columns the inverse reads are formed (0 .. N/2 with the half-spectrum inverse,
-            // the energies as above), and only their partner columns N - x are exchanged: with
-            // the half-spectrum inverse in one round (row ky of the plane holds Re Z[ky][m] in
-            // slot m - N/2 and Im Z[ky][m] in slot m for m = N/2 .. N-1, Re Z[ky][0] in slot N,
-            // and the window's reduction row holds Im Z[ky][0]), else real parts, then
-            // imaginary parts.
-            constexpr int XN = HALF_INV ? H2 + 1 : N;
+            // columns the half-spectrum inverse reads (0 .. N/2) are formed, the energies as
+            // above, and only their partner columns N - x are exchanged, in one round: row ky
+            // of the plane holds Re Z[ky][m] in slot m - N/2 and Im Z[ky][m] in slot m for
+            // m = N/2 .. N-1, Re Z[ky][0] in slot N, and the window's reduction row holds
+            // Im Z[ky][0].
+            static_assert(HALF_INV && !SPATIAL_E, "packed sizes (24, 48) invert half spectra");
             float ea2d = 0.0f, eb2d = 0.0f;  // Parseval terms counted twice
             auto cross = [&](int x, float qr, float qi) {
                 const float a2r = zr[x] + qr, a2i = zi[x] - qi;
                 const float b2r = zi[x] + qi, b2i = qr - zr[x];
                 const bool dc = (x == 0) && (lane == 0);
-                if (HALF_INV && x != 0 && x != H2) {
+                if (x != 0 && x != H2) {
                     ea2d = __builtin_fmaf(a2r, a2r, __builtin_fmaf(a2i, a2i, ea2d));
                     eb2d = __builtin_fmaf(b2r, b2r, __builtin_fmaf(b2i, b2i, eb2d));
                 } else {
@@ -983,34 +982,19 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
                 zi[x] = dc ? 0.0f : xi;
             };
             __syncthreads();
-            if constexpr (HALF_INV) {
 #pragma unroll
-                for (int m = H2; m < N; ++m) {
-                    pr[lane * P + m - H2] = zr[m];
-                    pr[lane * P + m] = zi[m];
-                }
-                pr[lane * P + N] = zr[0];
-                red[lane] = zi[0];
-                __syncthreads();
-                cross(0, pr[mrow * P + N], red[mrow]);
-#pragma unroll
-                for (int x = 1; x < XN; ++x) cross(x, pr[mrow * P + H2 - x], pr[mrow * P + N - x]);
-                ea2 = __builtin_fmaf(2.0f, ea2d, ea2);
-                eb2 = __builtin_fmaf(2.0f, eb2d, eb2);
-            } else {
-                float qr[XN];
-#pragma unroll
-                for (int x = 0; x < XN; ++x) pr[lane * P + x] = zr[x];
-                __syncthreads();
-#pragma unroll
-                for (int x = 0; x < XN; ++x) qr[x] = pr[mrow * P + (N - x) % N];
-                __syncthreads();
-#pragma unroll
-                for (int x = 0; x < XN; ++x) pr[lane * P + x] = zi[x];
-                __syncthreads();
-#pragma unroll
-                for (int x = 0; x < XN; ++x) cross(x, qr[x], pr[mrow * P + (N - x) % N]);
+            for (int m = H2; m < N; ++m) {
+                pr[lane * P + m - H2] = zr[m];
+                pr[lane * P + m] = zi[m];
             }
+            pr[lane * P + N] = zr[0];
+            red[lane] = zi[0];
+            __syncthreads();
+            cross(0, pr[mrow * P + N], red[mrow]);
+#pragma unroll
+            for (int x = 1; x <= H2; ++x) cross(x, pr[mrow * P + H2 - x], pr[mrow * P + N - x]);
+            ea2 = __builtin_fmaf(2.0f, ea2d, ea2);
+            eb2 = __builtin_fmaf(2.0f, eb2d, eb2);
         } else {
             __syncthreads();
 #pragma unroll
