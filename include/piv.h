@@ -145,13 +145,18 @@ piv_status piv_frame_prepare(piv_analyzer *a, const void *pixels, piv_pixel_form
 /* Same from DEVICE pixels already resident in HBM (no host copy).  When no conditioning is
  * configured (sigma 0, radius 0, no background) the frame references `pixels` without a copy
  * and the caller keeps them alive until piv_frame_release; otherwise it is conditioned into
- * frame-owned memory.  background: optional device float [h*w]. */
+ * frame-owned memory.  background: optional device float [h*w].  The preparation runs
+ * asynchronously on the analyzer's stream (as PreparedFrame::new's result, the frame is
+ * ready for every call of this library: analyses on the analyzer's stream follow it, one on
+ * a stream the caller passes waits for it, and download, release and the table query
+ * synchronise with it); `pixels` and `background` must stay unchanged until it has run. */
 piv_status piv_frame_prepare_device(piv_analyzer *a, const void *pixels, piv_pixel_format fmt,
                                     uint32_t height, uint32_t width, size_t row_stride,
                                     const float *background, piv_frame **out);
 /* `count` frames at pixels + i * frame_stride bytes (device memory, kept unchanged while the
  * frames live), prepared together: one launch per conditioning kernel for the whole set where
- * the configuration allows.  out[i] receives frame i (release each with piv_frame_release). */
+ * the configuration allows, asynchronously as piv_frame_prepare_device.  out[i] receives
+ * frame i (release each with piv_frame_release). */
 piv_status piv_frame_prepare_device_batch(piv_analyzer *a, const void *pixels,
                                           piv_pixel_format fmt, uint32_t count, uint32_t height,
                                           uint32_t width, size_t row_stride, size_t frame_stride,

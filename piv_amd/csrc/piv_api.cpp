@@ -777,6 +777,20 @@ static piv_status run_batch_impl(piv_analyzer *a, const piv_frame *const *fa, co
     HIP_TRY(hipStreamIsCapturing(st, &cap));
     if (cap != hipStreamCaptureStatusNone)
         return fail(PIV_ERR_UNSUPPORTED, "the analysis stream is being captured by the caller");
+    // Frames prepared (or last analysed) on another stream, e.g. the analyzer's own when the
+    // caller passes a stream: this stream waits for that point first.
+    {
+        const UseEvent *waited[8] = {};
+        int nw = 0;
+        for (const piv_frame *const *side : {fa, fb})
+            for (uint32_t i = 0; i < npairs; ++i) {
+                const UseEvent *ev = side[i]->last_use.get();
+                if (!ev || !ev->e || ev->st == st) continue;
+                if (std::find(waited, waited + nw, ev) != waited + nw) continue;
+                HIP_TRY(hipStreamWaitEvent(st, ev->e, 0));
+                if (nw < 8) waited[nw++] = ev;
+            }
+    }
     const bool same = !a->prof_on && sig == a->last_sig;
     if (same && a->graph_exec) {
         HIP_TRY(hipGraphLaunch(a->graph_exec, st));
@@ -878,6 +892,7 @@ piv_status frame_from_device(piv_analyzer *a, const void *pixels, int fmt, uint3
     (void)owned;
     piv_status s = prepare_frame(a, f, bg_dev);
     if (s != PIV_OK) {
+        (void)hipStreamSynchronize(a->stream);  // kernels of the failed preparation may still run
         delete f;
         return s;
     }
@@ -896,6 +911,7 @@ piv_status mark_use(piv_analyzer *a, hipStream_t st, const piv_frame *const *fs,
     if (!ev) {
         auto e = std::make_shared<UseEvent>();
         HIP_TRY(hipEventCreateWithFlags(&e->e, hipEventDisableTiming));
+        e->st = st;
         ev = e;
     }
     HIP_TRY(hipEventRecord(ev->e, st));
@@ -1117,7 +1133,6 @@ piv_status piv_frame_prepare_device(piv_analyzer *a, const void *pixels, piv_pix
         std::lock_guard<std::mutex> lk(a->mu);
         PIV_TRY(set_device(a->device));
         PIV_TRY(frame_from_device(a, pixels, fmt, height, width, row_stride, false, background, out));
-        HIP_TRY(hipStreamSynchronize(a->stream));
         return PIV_OK;
     });
 }
@@ -1153,9 +1168,9 @@ piv_status piv_frame_prepare_device_batch(piv_analyzer *a, const void *pixels,
             f->raw_pitch = (int)(row_stride / bytes_per_px((int)fmt));
         }
         if (s == PIV_OK) s = prepare_frames_batch(a, fs.data(), (int)count, background, (long)frame_stride);
-        if (s == PIV_OK && hipStreamSynchronize(a->stream) != hipSuccess)
-            s = fail(PIV_ERR_DEVICE, "frame preparation failed");
         if (s != PIV_OK) {
+            // kernels of a failed preparation may still run: the frames' memory is freed after
+            (void)hipStreamSynchronize(a->stream);
             for (piv_frame *f : fs) delete f;
             return s;
         }
@@ -1190,6 +1205,8 @@ piv_status piv_frame_download(const piv_frame *f, float *image, float *base)
     return pivi::guard(__func__, [&]() -> piv_status {
         if (!f) return fail(PIV_ERR_INVALID_ARGUMENT, "null frame");
         PIV_TRY(set_device(f->device));
+        // the preparation runs asynchronously on the analyzer's stream
+        if (f->last_use && f->last_use->e) HIP_TRY(hipEventSynchronize(f->last_use->e));
         const size_t n = (size_t)f->H * f->W;
         DevBuf tmp;
         auto normalized = [&](float **p) -> piv_status {

@@ -108,6 +108,7 @@ inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 // frame's memory may be reused once the event has completed.
 struct UseEvent {
     hipEvent_t e = nullptr;
+    hipStream_t st = nullptr;  // the stream the event is recorded on
     ~UseEvent()
     {
         if (e) (void)hipEventDestroy(e);

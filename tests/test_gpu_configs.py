@@ -291,6 +291,42 @@ def test_batch_lanes_api_and_autotune(piv, monkeypatch):
             an.batch_lanes = bad
 
 
+def test_async_preparation_then_caller_stream(piv):
+    """Device frames are prepared asynchronously on the preparing analyzer's stream (u16 frames
+    with the default mask: the tables screen runs there).  An analysis on a stream the caller
+    passes, issued at once, waits for that preparation and gives the analyzer-stream result
+    bitwise; so do downloads and the table query issued right after preparation."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    st = C.c_void_p()
+    assert hip.hipStreamCreateWithFlags(C.byref(st), 1) == 0  # non-blocking, default priority
+    try:
+        h, w, n = 256, 320, 9
+        frames, _ = S.video_frames(h, w, n, 23, bits=12, container=np.uint16)
+        cfg = piv.PivConfig(windows=[48, 24], final_iterations=1)
+        clip = piv.DeviceArray.from_numpy(np.stack(frames))
+        g = None
+        outs = []
+        for stream in (None, st.value):
+            an = piv.PivAnalyzer(cfg, piv.PreprocessConfig.raw())
+            fs = piv.PreparedFrame.from_device_batch(an, clip)
+            if stream is not None:
+                fs[0].tables_built()
+                img, _ = fs[1].download()
+                assert np.array_equal(img, frames[1].astype(np.float32) / np.float32(65535))
+            g = an.output_grid(h, w)
+            dx, dy, pr = (piv.DeviceArray((n - 1, g.ny, g.nx), np.float32) for _ in range(3))
+            fl = piv.DeviceArray((n - 1, g.ny, g.nx), np.uint8)
+            sums = an.analyze_batch_device(fs[:-1], fs[1:], dx, dy, pr, fl, stream=stream)
+            outs.append((bits(dx.numpy()), bits(dy.numpy()), bits(pr.numpy()), fl.numpy(), sums))
+        for x, y in zip(outs[1][:4], outs[0][:4]):
+            assert np.array_equal(x, y)
+        assert outs[1][4] == outs[0][4]
+    finally:
+        hip.hipStreamDestroy(st)
+
+
 def test_batch_replay_matches_eager(piv):
     """Repeated batch calls with identical frames and outputs replay one captured graph of the
     pass chain.  Every replay, with other eager work and synchronisations between replays,
