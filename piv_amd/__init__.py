@@ -528,11 +528,17 @@ class PreparedFrame:
         _check(an._lib.piv_frame_prepare_device_batch(
             an._h, C.c_void_p(ptr_), fmt, n, shape[1], shape[2], stride, fstride,
             None if background is None else L.ptr(background), hs))
+        # one shared attribute set per frame, copied into each object (a clip can hold
+        # thousands of frames: this loop is a visible share of a video's preparation time)
+        base = {"_lib": an._lib, "shape": tuple(shape[1:]), "_owner": an,
+                "_keep": (arr, background)}
+        new, vp = PreparedFrame.__new__, C.c_void_p
         out = []
-        for i in range(n):
-            f = PreparedFrame()
-            f._h, f._lib, f.shape, f._owner = C.c_void_p(hs[i]), an._lib, tuple(shape[1:]), an
-            f._keep = (arr, background)
+        for hv in hs:
+            f = new(PreparedFrame)
+            d = f.__dict__
+            d.update(base)
+            d["_h"] = vp(hv)
             out.append(f)
         return out
 

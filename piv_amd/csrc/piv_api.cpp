@@ -777,20 +777,6 @@ static piv_status run_batch_impl(piv_analyzer *a, const piv_frame *const *fa, co
     HIP_TRY(hipStreamIsCapturing(st, &cap));
     if (cap != hipStreamCaptureStatusNone)
         return fail(PIV_ERR_UNSUPPORTED, "the analysis stream is being captured by the caller");
-    // Frames prepared (or last analysed) on another stream, e.g. the analyzer's own when the
-    // caller passes a stream: this stream waits for that point first.
-    {
-        const UseEvent *waited[8] = {};
-        int nw = 0;
-        for (const piv_frame *const *side : {fa, fb})
-            for (uint32_t i = 0; i < npairs; ++i) {
-                const UseEvent *ev = side[i]->last_use.get();
-                if (!ev || !ev->e || ev->st == st) continue;
-                if (std::find(waited, waited + nw, ev) != waited + nw) continue;
-                HIP_TRY(hipStreamWaitEvent(st, ev->e, 0));
-                if (nw < 8) waited[nw++] = ev;
-            }
-    }
     const bool same = !a->prof_on && sig == a->last_sig;
     if (same && a->graph_exec) {
         HIP_TRY(hipGraphLaunch(a->graph_exec, st));
@@ -897,6 +883,32 @@ piv_status frame_from_device(piv_analyzer *a, const void *pixels, int fmt, uint3
         return s;
     }
     *out = f;
+    return PIV_OK;
+}
+
+// Helper streams (default priority) and fork / join events for `lanes` lanes, created when a
+// count first needs them, so that an analyzer that keeps the default holds one helper stream.
+piv_status ensure_lanes(piv_analyzer *a, int lanes)
+{
+    if (lanes > 1 && !a->ev_fork) HIP_TRY(hipEventCreateWithFlags(&a->ev_fork, hipEventDisableTiming));
+    for (int i = 0; i < lanes - 1; ++i) {
+        if (!a->side[i]) HIP_TRY(hipStreamCreateWithFlags(&a->side[i], hipStreamNonBlocking));
+        if (!a->ev_join[i]) HIP_TRY(hipEventCreateWithFlags(&a->ev_join[i], hipEventDisableTiming));
+    }
+    return PIV_OK;
+}
+
+piv_status wait_frames(hipStream_t st, const piv_frame *const *fs, size_t n)
+{
+    const UseEvent *waited[8] = {};
+    int nw = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const UseEvent *ev = fs[i] ? fs[i]->last_use.get() : nullptr;
+        if (!ev || !ev->e || ev->st == st) continue;
+        if (std::find(waited, waited + nw, ev) != waited + nw) continue;
+        HIP_TRY(hipStreamWaitEvent(st, ev->e, 0));
+        if (nw < 8) waited[nw++] = ev;
+    }
     return PIV_OK;
 }
 
@@ -1025,14 +1037,13 @@ piv_status piv_analyzer_create(const piv_config *cfg, int device, piv_analyzer *
                 a->lanes_pinned = true;
             }
         }
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&a->ev_fork, hipEventDisableTiming);
-        for (int i = 0; e == hipSuccess && i < piv_analyzer::MAX_LANES - 1; ++i) {
-            e = hipStreamCreateWithFlags(&a->side[i], hipStreamNonBlocking);
-            if (e == hipSuccess) e = hipEventCreateWithFlags(&a->ev_join[i], hipEventDisableTiming);
-        }
         if (e != hipSuccess) {
             delete a;
             return fail(PIV_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+        }
+        if (const piv_status ls = pivi::ensure_lanes(a, a->lanes); ls != PIV_OK) {
+            delete a;
+            return ls;
         }
         if (cfg->smooth_sigma > 0.0f) {
             a->taps_r = gauss_taps(cfg->smooth_sigma, a->taps);
@@ -1289,6 +1300,8 @@ static piv_status analyze_impl(piv_analyzer *a, const piv_frame *fa, const piv_f
     std::vector<int32_t *> pk;
     uint32_t *sums = nullptr;
     const piv_frame *A[1] = {fa}, *B[1] = {fb};
+    PIV_TRY(wait_frames(a->stream, A, 1));
+    PIV_TRY(wait_frames(a->stream, B, 1));
     PIV_TRY(run_batch(a, A, B, 1, BatchOut{}, traces != nullptr, plans, raw, val, org, pk, &sums));
     hipStream_t st = a->stream;
     const size_t np = plans.size();
@@ -1371,7 +1384,9 @@ piv_status piv_analyze_batch_device(piv_analyzer *a, const piv_frame *const *fa,
         bo.dy = dy;
         bo.ppr = ppr;
         bo.flags = flags;
-        piv_status s = run_batch(a, fa, fb, npairs, bo, false, plans, raw, val, org, pk, &sums);
+        piv_status s = wait_frames(a->stream, fa, npairs);
+        if (s == PIV_OK) s = wait_frames(a->stream, fb, npairs);
+        if (s == PIV_OK) s = run_batch(a, fa, fb, npairs, bo, false, plans, raw, val, org, pk, &sums);
         if (s == PIV_OK && summaries) {
             const size_t np = plans.size();
             s = a->h_summaries.reserve(np * npairs * 5 * sizeof(uint32_t));
@@ -1450,6 +1465,10 @@ piv_status piv_correlate_windows(piv_analyzer *a, const piv_frame *fa, const piv
         }
         std::lock_guard<std::mutex> lk(a->mu);
         PIV_TRY(set_device(a->device));
+        {
+            const piv_frame *fs[2] = {fa, fb};
+            PIV_TRY(wait_frames(a->stream, fs, 2));
+        }
         const size_t o_pp = align_up(count * 16, 256);
         const size_t o_sig = o_pp + align_up(sizeof(PassPair), 256);
         const size_t o_rec = o_sig + align_up(count, 256);
@@ -1803,6 +1822,8 @@ static piv_status tune_lanes(piv_analyzer *a, const piv_frame *const *fa,
     for (int sweep = 0; sweep < 2 && s == PIV_OK; ++sweep)
     for (int l = 1; l <= ML && s == PIV_OK; ++l) {
         if (l > 1 && npairs / 4 < (uint32_t)l) continue;  // same chains as a smaller count
+        s = ensure_lanes(a, l);
+        if (s != PIV_OK) break;
         a->lanes = l;
         std::vector<PassPlan> pl;
         std::vector<FieldPtrs> raw, val;
@@ -1852,6 +1873,8 @@ piv_status piv_autotune(piv_analyzer *a, const piv_frame *const *fa, const piv_f
         if (repeats == 0) repeats = 1;
         std::lock_guard<std::mutex> lk(a->mu);
         PIV_TRY(set_device(a->device));
+        PIV_TRY(wait_frames(a->stream, fa, npairs));
+        PIV_TRY(wait_frames(a->stream, fb, npairs));
         const int fmt = fa[0]->fmt;
         std::vector<PassPlan> plans;
         PIV_TRY(plan_passes(a, fa[0]->H, fa[0]->W, plans));
@@ -1945,6 +1968,8 @@ piv_status piv_batch_lanes_set(piv_analyzer *a, uint32_t lanes)
             return fail(PIV_ERR_INVALID_ARGUMENT, "batch lanes %u outside 1..%d", lanes,
                         piv_analyzer::MAX_LANES);
         std::lock_guard<std::mutex> lk(a->mu);
+        PIV_TRY(set_device(a->device));
+        PIV_TRY(pivi::ensure_lanes(a, (int)lanes));
         a->lanes = (int)lanes;
         return PIV_OK;
     });

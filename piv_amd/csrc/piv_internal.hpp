@@ -353,6 +353,13 @@ piv_status check_pixels_args(const void *pixels, int fmt, uint32_t h, uint32_t w
 // Records the analyzer's use event of stream `st` after the work enqueued there and makes it
 // the last use of the `n` frames.
 piv_status mark_use(piv_analyzer *a, hipStream_t st, const piv_frame *const *fs, size_t n);
+// Creates the helper streams and events that `lanes` batch lanes use, if missing.
+piv_status ensure_lanes(piv_analyzer *a, int lanes);
+// Makes `st` wait for the last use of frames last used on another stream: device frames are
+// prepared asynchronously on their analyzer's stream, and a caller may analyse them on its own
+// stream or through another analyzer.  The public analysis entry points call it; the serving
+// scheduler orders its frames itself.
+piv_status wait_frames(hipStream_t st, const piv_frame *const *fs, size_t n);
 
 }  // namespace pivi
 
