@@ -886,8 +886,7 @@ piv_status frame_from_device(piv_analyzer *a, const void *pixels, int fmt, uint3
     return PIV_OK;
 }
 
-// Helper streams (default priority) and fork / join events for `lanes` lanes, created when a
-// count first needs them, so that an analyzer that keeps the default holds one helper stream.
+// Helper streams (default priority) and fork / join events for `lanes` lanes, if missing.
 piv_status ensure_lanes(piv_analyzer *a, int lanes)
 {
     if (lanes > 1 && !a->ev_fork) HIP_TRY(hipEventCreateWithFlags(&a->ev_fork, hipEventDisableTiming));
@@ -1041,7 +1040,8 @@ piv_status piv_analyzer_create(const piv_config *cfg, int device, piv_analyzer *
             delete a;
             return fail(PIV_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
         }
-        if (const piv_status ls = pivi::ensure_lanes(a, a->lanes); ls != PIV_OK) {
+        // every lane count's helper streams at creation (the autotuner and the caller may switch)
+        if (const piv_status ls = pivi::ensure_lanes(a, piv_analyzer::MAX_LANES); ls != PIV_OK) {
             delete a;
             return ls;
         }
