@@ -1003,9 +1003,15 @@ piv_status piv_grid_new(uint32_t height, uint32_t width, uint32_t window, float 
     });
 }
 
-piv_status piv_analyzer_create(const piv_config *cfg, int device, piv_analyzer **out)
+}  // extern "C"
+
+namespace pivi {
+
+// An analyzer; `batch_lanes` false (the serving streams' analyzers, whose batches the server
+// launches itself) creates no helper streams and pins one lane.
+piv_status analyzer_create(const piv_config *cfg, int device, piv_analyzer **out, bool batch_lanes)
 {
-    return pivi::guard(__func__, [&]() -> piv_status {
+    {
         if (!cfg || !out) return fail(PIV_ERR_INVALID_ARGUMENT, "null argument");
         *out = nullptr;
         PIV_TRY(check_config(*cfg));
@@ -1040,8 +1046,13 @@ piv_status piv_analyzer_create(const piv_config *cfg, int device, piv_analyzer *
             delete a;
             return fail(PIV_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
         }
+        if (!batch_lanes) {
+            a->lanes = 1;
+            a->lanes_pinned = true;
+        }
         // every lane count's helper streams at creation (the autotuner and the caller may switch)
-        if (const piv_status ls = pivi::ensure_lanes(a, piv_analyzer::MAX_LANES); ls != PIV_OK) {
+        if (const piv_status ls = ensure_lanes(a, batch_lanes ? piv_analyzer::MAX_LANES : 1);
+            ls != PIV_OK) {
             delete a;
             return ls;
         }
@@ -1059,6 +1070,17 @@ piv_status piv_analyzer_create(const piv_config *cfg, int device, piv_analyzer *
         }
         *out = a;
         return PIV_OK;
+    }
+}
+
+}  // namespace pivi
+
+extern "C" {
+
+piv_status piv_analyzer_create(const piv_config *cfg, int device, piv_analyzer **out)
+{
+    return pivi::guard(__func__, [&]() -> piv_status {
+        return pivi::analyzer_create(cfg, device, out, true);
     });
 }
 

@@ -355,6 +355,8 @@ piv_status check_pixels_args(const void *pixels, int fmt, uint32_t h, uint32_t w
 piv_status mark_use(piv_analyzer *a, hipStream_t st, const piv_frame *const *fs, size_t n);
 // Creates the helper streams and events that `lanes` batch lanes use, if missing.
 piv_status ensure_lanes(piv_analyzer *a, int lanes);
+// piv_analyzer_create; batch_lanes false: no helper streams, one lane (serving analyzers).
+piv_status analyzer_create(const piv_config *cfg, int device, piv_analyzer **out, bool batch_lanes);
 // Makes `st` wait for the last use of frames last used on another stream: device frames are
 // prepared asynchronously on their analyzer's stream, and a caller may analyse them on its own
 // stream or through another analyzer.  The public analysis entry points call it; the serving

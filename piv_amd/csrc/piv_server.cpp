@@ -867,7 +867,8 @@ piv_status piv_server_add_stream(piv_server *s, const piv_config *cfg, uint32_t 
             return fail(PIV_ERR_INVALID_ARGUMENT, "unknown pixel format %d", (int)fmt);
         auto *st = new (std::nothrow) SrvStream();
         if (!st) return fail(PIV_ERR_OOM, "host allocation failed");
-        piv_status r = piv_analyzer_create(cfg, s->device, &st->an);
+        // the server launches the stream's batches on its own lanes: no helper streams
+        piv_status r = analyzer_create(cfg, s->device, &st->an, false);
         if (r != PIV_OK) {
             delete st;
             return r;
