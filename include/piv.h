@@ -405,8 +405,9 @@ piv_status piv_autotune(piv_analyzer *a, const piv_frame *const *frames_a,
  * 4 pairs each) whose pass chains run side by side on helper streams, so that one group's
  * short per-vector kernels and launch tails overlap another group's correlation.  Results do
  * not depend on it.  The default is 2, or PIV_BATCH_LANES in the environment at analyzer
- * creation, which also pins it; otherwise piv_autotune ends by timing the caller's batch
- * (graph replays, two sweeps) at each count and keeps the fastest. */
+ * creation, which also pins it.  With PIV_TUNE_LANES=1 at creation (and no pin),
+ * piv_autotune ends by timing the caller's batch (graph replays, two sweeps) at each count
+ * and keeps the fastest. */
 piv_status piv_batch_lanes_set(piv_analyzer *a, uint32_t lanes);
 piv_status piv_batch_lanes_get(const piv_analyzer *a, uint32_t *lanes);
 

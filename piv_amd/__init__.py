@@ -402,7 +402,8 @@ class PivAnalyzer:
     @property
     def batch_lanes(self):
         """Groups of pairs whose pass chains a batch runs side by side (1..4; results do not
-        depend on it): 2, PIV_BATCH_LANES, or what autotune() or the caller last chose."""
+        depend on it): 2, PIV_BATCH_LANES, or what the caller (or autotune() with
+        PIV_TUNE_LANES=1) last chose."""
         v = C.c_uint32(0)
         _check(self._lib.piv_batch_lanes_get(self._h, C.byref(v)))
         return int(v.value)
@@ -413,8 +414,8 @@ class PivAnalyzer:
 
     def autotune(self, frames_a, frames_b, repeats=3):
         """Time every correlation-kernel variant on this batch and keep the fastest per window
-        size (variants give bit-identical results), then the batch lanes (batch_lanes) unless
-        PIV_BATCH_LANES pins them.  Returns one dict per window size."""
+        size (variants give bit-identical results); with PIV_TUNE_LANES=1 at the analyzer's
+        creation, then the batch lanes (batch_lanes).  Returns one dict per window size."""
         n = len(frames_a)
         A = (C.c_void_p * n)(*[f._h.value for f in frames_a])
         B = (C.c_void_p * n)(*[f._h.value for f in frames_b])

@@ -1050,8 +1050,13 @@ piv_status analyzer_create(const piv_config *cfg, int device, piv_analyzer **out
             a->lanes = 1;
             a->lanes_pinned = true;
         }
-        // every lane count's helper streams at creation (the autotuner and the caller may switch)
-        if (const piv_status ls = ensure_lanes(a, batch_lanes ? piv_analyzer::MAX_LANES : 1);
+        // The autotuner's lane sweep is opt-in (PIV_TUNE_LANES=1): processes that had loaded
+        // torch's HIP runtime first ended twice in a host segmentation fault with three or four
+        // lanes in use, cause not found.
+        if (const char *env = std::getenv("PIV_TUNE_LANES"))
+            a->tune_lanes = batch_lanes && !a->lanes_pinned && std::atoi(env) == 1;
+        // helper streams for the lane count in use (all counts when the autotuner may switch)
+        if (const piv_status ls = ensure_lanes(a, a->tune_lanes ? piv_analyzer::MAX_LANES : a->lanes);
             ls != PIV_OK) {
             delete a;
             return ls;
@@ -1977,7 +1982,7 @@ piv_status piv_autotune(piv_analyzer *a, const piv_frame *const *fa, const piv_f
             ++i;
         }
         *count = i;
-        if (!a->lanes_pinned) PIV_TRY(tune_lanes(a, fa, fb, npairs, repeats));
+        if (a->tune_lanes) PIV_TRY(tune_lanes(a, fa, fb, npairs, repeats));
         return PIV_OK;
     });
 }

@@ -234,6 +234,7 @@ struct piv_analyzer {
     static constexpr int MAX_LANES = 4;
     int lanes = 2;
     bool lanes_pinned = false;  // PIV_BATCH_LANES set: the autotuner keeps `lanes`
+    bool tune_lanes = false;    // PIV_TUNE_LANES=1: piv_autotune also sweeps the lane count
     hipStream_t side[MAX_LANES - 1] = {};
     hipEvent_t ev_fork = nullptr, ev_join[MAX_LANES - 1] = {};
     std::mutex mu;
