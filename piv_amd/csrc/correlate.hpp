@@ -783,17 +783,17 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
     using C = KCfg<N>;
     constexpr int WPB = C::WPB, P = C::P, PL = C::PL, H2 = N / 2;
     __shared__ float s_re[WPB * PL];
-    // a second plane: windows of more than one wave, and the deformed loader (window B)
-    __shared__ float s_im[(C::NARROW && DEF == 0) ? 1 : WPB * PL];
+    // a second plane for the deformed loader (window B)
+    __shared__ float s_im[DEF == 0 ? 1 : WPB * PL];
     __shared__ float s_red[WPB * N];
     __shared__ float s_pkv[WPB * N];
     __shared__ int s_pki[WPB * N];
-    // inverse column pass split over two lanes per column pair (one-plane windows, N >= 24;
-    // at N = 16 the shorter transform saves less than the twiddles cost): entry (p, n) = 1 for
-    // the even half (p = 0), e^{+2 pi i n / N} for the odd half (p = 1), n < N / 2
-    constexpr bool SPLIT_INV = C::NARROW && N >= 24;
-    // one-plane windows: the inverse by half-spectrum columns and row pairs, every lane running
-    // two N/2-point transforms (inverse_half_spectrum)
+    // inverse column pass split over two lanes per column pair (N >= 24; at N = 16 the
+    // shorter transform saves less than the twiddles cost): entry (p, n) = 1 for the even half
+    // (p = 0), e^{+2 pi i n / N} for the odd half (p = 1), n < N / 2
+    constexpr bool SPLIT_INV = N >= 24;
+    // windows of at most one wave's width: the inverse by half-spectrum columns and row pairs,
+    // every lane running two N/2-point transforms (inverse_half_spectrum)
     constexpr bool HALF_INV = C::NARROW && N >= 16;
     __shared__ float2 s_tw[(SPLIT_INV || HALF_INV) ? N : 1];
 
@@ -819,7 +819,7 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
     const uint32_t k = blk * WPB + wl;
     const bool active = k < cells;
     float *pr = s_re + wl * PL;
-    float *pim = (C::NARROW && DEF == 0) ? s_im : s_im + wl * PL;
+    float *pim = DEF == 0 ? s_im : s_im + wl * PL;
     float *red = s_red + wl * N;
     float *pkv = s_pkv + wl * N;
     int *pki = s_pki + wl * N;
@@ -871,7 +871,7 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
         ea32 = window_sum<N>(qa, red, lane) - wsa * wsa * inv_n2;
         eb32 = window_sum<N>(qb, red, lane) - wsb * wsb * inv_n2;
     }
-    if constexpr (C::NARROW) {
+    {
 #pragma unroll
         for (int r = 0; r < N; ++r) pr[r * P + lane] = zr[r];
         __syncthreads();
@@ -883,19 +883,6 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
         __syncthreads();
 #pragma unroll
         for (int x = 0; x < N; ++x) zi[x] = pr[lane * P + x];
-    } else {
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < N; ++r) {
-            pr[r * P + lane] = zr[r];
-            pim[r * P + lane] = zi[r];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int x = 0; x < N; ++x) {
-            zr[x] = pr[lane * P + x];
-            zi[x] = pim[lane * P + x];
-        }
     }
     pivfft::fft<N, -1>(zr, zi);  // lane ky holds Z[ky][kx]
 
@@ -996,17 +983,12 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
             ea2 = __builtin_fmaf(2.0f, ea2d, ea2);
             eb2 = __builtin_fmaf(2.0f, eb2d, eb2);
         } else {
-            __syncthreads();
-#pragma unroll
-            for (int x = 0; x < N; ++x) {
-                pr[lane * P + x] = zr[x];
-                pim[lane * P + x] = zi[x];
-            }
-            __syncthreads();
-#pragma unroll
-            for (int x = 0; x < N; ++x) {
-                const int mx = (N - x) % N;
-                const float qr = pr[mrow * P + mx], qi = pim[mrow * P + mx];
+            // Windows of two waves (N > 64): Z[-ky][-kx] comes through the window's plane in
+            // rounds of S column pairs (x, N - x), real and imaginary parts of both columns in
+            // 4 S <= N + 1 slots per row, so that no column is rewritten before its partner
+            // has read it.
+            constexpr int S = (N + 1) / 4;
+            auto cross = [&](int x, float qr, float qi) {
                 const float a2r = zr[x] + qr, a2i = zi[x] - qi;
                 const float b2r = zi[x] + qi, b2i = qr - zr[x];
                 const bool dc = (x == 0) && (lane == 0);
@@ -1018,6 +1000,32 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
                 const float xi = __builtin_fmaf(a2r, b2i, -(a2i * b2r));
                 zr[x] = dc ? 0.0f : xr;
                 zi[x] = dc ? 0.0f : xi;
+            };
+#pragma unroll
+            for (int x0 = 0; x0 <= H2; x0 += S) {
+                __syncthreads();
+#pragma unroll
+                for (int j = 0; j < S; ++j) {
+                    const int x = x0 + j;
+                    if (x <= H2) {
+                        const int mx = (N - x) % N;
+                        pr[lane * P + 4 * j] = zr[x];
+                        pr[lane * P + 4 * j + 1] = zi[x];
+                        pr[lane * P + 4 * j + 2] = zr[mx];
+                        pr[lane * P + 4 * j + 3] = zi[mx];
+                    }
+                }
+                __syncthreads();
+#pragma unroll
+                for (int j = 0; j < S; ++j) {
+                    const int x = x0 + j;
+                    if (x <= H2) {
+                        const int mx = (N - x) % N;
+                        const float *q = pr + mrow * P + 4 * j;
+                        cross(x, q[2], q[3]);
+                        if (mx != x) cross(mx, q[0], q[1]);
+                    }
+                }
             }
         }
     }
@@ -1043,7 +1051,7 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
     //   V[ky] = Y[ky][2j] + i Y[ky][2j+1]  ->  C[.][2j] + i C[.][2j+1].
     pivfft::fft<N, 1>(zr, zi);  // lane ky (<= N/2) holds Y[ky][x]
     const int jj = lane < H2 ? lane : lane - H2;
-    if constexpr (C::NARROW) {
+    {
         // Row owners write V's real parts (virtual rows ky and N-ky of the half plane), keep
         // V's imaginary parts in zi, then the column-pair lanes gather them in two passes.
         constexpr int HW = H2 + 1;  // odd row stride of the half-width plane
@@ -1072,33 +1080,6 @@ void k_correlate(const PassPair *__restrict__ pairs, uint32_t bpp, uint32_t nblo
         __syncthreads();
 #pragma unroll
         for (int ky = 0; ky < N; ++ky) zi[ky] = pr[ky * HW + jj];
-    } else {
-        __syncthreads();
-        if (lane <= H2) {
-#pragma unroll
-            for (int x = 0; x < N; ++x) {
-                pr[lane * P + x] = zr[x];
-                pim[lane * P + x] = zi[x];
-            }
-        }
-        __syncthreads();
-        const int c0 = 2 * jj, c1 = 2 * jj + 1;
-#pragma unroll
-        for (int ky = 0; ky < N; ++ky) {
-            if (ky <= H2) {
-                const float y0r = pr[ky * P + c0], y0i = pim[ky * P + c0];
-                const float y1r = pr[ky * P + c1], y1i = pim[ky * P + c1];
-                zr[ky] = y0r - y1i;
-                zi[ky] = y0i + y1r;
-            } else {
-                const int s = N - ky;
-                const float y0r = pr[s * P + c0], y0i = pim[s * P + c0];
-                const float y1r = pr[s * P + c1], y1i = pim[s * P + c1];
-                zr[ky] = y0r + y1i;
-                zi[ky] = y1r - y0i;
-            }
-            if ((ky & 7) == 7) __builtin_amdgcn_sched_barrier(0);
-        }
     }
     const int c0 = 2 * jj, c1 = 2 * jj + 1;
     if constexpr (SPLIT_INV) {
