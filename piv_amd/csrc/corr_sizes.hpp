@@ -4,6 +4,7 @@
 #pragma once
 
 #define PIV_FFT_SIZES(X) \
-    X(8) X(10) X(12) X(14) X(16) X(20) X(24) X(28) X(32) X(40) X(48) X(56) X(64) X(70) X(72) \
-    X(80) X(84) X(90) X(96) X(98) X(100) X(108) X(112) X(120) X(126) X(128)
+    X(8) X(10) X(12) X(14) X(16) X(18) X(20) X(24) X(28) X(30) X(32) X(36) X(40) X(42) X(48) \
+    X(50) X(54) X(56) X(60) X(64) X(70) X(72) X(80) X(84) X(90) X(96) X(98) X(100) X(108)    \
+    X(112) X(120) X(126) X(128)
 #define PIV_MF_SIZES(X) X(16) X(32) X(64)
