@@ -6,7 +6,8 @@
 // exchanged between lanes only at the 2-D transpose points, through LDS.
 //
 // The transform is a Stockham autosort FFT (natural order in and out) with radix-4 stages
-// first, then radix 2, 3, 5 and 7, so every even N = 2^a 3^b 5^c 7^d is covered.  All loops
+// first, then radix 2, 3, 5 and 7, then any other odd prime up to 31 as one symmetric-pair
+// DFT stage, so every N whose prime factors are at most 31 is covered.  All loops
 // are fully unrolled, indices and twiddles are compile-time constants: twiddles come from a
 // constexpr table evaluated in double and rounded to float, and multiplications by 1, -1, +-i are
 // elided.  Complex products use explicit fma so the rest of the library can be built with
@@ -110,7 +111,7 @@ constexpr int pick_radix(int rem)
 constexpr bool supported(int n)
 {
     if (n < 2) return false;
-    for (int p : {2, 3, 5, 7})
+    for (int p : {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31})
         while (n % p == 0) n /= p;
     return n == 1;
 }
@@ -281,7 +282,7 @@ __device__ __forceinline__ void dft(float (&vr)[R], float (&vi)[R])
         vi[1] = mi + qi;
         vr[2] = mr - qr;
         vi[2] = mi - qi;
-    } else if constexpr (R == 5 || R == 7) {
+    } else if constexpr (R >= 5 && R % 2 == 1) {
         // odd prime R from the symmetric pairs j, R - j: with s_j = v_j + v_{R-j} and
         // d_j = v_j - v_{R-j},  Y_m = v_0 + sum_j cos(2 pi j m / R) s_j + S i sum_j
         // sin(2 pi j m / R) d_j,  and Y_{R-m} the same with -S
@@ -321,7 +322,7 @@ __device__ __forceinline__ void dft(float (&vr)[R], float (&vi)[R])
         vr[0] = y0r;
         vi[0] = y0i;
     } else {
-        static_assert(R == 2 || R == 3 || R == 4 || R == 5 || R == 7, "radix");
+        static_assert(R == 2 || R == 3 || R == 4 || (R >= 5 && R % 2 == 1), "radix");
     }
 }
 
@@ -374,7 +375,7 @@ __device__ __forceinline__ void stockham(float (&xr)[N], float (&xi)[N])
 template <int N, int S>
 __device__ __forceinline__ void fft(float (&xr)[N], float (&xi)[N])
 {
-    static_assert(supported(N), "N must be 2^a 3^b 5^c 7^d");
+    static_assert(supported(N), "N must have prime factors of at most 31");
     stockham<N, S, 1, N>(xr, xi);
 }
 
